@@ -1,0 +1,592 @@
+// C ABI of the 2D Euler DFR solver (include/gocfd_euler2d.h): host
+// orchestration of the device-resident solver.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "dfr_setup.hpp"
+#include "gocfd_euler2d.h"
+#include "kernels.hpp"
+
+using namespace gocfd;
+
+namespace {
+thread_local std::string g_last_error;
+
+int fail(int code, const std::string &msg) {
+  g_last_error = msg;
+  return code;
+}
+
+#define HIPCHECK(expr)                                                                     \
+  do {                                                                                     \
+    hipError_t e_ = (expr);                                                                \
+    if (e_ != hipSuccess)                                                                  \
+      throw SetupError(GOCFD_ERR_DEVICE, std::string(#expr) + ": " + hipGetErrorString(e_)); \
+  } while (0)
+
+template <class T>
+struct DevBuf {
+  T *p = nullptr;
+  size_t n = 0;
+  void alloc(size_t count) {
+    free();
+    if (count == 0) count = 1;
+    HIPCHECK(hipMalloc(&p, count * sizeof(T)));
+    n = count;
+  }
+  void upload(const std::vector<T> &v) {
+    alloc(v.size());
+    if (!v.empty()) HIPCHECK(hipMemcpy(p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice));
+  }
+  void free() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    n = 0;
+  }
+  ~DevBuf() { free(); }
+};
+
+// Freestream (model_problems/Euler2D/fluids.go:245-273)
+struct FreeStream {
+  double gamma = 1.4, minf = 0, alpha = 0, q[4] = {1, 0, 0, 0}, pinf = 0, cinf = 0;
+  void from_qinf(double g, const double qq[4]) {
+    gamma = g;
+    for (int i = 0; i < 4; ++i) q[i] = qq[i];
+    const double oorho = 1. / q[0], u = q[1] * oorho, v = q[2] * oorho;
+    pinf = (g - 1.) * (q[3] - 0.5 * q[0] * (u * u + v * v));
+    cinf = std::sqrt(std::fabs(g * pinf * oorho));
+  }
+  void from_mach(double M, double g, double a) {
+    const double ooggm1 = 1. / (g * (g - 1.));
+    const double qq[4] = {1, M * std::cos(a * M_PI / 180.), M * std::sin(a * M_PI / 180.), ooggm1 + 0.5 * M * M};
+    from_qinf(g, qq);
+    minf = M;
+    alpha = a;
+  }
+};
+
+// IVortex.GetStateC (isentropic_vortex/analytic_vortex.go:31-82), host copy for initialisation
+void ivortex_host(double t, double x, double y, double q[4]) {
+  const double beta = 5., X0 = 5., Y0 = 0., G = 1.4;
+  const double oo2pi = 0.5 * (1. / M_PI), GM1 = G - 1, OOGM1 = 1. / GM1, fac = 16 * G * (M_PI * M_PI);
+  double u = 1.0, v = 0.;
+  const double xm = x - u * t, ym = y - v * t;
+  const double r2 = (xm - X0) * (xm - X0) + (ym - Y0) * (ym - Y0);
+  const double ex1r = std::exp(1 - r2);
+  const double tv1 = 1.0 - (GM1 * beta * beta * std::exp(2.0 * (1.0 - r2)) / fac);
+  u -= beta * ex1r * (ym - Y0) * oo2pi;
+  v += beta * ex1r * (xm - X0) * oo2pi;
+  const double rho = std::pow(tv1, OOGM1), p = std::pow(rho, G);
+  q[0] = rho; q[1] = rho * u; q[2] = rho * v; q[3] = p * (1. / (G - 1.)) + 0.5 * rho * (u * u + v * v);
+}
+}  // namespace
+
+struct gocfd_mesh {
+  Mesh m;
+};
+
+struct gocfd_euler2d {
+  gocfd_params prm{};
+  Discretisation d;
+  DeviceLayout L;
+  FreeStream fs;
+  int device = 0;
+  int prec = 0;
+  size_t ssize = 8;  // bytes per stored state value
+  hipStream_t stream = nullptr;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  // state (element-major, Kp * NV values of the storage type)
+  void *q[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
+  void *R = nullptr;
+  void *scratch_in = nullptr;
+  DevBuf<double> rhs_out;
+  DevBuf<double> Jdet, Jinv, IInII, Nx, Ny, oohK, bxy, Interp64;
+  DevBuf<int> code, nbr_slot, halo_elem, halo_face;
+  DevBuf<double> DivInt_d, Interp_d;
+  DevBuf<float> DivInt_f, Interp_f;
+  dev::Ctl *ctl = nullptr;
+  long long gstage = 0;  // stage launches since the control block was initialised
+  int step_index = 0;    // steps since the control block was initialised
+  int steps_base = 0;    // RK.StepCount before the control block was initialised
+  double last_batch_ms = 0;
+  int last_batch_launches = 0;
+  int NV = 0;
+
+  ~gocfd_euler2d() {
+    if (stream) (void)hipStreamSynchronize(stream);
+    for (auto &p : q) if (p) (void)hipFree(p);
+    if (R) (void)hipFree(R);
+    if (scratch_in) (void)hipFree(scratch_in);
+    if (ctl) (void)hipFree(ctl);
+    if (ev0) (void)hipEventDestroy(ev0);
+    if (ev1) (void)hipEventDestroy(ev1);
+    if (stream) (void)hipStreamDestroy(stream);
+  }
+
+  dev::StageArgs args() const {
+    dev::StageArgs a{};
+    a.K = L.K; a.Kp = L.Kp; a.ntiles = L.ntiles; a.Hmax = L.Hmax;
+    a.q0 = q[0]; a.q1 = q[1]; a.q2 = q[2]; a.q3 = q[3]; a.q4 = q[4]; a.R = R;
+    a.ctl = ctl;
+    a.g.Jdet = Jdet.p; a.g.Jinv = Jinv.p; a.g.IInII = IInII.p; a.g.Nx = Nx.p; a.g.Ny = Ny.p;
+    a.g.oohK = oohK.p; a.g.code = code.p; a.g.nbr_slot = nbr_slot.p;
+    a.g.halo_elem = halo_elem.p; a.g.halo_face = halo_face.p; a.g.bxy = bxy.p;
+    if (prec == 0) { a.DivInt = DivInt_d.p; a.Interp = Interp_d.p; }
+    else { a.DivInt = DivInt_f.p; a.Interp = Interp_f.p; }
+    a.Interp64 = Interp64.p;
+    a.ph.gamma = fs.gamma; a.ph.minf = fs.minf; a.ph.pinf = fs.pinf; a.ph.cinf = fs.cinf;
+    for (int i = 0; i < 4; ++i) a.ph.qinf[i] = fs.q[i];
+    a.ph.cfl = prm.cfl; a.ph.final_time = prm.final_time; a.ph.flux = prm.flux_type;
+    return a;
+  }
+
+  // reference layout [4][NpInt][K] (double) <-> element-major device buffer
+  void to_device(const double *Q, void *dst) {
+    const int K = L.K, NpInt = d.ops.NpInt;
+    const size_t tot = size_t(L.Kp) * NV;
+    if (prec == 0) {
+      std::vector<double> h(tot);
+      fill_host(Q, h.data(), K, NpInt);
+      HIPCHECK(hipMemcpy(dst, h.data(), tot * sizeof(double), hipMemcpyHostToDevice));
+    } else {
+      std::vector<float> h(tot);
+      fill_host(Q, h.data(), K, NpInt);
+      HIPCHECK(hipMemcpy(dst, h.data(), tot * sizeof(float), hipMemcpyHostToDevice));
+    }
+  }
+  template <class T>
+  void fill_host(const double *Q, T *h, int K, int NpInt) {
+    const double pad[4] = {1.0, 0.0, 0.0, 2.5};
+    for (int kk = 0; kk < L.Kp; ++kk) {
+      const int k = L.perm[kk];
+      for (int n = 0; n < 4; ++n)
+        for (int i = 0; i < NpInt; ++i)
+          h[size_t(kk) * NV + n * NpInt + i] =
+              T(k < 0 ? pad[n] : Q[size_t(n) * NpInt * K + size_t(i) * K + k]);
+    }
+  }
+  void from_device(const void *src, double *Q, bool is_double) {
+    const int K = L.K, NpInt = d.ops.NpInt;
+    const size_t tot = size_t(L.Kp) * NV;
+    if (is_double) {
+      std::vector<double> h(tot);
+      HIPCHECK(hipMemcpy(h.data(), src, tot * sizeof(double), hipMemcpyDeviceToHost));
+      read_host(h.data(), Q, K, NpInt);
+    } else {
+      std::vector<float> h(tot);
+      HIPCHECK(hipMemcpy(h.data(), src, tot * sizeof(float), hipMemcpyDeviceToHost));
+      read_host(h.data(), Q, K, NpInt);
+    }
+  }
+  template <class T>
+  void read_host(const T *h, double *Q, int K, int NpInt) {
+    for (int kk = 0; kk < L.K; ++kk) {
+      const int k = L.perm[kk];
+      for (int n = 0; n < 4; ++n)
+        for (int i = 0; i < NpInt; ++i)
+          Q[size_t(n) * NpInt * K + size_t(i) * K + k] = double(h[size_t(kk) * NV + n * NpInt + i]);
+    }
+  }
+
+  dev::Ctl read_ctl() {
+    dev::Ctl c;
+    HIPCHECK(hipStreamSynchronize(stream));
+    HIPCHECK(hipMemcpy(&c, ctl, sizeof(c), hipMemcpyDeviceToHost));
+    return c;
+  }
+
+  // (Re)start the step sequence from the state in q[0] at `time`
+  void restart(double time) {
+    HIPCHECK(dev::launch_ctl_init(ctl, time, stream));
+    gstage = 0;
+    step_index = 0;
+    dev::StageArgs a = args();
+    a.qs = q[0];
+    a.ring = 0;
+    a.step_index = 0;
+    HIPCHECK(dev::launch_prologue(prm.polynomial_order, prec, a, stream));
+  }
+
+  void enqueue_steps(int nsteps) {
+    dev::StageArgs a = args();
+    HIPCHECK(hipEventRecord(ev0, stream));
+    for (int j = 0; j < nsteps; ++j) {
+      for (int s = 0; s < 5; ++s) {
+        a.stage = s;
+        a.step_index = step_index;
+        a.ring = int(gstage % 3);
+        a.qs = q[s];
+        HIPCHECK(dev::launch_stage(prm.polynomial_order, prec, a, stream));
+        ++gstage;
+      }
+      ++step_index;
+    }
+    HIPCHECK(hipEventRecord(ev1, stream));
+    last_batch_launches = 5 * nsteps;
+  }
+
+  int finish_batch() {
+    HIPCHECK(hipEventSynchronize(ev1));
+    float ms = 0;
+    HIPCHECK(hipEventElapsedTime(&ms, ev0, ev1));
+    last_batch_ms = ms;
+    dev::Ctl c = read_ctl();
+    if (c.nan) return fail(GOCFD_ERR_NAN, "NAN found");
+    return GOCFD_OK;
+  }
+};
+
+#define GUARD_BEGIN try {
+#define NEED_DEVICE(h) \
+  if ((h)->device < 0) return fail(GOCFD_ERR_DEVICE, "handle was created without a device");
+#define GUARD_END                                         \
+  }                                                       \
+  catch (const SetupError &e) { return fail(e.code, e.what()); } \
+  catch (const std::exception &e) { return fail(GOCFD_ERR_ARG, e.what()); }
+
+extern "C" {
+
+const char *gocfd_last_error(void) { return g_last_error.c_str(); }
+
+int gocfd_mesh_read(const char *path, gocfd_mesh **out) {
+  GUARD_BEGIN
+  if (!path || !out) return fail(GOCFD_ERR_ARG, "null argument");
+  auto m = std::make_unique<gocfd_mesh>();
+  m->m = read_mesh(path);
+  *out = m.release();
+  return GOCFD_OK;
+  GUARD_END
+}
+
+int gocfd_mesh_from_arrays(int32_t K, int32_t Nv, const double *VX, const double *VY, const int32_t *EToV,
+                           int32_t ntags, const char *const *tags, const int32_t *edge_offset,
+                           const int32_t *edge_count, const int32_t *edge_verts, gocfd_mesh **out) {
+  GUARD_BEGIN
+  if (K <= 0 || Nv <= 0 || !VX || !VY || !EToV || !out) return fail(GOCFD_ERR_ARG, "bad mesh arrays");
+  auto m = std::make_unique<gocfd_mesh>();
+  m->m.K = K;
+  m->m.Nv = Nv;
+  m->m.VX.assign(VX, VX + Nv);
+  m->m.VY.assign(VY, VY + Nv);
+  m->m.EToV.assign(EToV, EToV + size_t(3) * K);
+  for (int t = 0; t < ntags; ++t) {
+    std::vector<std::pair<int, int>> e(edge_count[t]);
+    for (int j = 0; j < edge_count[t]; ++j)
+      e[j] = {edge_verts[2 * (size_t(edge_offset[t]) + j)], edge_verts[2 * (size_t(edge_offset[t]) + j) + 1]};
+    std::string tag(tags[t]);
+    for (auto &c : tag) c = char(std::tolower((unsigned char)c));
+    m->m.add_tag(tag, std::move(e));
+  }
+  *out = m.release();
+  return GOCFD_OK;
+  GUARD_END
+}
+
+int gocfd_mesh_dims(const gocfd_mesh *m, int32_t *K, int32_t *Nv, int32_t *ntags) {
+  if (!m) return fail(GOCFD_ERR_ARG, "null mesh");
+  if (K) *K = m->m.K;
+  if (Nv) *Nv = m->m.Nv;
+  if (ntags) *ntags = int32_t(m->m.tags.size());
+  return GOCFD_OK;
+}
+
+int gocfd_mesh_arrays(const gocfd_mesh *m, double *VX, double *VY, int32_t *EToV) {
+  if (!m) return fail(GOCFD_ERR_ARG, "null mesh");
+  if (VX) std::memcpy(VX, m->m.VX.data(), sizeof(double) * m->m.Nv);
+  if (VY) std::memcpy(VY, m->m.VY.data(), sizeof(double) * m->m.Nv);
+  if (EToV) for (size_t i = 0; i < m->m.EToV.size(); ++i) EToV[i] = m->m.EToV[i];
+  return GOCFD_OK;
+}
+
+void gocfd_mesh_free(gocfd_mesh *m) { delete m; }
+
+int gocfd_euler2d_create(const gocfd_mesh *mesh, const gocfd_params *p, gocfd_euler2d **out) {
+  GUARD_BEGIN
+  if (!mesh || !p || !out) return fail(GOCFD_ERR_ARG, "null argument");
+  *out = nullptr;
+  if (p->polynomial_order < 0 || p->polynomial_order > 4)
+    return fail(GOCFD_ERR_ARG, "polynomial order must be 0..4 on the GPU path");
+  if (p->flux_type < 0 || p->flux_type > 3) return fail(GOCFD_ERR_ARG, "unable to use flux type");
+  if (p->init_type != GOCFD_INIT_FREESTREAM && p->init_type != GOCFD_INIT_IVORTEX)
+    return fail(GOCFD_ERR_ARG, "init type not supported on the GPU path (freestream, ivortex)");
+  if (p->local_time_stepping) return fail(GOCFD_ERR_ARG, "LocalTimeStepping is not supported on the GPU path");
+  if (p->limiter != 0) return fail(GOCFD_ERR_ARG, "the PerssonC0 limiter is not supported on the GPU path");
+  if (p->precision != GOCFD_PREC_FP64 && p->precision != GOCFD_PREC_FP32)
+    return fail(GOCFD_ERR_ARG, "unknown precision");
+  auto h = std::make_unique<gocfd_euler2d>();
+  h->prm = *p;
+  h->prec = p->precision;
+  h->ssize = h->prec == 0 ? sizeof(double) : sizeof(float);
+  h->d = build_discretisation(mesh->m, p->polynomial_order);
+  // InitializeSolution (euler.go:728-794)
+  std::vector<double> Q0(size_t(4) * h->d.ops.NpInt * h->d.K);
+  const size_t npts = size_t(h->d.ops.NpInt) * h->d.K;
+  if (p->init_type == GOCFD_INIT_IVORTEX) {
+    const double qq[4] = {1, 1, 0, 3};
+    h->fs.from_qinf(1.4, qq);
+    for (auto &E : h->d.edges)
+      if (E.bc == BC_Wall) E.bc = BC_IVortex;
+    for (size_t i = 0; i < npts; ++i) {
+      double q[4];
+      ivortex_host(0, h->d.SolX[i], h->d.SolY[i], q);
+      for (int n = 0; n < 4; ++n) Q0[n * npts + i] = q[n];
+    }
+  } else {
+    h->fs.from_mach(p->minf, p->gamma, p->alpha);
+    for (int n = 0; n < 4; ++n)
+      for (size_t i = 0; i < npts; ++i) Q0[n * npts + i] = 0.0 + h->fs.q[n];
+  }
+  h->L = build_device_layout(h->d, dev::kTile);
+  h->NV = 4 * h->d.ops.NpInt;
+  if (p->device < 0) {  // host-only handle: setup and introspection, no device work
+    h->device = -1;
+    *out = h.release();
+    return GOCFD_OK;
+  }
+  // Device
+  int ndev = 0;
+  HIPCHECK(hipGetDeviceCount(&ndev));
+  if (p->device < 0 || p->device >= ndev) return fail(GOCFD_ERR_DEVICE, "no such HIP device");
+  h->device = p->device;
+  HIPCHECK(hipSetDevice(h->device));
+  hipDeviceProp_t prop;
+  HIPCHECK(hipGetDeviceProperties(&prop, h->device));
+  if (std::strncmp(prop.gcnArchName, "gfx950", 6) != 0)
+    return fail(GOCFD_ERR_DEVICE, std::string("device is not gfx950: ") + prop.gcnArchName);
+  const size_t lds = dev::stage_lds_bytes(p->polynomial_order, h->prec, h->L.Hmax) + 64;
+  int lds_max = 0, lds_optin = 0;
+  HIPCHECK(hipDeviceGetAttribute(&lds_max, hipDeviceAttributeMaxSharedMemoryPerBlock, h->device));
+  if (hipDeviceGetAttribute(&lds_optin, hipDeviceAttributeSharedMemPerBlockOptin, h->device) == hipSuccess)
+    lds_max = std::max(lds_max, lds_optin);
+  if (lds > size_t(lds_max))
+    return fail(GOCFD_ERR_SETUP, "stage kernel needs " + std::to_string(lds) + " B of LDS, device allows " +
+                                     std::to_string(lds_max));
+  HIPCHECK(hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking));
+  HIPCHECK(hipEventCreate(&h->ev0));
+  HIPCHECK(hipEventCreate(&h->ev1));
+  const size_t tot = size_t(h->L.Kp) * h->NV * h->ssize;
+  for (auto &qq : h->q) HIPCHECK(hipMalloc(&qq, tot));
+  HIPCHECK(hipMalloc(&h->R, tot));
+  HIPCHECK(hipMemset(h->R, 0, tot));
+  for (int s = 1; s < 5; ++s) HIPCHECK(hipMemset(h->q[s], 0, tot));
+  HIPCHECK(hipMalloc(&h->ctl, sizeof(dev::Ctl)));
+  auto &L = h->L;
+  h->Jdet.upload(L.Jdet); h->Jinv.upload(L.Jinv); h->IInII.upload(L.IInII);
+  h->Nx.upload(L.Nx); h->Ny.upload(L.Ny); h->oohK.upload(L.oohK);
+  h->bxy.upload(L.bpoint_xy);
+  h->code.upload(L.face_code); h->nbr_slot.upload(L.nbr_slot);
+  h->halo_elem.upload(L.halo_elem); h->halo_face.upload(L.halo_face);
+  h->DivInt_d.upload(h->d.ops.DivInt); h->Interp_d.upload(h->d.ops.Interp);
+  h->Interp64.upload(h->d.ops.Interp);
+  h->DivInt_f.upload(std::vector<float>(h->d.ops.DivInt.begin(), h->d.ops.DivInt.end()));
+  h->Interp_f.upload(std::vector<float>(h->d.ops.Interp.begin(), h->d.ops.Interp.end()));
+  h->to_device(Q0.data(), h->q[0]);
+  h->restart(0.0);
+  HIPCHECK(hipStreamSynchronize(h->stream));
+  *out = h.release();
+  return GOCFD_OK;
+  GUARD_END
+}
+
+void gocfd_euler2d_destroy(gocfd_euler2d *h) { delete h; }
+
+int gocfd_euler2d_dims(const gocfd_euler2d *h, int32_t *K, int32_t *NpInt, int32_t *NpEdge, int32_t *NpFlux,
+                       int32_t *nedges) {
+  if (!h) return fail(GOCFD_ERR_ARG, "null handle");
+  if (K) *K = h->d.K;
+  if (NpInt) *NpInt = h->d.ops.NpInt;
+  if (NpEdge) *NpEdge = h->d.ops.NpEdge;
+  if (NpFlux) *NpFlux = h->d.ops.NpFlux;
+  if (nedges) *nedges = int32_t(h->d.edges.size());
+  return GOCFD_OK;
+}
+
+int gocfd_euler2d_set_state(gocfd_euler2d *h, const double *Q) {
+  GUARD_BEGIN
+  if (!h || !Q) return fail(GOCFD_ERR_ARG, "null argument");
+  NEED_DEVICE(h);
+  HIPCHECK(hipSetDevice(h->device));
+  dev::Ctl c = h->read_ctl();
+  h->steps_base += c.steps;
+  h->to_device(Q, h->q[0]);
+  h->restart(c.time[c.steps & 1]);
+  HIPCHECK(hipStreamSynchronize(h->stream));
+  return GOCFD_OK;
+  GUARD_END
+}
+
+int gocfd_euler2d_get_state(gocfd_euler2d *h, double *Q) {
+  GUARD_BEGIN
+  if (!h || !Q) return fail(GOCFD_ERR_ARG, "null argument");
+  NEED_DEVICE(h);
+  HIPCHECK(hipSetDevice(h->device));
+  HIPCHECK(hipStreamSynchronize(h->stream));
+  h->from_device(h->q[0], Q, h->prec == 0);
+  return GOCFD_OK;
+  GUARD_END
+}
+
+int gocfd_euler2d_get_residual(gocfd_euler2d *h, double *R) {
+  GUARD_BEGIN
+  if (!h || !R) return fail(GOCFD_ERR_ARG, "null argument");
+  NEED_DEVICE(h);
+  HIPCHECK(hipSetDevice(h->device));
+  HIPCHECK(hipStreamSynchronize(h->stream));
+  h->from_device(h->R, R, h->prec == 0);
+  return GOCFD_OK;
+  GUARD_END
+}
+
+int gocfd_euler2d_rhs(gocfd_euler2d *h, const double *Q, double t, double *RHS) {
+  GUARD_BEGIN
+  if (!h || !Q || !RHS) return fail(GOCFD_ERR_ARG, "null argument");
+  NEED_DEVICE(h);
+  HIPCHECK(hipSetDevice(h->device));
+  const size_t tot = size_t(h->L.Kp) * h->NV;
+  if (!h->scratch_in) HIPCHECK(hipMalloc(&h->scratch_in, tot * h->ssize));
+  if (h->rhs_out.n < tot) h->rhs_out.alloc(tot);
+  h->to_device(Q, h->scratch_in);
+  dev::StageArgs a = h->args();
+  a.stage = -1;
+  a.t_rhs = t;
+  a.qs = h->scratch_in;
+  a.rhs_out = h->rhs_out.p;
+  HIPCHECK(dev::launch_stage(h->prm.polynomial_order, h->prec, a, h->stream));
+  HIPCHECK(hipStreamSynchronize(h->stream));
+  h->from_device(h->rhs_out.p, RHS, true);
+  return GOCFD_OK;
+  GUARD_END
+}
+
+int gocfd_euler2d_step_async(gocfd_euler2d *h, int32_t nsteps) {
+  GUARD_BEGIN
+  if (!h || nsteps < 0) return fail(GOCFD_ERR_ARG, "bad argument");
+  NEED_DEVICE(h);
+  HIPCHECK(hipSetDevice(h->device));
+  h->enqueue_steps(nsteps);
+  return GOCFD_OK;
+  GUARD_END
+}
+
+int gocfd_euler2d_sync(gocfd_euler2d *h) {
+  GUARD_BEGIN
+  if (!h) return fail(GOCFD_ERR_ARG, "null handle");
+  NEED_DEVICE(h);
+  HIPCHECK(hipSetDevice(h->device));
+  return h->finish_batch();
+  GUARD_END
+}
+
+int gocfd_euler2d_step(gocfd_euler2d *h, int32_t nsteps) {
+  GUARD_BEGIN
+  if (!h || nsteps < 0) return fail(GOCFD_ERR_ARG, "bad argument");
+  NEED_DEVICE(h);
+  HIPCHECK(hipSetDevice(h->device));
+  h->enqueue_steps(nsteps);
+  return h->finish_batch();
+  GUARD_END
+}
+
+int gocfd_euler2d_time(gocfd_euler2d *h, double *time, double *global_dt, int32_t *steps) {
+  GUARD_BEGIN
+  if (!h) return fail(GOCFD_ERR_ARG, "null handle");
+  NEED_DEVICE(h);
+  HIPCHECK(hipSetDevice(h->device));
+  dev::Ctl c = h->read_ctl();
+  const int par = c.steps & 1;
+  if (time) *time = c.time[par];
+  if (global_dt) *global_dt = c.dt[4];
+  if (steps) *steps = h->steps_base + c.steps;
+  return GOCFD_OK;
+  GUARD_END
+}
+
+int gocfd_euler2d_set_time(gocfd_euler2d *h, double time) {
+  GUARD_BEGIN
+  if (!h) return fail(GOCFD_ERR_ARG, "null handle");
+  NEED_DEVICE(h);
+  HIPCHECK(hipSetDevice(h->device));
+  dev::Ctl c = h->read_ctl();
+  h->steps_base += c.steps;
+  h->restart(time);
+  HIPCHECK(hipStreamSynchronize(h->stream));
+  return GOCFD_OK;
+  GUARD_END
+}
+
+int gocfd_euler2d_get_edges(gocfd_euler2d *h, int64_t *out) {
+  if (!h || !out) return fail(GOCFD_ERR_ARG, "null argument");
+  for (size_t i = 0; i < h->d.edges.size(); ++i) {
+    const Edge &E = h->d.edges[i];
+    int64_t *r = out + 10 * i;
+    r[0] = int64_t(E.key & 0xffffffffu); r[1] = int64_t(E.key >> 32); r[2] = E.nconn;
+    r[3] = E.tri[0]; r[4] = E.nconn > 1 ? E.tri[1] : -1;
+    r[5] = E.enr[0]; r[6] = E.nconn > 1 ? E.enr[1] : -1;
+    r[7] = E.dir[0]; r[8] = E.nconn > 1 ? E.dir[1] : -1; r[9] = E.bc;
+  }
+  return GOCFD_OK;
+}
+
+int gocfd_euler2d_get_connectivity(gocfd_euler2d *h, int32_t *EtoE, int32_t *FaceEdge) {
+  if (!h) return fail(GOCFD_ERR_ARG, "null handle");
+  if (EtoE) for (size_t i = 0; i < h->d.EtoE.size(); ++i) EtoE[i] = h->d.EtoE[i];
+  if (FaceEdge) for (size_t i = 0; i < h->d.FaceEdge.size(); ++i) FaceEdge[i] = h->d.FaceEdge[i];
+  return GOCFD_OK;
+}
+
+int gocfd_euler2d_get_operators(gocfd_euler2d *h, double *DivInt, double *Div, double *Interp, double *R, double *S) {
+  if (!h) return fail(GOCFD_ERR_ARG, "null handle");
+  const auto &o = h->d.ops;
+  if (DivInt) std::memcpy(DivInt, o.DivInt.data(), o.DivInt.size() * sizeof(double));
+  if (Div) std::memcpy(Div, o.Div.data(), o.Div.size() * sizeof(double));
+  if (Interp) std::memcpy(Interp, o.Interp.data(), o.Interp.size() * sizeof(double));
+  if (R) std::memcpy(R, o.R.data(), o.R.size() * sizeof(double));
+  if (S) std::memcpy(S, o.S.data(), o.S.size() * sizeof(double));
+  return GOCFD_OK;
+}
+
+int gocfd_euler2d_get_geometry(gocfd_euler2d *h, double *Jdet, double *Jinv, double *FaceNx, double *FaceNy,
+                               double *IInII, double *EdgeLenMax) {
+  if (!h) return fail(GOCFD_ERR_ARG, "null handle");
+  const auto &d = h->d;
+  auto cp = [](double *dst, const std::vector<double> &v) {
+    if (dst) std::memcpy(dst, v.data(), v.size() * sizeof(double));
+  };
+  cp(Jdet, d.Jdet); cp(Jinv, d.Jinv); cp(FaceNx, d.FaceNx); cp(FaceNy, d.FaceNy);
+  cp(IInII, d.IInII); cp(EdgeLenMax, d.EdgeLenMax);
+  return GOCFD_OK;
+}
+
+int gocfd_euler2d_get_coords(gocfd_euler2d *h, double *SolX, double *SolY, double *FluxX, double *FluxY) {
+  if (!h) return fail(GOCFD_ERR_ARG, "null handle");
+  const auto &d = h->d;
+  auto cp = [](double *dst, const std::vector<double> &v) {
+    if (dst) std::memcpy(dst, v.data(), v.size() * sizeof(double));
+  };
+  cp(SolX, d.SolX); cp(SolY, d.SolY); cp(FluxX, d.FluxX); cp(FluxY, d.FluxY);
+  return GOCFD_OK;
+}
+
+int gocfd_euler2d_kernel_time(gocfd_euler2d *h, double *avg_stage_ms, int32_t *launches) {
+  if (!h) return fail(GOCFD_ERR_ARG, "null handle");
+  if (avg_stage_ms)
+    *avg_stage_ms = h->last_batch_launches ? h->last_batch_ms / h->last_batch_launches : 0.0;
+  if (launches) *launches = h->last_batch_launches;
+  return GOCFD_OK;
+}
+
+int gocfd_euler2d_tiling(gocfd_euler2d *h, int32_t *tile, int32_t *ntiles, int32_t *max_halo) {
+  if (!h) return fail(GOCFD_ERR_ARG, "null handle");
+  if (tile) *tile = h->L.E;
+  if (ntiles) *ntiles = h->L.ntiles;
+  if (max_halo) *max_halo = h->L.Hmax;
+  return GOCFD_OK;
+}
+
+}  // extern "C"
