@@ -1,0 +1,196 @@
+#!/usr/bin/env python3
+"""Benchmark of the 2D Euler DFR SSP54 step on one MI355X.
+
+Workload: isentropic vortex on [0,10]x[-5,5], 200x200 jittered quads split into
+80,000 triangles, P=4 (15 solution points per element, 1.2M DOF x 4
+variables), Roe flux, SSP54, CFL 1. One "step" is one SSP54 step (5 stage
+launches) over the whole mesh with the state resident in HBM.
+
+`value` is the FP32-bulk step's DOF-stage/s (DOF = K*NpInt; 5 stages per
+step); the FP64 drop-in is measured in the same run and reported beside it,
+with the density L2/Linf error of the FP32 run against the FP64 run and of
+both against the analytic convected vortex.
+
+python bench.py [--gpus 1] [--steps 500] [--warmup 10] [--no-cpu]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import math
+import os
+import sys
+import tempfile
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+METRIC = "DOF-stage/s at P=4 (FP32 vs FP64 GPU, L2 err vs FP64 oracle); % HBM peak"
+
+
+def alg_bytes_per_elem_stage(np_int: int, state_bytes: int) -> float:
+    """Algorithmic HBM bytes per element per SSP54 stage: 3.8 state-array
+    accesses of 4*NpInt values per stage on average (19 per step) plus 144 B
+    of per-element geometry, connectivity and face codes."""
+    return 4 * np_int * 3.8 * state_bytes + 144.0
+
+
+def analytic_density(t, X, Y):
+    """Isentropic vortex density (beta=5, centre (5,0), gamma 1.4, u=1)."""
+    g, beta = 1.4, 5.0
+    r2 = (X - t - 5.0) ** 2 + Y ** 2
+    tv1 = 1.0 - (g - 1) * beta * beta * np.exp(2.0 * (1.0 - r2)) / (16 * g * math.pi ** 2)
+    return tv1 ** (1.0 / (g - 1))
+
+
+def run_gpu(mesh, precision, steps, warmup):
+    from gocfd_amd import Euler, InputParameters2D
+    ip = InputParameters2D(Title="bench", CFL=1.0, FluxType="Roe", InitType="IVortex",
+                           PolynomialOrder=4, FinalTime=1e9, MaxIterations=warmup + steps)
+    e = Euler(ip, mesh, precision=precision)
+    e.step(warmup)
+    # timed region: inputs already resident in HBM; synchronised on both sides
+    e.sync()
+    t0 = time.perf_counter()
+    e.step(steps)  # enqueues 5*steps stage kernels and waits for them
+    elapsed = time.perf_counter() - t0
+    stage_ms, launches = e.kernel_time()
+    t, dt, nst = e.time_info()
+    q = e.Q
+    c = e.coords()
+    return dict(e=e, elapsed=elapsed, stage_ms=stage_ms, launches=launches, time=t, dt=dt,
+                steps_done=nst, rho=q[0], X=c["SolX"], Y=c["SolY"], K=e.K, NpInt=e.NpInt,
+                tiling=e.tiling())
+
+
+def cpu_baseline(mesh, target_seconds=15.0):
+    """The CPU oracle (plain C, FP64, OpenMP over the host cores) on the same mesh and order:
+    whole SSP54 steps until about `target_seconds` of CPU work."""
+    import oracle
+    s = oracle.OracleSolver(mesh, 4, flux="roe", init="ivortex", CFL=1.0, FinalTime=1e9)
+    n, t0 = 0, time.perf_counter()
+    while True:
+        s.step()
+        n += 1
+        el = time.perf_counter() - t0
+        if el >= target_seconds or n >= 50:
+            break
+    return dict(value=s.K * s.NpInt * 5 * n / el, steps=n, seconds=el, K=s.K, threads=oracle.threads())
+
+
+def copy_probe(mb: float) -> None:
+    """A plain device-to-device copy of `mb` MB with a torch elementwise copy
+    kernel, run 5 times: the reference point that scales the HBM byte counters."""
+    import torch
+    n = int(mb * 1e6 / 4)
+    a = torch.rand(n, device="cuda:0")
+    b = torch.empty_like(a)
+    for _ in range(5):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_probe_bytes_read": 4 * n, "copy_probe_bytes_written": 4 * n}))
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=500)
+    ap.add_argument("--warmup", type=int, default=10)
+    ap.add_argument("--n", type=int, default=200, help="quads per side (2n^2 triangles)")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--only", choices=["fp32", "fp64"], default=None,
+                    help="run one precision only (for profiler runs)")
+    ap.add_argument("--copy-probe", type=float, default=0.0, metavar="MB",
+                    help="run a plain device copy of MB megabytes (counter calibration) and exit")
+    args = ap.parse_args()
+    if args.copy_probe > 0:
+        copy_probe(args.copy_probe)
+        return
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU: --gpus {args.gpus} is not supported (the step "
+              "shards by element partition; see DESIGN.md)", file=sys.stderr)
+        sys.exit(2)
+
+    from gocfd_amd import meshgen
+    tmpdir = tempfile.mkdtemp(prefix="gocfd_bench_")
+    mesh = meshgen.vortex_mesh(os.path.join(tmpdir, f"vortex{args.n}.su2"), args.n, seed=1)
+
+    if args.only:
+        r = run_gpu(mesh, args.only, args.steps, args.warmup)
+        v = r["K"] * r["NpInt"] * 5 * args.steps / r["elapsed"]
+        print(json.dumps({"precision": args.only, "value": v, "stage_ms": r["stage_ms"],
+                          "ms_per_step": r["elapsed"] / args.steps * 1e3, "K": r["K"]}))
+        return
+    r64 = run_gpu(mesh, "fp64", args.steps, args.warmup)
+    r32 = run_gpu(mesh, "fp32", args.steps, args.warmup)
+    K, NpInt = r32["K"], r32["NpInt"]
+    dof_stages = K * NpInt * 5 * args.steps
+    v32 = dof_stages / r32["elapsed"]
+    v64 = dof_stages / r64["elapsed"]
+
+    # accuracy: FP32 vs FP64 GPU (FP64 GPU is parity-tested against the CPU
+    # oracle), both vs the analytic convected vortex
+    rho32, rho64 = r32["rho"], r64["rho"]
+    l2_32_64 = float(np.linalg.norm(rho32 - rho64) / np.linalg.norm(rho64))
+    linf_32_64 = float(np.abs(rho32 - rho64).max())
+    exact = analytic_density(r64["time"], r64["X"], r64["Y"])
+    l2_64_an = float(np.linalg.norm(rho64 - exact) / np.linalg.norm(exact))
+    l2_32_an = float(np.linalg.norm(rho32 - exact) / np.linalg.norm(exact))
+
+    b32 = alg_bytes_per_elem_stage(NpInt, 4) * K
+    b64 = alg_bytes_per_elem_stage(NpInt, 8) * K
+    ach32 = b32 / (r32["stage_ms"] * 1e-3) / 1e9
+    ach64 = b64 / (r64["stage_ms"] * 1e-3) / 1e9
+
+    traffic = None
+    tfile = os.path.join(ROOT, "profiles", "stage_fp32_p4_k80000_traffic.json")
+    if os.path.exists(tfile):
+        with open(tfile) as f:
+            traffic = json.load(f).get("bytes_per_launch")
+
+    cpu = None
+    if not args.no_cpu:
+        c = cpu_baseline(mesh, args.cpu_seconds)
+        cpu = {"value": c["value"], "unit": "DOF-stage/s", "cores": c["threads"], "kind": "port",
+               "sample": f"CPU oracle (plain C, FP64, OpenMP {c['threads']} threads), same {K}-triangle P=4 Roe "
+                         f"mesh, {c['steps']} SSP54 steps in {c['seconds']:.1f} s"}
+
+    line = {
+        "metric": METRIC,
+        "value": v32,
+        "unit": "DOF-stage/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": r32["elapsed"] / args.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32 (FP64 geometry, dt reduction and SSP sums)",
+        "data": "synthetic: deterministic jittered-quad isentropic-vortex mesh",
+        "config": {"workload": f"isentropic vortex, {K} triangles ({args.n}x{args.n} jittered quads), "
+                               f"P=4, Roe, SSP54, CFL 1", "K": K, "NpInt": NpInt,
+                   "dof": K * NpInt, "precision": "fp32-bulk", "tile": r32["tiling"][0]},
+        "fp64": {"value": v64, "ms_per_step": r64["elapsed"] / args.steps * 1e3,
+                 "stage_ms": r64["stage_ms"], "achieved_GBs": ach64,
+                 "frac_hbm_peak": ach64 / HBM_PEAK_GBS},
+        "fp32_over_fp64": v32 / v64,
+        "accuracy": {"rho_l2_fp32_vs_fp64": l2_32_64, "rho_linf_fp32_vs_fp64": linf_32_64,
+                     "rho_l2_fp64_vs_analytic": l2_64_an, "rho_l2_fp32_vs_analytic": l2_32_an,
+                     "time": r64["time"]},
+        "roofline": {"bound": "hbm", "achieved": ach32, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": ach32 / HBM_PEAK_GBS, "traffic": traffic,
+                     "kernel": "k_stage<4,float,float>", "avg_launch_ms": r32["stage_ms"],
+                     "alg_bytes_per_launch": b32},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line))
+
+
+if __name__ == "__main__":
+    main()

@@ -819,6 +819,19 @@ DeviceLayout build_device_layout(const Discretisation &d, int E) {
       L.face_code[fi] = code;
     }
   }
+  // Faces whose owner-side trace enters the CFL wave-speed maximum, per tile
+  std::vector<std::vector<int>> waves(L.ntiles);
+  for (int kk = 0; kk < K; ++kk)
+    for (int f = 0; f < 3; ++f)
+      if (L.face_code[size_t(f) * Kp + kk] & FC_WAVE) waves[kk / E].push_back((kk % E) | (f << 8));
+  L.Wmax = 1;
+  for (auto &w : waves) L.Wmax = std::max<int>(L.Wmax, int(w.size()));
+  L.wave_list.assign(size_t(L.ntiles) * L.Wmax, 0);
+  L.wave_cnt.assign(L.ntiles, 0);
+  for (int t = 0; t < L.ntiles; ++t) {
+    L.wave_cnt[t] = int(waves[t].size());
+    std::copy(waves[t].begin(), waves[t].end(), L.wave_list.begin() + size_t(t) * L.Wmax);
+  }
   L.Hmax = 1;
   for (auto &hl : halos) L.Hmax = std::max<int>(L.Hmax, int(hl.size()));
   L.halo_elem.assign(size_t(L.ntiles) * L.Hmax, -1);

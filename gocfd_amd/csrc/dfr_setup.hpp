@@ -101,6 +101,9 @@ struct DeviceLayout {
   std::vector<int> nbr_elem;        // internal index of the neighbour (or -1)
   std::vector<double> Jdet, Jinv, IInII, Nx, Ny, oohK;  // geometry in internal order
   std::vector<int> halo_elem, halo_face;                // [ntiles x Hmax], -1 unused
+  int Wmax = 0;                                         // owned (wave-speed) faces per tile, max
+  std::vector<int> wave_list;                           // [ntiles x Wmax]: e | face << 8
+  std::vector<int> wave_cnt;                            // [ntiles]
   std::vector<double> bpoint_xy;                        // IVortex boundary points [nb][NpEdge][2]
   int nbfaces = 0;
 };

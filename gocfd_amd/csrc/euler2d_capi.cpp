@@ -107,9 +107,7 @@ struct gocfd_euler2d {
   void *scratch_in = nullptr;
   DevBuf<double> rhs_out;
   DevBuf<double> Jdet, Jinv, IInII, Nx, Ny, oohK, bxy, Interp64;
-  DevBuf<int> code, nbr_slot, halo_elem, halo_face;
-  DevBuf<double> DivInt_d, Interp_d;
-  DevBuf<float> DivInt_f, Interp_f;
+  DevBuf<int> code, nbr_slot, halo_elem, halo_face, wave_list, wave_cnt;
   dev::Ctl *ctl = nullptr;
   long long gstage = 0;  // stage launches since the control block was initialised
   int step_index = 0;    // steps since the control block was initialised
@@ -131,14 +129,13 @@ struct gocfd_euler2d {
 
   dev::StageArgs args() const {
     dev::StageArgs a{};
-    a.K = L.K; a.Kp = L.Kp; a.ntiles = L.ntiles; a.Hmax = L.Hmax;
+    a.K = L.K; a.Kp = L.Kp; a.ntiles = L.ntiles; a.Hmax = L.Hmax; a.Wmax = L.Wmax;
     a.q0 = q[0]; a.q1 = q[1]; a.q2 = q[2]; a.q3 = q[3]; a.q4 = q[4]; a.R = R;
     a.ctl = ctl;
     a.g.Jdet = Jdet.p; a.g.Jinv = Jinv.p; a.g.IInII = IInII.p; a.g.Nx = Nx.p; a.g.Ny = Ny.p;
     a.g.oohK = oohK.p; a.g.code = code.p; a.g.nbr_slot = nbr_slot.p;
     a.g.halo_elem = halo_elem.p; a.g.halo_face = halo_face.p; a.g.bxy = bxy.p;
-    if (prec == 0) { a.DivInt = DivInt_d.p; a.Interp = Interp_d.p; }
-    else { a.DivInt = DivInt_f.p; a.Interp = Interp_f.p; }
+    a.g.wave_list = wave_list.p; a.g.wave_cnt = wave_cnt.p;
     a.Interp64 = Interp64.p;
     a.ph.gamma = fs.gamma; a.ph.minf = fs.minf; a.ph.pinf = fs.pinf; a.ph.cinf = fs.cinf;
     for (int i = 0; i < 4; ++i) a.ph.qinf[i] = fs.q[i];
@@ -382,10 +379,13 @@ int gocfd_euler2d_create(const gocfd_mesh *mesh, const gocfd_params *p, gocfd_eu
   h->bxy.upload(L.bpoint_xy);
   h->code.upload(L.face_code); h->nbr_slot.upload(L.nbr_slot);
   h->halo_elem.upload(L.halo_elem); h->halo_face.upload(L.halo_face);
-  h->DivInt_d.upload(h->d.ops.DivInt); h->Interp_d.upload(h->d.ops.Interp);
+  h->wave_list.upload(L.wave_list); h->wave_cnt.upload(L.wave_cnt);
+  std::vector<double> divT(h->d.ops.DivInt.size());  // [NpFlux][NpInt] for contiguous per-DOF rows
+  for (int i = 0; i < h->d.ops.NpInt; ++i)
+    for (int j = 0; j < h->d.ops.NpFlux; ++j)
+      divT[size_t(j) * h->d.ops.NpInt + i] = h->d.ops.DivInt[size_t(i) * h->d.ops.NpFlux + j];
+  HIPCHECK(dev::upload_operators(p->polynomial_order, divT.data(), h->d.ops.Interp.data()));
   h->Interp64.upload(h->d.ops.Interp);
-  h->DivInt_f.upload(std::vector<float>(h->d.ops.DivInt.begin(), h->d.ops.DivInt.end()));
-  h->Interp_f.upload(std::vector<float>(h->d.ops.Interp.begin(), h->d.ops.Interp.end()));
   h->to_device(Q0.data(), h->q[0]);
   h->restart(0.0);
   HIPCHECK(hipStreamSynchronize(h->stream));

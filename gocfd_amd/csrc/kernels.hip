@@ -18,12 +18,33 @@
 //      (calculateGlobalDT euler.go:945-971)
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+#include <vector>
+
 #include "dfr_setup.hpp"
 #include "kernels.hpp"
 #include "physics.hpp"
 
 namespace gocfd {
 namespace dev {
+
+// Reference-element operators of every order 0..4 in constant memory, so
+// that every coefficient is a uniform scalar load: per order, DivInt^T
+// [NpFlux][NpInt] followed by FluxEdgeInterp [3 NpEdge][NpInt]. They depend
+// on the order only, so all solvers on a device share them.
+constexpr int ops_size(int N) {
+  return ((N + 2) * (N + 4) + 3 * (N + 2)) * ((N + 1) * (N + 2) / 2);
+}
+constexpr int ops_offset(int N) { return N == 0 ? 0 : ops_offset(N - 1) + ops_size(N - 1); }
+constexpr int kOpsTotal = ops_offset(5);
+__constant__ double cOpsD[kOpsTotal];
+__constant__ float cOpsF[kOpsTotal];
+template <class C>
+__device__ __forceinline__ const C *ops_base();
+template <>
+__device__ __forceinline__ const double *ops_base<double>() { return cOpsD; }
+template <>
+__device__ __forceinline__ const float *ops_base<float>() { return cOpsF; }
 
 template <int N>
 struct Shape {
@@ -71,10 +92,10 @@ __device__ __forceinline__ double point_speed(double g, const double q[4], doubl
   return oohK * (U + c);
 }
 
-template <int N, class S, class C>
+template <int N, class S, class C, int FLUX, bool LAST>
 __global__ void __launch_bounds__(kThreads) k_stage(StageArgs a) {
   using Sh = Shape<N>;
-  constexpr int NpInt = Sh::NpInt, NpEdge = Sh::NpEdge, NpFlux = Sh::NpFlux, NF = Sh::NF, NV = Sh::NV;
+  constexpr int NpInt = Sh::NpInt, NpEdge = Sh::NpEdge, NF = Sh::NF, NV = Sh::NV;
   constexpr int E = kTile, P = kTile + 1;
   static_assert(kThreads == 4 * kTile, "one thread per (element, variable)");
   extern __shared__ __align__(16) unsigned char smem_raw[];
@@ -99,16 +120,29 @@ __global__ void __launch_bounds__(kThreads) k_stage(StageArgs a) {
   const int k0 = tile * E;
   const int Kp = a.Kp;
   const S *qs = static_cast<const S *>(a.qs);
-  const C *Itp = static_cast<const C *>(a.Interp);
-  const C *Dv = static_cast<const C *>(a.DivInt);
+  const C *DvT = ops_base<C>() + ops_offset(N);         // DivInt^T [NpFlux][NpInt]
+  const C *Itp = DvT + Sh::NpFlux * NpInt;              // FluxEdgeInterp [NF][NpInt]
   const C g = C(a.ph.gamma);
 
   // ---- A: tile state -> LDS (coalesced, contiguous chunk) ----
+  // Every load is issued before the first LDS store, so a thread's LPT
+  // loads are in flight together.
+  constexpr int LPT = (E * NV + kThreads - 1) / kThreads;
   {
     const S *qt = qs + size_t(k0) * NV;
-    for (int idx = tid; idx < E * NV; idx += kThreads) {
-      const int e = idx / NV, r = idx - e * NV;
-      sQ[r * P + e] = C(qt[idx]);
+    S v[LPT];
+#pragma unroll
+    for (int j = 0; j < LPT; ++j) {
+      const int idx = tid + j * kThreads;
+      v[j] = (idx < E * NV) ? qt[idx] : S(0);
+    }
+#pragma unroll
+    for (int j = 0; j < LPT; ++j) {
+      const int idx = tid + j * kThreads;
+      if (idx < E * NV) {
+        const int e = idx / NV, r = idx - e * NV;
+        sQ[r * P + e] = C(v[j]);
+      }
     }
   }
   __syncthreads();
@@ -146,6 +180,9 @@ __global__ void __launch_bounds__(kThreads) k_stage(StageArgs a) {
   __syncthreads();
 
   // ---- B: numerical flux at every face point ----
+  // Shared faces first (unrolled, results held in registers until every
+  // neighbour trace has been read), then boundary faces in a rolled loop: a
+  // boundary face point reads and writes only its own LDS slot.
   constexpr int ITB = (E * NF + kThreads - 1) / kThreads;
   C fl[ITB][4];
 #pragma unroll
@@ -155,49 +192,29 @@ __global__ void __launch_bounds__(kThreads) k_stage(StageArgs a) {
       const int e = it % E, p = it / E, f = p / NpEdge, pp = p - f * NpEdge;
       const size_t fi = size_t(f) * Kp + k0 + e;
       const int code = a.g.code[fi];
-      const int kind = code & 15;
-      C own[4];
-#pragma unroll
-      for (int c = 0; c < 4; ++c) own[c] = sT[c * TS + p * E + e];
-      const double nxd = a.g.Nx[fi], nyd = a.g.Ny[fi];
-      const C nx = C(nxd), ny = C(nyd);
-      C F[4] = {C(0), C(0), C(0), C(0)};
-      if (kind == FK_Shared) {
+      if ((code & 15) == FK_Shared) {
+        C own[4], oth[4], F[4];
         const unsigned slot = unsigned(a.g.nbr_slot[fi]);
         const int base = int(slot & 0x7fffffffu);
         const int stride = (slot >> 31) ? 1 : E;
         const int idx = base + (NpEdge - 1 - pp) * stride;
-        C oth[4];
 #pragma unroll
-        for (int c = 0; c < 4; ++c) oth[c] = sT[c * TS + idx];
-        if (code & FC_FLUX_OWNER) {
-          numflux(a.ph.flux, g, own, oth, nx, ny, F);
-        } else {
-          numflux(a.ph.flux, g, oth, own, nx, ny, F);
-#pragma unroll
-          for (int c = 0; c < 4; ++c) F[c] = -F[c];
+        for (int c = 0; c < 4; ++c) {
+          own[c] = sT[c * TS + p * E + e];
+          oth[c] = sT[c * TS + idx];
         }
-      } else if (kind == FK_Far || kind == FK_IVortex) {
-        double qb[4];
-        boundary_state(kind, a.ph, a.g.bxy, code >> 8, NpEdge, pp, time, own, nxd, nyd, qb);
-        const C q[4] = {C(qb[0]), C(qb[1]), C(qb[2]), C(qb[3])};
-        C Fx[4], Fy[4];
-        flux_xy(g, q, Fx, Fy);
+        const C nx = C(a.g.Nx[fi]), ny = C(a.g.Ny[fi]);
+        const C w = C(a.g.IInII[fi]);
+        if (code & FC_FLUX_OWNER) {
+          numflux(FLUX, g, own, oth, nx, ny, F);
 #pragma unroll
-        for (int c = 0; c < 4; ++c) F[c] = nx * Fx[c] + ny * Fy[c];
-      } else if (kind == FK_Wall) {
-        const C pr = pressure(g, own[0], own[1], own[2], own[3]);
-        F[1] = nx * pr;
-        F[2] = ny * pr;
-      } else if (kind == FK_Open) {
-        C Fx[4], Fy[4];
-        flux_xy(g, own, Fx, Fy);
+          for (int c = 0; c < 4; ++c) fl[j][c] = F[c] * w;
+        } else {
+          numflux(FLUX, g, oth, own, nx, ny, F);
 #pragma unroll
-        for (int c = 0; c < 4; ++c) F[c] = nx * Fx[c] + ny * Fy[c];
+          for (int c = 0; c < 4; ++c) fl[j][c] = -F[c] * w;
+        }
       }
-      const C w = C(a.g.IInII[fi]);
-#pragma unroll
-      for (int c = 0; c < 4; ++c) fl[j][c] = F[c] * w;
     }
   }
   __syncthreads();
@@ -205,10 +222,47 @@ __global__ void __launch_bounds__(kThreads) k_stage(StageArgs a) {
   for (int j = 0; j < ITB; ++j) {
     const int it = tid + j * kThreads;
     if (it < E * NF) {
-      const int e = it % E, p = it / E;
+      const int e = it % E, p = it / E, f = p / NpEdge;
+      if ((a.g.code[size_t(f) * Kp + k0 + e] & 15) == FK_Shared) {
 #pragma unroll
-      for (int c = 0; c < 4; ++c) sT[c * TS + p * E + e] = fl[j][c];
+        for (int c = 0; c < 4; ++c) sT[c * TS + p * E + e] = fl[j][c];
+      }
     }
+  }
+#pragma unroll 1
+  for (int it = tid; it < E * NF; it += kThreads) {
+    const int e = it % E, p = it / E, f = p / NpEdge, pp = p - f * NpEdge;
+    const size_t fi = size_t(f) * Kp + k0 + e;
+    const int code = a.g.code[fi];
+    const int kind = code & 15;
+    if (kind == FK_Shared) continue;
+    C own[4];
+#pragma unroll
+    for (int c = 0; c < 4; ++c) own[c] = sT[c * TS + p * E + e];
+    const double nxd = a.g.Nx[fi], nyd = a.g.Ny[fi];
+    const C nx = C(nxd), ny = C(nyd);
+    C F[4] = {C(0), C(0), C(0), C(0)};
+    if (kind == FK_Far || kind == FK_IVortex) {
+      double qb[4];
+      boundary_state(kind, a.ph, a.g.bxy, code >> 8, NpEdge, pp, time, own, nxd, nyd, qb);
+      const C q[4] = {C(qb[0]), C(qb[1]), C(qb[2]), C(qb[3])};
+      C Fx[4], Fy[4];
+      flux_xy(g, q, Fx, Fy);
+#pragma unroll
+      for (int c = 0; c < 4; ++c) F[c] = nx * Fx[c] + ny * Fy[c];
+    } else if (kind == FK_Wall) {
+      const C pr = pressure(g, own[0], own[1], own[2], own[3]);
+      F[1] = nx * pr;
+      F[2] = ny * pr;
+    } else if (kind == FK_Open) {
+      C Fx[4], Fy[4];
+      flux_xy(g, own, Fx, Fy);
+#pragma unroll
+      for (int c = 0; c < 4; ++c) F[c] = nx * Fx[c] + ny * Fy[c];
+    }
+    const C w = C(a.g.IInII[fi]);
+#pragma unroll
+    for (int c = 0; c < 4; ++c) sT[c * TS + p * E + e] = F[c] * w;
   }
   __syncthreads();
 
@@ -230,13 +284,13 @@ __global__ void __launch_bounds__(kThreads) k_stage(StageArgs a) {
       flux_component(tn, g, r0, r1, r2, r3, Fx, Fy);
       const C Fr = Jdc * (J0 * Fx + J1 * Fy), Fs = Jdc * (J2 * Fx + J3 * Fy);
 #pragma unroll
-      for (int i = 0; i < NpInt; ++i) acc[i] += Dv[i * NpFlux + j] * Fr + Dv[i * NpFlux + NpInt + j] * Fs;
+      for (int i = 0; i < NpInt; ++i) acc[i] += DvT[j * NpInt + i] * Fr + DvT[(NpInt + j) * NpInt + i] * Fs;
     }
 #pragma unroll 2
     for (int p = 0; p < NF; ++p) {
       const C Fe = sT[tn * TS + p * E + te];
 #pragma unroll
-      for (int i = 0; i < NpInt; ++i) acc[i] += Dv[i * NpFlux + 2 * NpInt + p] * Fe;
+      for (int i = 0; i < NpInt; ++i) acc[i] += DvT[(2 * NpInt + p) * NpInt + i] * Fe;
     }
     const C moojd = -C(1. / Jd);
     __syncthreads();  // every thread has read its fluxes; sR may now overwrite sT
@@ -255,54 +309,90 @@ __global__ void __launch_bounds__(kThreads) k_stage(StageArgs a) {
   }
 
   // ---- E: SSP54 stage combination (euler.go:511-565), FP64 arithmetic ----
+  // The operands of this stage are loaded for all LPT values of the thread
+  // first, then combined and stored.
   const int stage = a.stage;
   double dt = a.ph.cfl / __longlong_as_double((long long)ctl->wmax[a.ring]);
   if (time + dt > a.ph.final_time) dt = a.ph.final_time - time;
   int bad = 0;
   {
-    S *q0 = static_cast<S *>(a.q0) + size_t(k0) * NV;
-    S *q1 = static_cast<S *>(a.q1) + size_t(k0) * NV;
-    S *q2 = static_cast<S *>(a.q2) + size_t(k0) * NV;
-    S *q3 = static_cast<S *>(a.q3) + size_t(k0) * NV;
-    S *q4 = static_cast<S *>(a.q4) + size_t(k0) * NV;
-    S *Rr = static_cast<S *>(a.R) + size_t(k0) * NV;
+    const size_t base = size_t(k0) * NV;
+    S *q0 = static_cast<S *>(a.q0) + base;
+    S *q1 = static_cast<S *>(a.q1) + base;
+    S *q2 = static_cast<S *>(a.q2) + base;
+    S *q3 = static_cast<S *>(a.q3) + base;
+    S *q4 = static_cast<S *>(a.q4) + base;
+    S *Rr = static_cast<S *>(a.R) + base;
     const int nreal = min(E, a.K - k0);
-    for (int idx = tid; idx < E * NV; idx += kThreads) {
-      const int e = idx / NV, r = idx - e * NV;
-      const double rhs = double(sR[r * P + e]);
-      const double qsv = double(sQ[r * P + e]);
-      const double dtR = dt * rhs;
-      S nv;
-      switch (stage) {
-        case 0:
-          nv = S(qsv + 0.391752226571890 * dtR);
-          q1[idx] = nv;
-          break;
-        case 1:
-          nv = S(0.444370493651235 * double(q0[idx]) + 0.555629506348765 * qsv + 0.368410593050371 * dtR);
-          q2[idx] = nv;
-          break;
-        case 2:
-          nv = S(0.620101851488403 * double(q0[idx]) + 0.379898148511597 * qsv + 0.251891774271694 * dtR);
-          q3[idx] = nv;
-          break;
-        case 3:
-          Rr[idx] = S(rhs);
-          nv = S(0.178079954393132 * double(q0[idx]) + 0.821920045606868 * qsv + 0.544974750228521 * dtR);
-          q4[idx] = nv;
-          break;
-        default: {
-          const double q0v = double(q0[idx]);
-          const double dtR3 = dt * double(Rr[idx]);
-          const double Rn = -q0v + 0.517231671970585 * double(q2[idx]) + 0.096059710526146 * double(q3[idx]) +
-                            0.386708617503269 * qsv + 0.063692468666290 * dtR3 + 0.226007483236906 * dtR;
-          Rr[idx] = S(Rn);
-          nv = S(q0v + Rn);
-          q0[idx] = nv;
-        } break;
+    if constexpr (LAST) {
+      // stage 4: R = -q0 + a q2 + b q3 + c q4 + d dt R3 + e dt RHS, q0 += R
+      S v0[LPT], v2[LPT], v3[LPT], vR[LPT];
+#pragma unroll
+      for (int j = 0; j < LPT; ++j) {
+        const int idx = tid + j * kThreads;
+        const bool ok = idx < E * NV;
+        v0[j] = ok ? q0[idx] : S(0);
+        v2[j] = ok ? q2[idx] : S(0);
+        v3[j] = ok ? q3[idx] : S(0);
+        vR[j] = ok ? Rr[idx] : S(0);
       }
-      sQ[r * P + e] = C(nv);
-      if (e < nreal && !(nv == nv)) bad = 1;
+#pragma unroll
+      for (int j = 0; j < LPT; ++j) {
+        const int idx = tid + j * kThreads;
+        if (idx >= E * NV) continue;
+        const int e = idx / NV, r = idx - e * NV;
+        const double dtR = dt * double(sR[r * P + e]);
+        const double qsv = double(sQ[r * P + e]);
+        const double q0v = double(v0[j]);
+        const double dtR3 = dt * double(vR[j]);
+        const double Rn = -q0v + 0.517231671970585 * double(v2[j]) + 0.096059710526146 * double(v3[j]) +
+                          0.386708617503269 * qsv + 0.063692468666290 * dtR3 + 0.226007483236906 * dtR;
+        Rr[idx] = S(Rn);
+        const S nv = S(q0v + Rn);
+        q0[idx] = nv;
+        sQ[r * P + e] = C(nv);
+        if (e < nreal && !(nv == nv)) bad = 1;
+      }
+    } else {
+      S v0[LPT];
+      if (stage >= 1) {
+#pragma unroll
+        for (int j = 0; j < LPT; ++j) {
+          const int idx = tid + j * kThreads;
+          v0[j] = (idx < E * NV) ? q0[idx] : S(0);
+        }
+      }
+#pragma unroll
+      for (int j = 0; j < LPT; ++j) {
+        const int idx = tid + j * kThreads;
+        if (idx >= E * NV) continue;
+        const int e = idx / NV, r = idx - e * NV;
+        const double rhs = double(sR[r * P + e]);
+        const double qsv = double(sQ[r * P + e]);
+        const double dtR = dt * rhs;
+        S nv;
+        switch (stage) {
+          case 0:
+            nv = S(qsv + 0.391752226571890 * dtR);
+            q1[idx] = nv;
+            break;
+          case 1:
+            nv = S(0.444370493651235 * double(v0[j]) + 0.555629506348765 * qsv + 0.368410593050371 * dtR);
+            q2[idx] = nv;
+            break;
+          case 2:
+            nv = S(0.620101851488403 * double(v0[j]) + 0.379898148511597 * qsv + 0.251891774271694 * dtR);
+            q3[idx] = nv;
+            break;
+          default:
+            Rr[idx] = S(rhs);
+            nv = S(0.178079954393132 * double(v0[j]) + 0.821920045606868 * qsv + 0.544974750228521 * dtR);
+            q4[idx] = nv;
+            break;
+        }
+        sQ[r * P + e] = C(nv);
+        if (e < nreal && !(nv == nv)) bad = 1;
+      }
     }
   }
   __syncthreads();
@@ -323,25 +413,30 @@ __global__ void __launch_bounds__(kThreads) k_stage(StageArgs a) {
   __syncthreads();
 
   // ---- G: owner-side wave speed of the next stage, max over the tile ----
-  const double tnext = (stage == 4) ? time + dt : time;
+  const double tnext = LAST ? time + dt : time;
   double wl = 0.0;
-  for (int it = tid; it < E * NF; it += kThreads) {
-    const int e = it % E, p = it / E, f = p / NpEdge, pp = p - f * NpEdge;
-    const int kk = k0 + e;
-    if (kk >= a.K) continue;
-    const size_t fi = size_t(f) * Kp + kk;
-    const int code = a.g.code[fi];
-    if (!(code & FC_WAVE)) continue;
-    const int kind = code & 15;
-    C own[4];
+  {
+    const int nw = a.g.wave_cnt[tile] * NpEdge;
+    const int *wlist = a.g.wave_list + size_t(tile) * a.Wmax;
+#pragma unroll 1
+    for (int it = tid; it < nw; it += kThreads) {
+      const int w = it / NpEdge, pp = it - w * NpEdge;
+      const int packed = wlist[w];
+      const int e = packed & 255, f = packed >> 8, p = f * NpEdge + pp;
+      const int kk = k0 + e;
+      const size_t fi = size_t(f) * Kp + kk;
+      const int code = a.g.code[fi];
+      const int kind = code & 15;
+      C own[4];
 #pragma unroll
-    for (int c = 0; c < 4; ++c) own[c] = sT[c * TS + p * E + e];
-    double q[4] = {double(own[0]), double(own[1]), double(own[2]), double(own[3])};
-    if (kind == FK_Far || kind == FK_IVortex)
-      boundary_state(kind, a.ph, a.g.bxy, code >> 8, NpEdge, pp, tnext, own, a.g.Nx[fi], a.g.Ny[fi], q);
-    const double w = point_speed(a.ph.gamma, q, a.g.oohK[kk]);
-    if (!(w == w)) bad = 1;
-    wl = fmax(wl, w);
+      for (int c = 0; c < 4; ++c) own[c] = sT[c * TS + p * E + e];
+      double q[4] = {double(own[0]), double(own[1]), double(own[2]), double(own[3])};
+      if (kind == FK_Far || kind == FK_IVortex)
+        boundary_state(kind, a.ph, a.g.bxy, code >> 8, NpEdge, pp, tnext, own, a.g.Nx[fi], a.g.Ny[fi], q);
+      const double wsp = point_speed(a.ph.gamma, q, a.g.oohK[kk]);
+      if (!(wsp == wsp)) bad = 1;
+      wl = fmax(wl, wsp);
+    }
   }
   wl = wave_max(wl);
   if ((tid & 63) == 0) red[tid >> 6] = wl;
@@ -354,7 +449,7 @@ __global__ void __launch_bounds__(kThreads) k_stage(StageArgs a) {
     if (blockIdx.x == 0) {
       ctl->wmax[(a.ring + 2) % 3] = 0ull;
       ctl->dt[stage] = dt;
-      if (stage == 4) {
+      if (LAST) {
         ctl->time[par ^ 1] = tnext;
         ctl->steps += 1;
       }
@@ -433,19 +528,30 @@ size_t stage_lds_bytes(int N, int prec, int Hmax) {
   }
 }
 
-template <int N, class S, class C>
+template <int N, class S, class C, int FLUX, bool LAST>
 hipError_t launch_stage_t(const StageArgs &a, hipStream_t s) {
   const size_t lds = lds_bytes_n<N>(sizeof(C), a.Hmax);
   static size_t attr_bytes = 0;  // dynamic LDS the kernel is currently allowed
   if (lds > 64 * 1024 && lds > attr_bytes) {
-    hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void *>(&k_stage<N, S, C>),
+    hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void *>(&k_stage<N, S, C, FLUX, LAST>),
                                        hipFuncAttributeMaxDynamicSharedMemorySize, int(lds));
     if (e != hipSuccess) return e;
     attr_bytes = lds;
   }
   const int q = (a.ntiles + 7) / 8;
-  hipLaunchKernelGGL((k_stage<N, S, C>), dim3(8 * q), dim3(kThreads), lds, s, a);
+  hipLaunchKernelGGL((k_stage<N, S, C, FLUX, LAST>), dim3(8 * q), dim3(kThreads), lds, s, a);
   return hipGetLastError();
+}
+
+template <int N, class S, class C>
+hipError_t launch_stage_f(const StageArgs &a, hipStream_t s) {
+  switch (a.ph.flux) {
+    case 0: return a.stage == 4 ? launch_stage_t<N, S, C, 0, true>(a, s) : launch_stage_t<N, S, C, 0, false>(a, s);
+    case 1: return a.stage == 4 ? launch_stage_t<N, S, C, 1, true>(a, s) : launch_stage_t<N, S, C, 1, false>(a, s);
+    case 2: return a.stage == 4 ? launch_stage_t<N, S, C, 2, true>(a, s) : launch_stage_t<N, S, C, 2, false>(a, s);
+    case 3: return a.stage == 4 ? launch_stage_t<N, S, C, 3, true>(a, s) : launch_stage_t<N, S, C, 3, false>(a, s);
+  }
+  return hipErrorInvalidValue;
 }
 
 template <int N, class S>
@@ -459,19 +565,19 @@ hipError_t launch_prologue_t(const StageArgs &a, hipStream_t s) {
 hipError_t launch_stage(int N, int prec, const StageArgs &a, hipStream_t s) {
   if (prec == 0) {
     switch (N) {
-      case 0: return launch_stage_t<0, double, double>(a, s);
-      case 1: return launch_stage_t<1, double, double>(a, s);
-      case 2: return launch_stage_t<2, double, double>(a, s);
-      case 3: return launch_stage_t<3, double, double>(a, s);
-      case 4: return launch_stage_t<4, double, double>(a, s);
+      case 0: return launch_stage_f<0, double, double>(a, s);
+      case 1: return launch_stage_f<1, double, double>(a, s);
+      case 2: return launch_stage_f<2, double, double>(a, s);
+      case 3: return launch_stage_f<3, double, double>(a, s);
+      case 4: return launch_stage_f<4, double, double>(a, s);
     }
   } else {
     switch (N) {
-      case 0: return launch_stage_t<0, float, float>(a, s);
-      case 1: return launch_stage_t<1, float, float>(a, s);
-      case 2: return launch_stage_t<2, float, float>(a, s);
-      case 3: return launch_stage_t<3, float, float>(a, s);
-      case 4: return launch_stage_t<4, float, float>(a, s);
+      case 0: return launch_stage_f<0, float, float>(a, s);
+      case 1: return launch_stage_f<1, float, float>(a, s);
+      case 2: return launch_stage_f<2, float, float>(a, s);
+      case 3: return launch_stage_f<3, float, float>(a, s);
+      case 4: return launch_stage_f<4, float, float>(a, s);
     }
   }
   return hipErrorInvalidValue;
@@ -496,6 +602,20 @@ hipError_t launch_prologue(int N, int prec, const StageArgs &a, hipStream_t s) {
     }
   }
   return hipErrorInvalidValue;
+}
+
+hipError_t upload_operators(int N, const double *divT, const double *interp) {
+  if (N < 0 || N > 4) return hipErrorInvalidValue;
+  const int NpInt = (N + 1) * (N + 2) / 2, NpFlux = (N + 2) * (N + 4), NF = 3 * (N + 2);
+  std::vector<double> d(size_t(ops_size(N)));
+  std::copy(divT, divT + size_t(NpFlux) * NpInt, d.begin());
+  std::copy(interp, interp + size_t(NF) * NpInt, d.begin() + size_t(NpFlux) * NpInt);
+  std::vector<float> f(d.begin(), d.end());
+  hipError_t e = hipMemcpyToSymbol(HIP_SYMBOL(cOpsD), d.data(), d.size() * sizeof(double),
+                                   size_t(ops_offset(N)) * sizeof(double), hipMemcpyHostToDevice);
+  if (e != hipSuccess) return e;
+  return hipMemcpyToSymbol(HIP_SYMBOL(cOpsF), f.data(), f.size() * sizeof(float),
+                           size_t(ops_offset(N)) * sizeof(float), hipMemcpyHostToDevice);
 }
 
 hipError_t launch_ctl_init(Ctl *ctl, double time, hipStream_t s) {

@@ -28,6 +28,7 @@ struct Geom {
   const double *Jdet, *Jinv, *IInII, *Nx, *Ny, *oohK;  // [Kp], [4][Kp], [3][Kp] x3, [Kp]
   const int *code, *nbr_slot;                          // [3][Kp]
   const int *halo_elem, *halo_face;                    // [ntiles][Hmax]
+  const int *wave_list, *wave_cnt;                     // [ntiles][Wmax], [ntiles]
   const double *bxy;                                   // IVortex boundary points
 };
 
@@ -39,7 +40,7 @@ struct Phys {
 };
 
 struct StageArgs {
-  int K, Kp, ntiles, Hmax;
+  int K, Kp, ntiles, Hmax, Wmax;
   int stage;           // 0..4, or -1 for the RHS-only evaluation
   int step_index;      // steps since the state was set (selects time parity)
   int ring;            // global stage counter mod 3
@@ -49,7 +50,6 @@ struct StageArgs {
   double *rhs_out;     // RHS-only output (element-major, double)
   Ctl *ctl;
   Geom g;
-  const void *DivInt, *Interp;  // operators in the compute precision
   const double *Interp64;       // FluxEdgeInterp in FP64 (wave-speed prologue)
   Phys ph;
 };
@@ -59,6 +59,9 @@ struct StageArgs {
 hipError_t launch_stage(int N, int prec, const StageArgs &a, hipStream_t s);
 hipError_t launch_prologue(int N, int prec, const StageArgs &a, hipStream_t s);
 hipError_t launch_ctl_init(Ctl *ctl, double time, hipStream_t s);
+// Copy the order-N operators (DivInt^T [NpFlux][NpInt], FluxEdgeInterp
+// [3NpEdge][NpInt]) into the current device's constant memory.
+hipError_t upload_operators(int N, const double *divT, const double *interp);
 size_t stage_lds_bytes(int N, int prec, int Hmax);
 
 }  // namespace dev

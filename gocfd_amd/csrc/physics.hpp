@@ -9,10 +9,15 @@
 namespace gocfd {
 namespace dev {
 
+// Division in the bulk arithmetic type: IEEE division in FP64 (the drop-in
+// path), the hardware reciprocal in FP32 (the FP32-bulk path).
+__device__ __forceinline__ double fdiv(double a, double b) { return a / b; }
+__device__ __forceinline__ float fdiv(float a, float b) { return __fdividef(a, b); }
+
 // GetFlowFunctionBase StaticPressure (model_problems/Euler2D/fluids.go:289-336)
 template <class C>
 __device__ __forceinline__ C pressure(C g, C rho, C rhoU, C rhoV, C E) {
-  const C oorho = C(1) / rho;
+  const C oorho = fdiv(C(1), rho);
   const C u = rhoU * oorho, v = rhoV * oorho;
   const C q = C(0.5) * rho * (u * u + v * v);
   return (g - C(1)) * (E - q);
@@ -21,7 +26,7 @@ __device__ __forceinline__ C pressure(C g, C rho, C rhoU, C rhoV, C E) {
 // FluxCalcBase (model_problems/Euler2D/fluxes.go:76-87)
 template <class C>
 __device__ __forceinline__ void flux_xy(C g, const C q[4], C Fx[4], C Fy[4]) {
-  const C oorho = C(1) / q[0];
+  const C oorho = fdiv(C(1), q[0]);
   const C u = q[1] * oorho, v = q[2] * oorho;
   const C p = pressure(g, q[0], q[1], q[2], q[3]);
   Fx[0] = q[1]; Fx[1] = q[1] * u + p; Fx[2] = q[1] * v; Fx[3] = u * (q[3] + p);
@@ -31,7 +36,7 @@ __device__ __forceinline__ void flux_xy(C g, const C q[4], C Fx[4], C Fy[4]) {
 // One component n of the physical flux (n is wave-uniform in the kernels)
 template <class C>
 __device__ __forceinline__ void flux_component(int n, C g, C rho, C rhoU, C rhoV, C E, C &Fx, C &Fy) {
-  const C oorho = C(1) / rho;
+  const C oorho = fdiv(C(1), rho);
   const C u = rhoU * oorho, v = rhoV * oorho;
   const C p = (g - C(1)) * (E - C(0.5) * rho * (u * u + v * v));
   switch (n) {
@@ -45,7 +50,7 @@ __device__ __forceinline__ void flux_component(int n, C g, C rho, C rhoU, C rhoV
 template <class C>
 __device__ __forceinline__ C sound_speed(C g, const C q[4]) {
   const C p = pressure(g, q[0], q[1], q[2], q[3]);
-  return sqrt(fabs(g * p * (C(1) / q[0])));
+  return sqrt(fabs(g * p * (fdiv(C(1), q[0]))));
 }
 
 // AvgFlux (fluxes.go:135-159)
@@ -61,7 +66,7 @@ __device__ __forceinline__ void avg_flux(C g, const C L[4], const C R[4], C nx, 
 // LaxFlux (fluxes.go:161-190): dissipation uses max(|V|+c), not the normal speed
 template <class C>
 __device__ __forceinline__ void lax_flux(C g, const C L[4], const C R[4], C nx, C ny, C F[4]) {
-  const C uL = L[1] / L[0], vL = L[2] / L[0], uR = R[1] / R[0], vR = R[2] / R[0];
+  const C uL = fdiv(L[1], L[0]), vL = fdiv(L[2], L[0]), uR = fdiv(R[1], R[0]), vR = fdiv(R[2], R[0]);
   const C pL = pressure(g, L[0], L[1], L[2], L[3]), pR = pressure(g, R[0], R[1], R[2], R[3]);
   const C CL = sound_speed(g, L), CR = sound_speed(g, R);
   const C maxV = fmax(sqrt(uL * uL + vL * vL) + CL, sqrt(uR * uR + vR * vR) + CR);
@@ -79,21 +84,21 @@ __device__ __forceinline__ void roe_flux(C g, const C L[4], const C R[4], C nx, 
   const C GM1 = g - C(1);
   const C rULr = L[1] * nx + L[2] * ny, rVLr = L[1] * (-ny) + L[2] * nx;
   const C rURr = R[1] * nx + R[2] * ny, rVRr = R[1] * (-ny) + R[2] * nx;
-  const C rhoL = L[0], uL = rULr / L[0], vL = rVLr / L[0];
-  const C rhoR = R[0], uR = rURr / R[0], vR = rVRr / R[0];
+  const C rhoL = L[0], uL = fdiv(rULr, L[0]), vL = fdiv(rVLr, L[0]);
+  const C rhoR = R[0], uR = fdiv(rURr, R[0]), vR = fdiv(rVRr, R[0]);
   const C pL = pressure(g, L[0], L[1], L[2], L[3]), pR = pressure(g, R[0], R[1], R[2], R[3]);
-  const C hL = (L[3] + pL) / rhoL, hR = (R[3] + pR) / rhoR;
+  const C hL = fdiv(L[3] + pL, rhoL), hR = fdiv(R[3] + pR, rhoR);
   const C sL = sqrt(rhoL), sR = sqrt(rhoR), sLR = sL + sR;
   const C rho = sL * sR;
-  const C u = (sL * uL + sR * uR) / sLR;
-  const C v = (sL * vL + sR * vR) / sLR;
-  const C h = (sL * hL + sR * hR) / sLR;
+  const C u = fdiv(sL * uL + sR * uR, sLR);
+  const C v = fdiv(sL * vL + sR * vR, sLR);
+  const C h = fdiv(sL * hL + sR * hR, sLR);
   const C c2 = GM1 * (h - C(0.5) * (u * u + v * v));
   const C cc = sqrt(c2);
-  C dW1 = -C(0.5) * (rho * (uR - uL)) / cc + C(0.5) * (pR - pL) / c2;
-  C dW2 = (rhoR - rhoL) - (pR - pL) / c2;
+  C dW1 = fdiv(-C(0.5) * (rho * (uR - uL)), cc) + fdiv(C(0.5) * (pR - pL), c2);
+  C dW2 = (rhoR - rhoL) - fdiv(pR - pL, c2);
   C dW3 = rho * (vR - vL);
-  C dW4 = C(0.5) * (rho * (uR - uL)) / cc + C(0.5) * (pR - pL) / c2;
+  C dW4 = fdiv(C(0.5) * (rho * (uR - uL)), cc) + fdiv(C(0.5) * (pR - pL), c2);
   dW1 = fabs(u - cc) * dW1;
   dW2 = fabs(u) * dW2;
   dW3 = fabs(u) * dW3;
@@ -117,25 +122,25 @@ template <class C>
 __device__ __forceinline__ void roeer_flux(C g, const C L[4], const C R[4], C nx, C ny, C F[4]) {
   const C GM1 = g - C(1);
   const C rhoL = L[0], rhoR = R[0];
-  const C ooL = C(1) / rhoL, ooR = C(1) / rhoR;
+  const C ooL = fdiv(C(1), rhoL), ooR = fdiv(C(1), rhoR);
   const C sL = sqrt(rhoL), sR = sqrt(rhoR);
   const C uL = L[1] * ooL, vL = L[2] * ooL, uR = R[1] * ooR, vR = R[2] * ooR;
   const C EL = L[3], ER = R[3];
   const C UL = nx * uL + ny * vL, UR = nx * uR + ny * vR;
   const C pL = pressure(g, L[0], L[1], L[2], L[3]), pR = pressure(g, R[0], R[1], R[2], R[3]);
   const C HL = EL + pL, HR = ER + pR;
-  const C ooRs = C(1) / (sL + sR);
+  const C ooRs = fdiv(C(1), sL + sR);
   const C u = (sL * uL + sR * uR) * ooRs, v = (sL * vL + sR * vR) * ooRs, h = (HL + HR) * ooRs;
   const C rho = sL * sR, H = h * rho, U = nx * u + ny * v;
   const C C2 = GM1 * (h - C(0.5) * (u * u + v * v));
-  const C cc = sqrt(C2), ooC = C(1) / cc, Uabs = fabs(U);
+  const C cc = sqrt(C2), ooC = fdiv(C(1), cc), Uabs = fabs(U);
   F[0] = C(0.5) * (UL * rhoL + UR * rhoR);
   F[1] = C(0.5) * (UL * rhoL * uL + pL * nx + UR * rhoR * uR + pR * nx);
   F[2] = C(0.5) * (UL * rhoL * vL + pL * ny + UR * rhoR * vR + pR * ny);
   F[3] = C(0.5) * (UL * HL + UR * HR);
   const C Uef = C(0.05) * cc;
   const C du = uR - uL, dv = vR - vL, dV2 = du * du + dv * dv;
-  const C ooVmag = C(1) / sqrt(u * u + v * v);
+  const C ooVmag = fdiv(C(1), sqrt(u * u + v * v));
   C n1x, n1y;
   if (dV2 < C(0.01) * C2) { n1x = nx; n1y = ny; } else { n1x = ooVmag * du; n1y = ooVmag * dv; }
   const C n2x = n1y * (nx * n1y - n1x * ny), n2y = -n1x * (nx * n1y - n1x * ny);

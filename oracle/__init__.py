@@ -61,6 +61,7 @@ def lib():
         L.or_rhs.restype = C.c_int
         L.or_get_flux_dofs.argtypes = [C.c_void_p, _dp]
         L.or_step.argtypes = [C.c_void_p]
+        L.or_threads.restype = C.c_int
         L.or_step.restype = C.c_int
         L.or_get_operators.argtypes = [C.c_void_p, _dp, _dp, _dp, _dp, _dp]
         L.or_get_solution_element.argtypes = [C.c_void_p, _dp, _dp, _dp, _dp, _dp, _dp]
@@ -207,6 +208,11 @@ class OracleSolver:
         out = np.empty(self.K)
         lib().or_shock_indicator(self._h, _p(rho), kappa, _p(out))
         return out
+
+
+def threads() -> int:
+    """OpenMP threads of the oracle's parallel loops."""
+    return lib().or_threads()
 
 
 def ivortex_state(t, x, y):

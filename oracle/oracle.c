@@ -34,6 +34,19 @@
 
 #include "wsj_table.h"
 
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+/* Threads the parallel loops of the oracle use */
+int or_threads(void) {
+#ifdef _OPENMP
+  return omp_get_max_threads();
+#else
+  return 1;
+#endif
+}
+
 #define MAXTAGS 64
 
 static void seterr(char *err, int errlen, const char *fmt, ...) {
@@ -1294,6 +1307,7 @@ static void interp_to_edges(or_solver *s, const double *Q) {
   for (int n = 0; n < 4; n++) {
     const double *q = Q + (size_t)n * NpInt * K;
     double *qf = s->QFace + (size_t)n * ne * K;
+#pragma omp parallel for schedule(static)
     for (int p = 0; p < ne; p++)
       for (int k = 0; k < K; k++) {
         double acc = 0;
@@ -1313,6 +1327,7 @@ static inline double *QF(or_solver *s, int n, int p, int k) {
 static void edge_fluxes(or_solver *s, double t) {
   int K = s->K, NpEdge = s->NpEdge, NpInt = s->NpInt;
   double g = s->fs.gamma;
+#pragma omp parallel for schedule(static)
   for (int ei = 0; ei < s->ne; ei++) {
     edge_t *E = &s->E[ei];
     int kL = E->tri[0], enL = E->enr[0], shiftL = enL * NpEdge;
@@ -1385,6 +1400,7 @@ static void rt_flux_internal(or_solver *s, const double *Q) {
   int K = s->K, NpInt = s->NpInt, NpFlux = s->NpFlux;
   double g = s->fs.gamma;
   memset(s->FRT, 0, sizeof(double) * 4 * (size_t)NpFlux * K);
+#pragma omp parallel for schedule(static)
   for (int k = 0; k < K; k++) {
     double Jd = s->Jdet[k], *Ji = &s->Jinv[4 * k];
     for (int i = 0; i < NpInt; i++) {
@@ -1403,6 +1419,7 @@ static void rt_flux_internal(or_solver *s, const double *Q) {
 /* SetRTFluxOnEdges (edges.go:454-483) with GetEdgeValues (:93-113) */
 static void rt_flux_edges(or_solver *s) {
   int K = s->K, NpInt = s->NpInt, NpEdge = s->NpEdge, NpFlux = s->NpFlux;
+#pragma omp parallel for schedule(static)
   for (int k = 0; k < K; k++)
     for (int e = 0; e < 3; e++) {
       int ind2 = k + K * e;
@@ -1427,6 +1444,7 @@ static void rhs_internal(or_solver *s, double *RHS) {
   for (int n = 0; n < 4; n++) {
     const double *F = s->FRT + (size_t)n * NpFlux * K;
     double *R = RHS + (size_t)n * NpInt * K;
+#pragma omp parallel for schedule(static)
     for (int i = 0; i < NpInt; i++)
       for (int k = 0; k < K; k++) {
         double acc = 0;
@@ -1444,6 +1462,7 @@ static void rhs_internal(or_solver *s, double *RHS) {
 static void edge_aggregates(or_solver *s) {
   int K = s->K, NpEdge = s->NpEdge, NpInt = s->NpInt;
   double Np12 = (double)((s->p.N + 1) * (s->p.N + 1));
+#pragma omp parallel for schedule(static)
   for (int ei = 0; ei < s->ne; ei++) {
     edge_t *E = &s->E[ei];
     int k = E->tri[0], shift = E->enr[0] * NpEdge;
@@ -1549,6 +1568,7 @@ int or_step(or_solver *s) {
     for (int n = 0; n < 4; n++) {
       double *q0 = s->Q + n * nv, *q1 = s->Q1 + n * nv, *q2 = s->Q2 + n * nv, *q3 = s->Q3 + n * nv;
       double *q4 = s->Q4 + n * nv, *R = s->Resid + n * nv, *RHS = s->RHSQ + n * nv;
+#pragma omp parallel for schedule(static)
       for (int i = 0; i < NpInt; i++)
         for (int k = 0; k < K; k++) {
           size_t ind = k + (size_t)i * K;

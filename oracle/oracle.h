@@ -95,6 +95,9 @@ void or_shock_indicator(const or_solver *s, const double *rho, double kappa, dou
 /* Vertex max-merge of an element scalar (euler.go:1048-1065) */
 void or_vertex_max(const or_solver *s, const double *elem, double *vert);
 
+/* OpenMP threads used by the element and edge loops */
+int or_threads(void);
+
 /* Standalone pieces pinned by known-answer tests */
 void or_ivortex_state(double t, double x, double y, double *q4);
 void or_freestream(double Minf, double Gamma, double Alpha, double *q4, double *pinf,
