@@ -112,8 +112,10 @@ def main():
         copy_probe(args.copy_probe)
         return
     if args.gpus != 1:
-        print(f"bench.py measures one GPU: --gpus {args.gpus} is not supported (the step "
-              "shards by element partition; see DESIGN.md)", file=sys.stderr)
+        print(f"bench.py measures one GPU: --gpus {args.gpus} is not supported; the step "
+              "would shard by element partition with a per-stage face-trace halo exchange and "
+              "an 8-byte max all-reduce for dt, which this one-GPU build does not implement",
+              file=sys.stderr)
         sys.exit(2)
 
     from gocfd_amd import meshgen

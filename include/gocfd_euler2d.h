@@ -4,7 +4,7 @@
  * This is the drop-in boundary for the reference's Euler2D solver API
  * (Notargets/gocfd, model_problems/Euler2D). The reference has no FFI; each
  * entry point below states the reference interface it replaces. Plain
- * pointers, sizes and status codes only: a cgo binding (see INTEGRATION.md)
+ * pointers, sizes and status codes only: a cgo binding
  * calls these directly.
  *
  * Layout conventions at this boundary are the reference's:
