@@ -31,7 +31,7 @@ def splitmix64(seed: int, n: int) -> np.ndarray:
 
 
 def jittered_quads(nx: int, ny: int, x0=0.0, x1=10.0, y0=-5.0, y1=5.0, jitter=0.1, seed=1,
-                   periodic_x=True):
+                   periodic_x=True, periodic_y=False):
     """Returns (VX, VY, EToV [K,3], tags) with K = 2*nx*ny."""
     hx, hy = (x1 - x0) / nx, (y1 - y0) / ny
     ii, jj = np.meshgrid(np.arange(nx + 1), np.arange(ny + 1))
@@ -56,8 +56,12 @@ def jittered_quads(nx: int, ny: int, x0=0.0, x1=10.0, y0=-5.0, y1=5.0, jitter=0.
             tris.append((a, b, c))
             tris.append((a, c, d))
     EToV = np.array(tris, dtype=np.int32)
-    far = [(v(i, 0), v(i + 1, 0)) for i in range(nx)] + [(v(i, ny), v(i + 1, ny)) for i in range(nx)]
-    tags = {"far": far}
+    bottom = [(v(i, 0), v(i + 1, 0)) for i in range(nx)]
+    top = [(v(i, ny), v(i + 1, ny)) for i in range(nx)]
+    far = [] if periodic_y else bottom + top
+    tags = {"far": far} if far else {}
+    if periodic_y:
+        tags["periodic-2"] = bottom + top
     left = [(v(0, j), v(0, j + 1)) for j in range(ny)]
     right = [(v(nx, j), v(nx, j + 1)) for j in range(ny)]
     if periodic_x:
@@ -65,6 +69,13 @@ def jittered_quads(nx: int, ny: int, x0=0.0, x1=10.0, y0=-5.0, y1=5.0, jitter=0.
     else:
         tags["far"] = far + left + right
     return VX, VY, EToV, tags
+
+
+def periodic_mesh(path: str, n: int, seed: int = 1, jitter: float = 0.1) -> str:
+    """n x n jittered quads on [0,10]x[-5,5], periodic in x and y (no boundary)."""
+    VX, VY, EToV, tags = jittered_quads(n, n, seed=seed, jitter=jitter, periodic_y=True)
+    write_su2(path, VX, VY, EToV, tags)
+    return path
 
 
 def write_su2(path: str, VX, VY, EToV, tags, split_periodic=True) -> None:

@@ -72,6 +72,14 @@ def lib():
         L.or_get_shock.argtypes = [C.c_void_p, _dp, _dp, _dp]
         L.or_shock_indicator.argtypes = [C.c_void_p, _dp, C.c_double, _dp]
         L.or_vertex_max.argtypes = [C.c_void_p, _dp, _dp]
+        L.or_gradient.argtypes = [C.c_void_p, _dp, _dp, _dp]
+        L.or_sensor.argtypes = [C.c_void_p, _dp, _dp]
+        L.or_sensor.restype = C.c_int
+        L.or_filter.argtypes = [C.c_void_p, _dp, _dp, C.c_int]
+        L.or_filter.restype = C.c_int
+        L.or_get_dissipation.argtypes = [C.c_void_p, _dp, _dp, _dp, _dp]
+        L.or_get_dissipation.restype = C.c_int
+        L.or_get_dt.argtypes = [C.c_void_p, _dp]
         L.or_ivortex_state.argtypes = [C.c_double, C.c_double, C.c_double, _dp]
         L.or_freestream.argtypes = [C.c_double, C.c_double, C.c_double, _dp, _dp, _dp]
         L.or_numerical_flux.argtypes = [C.c_int, C.c_double, _dp, _dp, _dp, _dp]
@@ -208,6 +216,44 @@ class OracleSolver:
         out = np.empty(self.K)
         lib().or_shock_indicator(self._h, _p(rho), kappa, _p(out))
         return out
+
+
+    # ---- shock-capturing pieces (PerssonC0) ----
+    def gradient(self, q):
+        """RT gradient of the 4 variables with eps = 1: DX, DY [4][NpFlux][K]."""
+        q = np.ascontiguousarray(q, dtype=np.float64)
+        dx = np.empty((4, self.NpFlux, self.K)); dy = np.empty_like(dx)
+        lib().or_gradient(self._h, _p(q), _p(dx), _p(dy))
+        return dx, dy
+
+    def sensor(self, rho):
+        rho = np.ascontiguousarray(rho, dtype=np.float64)
+        se = np.empty(self.K)
+        if lib().or_sensor(self._h, _p(rho), _p(se)):
+            raise RuntimeError("solver made without the PerssonC0 limiter")
+        return se
+
+    def filter(self, u, sigma):
+        u = np.array(u, dtype=np.float64, order="C")
+        nv = u.size // (self.NpInt * self.K)
+        sig = np.ascontiguousarray(np.broadcast_to(sigma, (self.K,)), dtype=np.float64)
+        if lib().or_filter(self._h, _p(u), _p(sig), nv):
+            raise RuntimeError("solver made without the PerssonC0 limiter")
+        return u
+
+    def dissipation(self):
+        nf = (self.NpFlux, self.K)
+        eps = np.empty(nf); dx = np.empty((4,) + nf); dy = np.empty((4,) + nf)
+        ne = lib().or_num_edges(self._h)
+        ev = np.empty((ne, self.NpEdge, 4))
+        if lib().or_get_dissipation(self._h, _p(eps), _p(dx), _p(dy), _p(ev)):
+            raise RuntimeError("solver made without the PerssonC0 limiter")
+        return dict(EpsF=eps, DissX=dx, DissY=dy, EdgeVisc=ev)
+
+    def element_dt(self):
+        d = np.empty(self.K)
+        lib().or_get_dt(self._h, _p(d))
+        return d
 
 
 def threads() -> int:

@@ -1493,7 +1493,27 @@ static void shock_gradient_and_viscous_edges(or_solver *s, const double *QQQ);
 static void shock_add_dissipation(or_solver *s, double *RHS);
 static void limit_filter(or_solver *s, double *U, int nvars);
 
+/* RHS of a state at time t. With the shock path on, this is the stage RHS
+ * of StepWorker/rkAdvance for a stage other than rkStep 2 (no filtering of
+ * the state): sensor, vertex merge, viscosity field, Euler fluxes, viscous
+ * fluxes, RHS + dissipation, then the modal filter of the RHS. */
 int or_rhs(or_solver *s, const double *Q, double t, double *RHS) {
+  if (s->shock) {
+    size_t nq = 4 * (size_t)s->NpInt * s->K;
+    double *Qc = malloc(sizeof(double) * nq);
+    memcpy(Qc, Q, sizeof(double) * nq);
+    shock_stage_pre(s, Qc, -1);
+    interp_to_edges(s, Qc);
+    edge_fluxes(s, t);
+    shock_gradient_and_viscous_edges(s, Qc);
+    rt_flux_internal(s, Qc);
+    rt_flux_edges(s);
+    rhs_internal(s, RHS);
+    shock_add_dissipation(s, RHS);
+    limit_filter(s, RHS, 4);
+    free(Qc);
+    return 0;
+  }
   interp_to_edges(s, Q);
   edge_fluxes(s, t);
   rt_flux_internal(s, Q);
@@ -1783,40 +1803,49 @@ static void shock_stage_pre(or_solver *s, double *QQQ, int rkStep) {
  * (dfr_startup.go:213-254); then StoreEdgeViscousFlux (edges.go:151-244).
  * The viscous penalty jump uses the owner's stored trace for both sides
  * (edges.go:225-237). */
-static void shock_gradient_and_viscous_edges(or_solver *s, const double *QQQ) {
-  int K = s->K, NpInt = s->NpInt, NpEdge = s->NpEdge, NpFlux = s->NpFlux, N = s->p.N;
+/* The RT-element gradient of variable n of QQQ at the NpFlux points, times
+ * eps (eps == NULL: times 1), from GetSolutionGradientUsingRTElement
+ * (euler.go:864-918): interior DOFs Jinv*U, edge DOFs n*IInII/Jdet times the
+ * owner's stored trace (read reversed by the non-owner), then Div. */
+static void rt_gradient(or_solver *s, const double *QQQ, int n, const double *eps, double *DX, double *DY) {
+  int K = s->K, NpInt = s->NpInt, NpEdge = s->NpEdge, NpFlux = s->NpFlux;
   double *DOFX = malloc(sizeof(double) * NpFlux), *DOFY = malloc(sizeof(double) * NpFlux);
-  for (int n = 0; n < 4; n++) {
-    double *DX = s->DissX + (size_t)n * NpFlux * K, *DY = s->DissY + (size_t)n * NpFlux * K;
-    for (int k = 0; k < K; k++) {
-      double *Ji = &s->Jinv[4 * k], ooJd = 1. / s->Jdet[k];
-      for (int i = 0; i < NpInt; i++) {
-        double Un = QQQ[(size_t)n * NpInt * K + k + (size_t)i * K];
-        DOFX[i] = Ji[0] * Un; DOFY[i] = Ji[1] * Un;
-        DOFX[i + NpInt] = Ji[2] * Un; DOFY[i + NpInt] = Ji[3] * Un;
+  for (int k = 0; k < K; k++) {
+    double *Ji = &s->Jinv[4 * k], ooJd = 1. / s->Jdet[k];
+    for (int i = 0; i < NpInt; i++) {
+      double Un = QQQ[(size_t)n * NpInt * K + k + (size_t)i * K];
+      DOFX[i] = Ji[0] * Un; DOFY[i] = Ji[1] * Un;
+      DOFX[i + NpInt] = Ji[2] * Un; DOFY[i + NpInt] = Ji[3] * Un;
+    }
+    for (int e = 0; e < 3; e++) {
+      int faceInd = k + K * e, ei = s->FaceEdge[faceInd];
+      int sign = (s->E[ei].tri[0] == k) ? 1 : -1;
+      double nx = s->FNx[faceInd], ny = s->FNy[faceInd], IInII = s->IInII[faceInd];
+      const double *EQ = s->EdgeQ + (size_t)ei * NpEdge * 4;
+      for (int i = 0; i < NpEdge; i++) {
+        int ii = (sign < 0) ? NpEdge - 1 - i : i;
+        double Un = EQ[4 * ii + n];
+        DOFX[2 * NpInt + e * NpEdge + i] = ooJd * nx * IInII * Un;
+        DOFY[2 * NpInt + e * NpEdge + i] = ooJd * ny * IInII * Un;
       }
-      for (int e = 0; e < 3; e++) {
-        int faceInd = k + K * e, ei = s->FaceEdge[faceInd];
-        int sign = (s->E[ei].tri[0] == k) ? 1 : -1;
-        double nx = s->FNx[faceInd], ny = s->FNy[faceInd], IInII = s->IInII[faceInd];
-        const double *EQ = s->EdgeQ + (size_t)ei * NpEdge * 4;
-        for (int i = 0; i < NpEdge; i++) {
-          int ii = (sign < 0) ? NpEdge - 1 - i : i;
-          double Un = EQ[4 * ii + n];
-          DOFX[2 * NpInt + e * NpEdge + i] = ooJd * nx * IInII * Un;
-          DOFY[2 * NpInt + e * NpEdge + i] = ooJd * ny * IInII * Un;
-        }
-      }
-      for (int i = 0; i < NpFlux; i++) {
-        double ax = 0, ay = 0;
-        for (int j = 0; j < NpFlux; j++) { ax += s->o.Div[i * NpFlux + j] * DOFX[j]; ay += s->o.Div[i * NpFlux + j] * DOFY[j]; }
-        size_t ind = k + (size_t)i * K;
-        DX[ind] = ax * s->EpsF[ind];
-        DY[ind] = ay * s->EpsF[ind];
-      }
+    }
+    for (int i = 0; i < NpFlux; i++) {
+      double ax = 0, ay = 0;
+      for (int j = 0; j < NpFlux; j++) { ax += s->o.Div[i * NpFlux + j] * DOFX[j]; ay += s->o.Div[i * NpFlux + j] * DOFY[j]; }
+      size_t ind = k + (size_t)i * K;
+      double e = eps ? eps[ind] : 1.0;
+      DX[ind] = ax * e;
+      DY[ind] = ay * e;
     }
   }
   free(DOFX); free(DOFY);
+}
+
+static void shock_gradient_and_viscous_edges(or_solver *s, const double *QQQ) {
+  int K = s->K, NpInt = s->NpInt, NpEdge = s->NpEdge, NpFlux = s->NpFlux, N = s->p.N;
+#pragma omp parallel for schedule(static)
+  for (int n = 0; n < 4; n++)
+    rt_gradient(s, QQQ, n, s->EpsF, s->DissX + (size_t)n * NpFlux * K, s->DissY + (size_t)n * NpFlux * K);
   double Omega = 1. * (double)(N * N);
   for (int ei = 0; ei < s->ne; ei++) {
     edge_t *E = &s->E[ei];
@@ -1894,3 +1923,48 @@ void or_get_shock(const or_solver *s, double *Se, double *Sigma, double *Eps) {
   if (Sigma) memcpy(Sigma, s->Sigma, sizeof(double) * s->K);
   if (Eps) memcpy(Eps, s->Eps, sizeof(double) * s->K);
 }
+
+/* Gradient of the four variables of Q by the RT element with eps = 1
+ * (GetSolutionGradient, euler.go:864-918): DX, DY [4][NpFlux][K]. */
+void or_gradient(or_solver *s, const double *Q, double *DX, double *DY) {
+  size_t nf = (size_t)s->NpFlux * s->K;
+  interp_to_edges(s, Q);
+  edge_fluxes(s, 0.0);
+  for (int n = 0; n < 4; n++) rt_gradient(s, Q, n, NULL, DX + n * nf, DY + n * nf);
+}
+
+/* UpdateSeMoment sensor of a density field [NpInt][K]; needs the shock path. */
+int or_sensor(or_solver *s, const double *rho, double *Se) {
+  if (!s->shock) return 1;
+  sensor_se(s, rho, Se);
+  return 0;
+}
+
+/* limitAndFilterSolution of nvars fields [nvars][NpInt][K] with a given
+ * per-element sigma [K]; needs the shock path. */
+int or_filter(or_solver *s, double *U, const double *sigma, int nvars) {
+  if (!s->shock) return 1;
+  double *keep = malloc(sizeof(double) * s->K);
+  memcpy(keep, s->Sigma, sizeof(double) * s->K);
+  memcpy(s->Sigma, sigma, sizeof(double) * s->K);
+  limit_filter(s, U, nvars);
+  memcpy(s->Sigma, keep, sizeof(double) * s->K);
+  free(keep);
+  return 0;
+}
+
+/* Viscosity field and dissipative fluxes of the last shock-path evaluation:
+ * EpsF [NpFlux][K], DissX/DissY [4][NpFlux][K], viscous edge fluxes
+ * [nedges][NpEdge][4]. */
+int or_get_dissipation(const or_solver *s, double *EpsF, double *DissX, double *DissY, double *EdgeVisc) {
+  if (!s->shock) return 1;
+  size_t nf = (size_t)s->NpFlux * s->K;
+  if (EpsF) memcpy(EpsF, s->EpsF, sizeof(double) * nf);
+  if (DissX) memcpy(DissX, s->DissX, sizeof(double) * 4 * nf);
+  if (DissY) memcpy(DissY, s->DissY, sizeof(double) * 4 * nf);
+  if (EdgeVisc) memcpy(EdgeVisc, s->EdgeVisc, sizeof(double) * 4 * (size_t)s->NpEdge * s->ne);
+  return 0;
+}
+
+/* Per-element DT of the last stage (CalculateLocalDT) */
+void or_get_dt(const or_solver *s, double *DT) { memcpy(DT, s->DT, sizeof(double) * s->K); }

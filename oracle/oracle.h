@@ -94,6 +94,13 @@ void or_get_shock(const or_solver *s, double *Se, double *Sigma, double *Eps);
 void or_shock_indicator(const or_solver *s, const double *rho, double kappa, double *sigma);
 /* Vertex max-merge of an element scalar (euler.go:1048-1065) */
 void or_vertex_max(const or_solver *s, const double *elem, double *vert);
+/* RT-element gradient with eps = 1 (GetSolutionGradient, euler.go:864-918) */
+void or_gradient(or_solver *s, const double *Q, double *DX, double *DY);
+/* Shock-path pieces; return 1 if the solver was made without PerssonC0 */
+int or_sensor(or_solver *s, const double *rho, double *Se);
+int or_filter(or_solver *s, double *U, const double *sigma, int nvars);
+int or_get_dissipation(const or_solver *s, double *EpsF, double *DissX, double *DissY, double *EdgeVisc);
+void or_get_dt(const or_solver *s, double *DT);
 
 /* OpenMP threads used by the element and edge loops */
 int or_threads(void);
