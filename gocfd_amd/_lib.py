@@ -22,7 +22,7 @@ EXPORTS = [
     "gocfd_euler2d_time", "gocfd_euler2d_set_time", "gocfd_euler2d_get_residual",
     "gocfd_euler2d_get_edges", "gocfd_euler2d_get_connectivity", "gocfd_euler2d_get_operators",
     "gocfd_euler2d_get_geometry", "gocfd_euler2d_get_coords", "gocfd_euler2d_kernel_time",
-    "gocfd_euler2d_tiling",
+    "gocfd_euler2d_tiling", "gocfd_euler2d_get_shock", "gocfd_euler2d_get_element_dt",
 ]
 
 

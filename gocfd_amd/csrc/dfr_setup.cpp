@@ -744,6 +744,8 @@ DeviceLayout build_device_layout(const Discretisation &d, int E) {
   L.Jdet.assign(Kp, 1.0); L.Jinv.assign(size_t(4) * Kp, 0.0);
   L.IInII.assign(size_t(3) * Kp, 1.0); L.Nx.assign(size_t(3) * Kp, 1.0); L.Ny.assign(size_t(3) * Kp, 0.0);
   L.oohK.assign(Kp, 0.0);
+  L.hK.assign(Kp, 1.0);
+  L.evert.assign(size_t(3) * Kp, 0);
   const double Np12 = double((d.N + 1) * (d.N + 1));
   for (int kk = 0; kk < Kp; ++kk) {
     const int k = L.perm[kk];
@@ -752,11 +754,20 @@ DeviceLayout build_device_layout(const Discretisation &d, int E) {
     for (int c = 0; c < 4; ++c) L.Jinv[size_t(c) * Kp + kk] = d.Jinv[4 * size_t(k) + c];
     const double hK = d.EdgeLenMax[k] / Np12;  // GetHk (dfr_startup.go:89-98)
     L.oohK[kk] = 1. / hK;
+    L.hK[kk] = hK;
+    for (int c = 0; c < 3; ++c) L.evert[size_t(c) * Kp + kk] = d.EToV[3 * size_t(k) + c];
   }
   // Faces
   L.face_code.assign(size_t(3) * Kp, FK_Inert);
   L.nbr_slot.assign(size_t(3) * Kp, 0);
   L.nbr_elem.assign(size_t(3) * Kp, -1);
+  L.nbr_face.assign(size_t(3) * Kp, 0);
+  L.Nxo.assign(size_t(3) * Kp, 1.0);
+  L.Nyo.assign(size_t(3) * Kp, 0.0);
+  L.vooL.assign(size_t(3) * Kp, 0.0);
+  L.wsrc.assign(size_t(3) * Kp, 0);
+  for (int kk = 0; kk < Kp; ++kk)
+    for (int f = 0; f < 3; ++f) L.wsrc[size_t(f) * Kp + kk] = f * Kp + kk;
   std::vector<std::vector<std::pair<int, int>>> halos(L.ntiles);
   std::vector<std::map<std::pair<int, int>, int>> halo_index(L.ntiles);
   for (int kk = 0; kk < K; ++kk) {
@@ -768,7 +779,11 @@ DeviceLayout build_device_layout(const Discretisation &d, int E) {
       const Edge &own = d.edges[d.EtoEdge[3 * size_t(k) + f]];
       int code = 0;
       if (own.tri[0] == k && own.enr[0] == f) code |= FC_WAVE;
+      else  // the edge's speed is taken on its owner's face
+        L.wsrc[fi] = own.enr[0] * Kp + L.iperm[own.tri[0]];
       L.IInII[fi] = d.IInII[ri];
+      L.Nxo[fi] = d.FaceNx[ri];
+      L.Nyo[fi] = d.FaceNy[ri];
       // normal used by the Riemann solve: the flux owner's
       const size_t oi = size_t(Ed.tri[0]) + size_t(K) * Ed.enr[0];
       if (Ed.nconn == 2) {
@@ -778,6 +793,11 @@ DeviceLayout build_device_layout(const Discretisation &d, int E) {
         const int pk = Ed.tri[1 - side], pf = Ed.enr[1 - side];
         const int pkk = L.iperm[pk];
         L.nbr_elem[fi] = pkk;
+        L.nbr_face[fi] = pf;
+        // StoreEdgeViscousFlux's edge length (edges.go:195-201): owner face 1
+        // is the hypotenuse of the unit triangle
+        const double len = Ed.enr[0] == 1 ? (2. * std::sqrt(2.)) * Ed.IInII[0] : 2. * Ed.IInII[0];
+        L.vooL[fi] = 1. / len;
         if (pkk / E == tile) {
           L.nbr_slot[fi] = pf * NpEdge * E + (pkk - tile * E);
         } else {
@@ -842,6 +862,70 @@ DeviceLayout build_device_layout(const Discretisation &d, int E) {
       L.halo_face[size_t(t) * L.Hmax + i] = halos[t][i].second;
     }
   return L;
+}
+
+}  // namespace gocfd
+
+namespace gocfd {
+
+// NewAliasShockFinder (DG2D/dfr_shock_capturing.go:71-107), ModalFilter with
+// RecommendedFilterParameters (:43-69, :109-140), createInterpolationStencil
+// (model_problems/Euler2D/dissipation.go:414-449)
+std::vector<double> build_shock_operators(const Operators &o) {
+  const int N = o.N, Np = o.NpInt, NpEdge = o.NpEdge, NpFlux = o.NpFlux, NR = Np + 3 * NpEdge;
+  const size_t n2 = size_t(Np) * Np;
+  std::vector<double> out(5 * n2 + Np + size_t(NR) * NpFlux + 3 * size_t(NR), 0.0);
+  double *P = out.data(), *M = P + n2, *D = M + n2, *V = D + n2, *Vi = V + n2, *mf = Vi + n2;
+  double *Div = mf + Np, *Bary = Div + size_t(NR) * NpFlux;
+  // Clip = V diag(modes with i+j < N) Vinv, D = I - Clip, P = Mass D
+  std::vector<double> diag(Np), tmp(n2), clip(n2, 0.0);
+  int ii = 0;
+  for (int i = 0; i <= N; ++i)
+    for (int j = 0; j <= N - i; ++j) diag[ii++] = (i + j >= N) ? 0.0 : 1.0;
+  for (int i = 0; i < Np; ++i)
+    for (int j = 0; j < Np; ++j) tmp[size_t(i) * Np + j] = o.V[size_t(i) * Np + j] * diag[j];
+  for (int i = 0; i < Np; ++i)
+    for (int j = 0; j < Np; ++j) {
+      double acc = 0;
+      for (int l = 0; l < Np; ++l) acc += tmp[size_t(i) * Np + l] * o.Vinv[size_t(l) * Np + j];
+      clip[size_t(i) * Np + j] = acc;
+    }
+  for (int i = 0; i < Np; ++i)
+    for (int j = 0; j < Np; ++j) D[size_t(i) * Np + j] = (i == j ? 1.0 : 0.0) - clip[size_t(i) * Np + j];
+  for (int i = 0; i < Np; ++i)
+    for (int j = 0; j < Np; ++j) {
+      double acc = 0;
+      for (int l = 0; l < Np; ++l) acc += o.Mass[size_t(i) * Np + l] * D[size_t(l) * Np + j];
+      P[size_t(i) * Np + j] = acc;
+    }
+  std::copy(o.Mass.begin(), o.Mass.end(), M);
+  std::copy(o.V.begin(), o.V.end(), V);
+  std::copy(o.Vinv.begin(), o.Vinv.end(), Vi);
+  double alpha;
+  int sp;
+  if (N <= 2) { alpha = 4.0; sp = 4; }
+  else if (N == 3) { alpha = 6.0; sp = 4; }
+  else if (N == 4) { alpha = 8.0; sp = 5; }
+  else if (N == 5) { alpha = 10.0; sp = 6; }
+  else if (N == 6) { alpha = 14.0; sp = 6; }
+  else { alpha = 18.0; sp = 7; }
+  ii = 0;
+  for (int i = 0; i <= N; ++i)
+    for (int j = 0; j <= N - i; ++j) {
+      const int p = i + j;
+      mf[ii++] = (N <= 1 || p == 0) ? 1.0 : std::exp(-alpha * std::pow(double(p) / double(N), double(sp)));
+    }
+  // rows of Div at the used RT points, and their barycentric coordinates
+  // with respect to the vertices (-1,-1), (1,-1), (-1,1)
+  for (int r = 0; r < NR; ++r) {
+    const int row = r < Np ? r : r + Np;
+    for (int j = 0; j < NpFlux; ++j) Div[size_t(r) * NpFlux + j] = o.Div[size_t(row) * NpFlux + j];
+    const double rr = o.R[row], ss = o.S[row];
+    Bary[3 * r] = -0.5 * (rr + ss);
+    Bary[3 * r + 1] = 0.5 * (rr + 1.0);
+    Bary[3 * r + 2] = 0.5 * (ss + 1.0);
+  }
+  return out;
 }
 
 }  // namespace gocfd

@@ -106,7 +106,21 @@ struct DeviceLayout {
   std::vector<int> wave_cnt;                            // [ntiles]
   std::vector<double> bpoint_xy;                        // IVortex boundary points [nb][NpEdge][2]
   int nbfaces = 0;
+  // shock capturing and local time stepping, per face [3][Kp] / element [Kp]
+  std::vector<int> nbr_face;        // neighbour's face number (or 0)
+  std::vector<double> Nxo, Nyo;     // own outward normals
+  std::vector<double> vooL;         // 1 / owner edge length, (2 or 2 sqrt 2) IInII (shared faces)
+  std::vector<int> wsrc;            // face (f*Kp + kk) whose wave speed bounds this face's edge
+  std::vector<int> evert;           // [3][Kp] vertex ids
+  std::vector<double> hK;           // EdgeLenMax / (N+1)^2
 };
 DeviceLayout build_device_layout(const Discretisation &d, int tile_elems);
+
+// Operator block of the shock-capturing kernels for order N (layout of
+// gocfd::dev::ShockArgs::ops): P = Mass (I - Clip), Mass, D = I - Clip,
+// V, Vinv [NpInt^2] each, modal filter factors [NpInt], the rows of Div used
+// (0..NpInt-1 and 2NpInt..NpFlux-1) [NR][NpFlux], and the barycentric
+// coordinates of those RT points [NR][3]; NR = NpInt + 3 NpEdge.
+std::vector<double> build_shock_operators(const Operators &ops);
 
 }  // namespace gocfd

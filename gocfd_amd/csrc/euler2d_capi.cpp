@@ -108,6 +108,16 @@ struct gocfd_euler2d {
   DevBuf<double> rhs_out;
   DevBuf<double> Jdet, Jinv, IInII, Nx, Ny, oohK, bxy, Interp64;
   DevBuf<int> code, nbr_slot, halo_elem, halo_face, wave_list, wave_cnt;
+  // local time stepping
+  bool lts = false;
+  DevBuf<double> fw, fwv, DTk, DTv;
+  DevBuf<int> wsrc;
+  // shock capturing (PerssonC0)
+  bool shock = false;
+  double kappa = 0, eps0 = 0;
+  DevBuf<double> sh_ops, hK, Se, sigma, eps, epsF, diss, Nxo, Nyo, vooL;
+  DevBuf<int> evert, nbr_elem, nbr_face;
+  DevBuf<unsigned long long> sigV, epsV;
   dev::Ctl *ctl = nullptr;
   long long gstage = 0;  // stage launches since the control block was initialised
   int step_index = 0;    // steps since the control block was initialised
@@ -137,6 +147,19 @@ struct gocfd_euler2d {
     a.g.halo_elem = halo_elem.p; a.g.halo_face = halo_face.p; a.g.bxy = bxy.p;
     a.g.wave_list = wave_list.p; a.g.wave_cnt = wave_cnt.p;
     a.Interp64 = Interp64.p;
+    a.lts.on = lts ? 1 : 0;
+    a.lts.fw = fw.p; a.lts.fwv = fwv.p; a.lts.wsrc = wsrc.p; a.lts.DTk = DTk.p; a.lts.DTv = DTv.p;
+    if (shock) {
+      a.sh.ops = sh_ops.p; a.sh.evert = evert.p; a.sh.hK = hK.p;
+      a.sh.sigV = sigV.p; a.sh.epsV = epsV.p; a.sh.Nv = d.Nv;
+      a.sh.Se = Se.p; a.sh.sigma = sigma.p; a.sh.eps = eps.p; a.sh.epsF = epsF.p; a.sh.diss = diss.p;
+      a.sh.nbr_elem = nbr_elem.p; a.sh.nbr_face = nbr_face.p;
+      a.sh.Nxo = Nxo.p; a.sh.Nyo = Nyo.p; a.sh.vooL = vooL.p;
+      a.sh.kappa = kappa; a.sh.eps0 = eps0;
+      const double np1 = double(d.N + 1);
+      a.sh.cdiff = 1. / (np1 * np1);
+      a.sh.omega = 1. * double(d.N * d.N);
+    }
     a.ph.gamma = fs.gamma; a.ph.minf = fs.minf; a.ph.pinf = fs.pinf; a.ph.cinf = fs.cinf;
     for (int i = 0; i < 4; ++i) a.ph.qinf[i] = fs.q[i];
     a.ph.cfl = prm.cfl; a.ph.final_time = prm.final_time; a.ph.flux = prm.flux_type;
@@ -203,11 +226,30 @@ struct gocfd_euler2d {
     HIPCHECK(dev::launch_ctl_init(ctl, time, stream));
     gstage = 0;
     step_index = 0;
-    dev::StageArgs a = args();
-    a.qs = q[0];
-    a.ring = 0;
-    a.step_index = 0;
-    HIPCHECK(dev::launch_prologue(prm.polynomial_order, prec, a, stream));
+    if (shock) {
+      HIPCHECK(dev::launch_fill_u64(sigV.p, sigV.n, 0ull, stream));
+      HIPCHECK(dev::launch_fill_u64(epsV.p, epsV.n, 0ull, stream));
+    }
+    if (lts) HIPCHECK(dev::launch_fill_f64(DTv.p, DTv.n, 0.0, stream));
+    if (!shock) {  // with shock capturing each stage's gradient pass measures its wave speeds
+      dev::StageArgs a = args();
+      a.qs = q[0];
+      a.ring = 0;
+      a.par2 = 0;
+      a.step_index = 0;
+      HIPCHECK(dev::launch_prologue(prm.polynomial_order, prec, a, stream));
+    }
+  }
+
+  // One stage: the shock-capturing passes (if any), then the stage kernel
+  void launch_one_stage(dev::StageArgs &a) {
+    const int N = prm.polynomial_order;
+    if (shock) {
+      a.sh.vbuf = int(gstage & 1);
+      a.sh.filter_state = a.stage == 2 ? 1 : 0;
+      for (int pass = 0; pass < 3; ++pass) HIPCHECK(dev::launch_shock(N, pass, a, stream));
+    }
+    HIPCHECK(dev::launch_stage(N, prec, shock ? 1 : 0, a, stream));
   }
 
   void enqueue_steps(int nsteps) {
@@ -218,8 +260,9 @@ struct gocfd_euler2d {
         a.stage = s;
         a.step_index = step_index;
         a.ring = int(gstage % 3);
+        a.par2 = int(gstage & 1);
         a.qs = q[s];
-        HIPCHECK(dev::launch_stage(prm.polynomial_order, prec, a, stream));
+        launch_one_stage(a);
         ++gstage;
       }
       ++step_index;
@@ -312,12 +355,19 @@ int gocfd_euler2d_create(const gocfd_mesh *mesh, const gocfd_params *p, gocfd_eu
   if (p->flux_type < 0 || p->flux_type > 3) return fail(GOCFD_ERR_ARG, "unable to use flux type");
   if (p->init_type != GOCFD_INIT_FREESTREAM && p->init_type != GOCFD_INIT_IVORTEX)
     return fail(GOCFD_ERR_ARG, "init type not supported on the GPU path (freestream, ivortex)");
-  if (p->local_time_stepping) return fail(GOCFD_ERR_ARG, "LocalTimeStepping is not supported on the GPU path");
-  if (p->limiter != 0) return fail(GOCFD_ERR_ARG, "the PerssonC0 limiter is not supported on the GPU path");
+  if (p->limiter != 0 && p->limiter != 1) return fail(GOCFD_ERR_ARG, "unable to use limiter");
   if (p->precision != GOCFD_PREC_FP64 && p->precision != GOCFD_PREC_FP32)
     return fail(GOCFD_ERR_ARG, "unknown precision");
+  if (p->limiter == 1 && p->precision != GOCFD_PREC_FP64)
+    return fail(GOCFD_ERR_ARG, "the PerssonC0 limiter runs in FP64 only");
   auto h = std::make_unique<gocfd_euler2d>();
   h->prm = *p;
+  h->lts = p->local_time_stepping != 0;
+  // NewScalarDissipation (dissipation.go:111-165) for order >= 1; Kappa
+  // replaces the default 5 in the sensor ramp, the viscosity scale stays 5/1.5
+  h->shock = p->limiter == 1 && p->polynomial_order != 0;
+  h->kappa = p->kappa != 0. ? p->kappa : 5.;
+  h->eps0 = 5. / 1.5;
   h->prec = p->precision;
   h->ssize = h->prec == 0 ? sizeof(double) : sizeof(float);
   h->d = build_discretisation(mesh->m, p->polynomial_order);
@@ -356,7 +406,7 @@ int gocfd_euler2d_create(const gocfd_mesh *mesh, const gocfd_params *p, gocfd_eu
   HIPCHECK(hipGetDeviceProperties(&prop, h->device));
   if (std::strncmp(prop.gcnArchName, "gfx950", 6) != 0)
     return fail(GOCFD_ERR_DEVICE, std::string("device is not gfx950: ") + prop.gcnArchName);
-  const size_t lds = dev::stage_lds_bytes(p->polynomial_order, h->prec, h->L.Hmax) + 64;
+  const size_t lds = dev::stage_lds_bytes(p->polynomial_order, h->prec, h->L.Hmax) + 4096;
   int lds_max = 0, lds_optin = 0;
   HIPCHECK(hipDeviceGetAttribute(&lds_max, hipDeviceAttributeMaxSharedMemoryPerBlock, h->device));
   if (hipDeviceGetAttribute(&lds_optin, hipDeviceAttributeSharedMemPerBlockOptin, h->device) == hipSuccess)
@@ -380,6 +430,25 @@ int gocfd_euler2d_create(const gocfd_mesh *mesh, const gocfd_params *p, gocfd_eu
   h->code.upload(L.face_code); h->nbr_slot.upload(L.nbr_slot);
   h->halo_elem.upload(L.halo_elem); h->halo_face.upload(L.halo_face);
   h->wave_list.upload(L.wave_list); h->wave_cnt.upload(L.wave_cnt);
+  if (h->lts) {
+    h->fw.alloc(size_t(2) * 3 * L.Kp);
+    h->fwv.alloc(size_t(2) * 3 * L.Kp);
+    h->DTk.alloc(L.Kp);
+    h->DTv.alloc(L.Kp);
+    h->wsrc.upload(L.wsrc);
+  }
+  if (h->shock) {
+    const int NR = h->d.ops.NpInt + 3 * h->d.ops.NpEdge;
+    h->sh_ops.upload(build_shock_operators(h->d.ops));
+    h->evert.upload(L.evert); h->hK.upload(L.hK);
+    h->nbr_elem.upload(L.nbr_elem); h->nbr_face.upload(L.nbr_face);
+    h->Nxo.upload(L.Nxo); h->Nyo.upload(L.Nyo); h->vooL.upload(L.vooL);
+    h->sigV.alloc(size_t(2) * std::max(h->d.Nv, 1));
+    h->epsV.alloc(size_t(2) * std::max(h->d.Nv, 1));
+    h->Se.alloc(L.Kp); h->sigma.alloc(L.Kp); h->eps.alloc(L.Kp);
+    h->epsF.alloc(size_t(L.Kp) * NR);
+    h->diss.alloc(size_t(L.Kp) * 4 * 2 * NR);
+  }
   std::vector<double> divT(h->d.ops.DivInt.size());  // [NpFlux][NpInt] for contiguous per-DOF rows
   for (int i = 0; i < h->d.ops.NpInt; ++i)
     for (int j = 0; j < h->d.ops.NpFlux; ++j)
@@ -457,7 +526,14 @@ int gocfd_euler2d_rhs(gocfd_euler2d *h, const double *Q, double t, double *RHS) 
   a.t_rhs = t;
   a.qs = h->scratch_in;
   a.rhs_out = h->rhs_out.p;
-  HIPCHECK(dev::launch_stage(h->prm.polynomial_order, h->prec, a, h->stream));
+  a.ring = int(h->gstage % 3);
+  a.par2 = int(h->gstage & 1);
+  a.step_index = h->step_index;
+  h->launch_one_stage(a);
+  if (h->shock) {  // the vertex buffer this evaluation used must be clear for the next stage
+    HIPCHECK(dev::launch_fill_u64(h->sigV.p, h->sigV.n, 0ull, h->stream));
+    HIPCHECK(dev::launch_fill_u64(h->epsV.p, h->epsV.n, 0ull, h->stream));
+  }
   HIPCHECK(hipStreamSynchronize(h->stream));
   h->from_device(h->rhs_out.p, RHS, true);
   return GOCFD_OK;
@@ -571,6 +647,39 @@ int gocfd_euler2d_get_coords(gocfd_euler2d *h, double *SolX, double *SolY, doubl
   };
   cp(SolX, d.SolX); cp(SolY, d.SolY); cp(FluxX, d.FluxX); cp(FluxY, d.FluxY);
   return GOCFD_OK;
+}
+
+int gocfd_euler2d_get_shock(gocfd_euler2d *h, double *Se, double *Sigma, double *Eps) {
+  GUARD_BEGIN
+  if (!h) return fail(GOCFD_ERR_ARG, "null handle");
+  NEED_DEVICE(h);
+  if (!h->shock) return fail(GOCFD_ERR_ARG, "the solver was created without the PerssonC0 limiter");
+  HIPCHECK(hipSetDevice(h->device));
+  HIPCHECK(hipStreamSynchronize(h->stream));
+  const double *src[3] = {h->Se.p, h->sigma.p, h->eps.p};
+  double *dst[3] = {Se, Sigma, Eps};
+  std::vector<double> tmp(h->L.Kp);
+  for (int i = 0; i < 3; ++i) {
+    if (!dst[i]) continue;
+    HIPCHECK(hipMemcpy(tmp.data(), src[i], tmp.size() * sizeof(double), hipMemcpyDeviceToHost));
+    for (int kk = 0; kk < h->L.K; ++kk) dst[i][h->L.perm[kk]] = tmp[kk];
+  }
+  return GOCFD_OK;
+  GUARD_END
+}
+
+int gocfd_euler2d_get_element_dt(gocfd_euler2d *h, double *DT) {
+  GUARD_BEGIN
+  if (!h || !DT) return fail(GOCFD_ERR_ARG, "null argument");
+  NEED_DEVICE(h);
+  if (!h->lts) return fail(GOCFD_ERR_ARG, "the solver was created without LocalTimeStepping");
+  HIPCHECK(hipSetDevice(h->device));
+  HIPCHECK(hipStreamSynchronize(h->stream));
+  std::vector<double> tmp(h->L.Kp);
+  HIPCHECK(hipMemcpy(tmp.data(), h->DTk.p, tmp.size() * sizeof(double), hipMemcpyDeviceToHost));
+  for (int kk = 0; kk < h->L.K; ++kk) DT[h->L.perm[kk]] = tmp[kk];
+  return GOCFD_OK;
+  GUARD_END
 }
 
 int gocfd_euler2d_kernel_time(gocfd_euler2d *h, double *avg_stage_ms, int32_t *launches) {

@@ -12,13 +12,14 @@ constexpr int kTile = 64;      // elements per workgroup tile (one wavefront wid
 constexpr int kThreads = 256;  // threads per workgroup
 
 // Device control block. Time is double-buffered by step parity and the wave
-// speed maximum lives in a ring of three slots indexed by the global stage
+// speed maxima live in rings of three slots indexed by the global stage
 // counter, so that no kernel writes a value another block of the same launch
 // still reads.
 struct Ctl {
   double time[2];
   double dt[5];
-  unsigned long long wmax[3];
+  unsigned long long wmax[3];   // max (|U|+c)/hK, IEEE bits of a positive double
+  unsigned long long wmaxv[3];  // max eps/hK^2 (shock capturing)
   int nan;
   int steps;
   int pad[2];
@@ -39,30 +40,75 @@ struct Phys {
   int flux;
 };
 
+// Local time stepping (CalculateLocalDT, euler.go:973-1002)
+struct LtsArgs {
+  int on;
+  double *fw;        // [2][3][Kp] per-face max wave speed, buffer by stage parity
+  double *fwv;       // [2][3][Kp] per-face max viscous speed (shock capturing)
+  const int *wsrc;   // [3][Kp] face (f*Kp + kk) whose edge speed bounds this face
+  double *DTk;       // [Kp] element DT of the last stage
+  double *DTv;       // [Kp] element DTVisc (never reset, as in the reference)
+};
+
+// Shock capturing (PerssonC0): per-order operators and per-element fields.
+// Operator block (double): Pm, Mass, Dm, V, Vinv [NpInt^2] each, mf [NpInt],
+// DivR [NR][NpFlux] (rows 0..NpInt-1 and 2NpInt.. of Div), Bary [NR][3],
+// with NR = NpInt + 3 NpEdge.
+struct ShockArgs {
+  const double *ops;
+  const int *evert;                  // [3][Kp] vertex ids (original numbering)
+  const double *hK;                  // [Kp] EdgeLenMax / (N+1)^2
+  unsigned long long *sigV, *epsV;   // [2][Nv] vertex maxima (IEEE bits)
+  int Nv, vbuf;                      // vertex buffer used by this stage
+  double *Se, *sigma, *eps;          // [Kp]
+  double *epsF;                      // [Kp][NR] viscosity at the used RT points
+  double *diss;                      // [Kp][4][2][NR] eps * grad at the used RT points
+  const int *nbr_elem, *nbr_face;    // [3][Kp]
+  const double *Nxo, *Nyo;           // [3][Kp] own outward normals
+  const double *vooL;                // [3][Kp] 1 / owner edge length (shared faces)
+  double kappa, eps0, cdiff, omega;
+  int filter_state;                  // filter the stage state (rkStep == 2)
+};
+
 struct StageArgs {
   int K, Kp, ntiles, Hmax, Wmax;
   int stage;           // 0..4, or -1 for the RHS-only evaluation
   int step_index;      // steps since the state was set (selects time parity)
   int ring;            // global stage counter mod 3
+  int par2;            // global stage counter mod 2
   double t_rhs;        // time for the RHS-only evaluation
-  const void *qs;      // stage input state (element-major)
+  void *qs;            // stage input state (element-major)
   void *q0, *q1, *q2, *q3, *q4, *R;
   double *rhs_out;     // RHS-only output (element-major, double)
   Ctl *ctl;
   Geom g;
-  const double *Interp64;       // FluxEdgeInterp in FP64 (wave-speed prologue)
+  const double *Interp64;       // FluxEdgeInterp in FP64
   Phys ph;
+  LtsArgs lts;
+  ShockArgs sh;
 };
 
 // Precision variants: 0 = FP64 storage and arithmetic; 1 = FP32 storage and
 // bulk arithmetic with FP64 geometry, dt reduction and SSP combination.
-hipError_t launch_stage(int N, int prec, const StageArgs &a, hipStream_t s);
+// shock != 0 selects the PerssonC0 stage kernel (FP64 only).
+hipError_t launch_stage(int N, int prec, int shock, const StageArgs &a, hipStream_t s);
 hipError_t launch_prologue(int N, int prec, const StageArgs &a, hipStream_t s);
+// Shock-capturing passes of one stage: 0 sensor + vertex max, 1 element
+// fields (+ stage-state filter), 2 gradient, viscosity and wave speeds.
+hipError_t launch_shock(int N, int pass, const StageArgs &a, hipStream_t s);
 hipError_t launch_ctl_init(Ctl *ctl, double time, hipStream_t s);
+hipError_t launch_fill_u64(unsigned long long *p, size_t n, unsigned long long v, hipStream_t s);
+hipError_t launch_fill_f64(double *p, size_t n, double v, hipStream_t s);
 // Copy the order-N operators (DivInt^T [NpFlux][NpInt], FluxEdgeInterp
 // [3NpEdge][NpInt]) into the current device's constant memory.
 hipError_t upload_operators(int N, const double *divT, const double *interp);
 size_t stage_lds_bytes(int N, int prec, int Hmax);
+
+// Per-order entry points (one translation unit per order)
+template <int N> hipError_t launch_stage_n(int prec, int shock, const StageArgs &a, hipStream_t s);
+template <int N> hipError_t launch_prologue_n(int prec, const StageArgs &a, hipStream_t s);
+template <int N> hipError_t launch_shock_n(int pass, const StageArgs &a, hipStream_t s);
+template <int N> hipError_t upload_operators_n(const double *divT, const double *interp);
 
 }  // namespace dev
 }  // namespace gocfd

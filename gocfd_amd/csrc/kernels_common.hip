@@ -1,0 +1,96 @@
+// Order dispatch of the stage kernels, and the small control kernels.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+
+#include "kernels.hpp"
+
+namespace gocfd {
+namespace dev {
+
+#define GOCFD_DISPATCH_ORDER(N, CALL) \
+  switch (N) {                        \
+    case 0: return CALL(0);           \
+    case 1: return CALL(1);           \
+    case 2: return CALL(2);           \
+    case 3: return CALL(3);           \
+    case 4: return CALL(4);           \
+    default: return hipErrorInvalidValue; \
+  }
+
+hipError_t launch_stage(int N, int prec, int shock, const StageArgs &a, hipStream_t s) {
+#define CALL(n) launch_stage_n<n>(prec, shock, a, s)
+  GOCFD_DISPATCH_ORDER(N, CALL)
+#undef CALL
+}
+
+hipError_t launch_prologue(int N, int prec, const StageArgs &a, hipStream_t s) {
+#define CALL(n) launch_prologue_n<n>(prec, a, s)
+  GOCFD_DISPATCH_ORDER(N, CALL)
+#undef CALL
+}
+
+hipError_t launch_shock(int N, int pass, const StageArgs &a, hipStream_t s) {
+#define CALL(n) launch_shock_n<n>(pass, a, s)
+  GOCFD_DISPATCH_ORDER(N, CALL)
+#undef CALL
+}
+
+hipError_t upload_operators(int N, const double *divT, const double *interp) {
+#define CALL(n) upload_operators_n<n>(divT, interp)
+  GOCFD_DISPATCH_ORDER(N, CALL)
+#undef CALL
+}
+
+// Dynamic LDS of the stage kernel: the tile state [NV][kTile+1] and the
+// trace/flux buffer [4][3 NpEdge kTile + NpEdge Hmax].
+size_t stage_lds_bytes(int N, int prec, int Hmax) {
+  if (N < 0 || N > 4) return 0;
+  const size_t cs = prec == 0 ? sizeof(double) : sizeof(float);
+  const size_t NpInt = size_t(N + 1) * (N + 2) / 2, NpEdge = size_t(N) + 2;
+  return cs * (4 * NpInt * (kTile + 1) + 4 * (3 * NpEdge * kTile + NpEdge * size_t(Hmax)));
+}
+
+__global__ void k_ctl_init(Ctl *ctl, double time) {
+  ctl->time[0] = time;
+  ctl->time[1] = time;
+  for (int i = 0; i < 5; ++i) ctl->dt[i] = 0.0;
+  for (int i = 0; i < 3; ++i) {
+    ctl->wmax[i] = 0ull;
+    ctl->wmaxv[i] = 0ull;
+  }
+  ctl->nan = 0;
+  ctl->steps = 0;
+}
+
+__global__ void k_fill_u64(unsigned long long *p, size_t n, unsigned long long v) {
+  for (size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += size_t(gridDim.x) * blockDim.x)
+    p[i] = v;
+}
+
+__global__ void k_fill_f64(double *p, size_t n, double v) {
+  for (size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += size_t(gridDim.x) * blockDim.x)
+    p[i] = v;
+}
+
+hipError_t launch_ctl_init(Ctl *ctl, double time, hipStream_t s) {
+  hipLaunchKernelGGL(k_ctl_init, dim3(1), dim3(1), 0, s, ctl, time);
+  return hipGetLastError();
+}
+
+hipError_t launch_fill_u64(unsigned long long *p, size_t n, unsigned long long v, hipStream_t s) {
+  if (n == 0) return hipSuccess;
+  const int blocks = int(std::min<size_t>((n + 255) / 256, 4096));
+  hipLaunchKernelGGL(k_fill_u64, dim3(blocks), dim3(256), 0, s, p, n, v);
+  return hipGetLastError();
+}
+
+hipError_t launch_fill_f64(double *p, size_t n, double v, hipStream_t s) {
+  if (n == 0) return hipSuccess;
+  const int blocks = int(std::min<size_t>((n + 255) / 256, 4096));
+  hipLaunchKernelGGL(k_fill_f64, dim3(blocks), dim3(256), 0, s, p, n, v);
+  return hipGetLastError();
+}
+
+}  // namespace dev
+}  // namespace gocfd

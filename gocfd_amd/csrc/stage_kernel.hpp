@@ -1,0 +1,1001 @@
+// gfx950 kernels for the 2D Euler DFR right-hand side and SSP54 stages,
+// templated on the solution order N. Each stage_n<N>.hip translation unit
+// includes this header once and instantiates one order, so that the orders
+// compile in parallel and each holds only its own operators in constant
+// memory.
+//
+// One stage launch per Runge-Kutta stage. A workgroup owns a tile of kTile
+// consecutive elements (Hilbert order), whose element-major state is one
+// contiguous chunk of HBM. Per stage and tile:
+//   A. load the stage state into LDS; face traces by the FluxEdgeInterp
+//      contraction (model_problems/Euler2D/edges.go:485-491) for the tile's
+//      faces and for the out-of-tile neighbour faces listed in the halo table
+//   B. Riemann / boundary flux at every face point (edges.go:325-452,
+//      bcs.go:11-133), scaled by ||n|| (SetRTFluxOnEdges edges.go:454-483);
+//      with shock capturing, minus the viscous edge flux
+//      (StoreEdgeViscousFlux edges.go:151-244)
+//   C. interior physical flux (minus eps*grad Q with shock capturing),
+//      contravariant transform and the DivInt contraction (SetRTFluxInternal
+//      euler.go:701-726, RHSInternalPoints euler.go:665-699, AddDissipation
+//      dissipation.go:274-346), then the modal filter of the RHS with shock
+//      capturing (LimitFilterSolution dissipation.go:520-542)
+//   E. SSP54 combination (euler.go:502-565) with the global dt of this stage
+//      or the element's local dt (CalculateLocalDT euler.go:973-1002)
+//   F/G. traces of the new state, boundary states at the next stage's time
+//      and the owner-side CFL wave-speed maximum (edges.go:246-323), folded
+//      into one atomic per workgroup for the next stage's dt
+//      (calculateGlobalDT euler.go:945-971)
+// With shock capturing three passes precede each stage launch: the sensor
+// with the vertex max-merge, the element viscosity fields (and the stage-2
+// state filter), and the RT gradient with the stage's wave speeds.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <vector>
+
+#include "dfr_setup.hpp"
+#include "kernels.hpp"
+#include "physics.hpp"
+
+namespace gocfd {
+namespace dev {
+
+template <int N>
+struct Shape {
+  static constexpr int NpInt = (N + 1) * (N + 2) / 2;
+  static constexpr int NpEdge = N + 2;
+  static constexpr int NpFlux = (N + 2) * (N + 4);
+  static constexpr int NF = 3 * NpEdge;  // face points per element
+  static constexpr int NV = 4 * NpInt;   // state values per element
+  static constexpr int NR = NpInt + NF;  // RT points whose gradient is used
+  // shock-operator block offsets (see ShockArgs)
+  static constexpr int oP = 0, oM = NpInt * NpInt, oD = 2 * NpInt * NpInt, oV = 3 * NpInt * NpInt,
+                       oVi = 4 * NpInt * NpInt, omf = 5 * NpInt * NpInt, oDiv = omf + NpInt,
+                       oBary = oDiv + NR * NpFlux, shock_ops = oBary + 3 * NR;
+  static constexpr int ops = (NpFlux + NF) * NpInt;
+};
+
+// The operators of this translation unit's order in constant memory, so that
+// every coefficient is a uniform scalar load: DivInt^T [NpFlux][NpInt]
+// followed by FluxEdgeInterp [3 NpEdge][NpInt], in FP64 and FP32.
+constexpr int kOpsMax = ((4 + 2) * (4 + 4) + 3 * (4 + 2)) * 15;
+static __constant__ double cOpsD[kOpsMax];
+static __constant__ float cOpsF[kOpsMax];
+template <class C>
+__device__ __forceinline__ const C *ops_base();
+template <>
+__device__ __forceinline__ const double *ops_base<double>() { return cOpsD; }
+template <>
+__device__ __forceinline__ const float *ops_base<float>() { return cOpsF; }
+
+// Tile index of a workgroup: with workgroups dealt round-robin to the 8 XCDs,
+// XCD x processes the contiguous tile range [x*q, (x+1)*q), so neighbouring
+// tiles share an L2.
+__device__ __forceinline__ int tile_of_block() {
+  const int q = gridDim.x >> 3;
+  return (blockIdx.x & 7) * q + (blockIdx.x >> 3);
+}
+
+__device__ __forceinline__ double wave_max(double v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v = fmax(v, __shfl_xor(v, off, 64));
+  return v;
+}
+
+__device__ __forceinline__ unsigned long long bits(double v) {
+  return (unsigned long long)__double_as_longlong(v);
+}
+__device__ __forceinline__ double from_bits(unsigned long long b) { return __longlong_as_double((long long)b); }
+
+template <class C>
+__device__ __forceinline__ void boundary_state(int kind, const Phys &ph, const double *bxy, int bidx,
+                                               int NpEdge, int pp, double t, const C own[4],
+                                               double nx, double ny, double qb[4]) {
+  double qi[4] = {double(own[0]), double(own[1]), double(own[2]), double(own[3])};
+  double qf[4];
+  if (kind == FK_Far) {
+    qf[0] = ph.qinf[0]; qf[1] = ph.qinf[1]; qf[2] = ph.qinf[2]; qf[3] = ph.qinf[3];
+  } else {
+    const double *xy = bxy + 2 * (size_t(bidx) * NpEdge + pp);
+    ivortex_state(t, xy[0], xy[1], qf);
+  }
+  riemann_bc(ph.gamma, ph.minf, ph.pinf, ph.cinf, qi, qf, nx, ny, qb);
+}
+
+// Wave speed (|U| + c) / hK of a face point (StoreEdgeAggregates, edges.go:264-272)
+__device__ __forceinline__ double point_speed(double g, const double q[4], double oohK) {
+  const double oorho = 1. / q[0], u = q[1] * oorho, v = q[2] * oorho;
+  const double U = sqrt(u * u + v * v);
+  const double c = sound_speed(g, q);
+  return oohK * (U + c);
+}
+
+// CheckIfFinished (euler.go:796-801): no further steps once Time >= FinalTime
+__device__ __forceinline__ bool step_skipped(const StageArgs &a) {
+  return a.stage >= 0 && a.step_index > 0 && a.ctl->time[a.step_index & 1] >= a.ph.final_time;
+}
+
+template <int N, class S, class C, int FLUX, bool LAST, bool SHOCK>
+__global__ void __launch_bounds__(kThreads) k_stage(StageArgs a) {
+  using Sh = Shape<N>;
+  constexpr int NpInt = Sh::NpInt, NpEdge = Sh::NpEdge, NF = Sh::NF, NV = Sh::NV, NR = Sh::NR;
+  constexpr int E = kTile, P = kTile + 1;
+  static_assert(kThreads == 4 * kTile, "one thread per (element, variable)");
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  C *sQ = reinterpret_cast<C *>(smem_raw);  // [NV][P]
+  const int TS = NF * E + NpEdge * a.Hmax;
+  C *sT = sQ + NV * P;                       // [4][TS]: traces, then fluxes
+  C *sR = sT;                                // [NV][P]: RHS (aliases sT)
+  __shared__ double red[kThreads / 64];
+  __shared__ double sDT[kTile];
+  __shared__ unsigned long long sWf[3 * kTile];
+
+  const int tile = tile_of_block();
+  if (tile >= a.ntiles) return;
+  const int tid = threadIdx.x;
+  const bool rhs_only = a.stage < 0;
+  const int par = a.step_index & 1;
+  Ctl *ctl = a.ctl;
+  const double time = rhs_only ? a.t_rhs : ctl->time[par];
+  if (step_skipped(a)) {
+    if (a.stage == 4 && blockIdx.x == 0 && threadIdx.x == 0) ctl->time[par ^ 1] = time;
+    return;
+  }
+  const int k0 = tile * E;
+  const int Kp = a.Kp;
+  const S *qs = static_cast<const S *>(a.qs);
+  const C *DvT = ops_base<C>();                         // DivInt^T [NpFlux][NpInt]
+  const C *Itp = DvT + Sh::NpFlux * NpInt;              // FluxEdgeInterp [NF][NpInt]
+  const C g = C(a.ph.gamma);
+
+  // ---- A: tile state -> LDS (coalesced, contiguous chunk) ----
+  // Every load is issued before the first LDS store, so a thread's LPT
+  // loads are in flight together.
+  constexpr int LPT = (E * NV + kThreads - 1) / kThreads;
+  {
+    const S *qt = qs + size_t(k0) * NV;
+    S v[LPT];
+#pragma unroll
+    for (int j = 0; j < LPT; ++j) {
+      const int idx = tid + j * kThreads;
+      v[j] = (idx < E * NV) ? qt[idx] : S(0);
+    }
+#pragma unroll
+    for (int j = 0; j < LPT; ++j) {
+      const int idx = tid + j * kThreads;
+      if (idx < E * NV) {
+        const int e = idx / NV, r = idx - e * NV;
+        sQ[r * P + e] = C(v[j]);
+      }
+    }
+  }
+  __syncthreads();
+  const int te = tid % E, tn = tid / E;  // (element, variable) of this thread
+  {
+    C q[NpInt];
+#pragma unroll
+    for (int i = 0; i < NpInt; ++i) q[i] = sQ[(tn * NpInt + i) * P + te];
+#pragma unroll 2
+    for (int p = 0; p < NF; ++p) {
+      C t = C(0);
+#pragma unroll
+      for (int i = 0; i < NpInt; ++i) t += Itp[p * NpInt + i] * q[i];
+      sT[tn * TS + p * E + te] = t;
+    }
+  }
+  for (int it = tid; it < a.Hmax * 4; it += kThreads) {
+    const int h = it % a.Hmax, n = it / a.Hmax;
+    const int kh = a.g.halo_elem[size_t(tile) * a.Hmax + h];
+    if (kh < 0) continue;
+    const int f = a.g.halo_face[size_t(tile) * a.Hmax + h];
+    const S *qn = qs + size_t(kh) * NV + n * NpInt;
+    C q[NpInt];
+#pragma unroll
+    for (int i = 0; i < NpInt; ++i) q[i] = C(qn[i]);
+#pragma unroll
+    for (int pp = 0; pp < NpEdge; ++pp) {
+      const C *row = Itp + (f * NpEdge + pp) * NpInt;
+      C t = C(0);
+#pragma unroll
+      for (int i = 0; i < NpInt; ++i) t += row[i] * q[i];
+      sT[n * TS + NF * E + h * NpEdge + pp] = t;
+    }
+  }
+  __syncthreads();
+
+  // ---- B: numerical flux at every face point ----
+  // Shared faces first (unrolled, results held in registers until every
+  // neighbour trace has been read), then boundary faces in a rolled loop: a
+  // boundary face point reads and writes only its own LDS slot.
+  constexpr int ITB = (E * NF + kThreads - 1) / kThreads;
+  C fl[ITB][4];
+#pragma unroll
+  for (int j = 0; j < ITB; ++j) {
+    const int it = tid + j * kThreads;
+    if (it < E * NF) {
+      const int e = it % E, p = it / E, f = p / NpEdge, pp = p - f * NpEdge;
+      const size_t fi = size_t(f) * Kp + k0 + e;
+      const int code = a.g.code[fi];
+      if ((code & 15) == FK_Shared) {
+        C own[4], oth[4], F[4];
+        const unsigned slot = unsigned(a.g.nbr_slot[fi]);
+        const int base = int(slot & 0x7fffffffu);
+        const int stride = (slot >> 31) ? 1 : E;
+        const int rp = NpEdge - 1 - pp;
+        const int idx = base + rp * stride;
+#pragma unroll
+        for (int c = 0; c < 4; ++c) {
+          own[c] = sT[c * TS + p * E + e];
+          oth[c] = sT[c * TS + idx];
+        }
+        const C nx = C(a.g.Nx[fi]), ny = C(a.g.Ny[fi]);
+        const C w = C(a.g.IInII[fi]);
+        const bool owner = code & FC_FLUX_OWNER;
+        if (owner) numflux(FLUX, g, own, oth, nx, ny, F);
+        else numflux(FLUX, g, oth, own, nx, ny, F);
+        if constexpr (SHOCK) {
+          // Viscous edge flux at owner point jo (StoreEdgeViscousFlux,
+          // edges.go:151-244): average of the two sides' eps*grad Q along
+          // the owner normal, minus the penalty on the owner's trace jump.
+          const int kk = k0 + e, nb = a.sh.nbr_elem[fi], nf = a.sh.nbr_face[fi];
+          const int kL = owner ? kk : nb, kR = owner ? nb : kk;
+          const int rowL = NpInt + (owner ? f * NpEdge + pp : nf * NpEdge + rp);
+          const int rowR = NpInt + (owner ? nf * NpEdge + rp : f * NpEdge + pp);
+          const double lam = 0.5 * (a.sh.epsF[size_t(kL) * NR + rowL] + a.sh.epsF[size_t(kR) * NR + rowR]);
+          const double coef = (a.sh.omega * lam) * a.sh.vooL[fi];
+#pragma unroll
+          for (int c = 0; c < 4; ++c) {
+            // owner traces at owner points jo and NpEdge-1-jo
+            const double EQj = owner ? double(own[c]) : double(oth[c]);
+            const double EQr = owner ? double(sT[c * TS + (f * NpEdge + rp) * E + e])
+                                     : double(sT[c * TS + base + pp * stride]);
+            const double *dL = a.sh.diss + (size_t(kL) * 4 + c) * 2 * NR;
+            const double *dR = a.sh.diss + (size_t(kR) * 4 + c) * 2 * NR;
+            const double vFL = double(nx) * dL[rowL] + double(ny) * dL[NR + rowL];
+            const double vFR = double(nx) * dR[rowR] + double(ny) * dR[NR + rowR];
+            const double VF = 0.5 * (vFL + vFR) - coef * (EQj - EQr);
+            F[c] = C(double(F[c]) - VF);
+          }
+        }
+        if (owner) {
+#pragma unroll
+          for (int c = 0; c < 4; ++c) fl[j][c] = F[c] * w;
+        } else {
+#pragma unroll
+          for (int c = 0; c < 4; ++c) fl[j][c] = -F[c] * w;
+        }
+      }
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int j = 0; j < ITB; ++j) {
+    const int it = tid + j * kThreads;
+    if (it < E * NF) {
+      const int e = it % E, p = it / E, f = p / NpEdge;
+      if ((a.g.code[size_t(f) * Kp + k0 + e] & 15) == FK_Shared) {
+#pragma unroll
+        for (int c = 0; c < 4; ++c) sT[c * TS + p * E + e] = fl[j][c];
+      }
+    }
+  }
+#pragma unroll 1
+  for (int it = tid; it < E * NF; it += kThreads) {
+    const int e = it % E, p = it / E, f = p / NpEdge, pp = p - f * NpEdge;
+    const size_t fi = size_t(f) * Kp + k0 + e;
+    const int code = a.g.code[fi];
+    const int kind = code & 15;
+    if (kind == FK_Shared) continue;
+    C own[4];
+#pragma unroll
+    for (int c = 0; c < 4; ++c) own[c] = sT[c * TS + p * E + e];
+    const double nxd = a.g.Nx[fi], nyd = a.g.Ny[fi];
+    const C nx = C(nxd), ny = C(nyd);
+    C F[4] = {C(0), C(0), C(0), C(0)};
+    if (kind == FK_Far || kind == FK_IVortex) {
+      double qb[4];
+      boundary_state(kind, a.ph, a.g.bxy, code >> 8, NpEdge, pp, time, own, nxd, nyd, qb);
+      const C q[4] = {C(qb[0]), C(qb[1]), C(qb[2]), C(qb[3])};
+      C Fx[4], Fy[4];
+      flux_xy(g, q, Fx, Fy);
+#pragma unroll
+      for (int c = 0; c < 4; ++c) F[c] = nx * Fx[c] + ny * Fy[c];
+    } else if (kind == FK_Wall) {
+      const C pr = pressure(g, own[0], own[1], own[2], own[3]);
+      F[1] = nx * pr;
+      F[2] = ny * pr;
+    } else if (kind == FK_Open) {
+      C Fx[4], Fy[4];
+      flux_xy(g, own, Fx, Fy);
+#pragma unroll
+      for (int c = 0; c < 4; ++c) F[c] = nx * Fx[c] + ny * Fy[c];
+    }
+    if constexpr (SHOCK) {
+      // unshared edge: the own side's eps*grad Q along its normal
+      const int row = NpInt + f * NpEdge + pp;
+#pragma unroll
+      for (int c = 0; c < 4; ++c) {
+        const double *d = a.sh.diss + (size_t(k0 + e) * 4 + c) * 2 * NR;
+        F[c] = C(double(F[c]) - (nxd * d[row] + nyd * d[NR + row]));
+      }
+    }
+    const C w = C(a.g.IInII[fi]);
+#pragma unroll
+    for (int c = 0; c < 4; ++c) sT[c * TS + p * E + e] = F[c] * w;
+  }
+  __syncthreads();
+
+  // ---- C: interior flux + divergence, thread (element te, variable tn) ----
+  {
+    const int kk = k0 + te;
+    const double Jd = a.g.Jdet[kk];
+    const C Jdc = C(Jd);
+    const C J0 = C(a.g.Jinv[kk]), J1 = C(a.g.Jinv[size_t(Kp) + kk]);
+    const C J2 = C(a.g.Jinv[2 * size_t(Kp) + kk]), J3 = C(a.g.Jinv[3 * size_t(Kp) + kk]);
+    C acc[NpInt];
+#pragma unroll
+    for (int i = 0; i < NpInt; ++i) acc[i] = C(0);
+#pragma unroll 1
+    for (int j = 0; j < NpInt; ++j) {
+      const C r0 = sQ[(0 * NpInt + j) * P + te], r1 = sQ[(1 * NpInt + j) * P + te];
+      const C r2 = sQ[(2 * NpInt + j) * P + te], r3 = sQ[(3 * NpInt + j) * P + te];
+      C Fx, Fy;
+      flux_component(tn, g, r0, r1, r2, r3, Fx, Fy);
+      if constexpr (SHOCK) {
+        const double *d = a.sh.diss + (size_t(kk) * 4 + tn) * 2 * NR;
+        Fx -= d[j];
+        Fy -= d[NR + j];
+      }
+      const C Fr = Jdc * (J0 * Fx + J1 * Fy), Fs = Jdc * (J2 * Fx + J3 * Fy);
+#pragma unroll
+      for (int i = 0; i < NpInt; ++i) acc[i] += DvT[j * NpInt + i] * Fr + DvT[(NpInt + j) * NpInt + i] * Fs;
+    }
+#pragma unroll 2
+    for (int p = 0; p < NF; ++p) {
+      const C Fe = sT[tn * TS + p * E + te];
+#pragma unroll
+      for (int i = 0; i < NpInt; ++i) acc[i] += DvT[(2 * NpInt + p) * NpInt + i] * Fe;
+    }
+    const C moojd = -C(1. / Jd);
+#pragma unroll
+    for (int i = 0; i < NpInt; ++i) acc[i] *= moojd;
+    if constexpr (SHOCK) {
+      // modal filter of the RHS (limitAndFilterSolution dissipation.go:606-622)
+      const double *V = a.sh.ops + Sh::oV, *Vi = a.sh.ops + Sh::oVi, *mf = a.sh.ops + Sh::omf;
+      const double fac = 1. - sin(0.5 * M_PI * a.sh.sigma[kk]);
+      double m[NpInt];
+#pragma unroll
+      for (int i = 0; i < NpInt; ++i) {
+        double t = 0;
+#pragma unroll
+        for (int j = 0; j < NpInt; ++j) t += Vi[i * NpInt + j] * double(acc[j]);
+        m[i] = i == 0 ? t : t * (mf[i] * fac);
+      }
+#pragma unroll
+      for (int i = 0; i < NpInt; ++i) {
+        double t = 0;
+#pragma unroll
+        for (int j = 0; j < NpInt; ++j) t += V[i * NpInt + j] * m[j];
+        acc[i] = C(t);
+      }
+    }
+    __syncthreads();  // every thread has read its fluxes; sR may now overwrite sT
+#pragma unroll
+    for (int i = 0; i < NpInt; ++i) sR[(tn * NpInt + i) * P + te] = acc[i];
+  }
+  __syncthreads();
+
+  if (rhs_only) {
+    double *out = a.rhs_out + size_t(k0) * NV;
+    for (int idx = tid; idx < E * NV; idx += kThreads) {
+      const int e = idx / NV, r = idx - e * NV;
+      out[idx] = double(sR[r * P + e]);
+    }
+    return;
+  }
+
+  // ---- E: SSP54 stage combination (euler.go:511-565), FP64 arithmetic ----
+  const int stage = a.stage;
+  const bool lts = a.lts.on;
+  double dt = 0.0;  // GlobalDT; stays 0 with local time stepping (never computed)
+  if (!lts) {
+    dt = a.ph.cfl / from_bits(ctl->wmax[a.ring]);
+    if constexpr (SHOCK) {
+      const double dtv = a.sh.cdiff / from_bits(ctl->wmaxv[a.ring]);
+      dt = fmin(dt, dtv);
+    }
+    if (time + dt > a.ph.final_time) dt = a.ph.final_time - time;
+  } else {
+    // CalcElementMaxWaveSpeed (edges.go:291-323) + CalculateLocalDT: the
+    // element DT is CFL over the max of its previous value (-100 at stage 0)
+    // and the speeds of its three edges
+    if (tid < E) {
+      const int kk = k0 + tid;
+      double d = 0.0;
+      if (kk < a.K) {
+        const double *fw = a.lts.fw + size_t(a.par2) * 3 * Kp;
+        double w = stage == 0 ? -100.0 : a.lts.DTk[kk];
+#pragma unroll
+        for (int f = 0; f < 3; ++f) w = fmax(w, fw[a.lts.wsrc[size_t(f) * Kp + kk]]);
+        d = a.ph.cfl / w;
+        if constexpr (SHOCK) {
+          const double *fwv = a.lts.fwv + size_t(a.par2) * 3 * Kp;
+          double dv = a.lts.DTv[kk];
+#pragma unroll
+          for (int f = 0; f < 3; ++f) dv = fmax(dv, fwv[a.lts.wsrc[size_t(f) * Kp + kk]]);
+          if (dv > 1.e-9) {
+            dv = a.sh.cdiff / dv;
+            d = fmin(d, dv);
+          }
+          a.lts.DTv[kk] = dv;
+        }
+        a.lts.DTk[kk] = d;
+      }
+      sDT[tid] = d;
+    }
+    __syncthreads();
+  }
+  int bad = 0;
+  {
+    // The operands of this stage are loaded for all LPT values of the
+    // thread first, then combined and stored.
+    const size_t base = size_t(k0) * NV;
+    S *q0 = static_cast<S *>(a.q0) + base;
+    S *q1 = static_cast<S *>(a.q1) + base;
+    S *q2 = static_cast<S *>(a.q2) + base;
+    S *q3 = static_cast<S *>(a.q3) + base;
+    S *q4 = static_cast<S *>(a.q4) + base;
+    S *Rr = static_cast<S *>(a.R) + base;
+    const int nreal = min(E, a.K - k0);
+    if constexpr (LAST) {
+      // stage 4: R = -q0 + a q2 + b q3 + c q4 + d dt R3 + e dt RHS, q0 += R
+      S v0[LPT], v2[LPT], v3[LPT], vR[LPT];
+#pragma unroll
+      for (int j = 0; j < LPT; ++j) {
+        const int idx = tid + j * kThreads;
+        const bool ok = idx < E * NV;
+        v0[j] = ok ? q0[idx] : S(0);
+        v2[j] = ok ? q2[idx] : S(0);
+        v3[j] = ok ? q3[idx] : S(0);
+        vR[j] = ok ? Rr[idx] : S(0);
+      }
+#pragma unroll
+      for (int j = 0; j < LPT; ++j) {
+        const int idx = tid + j * kThreads;
+        if (idx >= E * NV) continue;
+        const int e = idx / NV, r = idx - e * NV;
+        const double dte = lts ? sDT[e] : dt;
+        const double dtR = dte * double(sR[r * P + e]);
+        const double qsv = double(sQ[r * P + e]);
+        const double q0v = double(v0[j]);
+        const double dtR3 = dte * double(vR[j]);
+        const double Rn = -q0v + 0.517231671970585 * double(v2[j]) + 0.096059710526146 * double(v3[j]) +
+                          0.386708617503269 * qsv + 0.063692468666290 * dtR3 + 0.226007483236906 * dtR;
+        Rr[idx] = S(Rn);
+        const S nv = S(q0v + Rn);
+        q0[idx] = nv;
+        sQ[r * P + e] = C(nv);
+        if (e < nreal && !(nv == nv)) bad = 1;
+      }
+    } else {
+      S v0[LPT];
+      if (stage >= 1) {
+#pragma unroll
+        for (int j = 0; j < LPT; ++j) {
+          const int idx = tid + j * kThreads;
+          v0[j] = (idx < E * NV) ? q0[idx] : S(0);
+        }
+      }
+#pragma unroll
+      for (int j = 0; j < LPT; ++j) {
+        const int idx = tid + j * kThreads;
+        if (idx >= E * NV) continue;
+        const int e = idx / NV, r = idx - e * NV;
+        const double rhs = double(sR[r * P + e]);
+        const double qsv = double(sQ[r * P + e]);
+        const double dtR = (lts ? sDT[e] : dt) * rhs;
+        S nv;
+        switch (stage) {
+          case 0:
+            nv = S(qsv + 0.391752226571890 * dtR);
+            q1[idx] = nv;
+            break;
+          case 1:
+            nv = S(0.444370493651235 * double(v0[j]) + 0.555629506348765 * qsv + 0.368410593050371 * dtR);
+            q2[idx] = nv;
+            break;
+          case 2:
+            nv = S(0.620101851488403 * double(v0[j]) + 0.379898148511597 * qsv + 0.251891774271694 * dtR);
+            q3[idx] = nv;
+            break;
+          default:
+            Rr[idx] = S(rhs);
+            nv = S(0.178079954393132 * double(v0[j]) + 0.821920045606868 * qsv + 0.544974750228521 * dtR);
+            q4[idx] = nv;
+            break;
+        }
+        sQ[r * P + e] = C(nv);
+        if (e < nreal && !(nv == nv)) bad = 1;
+      }
+    }
+  }
+  const double tnext = LAST ? time + dt : time;
+  double wl = 0.0;
+  if constexpr (!SHOCK) {
+    // With shock capturing the next stage state may be filtered before its
+    // traces are taken, so its wave speeds come from the gradient pass.
+    if (lts && tid < 3 * E) sWf[tid] = 0ull;
+    __syncthreads();
+
+    // ---- F: traces of the new state (own faces) ----
+    {
+      C q[NpInt];
+#pragma unroll
+      for (int i = 0; i < NpInt; ++i) q[i] = sQ[(tn * NpInt + i) * P + te];
+#pragma unroll 2
+      for (int p = 0; p < NF; ++p) {
+        C t = C(0);
+#pragma unroll
+        for (int i = 0; i < NpInt; ++i) t += Itp[p * NpInt + i] * q[i];
+        sT[tn * TS + p * E + te] = t;
+      }
+    }
+    __syncthreads();
+
+    // ---- G: owner-side wave speed of the next stage, max over the tile ----
+    {
+      const int nw = a.g.wave_cnt[tile] * NpEdge;
+      const int *wlist = a.g.wave_list + size_t(tile) * a.Wmax;
+#pragma unroll 1
+      for (int it = tid; it < nw; it += kThreads) {
+        const int w = it / NpEdge, pp = it - w * NpEdge;
+        const int packed = wlist[w];
+        const int e = packed & 255, f = packed >> 8, p = f * NpEdge + pp;
+        const int kk = k0 + e;
+        const size_t fi = size_t(f) * Kp + kk;
+        const int code = a.g.code[fi];
+        const int kind = code & 15;
+        C own[4];
+#pragma unroll
+        for (int c = 0; c < 4; ++c) own[c] = sT[c * TS + p * E + e];
+        double q[4] = {double(own[0]), double(own[1]), double(own[2]), double(own[3])};
+        if (kind == FK_Far || kind == FK_IVortex)
+          boundary_state(kind, a.ph, a.g.bxy, code >> 8, NpEdge, pp, tnext, own, a.g.Nx[fi], a.g.Ny[fi], q);
+        const double wsp = point_speed(a.ph.gamma, q, a.g.oohK[kk]);
+        if (!(wsp == wsp)) bad = 1;
+        wl = fmax(wl, wsp);
+        if (lts) atomicMax(&sWf[e * 3 + f], bits(wsp));
+      }
+    }
+  }
+  wl = wave_max(wl);
+  if ((tid & 63) == 0) red[tid >> 6] = wl;
+  bad = __syncthreads_or(bad);
+  if constexpr (!SHOCK) {
+    if (lts) {
+      double *fw = a.lts.fw + size_t(a.par2 ^ 1) * 3 * Kp;
+      const int *wlist = a.g.wave_list + size_t(tile) * a.Wmax;
+      for (int w = tid; w < a.g.wave_cnt[tile]; w += kThreads) {
+        const int packed = wlist[w];
+        const int e = packed & 255, f = packed >> 8;
+        fw[size_t(f) * Kp + k0 + e] = from_bits(sWf[e * 3 + f]);
+      }
+    }
+  }
+  if (tid == 0) {
+    if constexpr (!SHOCK) {
+      double m = red[0];
+      for (int w = 1; w < kThreads / 64; ++w) m = fmax(m, red[w]);
+      atomicMax(&ctl->wmax[(a.ring + 1) % 3], bits(m));
+    }
+    if (bad) atomicOr(&ctl->nan, 1);
+    if (blockIdx.x == 0) {
+      ctl->wmax[(a.ring + 2) % 3] = 0ull;
+      ctl->wmaxv[(a.ring + 2) % 3] = 0ull;
+      ctl->dt[stage] = dt;
+      if (LAST) {
+        ctl->time[par ^ 1] = tnext;
+        ctl->steps += 1;
+      }
+    }
+  }
+}
+
+// Wave speeds of the current state (start of a step sequence): one thread per
+// (element, face), traces straight from HBM; the per-face maximum feeds local
+// time stepping.
+template <int N, class S>
+__global__ void __launch_bounds__(256) k_prologue(StageArgs a) {
+  using Sh = Shape<N>;
+  constexpr int NpInt = Sh::NpInt, NpEdge = Sh::NpEdge, NV = Sh::NV;
+  __shared__ double red[4];
+  const int it = blockIdx.x * blockDim.x + threadIdx.x;
+  const int kk = it / 3, f = it - 3 * kk;
+  double wl = 0.0;
+  int bad = 0;
+  const double *Itp = a.Interp64;
+  if (kk < a.K) {
+    const size_t fi = size_t(f) * a.Kp + kk;
+    const int code = a.g.code[fi];
+    if (code & FC_WAVE) {
+      const S *q = static_cast<const S *>(a.qs) + size_t(kk) * NV;
+      for (int pp = 0; pp < NpEdge; ++pp) {
+        const int p = f * NpEdge + pp;
+        double tr[4];
+        for (int n = 0; n < 4; ++n) {
+          double t = 0;
+          for (int i = 0; i < NpInt; ++i) t += Itp[p * NpInt + i] * double(q[n * NpInt + i]);
+          tr[n] = t;
+        }
+        const int kind = code & 15;
+        if (kind == FK_Far || kind == FK_IVortex) {
+          const double own[4] = {tr[0], tr[1], tr[2], tr[3]};
+          boundary_state(kind, a.ph, a.g.bxy, code >> 8, NpEdge, pp, a.ctl->time[a.step_index & 1], own,
+                         a.g.Nx[fi], a.g.Ny[fi], tr);
+        }
+        const double w = point_speed(a.ph.gamma, tr, a.g.oohK[kk]);
+        if (!(w == w)) bad = 1;
+        wl = fmax(wl, w);
+      }
+      if (a.lts.on) a.lts.fw[size_t(a.par2) * 3 * a.Kp + fi] = wl;
+    }
+  }
+  wl = wave_max(wl);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = wl;
+  bad = __syncthreads_or(bad);
+  if (threadIdx.x == 0) {
+    double m = fmax(fmax(red[0], red[1]), fmax(red[2], red[3]));
+    atomicMax(&a.ctl->wmax[a.ring], bits(m));
+    if (bad) atomicOr(&a.ctl->nan, 1);
+  }
+}
+
+// ---- Shock capturing (FP64) ----
+
+// Pass 0: Persson sensor on the density (UpdateSeMoment,
+// DG2D/dfr_shock_capturing.go:142-166), sigma (UpdateShockFinderSigma,
+// dissipation.go:491-518, S0 = 4/(N+1)^4), element viscosity
+// (CalculateElementViscosity :397-412), and the max-merge of both to the
+// vertices (MergeElementScalarToVertices euler.go:1048-1065) by atomics on
+// the IEEE bits of the non-negative values. One thread per element.
+template <int N>
+__global__ void __launch_bounds__(256) k_shock_sensor(StageArgs a) {
+  using Sh = Shape<N>;
+  constexpr int NpInt = Sh::NpInt, NV = Sh::NV;
+  if (step_skipped(a)) return;
+  const int kk = blockIdx.x * blockDim.x + threadIdx.x;
+  if (kk >= a.K) return;
+  const double *rho = static_cast<const double *>(a.qs) + size_t(kk) * NV;
+  const double *Pm = a.sh.ops + Sh::oP, *Mm = a.sh.ops + Sh::oM, *Dm = a.sh.ops + Sh::oD;
+  double r[NpInt];
+#pragma unroll
+  for (int i = 0; i < NpInt; ++i) r[i] = rho[i];
+  double num = 0, den = 0;
+#pragma unroll
+  for (int i = 0; i < NpInt; ++i) {
+    double X = 0, Y = 0, DQ = 0;
+#pragma unroll
+    for (int j = 0; j < NpInt; ++j) {
+      X += Pm[i * NpInt + j] * r[j];
+      Y += Mm[i * NpInt + j] * r[j];
+      DQ += Dm[i * NpInt + j] * r[j];
+    }
+    num += DQ * X;
+    den += r[i] * Y;
+  }
+  const double se = log10(num / den);
+  const double kappa = a.sh.kappa, S0 = 4. / double((N + 1) * (N + 1) * (N + 1) * (N + 1));
+  const double left = S0 - kappa, right = S0 + kappa, ook = 0.5 / kappa;
+  double sg = 0.;
+  if (se < left) sg = 0.;
+  else if (se >= left && se <= right) sg = 0.5 * (1. + sin(M_PI * ook * (se - S0)));
+  else if (se > right) sg = 1.;
+  const double eps = a.sh.eps0 * a.sh.hK[kk] * sg;
+  a.sh.Se[kk] = se;
+  a.sh.eps[kk] = eps;
+  unsigned long long *sv = a.sh.sigV + size_t(a.sh.vbuf) * a.sh.Nv;
+  unsigned long long *ev = a.sh.epsV + size_t(a.sh.vbuf) * a.sh.Nv;
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    const int v = a.sh.evert[size_t(c) * a.Kp + kk];
+    atomicMax(&sv[v], bits(sg));
+    atomicMax(&ev[v], bits(eps));
+  }
+}
+
+// Pass 1: sigma = mean of the vertex maxima (MergeVertexScalarToElement),
+// eps at the RT points by barycentric interpolation of the vertex maxima
+// (InterpolateEpsilonSigma dissipation.go:219-242), the modal filter of the
+// stage state at rkStep 2 (euler.go:603-608), and clearing of the other
+// vertex buffer for the next stage. One thread per element (padded range).
+template <int N>
+__global__ void __launch_bounds__(256) k_shock_prep(StageArgs a) {
+  using Sh = Shape<N>;
+  constexpr int NpInt = Sh::NpInt, NV = Sh::NV, NR = Sh::NR;
+  if (step_skipped(a)) return;
+  const int gid = blockIdx.x * blockDim.x + threadIdx.x;
+  {
+    unsigned long long *so = a.sh.sigV + size_t(a.sh.vbuf ^ 1) * a.sh.Nv;
+    unsigned long long *eo = a.sh.epsV + size_t(a.sh.vbuf ^ 1) * a.sh.Nv;
+    for (int v = gid; v < a.sh.Nv; v += gridDim.x * blockDim.x) {
+      so[v] = 0ull;
+      eo[v] = 0ull;
+    }
+  }
+  const int kk = gid;
+  if (kk >= a.Kp) return;
+  double *epsF = a.sh.epsF + size_t(kk) * NR;
+  if (kk >= a.K) {
+    a.sh.sigma[kk] = 0.;
+    for (int r = 0; r < NR; ++r) epsF[r] = 0.;
+    return;
+  }
+  const unsigned long long *sv = a.sh.sigV + size_t(a.sh.vbuf) * a.sh.Nv;
+  const unsigned long long *ev = a.sh.epsV + size_t(a.sh.vbuf) * a.sh.Nv;
+  const int v0 = a.sh.evert[kk], v1 = a.sh.evert[size_t(a.Kp) + kk], v2 = a.sh.evert[2 * size_t(a.Kp) + kk];
+  const double sg = ((from_bits(sv[v0]) + from_bits(sv[v1])) + from_bits(sv[v2])) / 3.;
+  a.sh.sigma[kk] = sg;
+  const double e0 = from_bits(ev[v0]), e1 = from_bits(ev[v1]), e2 = from_bits(ev[v2]);
+  const double *B = a.sh.ops + Sh::oBary;
+#pragma unroll
+  for (int r = 0; r < NR; ++r) epsF[r] = B[3 * r] * e0 + B[3 * r + 1] * e1 + B[3 * r + 2] * e2;
+  if (a.sh.filter_state) {
+    const double *V = a.sh.ops + Sh::oV, *Vi = a.sh.ops + Sh::oVi, *mf = a.sh.ops + Sh::omf;
+    const double fac = 1. - sin(0.5 * M_PI * sg);
+    double *q = static_cast<double *>(a.qs) + size_t(kk) * NV;
+    for (int n = 0; n < 4; ++n) {
+      double u[NpInt], m[NpInt];
+#pragma unroll
+      for (int i = 0; i < NpInt; ++i) u[i] = q[n * NpInt + i];
+#pragma unroll
+      for (int i = 0; i < NpInt; ++i) {
+        double t = 0;
+#pragma unroll
+        for (int j = 0; j < NpInt; ++j) t += Vi[i * NpInt + j] * u[j];
+        m[i] = i == 0 ? t : t * (mf[i] * fac);
+      }
+#pragma unroll
+      for (int i = 0; i < NpInt; ++i) {
+        double t = 0;
+#pragma unroll
+        for (int j = 0; j < NpInt; ++j) t += V[i * NpInt + j] * m[j];
+        q[n * NpInt + i] = t;
+      }
+    }
+  }
+}
+
+// Pass 2: eps * grad Q by the RT element (CalculateEpsilonGradient
+// dissipation.go:244-272 via GetSolutionGradientUsingRTElement
+// euler.go:864-918) at the interior and edge RT points, from interior DOFs
+// Jinv*Q and edge DOFs n*IInII/Jdet times the owner's trace (read reversed
+// by the non-owner); then the stage's owner-side wave speeds and viscous
+// speeds eps/hK^2 (StoreEdgeAggregates edges.go:246-289). Workgroup = tile of
+// 64 elements x 4 variables.
+template <int N>
+__global__ void __launch_bounds__(kThreads) k_shock_grad(StageArgs a) {
+  using Sh = Shape<N>;
+  constexpr int NpInt = Sh::NpInt, NpEdge = Sh::NpEdge, NpFlux = Sh::NpFlux, NF = Sh::NF, NV = Sh::NV,
+                NR = Sh::NR, E = kTile;
+  __shared__ double sTr[4][NF][E];
+  __shared__ double red[2][kThreads / 64];
+  if (step_skipped(a)) return;
+  const int tile = blockIdx.x, tid = threadIdx.x;
+  const int te = tid % E, tn = tid / E, k0 = tile * E, kk = k0 + te, Kp = a.Kp;
+  const double *Q = static_cast<const double *>(a.qs);
+  const double *Itp = ops_base<double>() + NpFlux * NpInt;
+  const double *Div = a.sh.ops + Sh::oDiv;
+  double u[NpInt];
+#pragma unroll
+  for (int i = 0; i < NpInt; ++i) u[i] = Q[size_t(kk) * NV + tn * NpInt + i];
+  // edge values: the owner's trace at this face's points, in this face's order
+  double ue[NF];
+#pragma unroll
+  for (int f = 0; f < 3; ++f) {
+    const size_t fi = size_t(f) * Kp + kk;
+    const int code = a.g.code[fi];
+    // unshared faces (boundary, padding) are always their own edge's owner
+    const bool owner = (code & FC_FLUX_OWNER) || (code & 15) != FK_Shared;
+    const int nb = a.sh.nbr_elem[fi], nf = a.sh.nbr_face[fi];
+#pragma unroll
+    for (int i = 0; i < NpEdge; ++i) {
+      double t = 0;
+#pragma unroll
+      for (int j = 0; j < NpInt; ++j) t += Itp[(f * NpEdge + i) * NpInt + j] * u[j];
+      sTr[tn][f * NpEdge + i][te] = t;
+      ue[f * NpEdge + i] = t;
+    }
+    if (!owner) {
+      const double *qn = Q + size_t(nb) * NV + tn * NpInt;
+#pragma unroll
+      for (int i = 0; i < NpEdge; ++i) {
+        const int p = nf * NpEdge + (NpEdge - 1 - i);
+        double t = 0;
+#pragma unroll
+        for (int j = 0; j < NpInt; ++j) t += Itp[p * NpInt + j] * qn[j];
+        ue[f * NpEdge + i] = t;
+      }
+    }
+  }
+  const double Jd = a.g.Jdet[kk], ooJd = 1. / Jd;
+  const double Ji[4] = {a.g.Jinv[kk], a.g.Jinv[size_t(Kp) + kk], a.g.Jinv[2 * size_t(Kp) + kk],
+                        a.g.Jinv[3 * size_t(Kp) + kk]};
+  const double *epsF = a.sh.epsF + size_t(kk) * NR;
+  double *dout = a.sh.diss + (size_t(kk) * 4 + tn) * 2 * NR;
+#pragma unroll 1
+  for (int dir = 0; dir < 2; ++dir) {
+    double dof[NpFlux];
+#pragma unroll
+    for (int i = 0; i < NpInt; ++i) {
+      dof[i] = Ji[dir] * u[i];
+      dof[NpInt + i] = Ji[2 + dir] * u[i];
+    }
+#pragma unroll
+    for (int f = 0; f < 3; ++f) {
+      const size_t fi = size_t(f) * Kp + kk;
+      const double nrm = dir == 0 ? a.sh.Nxo[fi] : a.sh.Nyo[fi];
+      const double w = a.g.IInII[fi];
+#pragma unroll
+      for (int i = 0; i < NpEdge; ++i) dof[2 * NpInt + f * NpEdge + i] = ooJd * nrm * w * ue[f * NpEdge + i];
+    }
+#pragma unroll 3
+    for (int r = 0; r < NR; ++r) {
+      double acc = 0;
+#pragma unroll
+      for (int j = 0; j < NpFlux; ++j) acc += Div[r * NpFlux + j] * dof[j];
+      dout[dir * NR + r] = acc * epsF[r];
+    }
+  }
+  if (a.stage < 0) return;
+  __syncthreads();
+  // wave speeds of the owned faces: thread (element, face)
+  double wl = 0.0, wv = 0.0;
+  int bad = 0;
+  const double time = a.ctl->time[a.step_index & 1];
+  if (tid < 3 * E) {
+    const int e = tid / 3, f = tid - 3 * e, k = k0 + e;
+    const size_t fi = size_t(f) * Kp + k;
+    const int code = a.g.code[fi];
+    if (k < a.K && (code & FC_WAVE)) {
+      const int kind = code & 15;
+      const double oohK = a.g.oohK[k];
+      const double *ef = a.sh.epsF + size_t(k) * NR + NpInt + f * NpEdge;
+      double fwl = 0.0, fwv = 0.0;
+      for (int pp = 0; pp < NpEdge; ++pp) {
+        double q[4] = {sTr[0][f * NpEdge + pp][e], sTr[1][f * NpEdge + pp][e], sTr[2][f * NpEdge + pp][e],
+                       sTr[3][f * NpEdge + pp][e]};
+        if (kind == FK_Far || kind == FK_IVortex) {
+          const double own[4] = {q[0], q[1], q[2], q[3]};
+          boundary_state(kind, a.ph, a.g.bxy, code >> 8, NpEdge, pp, time, own, a.g.Nx[fi], a.g.Ny[fi], q);
+        }
+        const double s = point_speed(a.ph.gamma, q, oohK);
+        if (!(s == s)) bad = 1;
+        fwl = fmax(fwl, s);
+        fwv = fmax(fwv, oohK * oohK * ef[pp]);
+      }
+      if (a.lts.on) {
+        a.lts.fw[size_t(a.par2) * 3 * Kp + fi] = fwl;
+        a.lts.fwv[size_t(a.par2) * 3 * Kp + fi] = fwv;
+      }
+      wl = fwl;
+      wv = fwv;
+    }
+  }
+  wl = wave_max(wl);
+  wv = wave_max(wv);
+  if ((tid & 63) == 0) {
+    red[0][tid >> 6] = wl;
+    red[1][tid >> 6] = wv;
+  }
+  bad = __syncthreads_or(bad);
+  if (tid == 0) {
+    double m = red[0][0], mv = red[1][0];
+    for (int w = 1; w < kThreads / 64; ++w) {
+      m = fmax(m, red[0][w]);
+      mv = fmax(mv, red[1][w]);
+    }
+    atomicMax(&a.ctl->wmax[a.ring], bits(m));
+    atomicMax(&a.ctl->wmaxv[a.ring], bits(mv));
+    if (bad) atomicOr(&a.ctl->nan, 1);
+  }
+}
+
+// ---- launchers ----
+
+template <int N>
+size_t lds_bytes_n(size_t csize, int Hmax) {
+  using Sh = Shape<N>;
+  return csize * (size_t(Sh::NV) * (kTile + 1) + 4 * (size_t(Sh::NF) * kTile + size_t(Sh::NpEdge) * Hmax));
+}
+
+template <int N, class S, class C, int FLUX, bool LAST, bool SHOCK>
+hipError_t launch_stage_t(const StageArgs &a, hipStream_t s) {
+  const size_t lds = lds_bytes_n<N>(sizeof(C), a.Hmax);
+  static size_t attr_bytes = 0;  // dynamic LDS the kernel is currently allowed
+  if (lds > 64 * 1024 && lds > attr_bytes) {
+    hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void *>(&k_stage<N, S, C, FLUX, LAST, SHOCK>),
+                                       hipFuncAttributeMaxDynamicSharedMemorySize, int(lds));
+    if (e != hipSuccess) return e;
+    attr_bytes = lds;
+  }
+  const int q = (a.ntiles + 7) / 8;
+  hipLaunchKernelGGL((k_stage<N, S, C, FLUX, LAST, SHOCK>), dim3(8 * q), dim3(kThreads), lds, s, a);
+  return hipGetLastError();
+}
+
+template <int N, class S, class C, bool SHOCK>
+hipError_t launch_stage_f(const StageArgs &a, hipStream_t s) {
+  const bool last = a.stage == 4;
+  switch (a.ph.flux) {
+    case 0: return last ? launch_stage_t<N, S, C, 0, true, SHOCK>(a, s) : launch_stage_t<N, S, C, 0, false, SHOCK>(a, s);
+    case 1: return last ? launch_stage_t<N, S, C, 1, true, SHOCK>(a, s) : launch_stage_t<N, S, C, 1, false, SHOCK>(a, s);
+    case 2: return last ? launch_stage_t<N, S, C, 2, true, SHOCK>(a, s) : launch_stage_t<N, S, C, 2, false, SHOCK>(a, s);
+    case 3: return last ? launch_stage_t<N, S, C, 3, true, SHOCK>(a, s) : launch_stage_t<N, S, C, 3, false, SHOCK>(a, s);
+  }
+  return hipErrorInvalidValue;
+}
+
+template <int N>
+hipError_t launch_stage_n(int prec, int shock, const StageArgs &a, hipStream_t s) {
+  if (shock) {
+    if (prec != 0) return hipErrorInvalidValue;
+    return launch_stage_f<N, double, double, true>(a, s);
+  }
+  return prec == 0 ? launch_stage_f<N, double, double, false>(a, s) : launch_stage_f<N, float, float, false>(a, s);
+}
+
+template <int N>
+hipError_t launch_prologue_n(int prec, const StageArgs &a, hipStream_t s) {
+  const long items = long(a.K) * 3;
+  const int blocks = int((items + 255) / 256);
+  if (prec == 0) hipLaunchKernelGGL((k_prologue<N, double>), dim3(blocks), dim3(256), 0, s, a);
+  else hipLaunchKernelGGL((k_prologue<N, float>), dim3(blocks), dim3(256), 0, s, a);
+  return hipGetLastError();
+}
+
+template <int N>
+hipError_t launch_shock_n(int pass, const StageArgs &a, hipStream_t s) {
+  switch (pass) {
+    case 0:
+      hipLaunchKernelGGL((k_shock_sensor<N>), dim3((a.K + 255) / 256), dim3(256), 0, s, a);
+      break;
+    case 1:
+      hipLaunchKernelGGL((k_shock_prep<N>), dim3((a.Kp + 255) / 256), dim3(256), 0, s, a);
+      break;
+    case 2:
+      hipLaunchKernelGGL((k_shock_grad<N>), dim3(a.ntiles), dim3(kThreads), 0, s, a);
+      break;
+    default:
+      return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
+template <int N>
+hipError_t upload_operators_n(const double *divT, const double *interp) {
+  using Sh = Shape<N>;
+  std::vector<double> d(static_cast<size_t>(Sh::ops));
+  std::copy(divT, divT + size_t(Sh::NpFlux) * Sh::NpInt, d.begin());
+  std::copy(interp, interp + size_t(Sh::NF) * Sh::NpInt, d.begin() + size_t(Sh::NpFlux) * Sh::NpInt);
+  std::vector<float> f(d.begin(), d.end());
+  hipError_t e = hipMemcpyToSymbol(HIP_SYMBOL(cOpsD), d.data(), d.size() * sizeof(double), 0,
+                                   hipMemcpyHostToDevice);
+  if (e != hipSuccess) return e;
+  return hipMemcpyToSymbol(HIP_SYMBOL(cOpsF), f.data(), f.size() * sizeof(float), 0, hipMemcpyHostToDevice);
+}
+
+}  // namespace dev
+}  // namespace gocfd
+
+// Explicit instantiation of the per-order entry points of one order.
+#define GOCFD_INSTANTIATE_ORDER(NN)                                                                      \
+  namespace gocfd {                                                                                      \
+  namespace dev {                                                                                        \
+  template hipError_t launch_stage_n<NN>(int, int, const StageArgs &, hipStream_t);                      \
+  template hipError_t launch_prologue_n<NN>(int, const StageArgs &, hipStream_t);                        \
+  template hipError_t launch_shock_n<NN>(int, const StageArgs &, hipStream_t);                           \
+  template hipError_t upload_operators_n<NN>(const double *, const double *);                           \
+  }                                                                                                      \
+  }

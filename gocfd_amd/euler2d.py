@@ -207,6 +207,18 @@ class Euler:
     def set_time(self, t: float) -> None:
         _lib.check(_lib.load().gocfd_euler2d_set_time(self._h, t))
 
+    def shock(self):
+        """Se, sigma, eps per element of the last stage (PerssonC0 only)."""
+        se, sg, ep = np.empty(self.K), np.empty(self.K), np.empty(self.K)
+        _lib.check(_lib.load().gocfd_euler2d_get_shock(self._h, _dp(se), _dp(sg), _dp(ep)))
+        return se, sg, ep
+
+    def element_dt(self):
+        """Element time steps of the last stage (LocalTimeStepping only)."""
+        d = np.empty(self.K)
+        _lib.check(_lib.load().gocfd_euler2d_get_element_dt(self._h, _dp(d)))
+        return d
+
     def kernel_time(self):
         ms, n = C.c_double(), C.c_int32()
         _lib.check(_lib.load().gocfd_euler2d_kernel_time(self._h, C.byref(ms), C.byref(n)))

@@ -57,8 +57,8 @@ typedef struct gocfd_params {
   double cfl;                   /* CFL */
   double final_time;            /* FinalTime */
   double minf, gamma, alpha;    /* Minf, Gamma, Alpha (degrees) */
-  int32_t local_time_stepping;  /* LocalTimeStepping (must be 0 on the GPU path) */
-  int32_t limiter;              /* 0 none, 1 PerssonC0 (Limiter) */
+  int32_t local_time_stepping;  /* LocalTimeStepping */
+  int32_t limiter;              /* 0 none, 1 PerssonC0 (Limiter; FP64 precision only) */
   double kappa;                 /* Kappa */
   int32_t precision;            /* GOCFD_PREC_* */
   int32_t device;               /* HIP device ordinal; -1 = host-only handle (setup and
@@ -141,6 +141,17 @@ int gocfd_euler2d_get_geometry(gocfd_euler2d *h, double *Jdet, double *Jinv, dou
  * (DFR2D.SolutionX/Y, FluxX/Y, dfr_startup.go:154-159) */
 int gocfd_euler2d_get_coords(gocfd_euler2d *h, double *SolX, double *SolY, double *FluxX,
                              double *FluxY);
+
+/* ---- Shock capturing (Limiter: PerssonC0) and local time stepping ---- */
+/* Per element, of the last stage evaluated: the sensor Se (UpdateSeMoment,
+ * DG2D/dfr_shock_capturing.go:142-166), sigma after the vertex averaging
+ * (sd.SigmaScalar, model_problems/Euler2D/euler.go:594-599) and the element
+ * viscosity before it (sd.EpsilonScalar, dissipation.go:397-412). [K] each;
+ * any pointer may be NULL. */
+int gocfd_euler2d_get_shock(gocfd_euler2d *h, double *Se, double *Sigma, double *Eps);
+/* Element time steps of the last stage with LocalTimeStepping (rk.DT after
+ * CalculateLocalDT, euler.go:973-1002) [K] */
+int gocfd_euler2d_get_element_dt(gocfd_euler2d *h, double *DT);
 
 /* ---- Device-level figures for measurement (no reference counterpart) ---- */
 /* Average duration in ms of the last timed batch of stage kernels, measured

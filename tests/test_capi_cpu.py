@@ -123,17 +123,29 @@ def test_host_handle_rejects_device_calls(vortex_meshes):
         h.step(1)
 
 
-@pytest.mark.parametrize("kw,msg", [
-    (dict(PolynomialOrder=5), "polynomial order"),
-    (dict(LocalTimeStepping=True), "LocalTimeStepping"),
-    (dict(InitType="shocktube"), "init type"),
+@pytest.mark.parametrize("kw,msg,prec", [
+    (dict(PolynomialOrder=5), "polynomial order", "fp64"),
+    (dict(InitType="shocktube"), "init type", "fp64"),
+    (dict(Limiter="PerssonC0"), "FP64 only", "fp32"),
 ])
-def test_parameter_errors(vortex_meshes, kw, msg):
+def test_parameter_errors(vortex_meshes, kw, msg, prec):
     base = dict(CFL=1.0, FluxType="Lax", InitType="ivortex", PolynomialOrder=2, FinalTime=1.0)
     base.update(kw)
     with pytest.raises(GocfdError) as ei:
-        Euler(InputParameters2D(**base), vortex_meshes[4], device=-1)
+        Euler(InputParameters2D(**base), vortex_meshes[4], precision=prec, device=-1)
     assert ei.value.code == _lib.ERR_ARG and msg.lower() in str(ei.value).lower()
+
+
+def test_shock_and_lts_handles_build_on_host(vortex_meshes):
+    """PerssonC0 and LocalTimeStepping are accepted (FP64); host-only handles
+    refuse the device-only introspection."""
+    ip = InputParameters2D(CFL=1.0, FluxType="Roe", InitType="ivortex", PolynomialOrder=2, FinalTime=1.0,
+                           Limiter="PerssonC0", Kappa=3.0, LocalTimeStepping=True)
+    h = Euler(ip, vortex_meshes[4], device=-1)
+    with pytest.raises(GocfdError):
+        h.shock()
+    with pytest.raises(GocfdError):
+        h.element_dt()
 
 
 def test_unknown_names_raise_like_the_reference(vortex_meshes):

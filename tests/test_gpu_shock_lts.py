@@ -1,0 +1,151 @@
+"""GPU parity of the shock-capturing path (Limiter: PerssonC0) and of local
+time stepping, through the C ABI, against the CPU oracle.
+
+Bars (FP64): RHS within 1e-10 of its inf-norm (the modal filter of the RHS
+mixes every term of an element, so the bound is taken on the whole RHS);
+sensor sigma/eps within 1e-10; density after the steps within 1e-10
+relative L2; time and dt within 1e-12 (1e-10 with an active shock sensor).
+"""
+import os
+
+import numpy as np
+import pytest
+
+import oracle
+from gocfd_amd import Euler, InputParameters2D
+
+pytestmark = pytest.mark.gpu
+
+G = os.path.join(os.path.dirname(__file__), "golden")
+
+
+def _pair(mesh, N, flux="Roe", init="freestream", **kw):
+    ip = InputParameters2D(CFL=kw.get("CFL", 0.5), FluxType=flux, InitType=init, PolynomialOrder=N,
+                           FinalTime=kw.get("FinalTime", 1e9), Minf=kw.get("Minf", 0.5),
+                           Alpha=kw.get("Alpha", 0.0), MaxIterations=100,
+                           Limiter=kw.get("Limiter", "PerssonC0"), Kappa=kw.get("Kappa", 3.0),
+                           LocalTimeStepping=kw.get("LTS", False))
+    gpu = Euler(ip, mesh)
+    ora = oracle.OracleSolver(mesh, N, flux=flux.lower(), init=init, CFL=ip.CFL, FinalTime=ip.FinalTime,
+                              Minf=ip.Minf, Alpha=ip.Alpha, limiter=ip.Limiter, Kappa=ip.Kappa,
+                              LocalTimeStepping=ip.LocalTimeStepping)
+    return gpu, ora
+
+
+def _noisy(q, seed, amp=0.05):
+    rng = np.random.default_rng(seed)
+    return q * (1.0 + amp * rng.standard_normal(q.shape))
+
+
+def _sod(coords, lo):
+    """Density/pressure jump at x = 5 at rest (a Sod-type problem)."""
+    X = coords["SolX"]
+    rho = np.where(X < 5, 1.0, lo)
+    p = np.where(X < 5, 1.0, lo)
+    return np.stack([rho, 0 * rho, 0 * rho, p / 0.4])
+
+
+def _rel(a, b):
+    return np.abs(a - b).max() / max(np.abs(b).max(), 1e-300)
+
+
+@pytest.mark.parametrize("N", [1, 2, 3, 4])
+@pytest.mark.parametrize("flux", ["Lax", "Roe"])
+def test_shock_rhs_parity(vortex_meshes, N, flux):
+    gpu, ora = _pair(vortex_meshes[20], N, flux)
+    q = _noisy(ora.state(), 20 + N)
+    r_gpu, r_ora = gpu.rhs(q, 0.0), ora.rhs(q, 0.0)
+    se_g, sg_g, ep_g = gpu.shock()
+    se_o, sg_o, ep_o = ora.shock()
+    assert (sg_o > 0).any() and (ep_o > 0).any()
+    assert np.abs(sg_g - sg_o).max() < 1e-10
+    assert _rel(ep_g, ep_o) < 1e-10
+    assert np.abs(se_g - se_o).max() < 1e-10 * np.abs(se_o).max()
+    assert _rel(r_gpu, r_ora) < 1e-10, _rel(r_gpu, r_ora)
+
+
+@pytest.mark.parametrize("mesh,N,init,Minf", [
+    ("fstepA001.neu", 2, "freestream", 2.0),   # walls, in/out/far, supersonic
+    ("fstepA001.neu", 1, "freestream", 0.6),
+    ("test_tris_6.neu", 2, "ivortex", 0.5),    # walls -> IVortex BC
+])
+def test_shock_rhs_parity_reference_meshes(mesh, N, init, Minf):
+    gpu, ora = _pair(os.path.join(G, mesh), N, "Roe", init, Minf=Minf)
+    q = _noisy(ora.state(), 7)
+    r_gpu, r_ora = gpu.rhs(q, 0.1), ora.rhs(q, 0.1)
+    assert _rel(r_gpu, r_ora) < 1e-10, _rel(r_gpu, r_ora)
+
+
+@pytest.mark.parametrize("N,kappa,cfl,lo,nsteps", [
+    (1, 3.0, 0.5, 0.125, 30),
+    (2, 8.0, 0.5, 0.125, 30),
+    (3, 10.5, 0.3, 0.5, 15),
+])
+def test_shock_step_parity(vortex_meshes, N, kappa, cfl, lo, nsteps):
+    """A Sod-type jump on the 20x20 vortex mesh with PerssonC0: sensor,
+    vertex merge, viscous fluxes, the stage-2 state filter and the viscous
+    dt limit all active over the steps."""
+    gpu, ora = _pair(vortex_meshes[20], N, "Roe", Kappa=kappa, CFL=cfl)
+    q = _sod(ora.coords(), lo)
+    gpu.Q = q
+    ora.set_state(q)
+    gpu.step(nsteps)
+    for _ in range(nsteps):
+        ora.step()
+    t, dt, steps = gpu.time_info()
+    assert steps == nsteps
+    # dt = min(CFL/max speed, C_diff/max eps/hK^2): eps passes through
+    # log10, sin and the vertex max of a developing shock, which amplifies
+    # last-bit differences over the steps; the bar is 1e-10 here
+    assert abs(t - ora.time) <= 1e-10 * ora.time
+    assert abs(dt - ora.dt) <= 1e-10 * ora.dt
+    se_o, sg_o, ep_o = ora.shock()
+    assert (sg_o > 0).sum() > 50
+    qg, qo = gpu.Q, ora.state()
+    l2 = np.linalg.norm(qg[0] - qo[0]) / np.linalg.norm(qo[0])
+    assert l2 < 1e-10, l2
+    _, sg_g, _ = gpu.shock()
+    assert np.abs(sg_g - sg_o).max() < 1e-8
+
+
+@pytest.mark.parametrize("flux,N", [("Lax", 2), ("Roe", 4)])
+def test_lts_step_parity(vortex_meshes, flux, N):
+    """LocalTimeStepping: per-element DT = CFL / max edge speed; Time stays 0."""
+    gpu, ora = _pair(vortex_meshes[20], N, flux, "ivortex", Limiter="", LTS=True, CFL=0.8)
+    nsteps = 20
+    gpu.step(nsteps)
+    for _ in range(nsteps):
+        ora.step()
+    t, dt, steps = gpu.time_info()
+    assert steps == nsteps and t == 0.0 and ora.time == 0.0
+    assert _rel(gpu.element_dt(), ora.element_dt()) < 1e-12
+    qg, qo = gpu.Q, ora.state()
+    l2 = np.linalg.norm(qg[0] - qo[0]) / np.linalg.norm(qo[0])
+    assert l2 < 1e-10, l2
+
+
+def test_lts_shock_step_parity(vortex_meshes):
+    """LocalTimeStepping with PerssonC0: the viscous element DT enters."""
+    gpu, ora = _pair(vortex_meshes[20], 1, "Roe", LTS=True, Kappa=3.0, CFL=0.5)
+    q = _sod(ora.coords(), 0.125)
+    gpu.Q = q
+    ora.set_state(q)
+    for _ in range(20):
+        ora.step()
+    gpu.step(20)
+    assert _rel(gpu.element_dt(), ora.element_dt()) < 1e-11
+    qg, qo = gpu.Q, ora.state()
+    l2 = np.linalg.norm(qg[0] - qo[0]) / np.linalg.norm(qo[0])
+    assert l2 < 1e-10, l2
+
+
+def test_lts_fp32(vortex_meshes):
+    ip = InputParameters2D(CFL=0.8, FluxType="Roe", InitType="ivortex", PolynomialOrder=3, FinalTime=1e9,
+                           LocalTimeStepping=True)
+    gpu = Euler(ip, vortex_meshes[20], precision="fp32")
+    ora = oracle.OracleSolver(vortex_meshes[20], 3, flux="roe", init="ivortex", CFL=0.8, LocalTimeStepping=True)
+    gpu.step(20)
+    for _ in range(20):
+        ora.step()
+    l2 = np.linalg.norm(gpu.Q[0] - ora.state()[0]) / np.linalg.norm(ora.state()[0])
+    assert l2 < 1e-5, l2
