@@ -11,7 +11,11 @@ step); the FP64 drop-in is measured in the same run and reported beside it,
 with the density L2/Linf error of the FP32 run against the FP64 run and of
 both against the analytic convected vortex.
 
-python bench.py [--gpus 1] [--steps 500] [--warmup 10] [--no-cpu]
+The same run measures the shock-capturing path (PerssonC0) on a Mach 2
+wedge-channel mesh of about 250k triangles at P=2 (FP64, 2000 steps) and
+reports it under "shock_path".
+
+python bench.py [--gpus 1] [--steps 500] [--warmup 10] [--no-cpu] [--no-shock]
 """
 from __future__ import annotations
 
@@ -67,6 +71,43 @@ def run_gpu(mesh, precision, steps, warmup):
                 tiling=e.tiling())
 
 
+def shock_alg_bytes_per_elem_stage(np_int: int, np_edge: int) -> float:
+    """Algorithmic HBM bytes per element per stage of the FP64 shock path:
+    the plain stage (4 NpInt 3.8 x 8 + 144) plus the sensor (density in,
+    Se/eps out, 3 vertex maxima x 2), the element fields (3 vertex pairs in,
+    eps at the NR used RT points and sigma out), the gradient pass (state,
+    eps, 144 B geometry in; eps*grad at 2 x 4 x NR points out) and the stage
+    kernel's reads of eps and eps*grad; NR = NpInt + 3 NpEdge."""
+    nr = np_int + 3 * np_edge
+    nv = 4 * np_int
+    plain = nv * 3.8 * 8 + 144.0
+    sensor = np_int * 8 + 16 + 48
+    prep = 48 + nr * 8 + 8
+    grad = nv * 8 + nr * 8 + 144 + 8 * nr * 8
+    stage_extra = nr * 8 + 8 * nr * 8 + 8
+    return plain + sensor + prep + grad + stage_extra
+
+
+def run_gpu_shock(mesh, steps, warmup):
+    """The shock-capturing path: Mach 2 wedge channel, P=2, Roe, PerssonC0
+    (Kappa 6), FP64, uniform freestream start."""
+    from gocfd_amd import Euler, InputParameters2D
+    ip = InputParameters2D(Title="wedge", CFL=1.0, FluxType="Roe", InitType="Freestream",
+                           PolynomialOrder=2, Minf=2.0, FinalTime=1e9, MaxIterations=warmup + steps,
+                           Limiter="PerssonC0", Kappa=6.0)
+    e = Euler(ip, mesh, precision="fp64")
+    e.step(warmup)
+    e.sync()
+    t0 = time.perf_counter()
+    e.step(steps)
+    elapsed = time.perf_counter() - t0
+    stage_ms, launches = e.kernel_time()
+    t, dt, nst = e.time_info()
+    se, sigma, eps = e.shock()
+    return dict(elapsed=elapsed, stage_ms=stage_ms, time=t, dt=dt, K=e.K, NpInt=e.NpInt,
+                NpEdge=e.NpEdge, flagged=int((sigma > 0).sum()), sigma_max=float(sigma.max()))
+
+
 def cpu_baseline(mesh, target_seconds=15.0):
     """The CPU oracle (plain C, FP64, OpenMP over the host cores) on the same mesh and order:
     whole SSP54 steps until about `target_seconds` of CPU work."""
@@ -103,8 +144,12 @@ def main():
     ap.add_argument("--n", type=int, default=200, help="quads per side (2n^2 triangles)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
-    ap.add_argument("--only", choices=["fp32", "fp64"], default=None,
-                    help="run one precision only (for profiler runs)")
+    ap.add_argument("--only", choices=["fp32", "fp64", "shock"], default=None,
+                    help="run one leg only (for profiler runs)")
+    ap.add_argument("--shock-points", type=int, default=125_000,
+                    help="interior points of the wedge mesh (about twice as many triangles)")
+    ap.add_argument("--shock-steps", type=int, default=2000)
+    ap.add_argument("--no-shock", action="store_true", help="skip the shock-capturing leg")
     ap.add_argument("--copy-probe", type=float, default=0.0, metavar="MB",
                     help="run a plain device copy of MB megabytes (counter calibration) and exit")
     args = ap.parse_args()
@@ -122,6 +167,15 @@ def main():
     tmpdir = tempfile.mkdtemp(prefix="gocfd_bench_")
     mesh = meshgen.vortex_mesh(os.path.join(tmpdir, f"vortex{args.n}.su2"), args.n, seed=1)
 
+    wedge = None
+    if args.only == "shock" or (args.only is None and not args.no_shock):
+        wedge = meshgen.wedge_mesh(os.path.join(tmpdir, "wedge.su2"), args.shock_points, seed=1)
+    if args.only == "shock":
+        r = run_gpu_shock(wedge, args.shock_steps, args.warmup)
+        v = r["K"] * r["NpInt"] * 5 * args.shock_steps / r["elapsed"]
+        print(json.dumps({"leg": "shock", "value": v, "stage_ms": r["stage_ms"],
+                          "ms_per_step": r["elapsed"] / args.shock_steps * 1e3, "K": r["K"]}))
+        return
     if args.only:
         r = run_gpu(mesh, args.only, args.steps, args.warmup)
         v = r["K"] * r["NpInt"] * 5 * args.steps / r["elapsed"]
@@ -154,6 +208,20 @@ def main():
     if os.path.exists(tfile):
         with open(tfile) as f:
             traffic = json.load(f).get("bytes_per_launch")
+
+    shock = None
+    if wedge is not None:
+        rs = run_gpu_shock(wedge, args.shock_steps, args.warmup)
+        bs = shock_alg_bytes_per_elem_stage(rs["NpInt"], rs["NpEdge"]) * rs["K"]
+        shock = {"workload": f"Mach 2 wedge channel 3x1, 15 deg ramp, {rs['K']} triangles (Delaunay of a "
+                             f"jittered lattice), P=2, Roe, PerssonC0 Kappa 6, FP64, {args.shock_steps} steps",
+                 "value": rs["K"] * rs["NpInt"] * 5 * args.shock_steps / rs["elapsed"], "unit": "DOF-stage/s",
+                 "ms_per_step": rs["elapsed"] / args.shock_steps * 1e3,
+                 "stage_ms": rs["stage_ms"], "alg_bytes_per_stage": bs,
+                 "achieved_GBs": bs / (rs["stage_ms"] * 1e-3) / 1e9,
+                 "frac_hbm_peak": bs / (rs["stage_ms"] * 1e-3) / 1e9 / HBM_PEAK_GBS,
+                 "time": rs["time"], "dt": rs["dt"], "sensor_flagged_elements": rs["flagged"],
+                 "sigma_max": rs["sigma_max"]}
 
     cpu = None
     if not args.no_cpu:
@@ -190,6 +258,7 @@ def main():
                      "kernel": "k_stage<4,float,float>", "avg_launch_ms": r32["stage_ms"],
                      "alg_bytes_per_launch": b32},
         "cpu_baseline": cpu,
+        "shock_path": shock,
     }
     print(json.dumps(line))
 

@@ -8,8 +8,12 @@
 namespace gocfd {
 namespace dev {
 
-constexpr int kTile = 64;      // elements per workgroup tile (one wavefront wide)
-constexpr int kThreads = 256;  // threads per workgroup
+#ifndef GOCFD_TILE
+#define GOCFD_TILE 64
+#endif
+constexpr int kTile = GOCFD_TILE;  // elements per workgroup tile
+constexpr int kThreads = 4 * kTile; // threads per workgroup: one per (element, variable)
+static_assert(kThreads % 64 == 0, "whole wavefronts per workgroup");
 
 // Device control block. Time is double-buffered by step parity and the wave
 // speed maxima live in rings of three slots indexed by the global stage

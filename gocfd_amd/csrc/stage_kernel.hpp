@@ -117,8 +117,24 @@ __device__ __forceinline__ bool step_skipped(const StageArgs &a) {
   return a.stage >= 0 && a.step_index > 0 && a.ctl->time[a.step_index & 1] >= a.ph.final_time;
 }
 
+// Waves per SIMD the register allocator targets. FP64 tiles need
+// 80 KB of LDS at N=4 (two workgroups per CU), so FP64 gains nothing above
+// 2; the FP32 stages 0-3 fit 128 VGPRs without spilling, which lets four
+// 40 KB workgroups share a CU.
+#ifndef GOCFD_WAVES_F32
+#define GOCFD_WAVES_F32 4
+#endif
+#ifndef GOCFD_WAVES_F32_LAST
+#define GOCFD_WAVES_F32_LAST 3
+#endif
+template <class S, bool LAST>
+struct StageWaves {
+  static constexpr int value = sizeof(S) == 4 ? (LAST ? GOCFD_WAVES_F32_LAST : GOCFD_WAVES_F32) : 1;
+};
+
 template <int N, class S, class C, int FLUX, bool LAST, bool SHOCK>
-__global__ void __launch_bounds__(kThreads) k_stage(StageArgs a) {
+__global__ void __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(StageWaves<S, LAST>::value)))
+k_stage(StageArgs a) {
   using Sh = Shape<N>;
   constexpr int NpInt = Sh::NpInt, NpEdge = Sh::NpEdge, NF = Sh::NF, NV = Sh::NV, NR = Sh::NR;
   constexpr int E = kTile, P = kTile + 1;

@@ -1,4 +1,4 @@
-"""Deterministic synthetic meshes for the isentropic-vortex workloads.
+"""Deterministic synthetic meshes for the benchmark and test workloads.
 
 jittered_quads(): an nx x ny grid of quads on [x0,x1] x [y0,y1]; interior
 nodes are moved by up to +-jitter*h in each coordinate with a SplitMix64
@@ -105,4 +105,64 @@ def vortex_mesh(path: str, n: int, seed: int = 1, jitter: float = 0.1) -> str:
     """The isentropic-vortex mesh on [0,10]x[-5,5]: n x n jittered quads -> 2n^2 triangles."""
     VX, VY, EToV, tags = jittered_quads(n, n, seed=seed, jitter=jitter)
     write_su2(path, VX, VY, EToV, tags)
+    return path
+
+
+def wedge_mesh(path: str, n_points: int = 125_000, seed: int = 1, x_ramp: float = 0.5,
+               angle_deg: float = 15.0, jitter: float = 0.2) -> str:
+    """Mach-2 wedge channel: [0,3] x [0,1] with a ramp rising at angle_deg from
+    x = x_ramp to the outlet. Delaunay triangulation (scipy) of about n_points
+    interior points -- a hexagonal lattice of spacing h, each point moved by
+    up to +-jitter*h per coordinate with a SplitMix64 stream from `seed` --
+    plus boundary points at spacing h; interior points closer than h/2 to the
+    boundary are dropped. (Purely uniform random points give sliver
+    triangles whose CFL limit stalls the time step.) Tags: "wall" on the
+    floor, ramp and roof, "far" on the inlet and outlet (the reference's
+    In/Out need FSIn/FSOut, which only the shock-tube case sets). About
+    2 n_points triangles."""
+    from scipy.spatial import Delaunay
+    t = np.tan(np.radians(angle_deg))
+    L, H = 3.0, 1.0
+    area = L * H - 0.5 * (L - x_ramp) ** 2 * t
+    h = np.sqrt(2.0 * area / (np.sqrt(3.0) * n_points))
+    corners = [(0.0, 0.0), (x_ramp, 0.0), (L, (L - x_ramp) * t), (L, H), (0.0, H)]
+    bpts, segs = [], []
+    for a, b in zip(corners, corners[1:] + corners[:1]):
+        a, b = np.array(a), np.array(b)
+        m = max(1, int(np.ceil(np.linalg.norm(b - a) / h)))
+        for i in range(m):
+            bpts.append(a + (b - a) * (i / m))
+        segs.append(m)
+    bpts = np.array(bpts)
+    dy = h * np.sqrt(3.0) / 2
+    ny, nx = int(H / dy) + 2, int(L / h) + 2
+    jj, ii = np.meshgrid(np.arange(ny), np.arange(nx), indexing="ij")
+    x = (ii + 0.5 * (jj % 2)).ravel() * h
+    y = jj.ravel() * dy
+    u = splitmix64(seed, 2 * x.size)
+    x = x + (2 * u[:x.size] - 1) * jitter * h
+    y = y + (2 * u[x.size:] - 1) * jitter * h
+    c, sn = np.cos(np.radians(angle_deg)), np.sin(np.radians(angle_deg))
+    d_ramp = np.where(x >= x_ramp, (y - (x - x_ramp) * t) * c, np.inf)
+    d_floor = np.where(x < x_ramp, y, np.hypot(x - x_ramp, y))
+    dist = np.minimum.reduce([d_floor, d_ramp, H - y, x, L - x])
+    inside = (y > np.where(x < x_ramp, 0.0, (x - x_ramp) * t)) & (y < H) & (x > 0) & (x < L)
+    pts = np.stack([x, y], axis=1)[inside & (dist > 0.5 * h)]
+    P = np.vstack([bpts, pts])
+    tri = Delaunay(P).simplices.astype(np.int64)
+    a, b, cc = P[tri[:, 0]], P[tri[:, 1]], P[tri[:, 2]]
+    cross = (b[:, 0] - a[:, 0]) * (cc[:, 1] - a[:, 1]) - (b[:, 1] - a[:, 1]) * (cc[:, 0] - a[:, 0])
+    tri = tri[np.abs(cross) > 1e-12 * h * h]
+    cross = cross[np.abs(cross) > 1e-12 * h * h]
+    tri[cross < 0] = tri[cross < 0][:, [0, 2, 1]]
+    # boundary edges, walked along the boundary point list
+    nb = len(bpts)
+    start = np.cumsum([0] + segs)
+    walls, far = [], []
+    for s_i, (s0, m) in enumerate(zip(start[:-1], segs)):
+        edges = [(s0 + i, (s0 + i + 1) % nb) for i in range(m)]
+        # segments: floor, ramp, outlet, roof, inlet
+        (far if s_i in (2, 4) else walls).extend(edges)
+    tags = {"wall": walls, "far": far}
+    write_su2(path, P[:, 0], P[:, 1], tri.astype(np.int32), tags)
     return path

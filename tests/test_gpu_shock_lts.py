@@ -106,6 +106,7 @@ def test_shock_step_parity(vortex_meshes, N, kappa, cfl, lo, nsteps):
     assert l2 < 1e-10, l2
     _, sg_g, _ = gpu.shock()
     assert np.abs(sg_g - sg_o).max() < 1e-8
+    assert np.array_equal(sg_g > 0, sg_o > 0)  # shock-cell flags identical
 
 
 @pytest.mark.parametrize("flux,N", [("Lax", 2), ("Roe", 4)])
@@ -149,3 +150,29 @@ def test_lts_fp32(vortex_meshes):
         ora.step()
     l2 = np.linalg.norm(gpu.Q[0] - ora.state()[0]) / np.linalg.norm(ora.state()[0])
     assert l2 < 1e-5, l2
+
+
+def test_wedge_shock_path_parity(tmp_path):
+    """The Mach 2 wedge-channel workload at a small size (2k lattice points,
+    about 4k triangles): walls on floor, ramp and roof, far field at inlet and
+    outlet, P=2, Roe, PerssonC0 with Kappa 10 (wide enough a ramp that the
+    sensor flags elements under the P=2 modal filter); 40 steps from a
+    perturbed freestream."""
+    from gocfd_amd import meshgen
+    mesh = meshgen.wedge_mesh(str(tmp_path / "wedge2k.su2"), 2000)
+    gpu, ora = _pair(mesh, 2, "Roe", Minf=2.0, Kappa=10.0, CFL=1.0)
+    q = _noisy(ora.state(), 11, amp=0.02)
+    gpu.Q = q
+    ora.set_state(q)
+    gpu.step(40)
+    for _ in range(40):
+        ora.step()
+    t, dt, steps = gpu.time_info()
+    assert abs(t - ora.time) <= 1e-10 * ora.time
+    qg, qo = gpu.Q, ora.state()
+    l2 = np.linalg.norm(qg[0] - qo[0]) / np.linalg.norm(qo[0])
+    assert l2 < 1e-10, l2
+    _, sg_g, _ = gpu.shock()
+    _, sg_o, _ = ora.shock()
+    assert (sg_o > 0).sum() > 100
+    assert np.array_equal(sg_g > 0, sg_o > 0)
