@@ -7,7 +7,9 @@ variables), Roe flux, SSP54, CFL 1. One "step" is one SSP54 step (5 stage
 launches) over the whole mesh with the state resident in HBM.
 
 `value` is the FP32-bulk step's DOF-stage/s (DOF = K*NpInt; 5 stages per
-step); the FP64 drop-in is measured in the same run and reported beside it,
+step; FP32 stage states and flux/divergence arithmetic, FP64 accumulated
+state q0 and residual R); the FP64 drop-in is measured in the same run and
+reported beside it,
 with the density L2/Linf error of the FP32 run against the FP64 run and of
 both against the analytic convected vortex.
 
@@ -37,10 +39,14 @@ METRIC = "DOF-stage/s at P=4 (FP32 vs FP64 GPU, L2 err vs FP64 oracle); % HBM pe
 
 
 def alg_bytes_per_elem_stage(np_int: int, state_bytes: int) -> float:
-    """Algorithmic HBM bytes per element per SSP54 stage: 3.8 state-array
-    accesses of 4*NpInt values per stage on average (19 per step) plus 144 B
-    of per-element geometry, connectivity and face codes."""
-    return 4 * np_int * 3.8 * state_bytes + 144.0
+    """Algorithmic HBM bytes per element per SSP54 stage: 19 state-array
+    accesses of 4*NpInt values per step -- 6 of q0 and 3 of R, which are FP64
+    in every precision, and 10 of the stage states q1..q4 (state_bytes each)
+    -- divided over the 5 stages, plus 144 B of per-element geometry,
+    connectivity and face codes. FP64: 30.4 B per value and stage; FP32
+    stage states: 22.4 B."""
+    per_value = (6 * 8 + 3 * 8 + 10 * state_bytes) / 5.0
+    return 4 * np_int * per_value + 144.0
 
 
 def analytic_density(t, X, Y):
@@ -241,7 +247,8 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32 (FP64 geometry, dt reduction and SSP sums)",
+        "dtype": "f32 (stage states q1..q4 and bulk flux/divergence arithmetic; FP64 geometry, dt "
+                 "reduction, SSP sums and accumulated state q0/R)",
         "data": "synthetic: deterministic jittered-quad isentropic-vortex mesh",
         "config": {"workload": f"isentropic vortex, {K} triangles ({args.n}x{args.n} jittered quads), "
                                f"P=4, Roe, SSP54, CFL 1", "K": K, "NpInt": NpInt,

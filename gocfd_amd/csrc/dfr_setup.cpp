@@ -839,6 +839,28 @@ DeviceLayout build_device_layout(const Discretisation &d, int E) {
       L.face_code[fi] = code;
     }
   }
+  // Shared faces with Riemann work, per tile
+  {
+    std::vector<std::vector<int>> fl(L.ntiles);
+    for (int kk = 0; kk < K; ++kk)
+      for (int f = 0; f < 3; ++f) {
+        const size_t fi = size_t(f) * Kp + kk;
+        const int code = L.face_code[fi];
+        if ((code & 15) != FK_Shared) continue;
+        const bool in_tile = !(uint32_t(L.nbr_slot[fi]) >> 31);
+        const bool owner = code & FC_FLUX_OWNER;
+        if (in_tile && !owner) continue;  // the in-tile owner fills this slot
+        fl[kk / E].push_back((kk % E) | (f << 8) | ((in_tile && owner) ? 1 << 10 : 0));
+      }
+    L.Fmax = 1;
+    for (auto &v : fl) L.Fmax = std::max<int>(L.Fmax, int(v.size()));
+    L.flux_list.assign(size_t(L.ntiles) * L.Fmax, 0);
+    L.flux_cnt.assign(L.ntiles, 0);
+    for (int t = 0; t < L.ntiles; ++t) {
+      L.flux_cnt[t] = int(fl[t].size());
+      std::copy(fl[t].begin(), fl[t].end(), L.flux_list.begin() + size_t(t) * L.Fmax);
+    }
+  }
   // Faces whose owner-side trace enters the CFL wave-speed maximum, per tile
   std::vector<std::vector<int>> waves(L.ntiles);
   for (int kk = 0; kk < K; ++kk)

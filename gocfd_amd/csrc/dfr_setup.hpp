@@ -104,6 +104,12 @@ struct DeviceLayout {
   int Wmax = 0;                                         // owned (wave-speed) faces per tile, max
   std::vector<int> wave_list;                           // [ntiles x Wmax]: e | face << 8
   std::vector<int> wave_cnt;                            // [ntiles]
+  // Shared faces whose Riemann flux this tile evaluates: every face with an
+  // out-of-tile neighbour, and the owner side of in-tile pairs (which also
+  // fills the partner's slot). [ntiles x Fmax]: e | face << 8 | partner << 10
+  int Fmax = 0;
+  std::vector<int> flux_list;
+  std::vector<int> flux_cnt;                            // [ntiles]
   std::vector<double> bpoint_xy;                        // IVortex boundary points [nb][NpEdge][2]
   int nbfaces = 0;
   // shock capturing and local time stepping, per face [3][Kp] / element [Kp]

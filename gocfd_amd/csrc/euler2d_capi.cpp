@@ -98,7 +98,7 @@ struct gocfd_euler2d {
   FreeStream fs;
   int device = 0;
   int prec = 0;
-  size_t ssize = 8;  // bytes per stored state value
+  size_t ssize = 8;  // bytes per stored stage-state value (q1..q4); q0 and R are FP64
   hipStream_t stream = nullptr;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   // state (element-major, Kp * NV values of the storage type)
@@ -107,7 +107,7 @@ struct gocfd_euler2d {
   void *scratch_in = nullptr;
   DevBuf<double> rhs_out;
   DevBuf<double> Jdet, Jinv, IInII, Nx, Ny, oohK, bxy, Interp64;
-  DevBuf<int> code, nbr_slot, halo_elem, halo_face, wave_list, wave_cnt;
+  DevBuf<int> code, nbr_slot, halo_elem, halo_face, wave_list, wave_cnt, flux_list, flux_cnt;
   // local time stepping
   bool lts = false;
   DevBuf<double> fw, fwv, DTk, DTv;
@@ -139,13 +139,14 @@ struct gocfd_euler2d {
 
   dev::StageArgs args() const {
     dev::StageArgs a{};
-    a.K = L.K; a.Kp = L.Kp; a.ntiles = L.ntiles; a.Hmax = L.Hmax; a.Wmax = L.Wmax;
+    a.K = L.K; a.Kp = L.Kp; a.ntiles = L.ntiles; a.Hmax = L.Hmax; a.Wmax = L.Wmax; a.Fmax = L.Fmax;
     a.q0 = q[0]; a.q1 = q[1]; a.q2 = q[2]; a.q3 = q[3]; a.q4 = q[4]; a.R = R;
     a.ctl = ctl;
     a.g.Jdet = Jdet.p; a.g.Jinv = Jinv.p; a.g.IInII = IInII.p; a.g.Nx = Nx.p; a.g.Ny = Ny.p;
     a.g.oohK = oohK.p; a.g.code = code.p; a.g.nbr_slot = nbr_slot.p;
     a.g.halo_elem = halo_elem.p; a.g.halo_face = halo_face.p; a.g.bxy = bxy.p;
     a.g.wave_list = wave_list.p; a.g.wave_cnt = wave_cnt.p;
+    a.g.flux_list = flux_list.p; a.g.flux_cnt = flux_cnt.p;
     a.Interp64 = Interp64.p;
     a.lts.on = lts ? 1 : 0;
     a.lts.fw = fw.p; a.lts.fwv = fwv.p; a.lts.wsrc = wsrc.p; a.lts.DTk = DTk.p; a.lts.DTv = DTv.p;
@@ -166,19 +167,14 @@ struct gocfd_euler2d {
     return a;
   }
 
-  // reference layout [4][NpInt][K] (double) <-> element-major device buffer
+  // reference layout [4][NpInt][K] (double) <-> element-major FP64 device
+  // buffer (q0 and the RHS-evaluation input)
   void to_device(const double *Q, void *dst) {
     const int K = L.K, NpInt = d.ops.NpInt;
     const size_t tot = size_t(L.Kp) * NV;
-    if (prec == 0) {
-      std::vector<double> h(tot);
-      fill_host(Q, h.data(), K, NpInt);
-      HIPCHECK(hipMemcpy(dst, h.data(), tot * sizeof(double), hipMemcpyHostToDevice));
-    } else {
-      std::vector<float> h(tot);
-      fill_host(Q, h.data(), K, NpInt);
-      HIPCHECK(hipMemcpy(dst, h.data(), tot * sizeof(float), hipMemcpyHostToDevice));
-    }
+    std::vector<double> h(tot);
+    fill_host(Q, h.data(), K, NpInt);
+    HIPCHECK(hipMemcpy(dst, h.data(), tot * sizeof(double), hipMemcpyHostToDevice));
   }
   template <class T>
   void fill_host(const double *Q, T *h, int K, int NpInt) {
@@ -417,11 +413,14 @@ int gocfd_euler2d_create(const gocfd_mesh *mesh, const gocfd_params *p, gocfd_eu
   HIPCHECK(hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking));
   HIPCHECK(hipEventCreate(&h->ev0));
   HIPCHECK(hipEventCreate(&h->ev1));
-  const size_t tot = size_t(h->L.Kp) * h->NV * h->ssize;
-  for (auto &qq : h->q) HIPCHECK(hipMalloc(&qq, tot));
-  HIPCHECK(hipMalloc(&h->R, tot));
-  HIPCHECK(hipMemset(h->R, 0, tot));
-  for (int s = 1; s < 5; ++s) HIPCHECK(hipMemset(h->q[s], 0, tot));
+  const size_t nval = size_t(h->L.Kp) * h->NV;
+  HIPCHECK(hipMalloc(&h->q[0], nval * sizeof(double)));
+  for (int s = 1; s < 5; ++s) {
+    HIPCHECK(hipMalloc(&h->q[s], nval * h->ssize));
+    HIPCHECK(hipMemset(h->q[s], 0, nval * h->ssize));
+  }
+  HIPCHECK(hipMalloc(&h->R, nval * sizeof(double)));
+  HIPCHECK(hipMemset(h->R, 0, nval * sizeof(double)));
   HIPCHECK(hipMalloc(&h->ctl, sizeof(dev::Ctl)));
   auto &L = h->L;
   h->Jdet.upload(L.Jdet); h->Jinv.upload(L.Jinv); h->IInII.upload(L.IInII);
@@ -430,6 +429,7 @@ int gocfd_euler2d_create(const gocfd_mesh *mesh, const gocfd_params *p, gocfd_eu
   h->code.upload(L.face_code); h->nbr_slot.upload(L.nbr_slot);
   h->halo_elem.upload(L.halo_elem); h->halo_face.upload(L.halo_face);
   h->wave_list.upload(L.wave_list); h->wave_cnt.upload(L.wave_cnt);
+  h->flux_list.upload(L.flux_list); h->flux_cnt.upload(L.flux_cnt);
   if (h->lts) {
     h->fw.alloc(size_t(2) * 3 * L.Kp);
     h->fwv.alloc(size_t(2) * 3 * L.Kp);
@@ -496,7 +496,7 @@ int gocfd_euler2d_get_state(gocfd_euler2d *h, double *Q) {
   NEED_DEVICE(h);
   HIPCHECK(hipSetDevice(h->device));
   HIPCHECK(hipStreamSynchronize(h->stream));
-  h->from_device(h->q[0], Q, h->prec == 0);
+  h->from_device(h->q[0], Q, true);
   return GOCFD_OK;
   GUARD_END
 }
@@ -507,7 +507,7 @@ int gocfd_euler2d_get_residual(gocfd_euler2d *h, double *R) {
   NEED_DEVICE(h);
   HIPCHECK(hipSetDevice(h->device));
   HIPCHECK(hipStreamSynchronize(h->stream));
-  h->from_device(h->R, R, h->prec == 0);
+  h->from_device(h->R, R, true);
   return GOCFD_OK;
   GUARD_END
 }
@@ -518,7 +518,7 @@ int gocfd_euler2d_rhs(gocfd_euler2d *h, const double *Q, double t, double *RHS) 
   NEED_DEVICE(h);
   HIPCHECK(hipSetDevice(h->device));
   const size_t tot = size_t(h->L.Kp) * h->NV;
-  if (!h->scratch_in) HIPCHECK(hipMalloc(&h->scratch_in, tot * h->ssize));
+  if (!h->scratch_in) HIPCHECK(hipMalloc(&h->scratch_in, tot * sizeof(double)));
   if (h->rhs_out.n < tot) h->rhs_out.alloc(tot);
   h->to_device(Q, h->scratch_in);
   dev::StageArgs a = h->args();

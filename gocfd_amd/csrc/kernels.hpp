@@ -34,6 +34,7 @@ struct Geom {
   const int *code, *nbr_slot;                          // [3][Kp]
   const int *halo_elem, *halo_face;                    // [ntiles][Hmax]
   const int *wave_list, *wave_cnt;                     // [ntiles][Wmax], [ntiles]
+  const int *flux_list, *flux_cnt;                     // [ntiles][Fmax], [ntiles]
   const double *bxy;                                   // IVortex boundary points
 };
 
@@ -65,8 +66,8 @@ struct ShockArgs {
   unsigned long long *sigV, *epsV;   // [2][Nv] vertex maxima (IEEE bits)
   int Nv, vbuf;                      // vertex buffer used by this stage
   double *Se, *sigma, *eps;          // [Kp]
-  double *epsF;                      // [Kp][NR] viscosity at the used RT points
-  double *diss;                      // [Kp][4][2][NR] eps * grad at the used RT points
+  double *epsF;                      // [NR][Kp] viscosity at the used RT points
+  double *diss;                      // [4][2][NR][Kp] eps * grad at the used RT points
   const int *nbr_elem, *nbr_face;    // [3][Kp]
   const double *Nxo, *Nyo;           // [3][Kp] own outward normals
   const double *vooL;                // [3][Kp] 1 / owner edge length (shared faces)
@@ -75,7 +76,7 @@ struct ShockArgs {
 };
 
 struct StageArgs {
-  int K, Kp, ntiles, Hmax, Wmax;
+  int K, Kp, ntiles, Hmax, Wmax, Fmax;
   int stage;           // 0..4, or -1 for the RHS-only evaluation
   int step_index;      // steps since the state was set (selects time parity)
   int ring;            // global stage counter mod 3
@@ -92,8 +93,10 @@ struct StageArgs {
   ShockArgs sh;
 };
 
-// Precision variants: 0 = FP64 storage and arithmetic; 1 = FP32 storage and
-// bulk arithmetic with FP64 geometry, dt reduction and SSP combination.
+// Precision variants: 0 = FP64 storage and arithmetic; 1 = FP32 storage of
+// the stage states q1..q4 and FP32 bulk arithmetic (fluxes, divergence),
+// with FP64 geometry, dt reduction, SSP combination and FP64 storage of the
+// accumulated state q0 and the stage-3 RHS R.
 // shock != 0 selects the PerssonC0 stage kernel (FP64 only).
 hipError_t launch_stage(int N, int prec, int shock, const StageArgs &a, hipStream_t s);
 hipError_t launch_prologue(int N, int prec, const StageArgs &a, hipStream_t s);

@@ -117,23 +117,29 @@ __device__ __forceinline__ bool step_skipped(const StageArgs &a) {
   return a.stage >= 0 && a.step_index > 0 && a.ctl->time[a.step_index & 1] >= a.ph.final_time;
 }
 
-// Waves per SIMD the register allocator targets. FP64 tiles need
-// 80 KB of LDS at N=4 (two workgroups per CU), so FP64 gains nothing above
-// 2; the FP32 stages 0-3 fit 128 VGPRs without spilling, which lets four
-// 40 KB workgroups share a CU.
+// Waves per SIMD the register allocator targets. FP64 tiles need 80 KB of
+// LDS at N=4 (two workgroups per CU), so FP64 gains nothing above 2; the FP32
+// stages 0-3 fit 128 VGPRs without spilling.
 #ifndef GOCFD_WAVES_F32
 #define GOCFD_WAVES_F32 4
 #endif
 #ifndef GOCFD_WAVES_F32_LAST
 #define GOCFD_WAVES_F32_LAST 3
 #endif
-template <class S, bool LAST>
+#ifndef GOCFD_WAVES_F64
+#define GOCFD_WAVES_F64 1
+#endif
+template <class S, bool LAST, bool SHOCK>
 struct StageWaves {
-  static constexpr int value = sizeof(S) == 4 ? (LAST ? GOCFD_WAVES_F32_LAST : GOCFD_WAVES_F32) : 1;
+  static constexpr int value =
+      SHOCK ? 1 : sizeof(S) == 4 ? (LAST ? GOCFD_WAVES_F32_LAST : GOCFD_WAVES_F32) : GOCFD_WAVES_F64;
 };
 
-template <int N, class S, class C, int FLUX, bool LAST, bool SHOCK>
-__global__ void __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(StageWaves<S, LAST>::value)))
+// S: storage of the stage states q1..q4; SIN: storage of this stage's
+// input (q0 at stage 0 and for the RHS evaluation, else S). q0 and the
+// stage-3 RHS R, which accumulate the step, are always FP64.
+template <int N, class S, class SIN, class C, int FLUX, bool LAST, bool SHOCK>
+__global__ void __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(StageWaves<S, LAST, SHOCK>::value)))
 k_stage(StageArgs a) {
   using Sh = Shape<N>;
   constexpr int NpInt = Sh::NpInt, NpEdge = Sh::NpEdge, NF = Sh::NF, NV = Sh::NV, NR = Sh::NR;
@@ -161,7 +167,7 @@ k_stage(StageArgs a) {
   }
   const int k0 = tile * E;
   const int Kp = a.Kp;
-  const S *qs = static_cast<const S *>(a.qs);
+  const SIN *qs = static_cast<const SIN *>(a.qs);
   const C *DvT = ops_base<C>();                         // DivInt^T [NpFlux][NpInt]
   const C *Itp = DvT + Sh::NpFlux * NpInt;              // FluxEdgeInterp [NF][NpInt]
   const C g = C(a.ph.gamma);
@@ -171,8 +177,8 @@ k_stage(StageArgs a) {
   // loads are in flight together.
   constexpr int LPT = (E * NV + kThreads - 1) / kThreads;
   {
-    const S *qt = qs + size_t(k0) * NV;
-    S v[LPT];
+    const SIN *qt = qs + size_t(k0) * NV;
+    SIN v[LPT];
 #pragma unroll
     for (int j = 0; j < LPT; ++j) {
       const int idx = tid + j * kThreads;
@@ -206,7 +212,7 @@ k_stage(StageArgs a) {
     const int kh = a.g.halo_elem[size_t(tile) * a.Hmax + h];
     if (kh < 0) continue;
     const int f = a.g.halo_face[size_t(tile) * a.Hmax + h];
-    const S *qn = qs + size_t(kh) * NV + n * NpInt;
+    const SIN *qn = qs + size_t(kh) * NV + n * NpInt;
     C q[NpInt];
 #pragma unroll
     for (int i = 0; i < NpInt; ++i) q[i] = C(qn[i]);
@@ -222,78 +228,94 @@ k_stage(StageArgs a) {
   __syncthreads();
 
   // ---- B: numerical flux at every face point ----
-  // Shared faces first (unrolled, results held in registers until every
-  // neighbour trace has been read), then boundary faces in a rolled loop: a
-  // boundary face point reads and writes only its own LDS slot.
+  // Shared faces from the tile's work list: a face with an out-of-tile
+  // neighbour evaluates its own side; of an in-tile pair only the owner
+  // evaluates F* and also fills the partner's slot (negated, at the reversed
+  // point, scaled by the partner's ||n||), which is the value the reference
+  // stores once per edge and reads back from both sides (edges.go:454-483).
+  // Results stay in registers until every trace has been read. Boundary
+  // faces follow in a rolled loop: a boundary face point reads and writes
+  // only its own LDS slot.
   constexpr int ITB = (E * NF + kThreads - 1) / kThreads;
   C fl[ITB][4];
+  const int nfl = a.g.flux_cnt[tile] * NpEdge;
+  const int *flist = a.g.flux_list + size_t(tile) * a.Fmax;
 #pragma unroll
   for (int j = 0; j < ITB; ++j) {
     const int it = tid + j * kThreads;
-    if (it < E * NF) {
-      const int e = it % E, p = it / E, f = p / NpEdge, pp = p - f * NpEdge;
+    if (it < nfl) {
+      const int w = it / NpEdge, pp = it - w * NpEdge;
+      const int packed = flist[w];
+      const int e = packed & 255, f = (packed >> 8) & 3, p = f * NpEdge + pp;
       const size_t fi = size_t(f) * Kp + k0 + e;
       const int code = a.g.code[fi];
-      if ((code & 15) == FK_Shared) {
-        C own[4], oth[4], F[4];
-        const unsigned slot = unsigned(a.g.nbr_slot[fi]);
-        const int base = int(slot & 0x7fffffffu);
-        const int stride = (slot >> 31) ? 1 : E;
-        const int rp = NpEdge - 1 - pp;
-        const int idx = base + rp * stride;
+      C own[4], oth[4], F[4];
+      const unsigned slot = unsigned(a.g.nbr_slot[fi]);
+      const int base = int(slot & 0x7fffffffu);
+      const int stride = (slot >> 31) ? 1 : E;
+      const int rp = NpEdge - 1 - pp;
+      const int idx = base + rp * stride;
+#pragma unroll
+      for (int c = 0; c < 4; ++c) {
+        own[c] = sT[c * TS + p * E + e];
+        oth[c] = sT[c * TS + idx];
+      }
+      const C nx = C(a.g.Nx[fi]), ny = C(a.g.Ny[fi]);
+      const bool owner = code & FC_FLUX_OWNER;
+      if (owner) numflux(FLUX, g, own, oth, nx, ny, F);
+      else numflux(FLUX, g, oth, own, nx, ny, F);
+      if constexpr (SHOCK) {
+        // Viscous edge flux at owner point jo (StoreEdgeViscousFlux,
+        // edges.go:151-244): average of the two sides' eps*grad Q along
+        // the owner normal, minus the penalty on the owner's trace jump.
+        const int kk = k0 + e, nb = a.sh.nbr_elem[fi], nf = a.sh.nbr_face[fi];
+        const int kL = owner ? kk : nb, kR = owner ? nb : kk;
+        const int rowL = NpInt + (owner ? f * NpEdge + pp : nf * NpEdge + rp);
+        const int rowR = NpInt + (owner ? nf * NpEdge + rp : f * NpEdge + pp);
+        const double lam = 0.5 * (a.sh.epsF[size_t(rowL) * Kp + kL] + a.sh.epsF[size_t(rowR) * Kp + kR]);
+        const double coef = (a.sh.omega * lam) * a.sh.vooL[fi];
 #pragma unroll
         for (int c = 0; c < 4; ++c) {
-          own[c] = sT[c * TS + p * E + e];
-          oth[c] = sT[c * TS + idx];
-        }
-        const C nx = C(a.g.Nx[fi]), ny = C(a.g.Ny[fi]);
-        const C w = C(a.g.IInII[fi]);
-        const bool owner = code & FC_FLUX_OWNER;
-        if (owner) numflux(FLUX, g, own, oth, nx, ny, F);
-        else numflux(FLUX, g, oth, own, nx, ny, F);
-        if constexpr (SHOCK) {
-          // Viscous edge flux at owner point jo (StoreEdgeViscousFlux,
-          // edges.go:151-244): average of the two sides' eps*grad Q along
-          // the owner normal, minus the penalty on the owner's trace jump.
-          const int kk = k0 + e, nb = a.sh.nbr_elem[fi], nf = a.sh.nbr_face[fi];
-          const int kL = owner ? kk : nb, kR = owner ? nb : kk;
-          const int rowL = NpInt + (owner ? f * NpEdge + pp : nf * NpEdge + rp);
-          const int rowR = NpInt + (owner ? nf * NpEdge + rp : f * NpEdge + pp);
-          const double lam = 0.5 * (a.sh.epsF[size_t(kL) * NR + rowL] + a.sh.epsF[size_t(kR) * NR + rowR]);
-          const double coef = (a.sh.omega * lam) * a.sh.vooL[fi];
-#pragma unroll
-          for (int c = 0; c < 4; ++c) {
-            // owner traces at owner points jo and NpEdge-1-jo
-            const double EQj = owner ? double(own[c]) : double(oth[c]);
-            const double EQr = owner ? double(sT[c * TS + (f * NpEdge + rp) * E + e])
-                                     : double(sT[c * TS + base + pp * stride]);
-            const double *dL = a.sh.diss + (size_t(kL) * 4 + c) * 2 * NR;
-            const double *dR = a.sh.diss + (size_t(kR) * 4 + c) * 2 * NR;
-            const double vFL = double(nx) * dL[rowL] + double(ny) * dL[NR + rowL];
-            const double vFR = double(nx) * dR[rowR] + double(ny) * dR[NR + rowR];
-            const double VF = 0.5 * (vFL + vFR) - coef * (EQj - EQr);
-            F[c] = C(double(F[c]) - VF);
-          }
-        }
-        if (owner) {
-#pragma unroll
-          for (int c = 0; c < 4; ++c) fl[j][c] = F[c] * w;
-        } else {
-#pragma unroll
-          for (int c = 0; c < 4; ++c) fl[j][c] = -F[c] * w;
+          // owner traces at owner points jo and NpEdge-1-jo
+          const double EQj = owner ? double(own[c]) : double(oth[c]);
+          const double EQr = owner ? double(sT[c * TS + (f * NpEdge + rp) * E + e])
+                                   : double(sT[c * TS + base + pp * stride]);
+          const double *dX = a.sh.diss + size_t(c * 2 * NR) * Kp, *dY = dX + size_t(NR) * Kp;
+          const double vFL = double(nx) * dX[size_t(rowL) * Kp + kL] + double(ny) * dY[size_t(rowL) * Kp + kL];
+          const double vFR = double(nx) * dX[size_t(rowR) * Kp + kR] + double(ny) * dY[size_t(rowR) * Kp + kR];
+          const double VF = 0.5 * (vFL + vFR) - coef * (EQj - EQr);
+          F[c] = C(double(F[c]) - VF);
         }
       }
+#pragma unroll
+      for (int c = 0; c < 4; ++c) fl[j][c] = F[c];
     }
   }
   __syncthreads();
 #pragma unroll
   for (int j = 0; j < ITB; ++j) {
     const int it = tid + j * kThreads;
-    if (it < E * NF) {
-      const int e = it % E, p = it / E, f = p / NpEdge;
-      if ((a.g.code[size_t(f) * Kp + k0 + e] & 15) == FK_Shared) {
+    if (it < nfl) {
+      const int w = it / NpEdge, pp = it - w * NpEdge;
+      const int packed = flist[w];
+      const int e = packed & 255, f = (packed >> 8) & 3, p = f * NpEdge + pp;
+      const size_t fi = size_t(f) * Kp + k0 + e;
+      const C wn = C(a.g.IInII[fi]);
+      if (a.g.code[fi] & FC_FLUX_OWNER) {
 #pragma unroll
-        for (int c = 0; c < 4; ++c) sT[c * TS + p * E + e] = fl[j][c];
+        for (int c = 0; c < 4; ++c) sT[c * TS + p * E + e] = fl[j][c] * wn;
+      } else {
+#pragma unroll
+        for (int c = 0; c < 4; ++c) sT[c * TS + p * E + e] = -fl[j][c] * wn;
+      }
+      if (packed & (1 << 10)) {
+        // the in-tile partner: slot pf*NpEdge*E + e2, at the reversed point
+        const int base = a.g.nbr_slot[fi];
+        const int pf = base / (NpEdge * E), e2 = base - pf * NpEdge * E;
+        const C w2 = C(a.g.IInII[size_t(pf) * Kp + k0 + e2]);
+        const int idx = base + (NpEdge - 1 - pp) * E;
+#pragma unroll
+        for (int c = 0; c < 4; ++c) sT[c * TS + idx] = -fl[j][c] * w2;
       }
     }
   }
@@ -333,8 +355,8 @@ k_stage(StageArgs a) {
       const int row = NpInt + f * NpEdge + pp;
 #pragma unroll
       for (int c = 0; c < 4; ++c) {
-        const double *d = a.sh.diss + (size_t(k0 + e) * 4 + c) * 2 * NR;
-        F[c] = C(double(F[c]) - (nxd * d[row] + nyd * d[NR + row]));
+        const double *dX = a.sh.diss + size_t(c * 2 * NR + row) * Kp + k0 + e;
+        F[c] = C(double(F[c]) - (nxd * dX[0] + nyd * dX[size_t(NR) * Kp]));
       }
     }
     const C w = C(a.g.IInII[fi]);
@@ -360,9 +382,9 @@ k_stage(StageArgs a) {
       C Fx, Fy;
       flux_component(tn, g, r0, r1, r2, r3, Fx, Fy);
       if constexpr (SHOCK) {
-        const double *d = a.sh.diss + (size_t(kk) * 4 + tn) * 2 * NR;
-        Fx -= d[j];
-        Fy -= d[NR + j];
+        const double *dX = a.sh.diss + size_t(tn * 2 * NR + j) * Kp + kk;
+        Fx -= dX[0];
+        Fy -= dX[size_t(NR) * Kp];
       }
       const C Fr = Jdc * (J0 * Fx + J1 * Fy), Fs = Jdc * (J2 * Fx + J3 * Fy);
 #pragma unroll
@@ -458,24 +480,25 @@ k_stage(StageArgs a) {
     // The operands of this stage are loaded for all LPT values of the
     // thread first, then combined and stored.
     const size_t base = size_t(k0) * NV;
-    S *q0 = static_cast<S *>(a.q0) + base;
+    double *q0 = static_cast<double *>(a.q0) + base;
     S *q1 = static_cast<S *>(a.q1) + base;
     S *q2 = static_cast<S *>(a.q2) + base;
     S *q3 = static_cast<S *>(a.q3) + base;
     S *q4 = static_cast<S *>(a.q4) + base;
-    S *Rr = static_cast<S *>(a.R) + base;
+    double *Rr = static_cast<double *>(a.R) + base;
     const int nreal = min(E, a.K - k0);
     if constexpr (LAST) {
       // stage 4: R = -q0 + a q2 + b q3 + c q4 + d dt R3 + e dt RHS, q0 += R
-      S v0[LPT], v2[LPT], v3[LPT], vR[LPT];
+      double v0[LPT], vR[LPT];
+      S v2[LPT], v3[LPT];
 #pragma unroll
       for (int j = 0; j < LPT; ++j) {
         const int idx = tid + j * kThreads;
         const bool ok = idx < E * NV;
-        v0[j] = ok ? q0[idx] : S(0);
+        v0[j] = ok ? q0[idx] : 0.0;
         v2[j] = ok ? q2[idx] : S(0);
         v3[j] = ok ? q3[idx] : S(0);
-        vR[j] = ok ? Rr[idx] : S(0);
+        vR[j] = ok ? Rr[idx] : 0.0;
       }
 #pragma unroll
       for (int j = 0; j < LPT; ++j) {
@@ -489,19 +512,19 @@ k_stage(StageArgs a) {
         const double dtR3 = dte * double(vR[j]);
         const double Rn = -q0v + 0.517231671970585 * double(v2[j]) + 0.096059710526146 * double(v3[j]) +
                           0.386708617503269 * qsv + 0.063692468666290 * dtR3 + 0.226007483236906 * dtR;
-        Rr[idx] = S(Rn);
-        const S nv = S(q0v + Rn);
+        Rr[idx] = Rn;
+        const double nv = q0v + Rn;
         q0[idx] = nv;
         sQ[r * P + e] = C(nv);
         if (e < nreal && !(nv == nv)) bad = 1;
       }
     } else {
-      S v0[LPT];
+      double v0[LPT];
       if (stage >= 1) {
 #pragma unroll
         for (int j = 0; j < LPT; ++j) {
           const int idx = tid + j * kThreads;
-          v0[j] = (idx < E * NV) ? q0[idx] : S(0);
+          v0[j] = (idx < E * NV) ? q0[idx] : 0.0;
         }
       }
 #pragma unroll
@@ -527,7 +550,7 @@ k_stage(StageArgs a) {
             q3[idx] = nv;
             break;
           default:
-            Rr[idx] = S(rhs);
+            Rr[idx] = rhs;
             nv = S(0.178079954393132 * double(v0[j]) + 0.821920045606868 * qsv + 0.544974750228521 * dtR);
             q4[idx] = nv;
             break;
@@ -742,10 +765,10 @@ __global__ void __launch_bounds__(256) k_shock_prep(StageArgs a) {
   }
   const int kk = gid;
   if (kk >= a.Kp) return;
-  double *epsF = a.sh.epsF + size_t(kk) * NR;
+  double *epsF = a.sh.epsF + kk;  // [NR][Kp]
   if (kk >= a.K) {
     a.sh.sigma[kk] = 0.;
-    for (int r = 0; r < NR; ++r) epsF[r] = 0.;
+    for (int r = 0; r < NR; ++r) epsF[size_t(r) * a.Kp] = 0.;
     return;
   }
   const unsigned long long *sv = a.sh.sigV + size_t(a.sh.vbuf) * a.sh.Nv;
@@ -756,7 +779,7 @@ __global__ void __launch_bounds__(256) k_shock_prep(StageArgs a) {
   const double e0 = from_bits(ev[v0]), e1 = from_bits(ev[v1]), e2 = from_bits(ev[v2]);
   const double *B = a.sh.ops + Sh::oBary;
 #pragma unroll
-  for (int r = 0; r < NR; ++r) epsF[r] = B[3 * r] * e0 + B[3 * r + 1] * e1 + B[3 * r + 2] * e2;
+  for (int r = 0; r < NR; ++r) epsF[size_t(r) * a.Kp] = B[3 * r] * e0 + B[3 * r + 1] * e1 + B[3 * r + 2] * e2;
   if (a.sh.filter_state) {
     const double *V = a.sh.ops + Sh::oV, *Vi = a.sh.ops + Sh::oVi, *mf = a.sh.ops + Sh::omf;
     const double fac = 1. - sin(0.5 * M_PI * sg);
@@ -838,8 +861,8 @@ __global__ void __launch_bounds__(kThreads) k_shock_grad(StageArgs a) {
   const double Jd = a.g.Jdet[kk], ooJd = 1. / Jd;
   const double Ji[4] = {a.g.Jinv[kk], a.g.Jinv[size_t(Kp) + kk], a.g.Jinv[2 * size_t(Kp) + kk],
                         a.g.Jinv[3 * size_t(Kp) + kk]};
-  const double *epsF = a.sh.epsF + size_t(kk) * NR;
-  double *dout = a.sh.diss + (size_t(kk) * 4 + tn) * 2 * NR;
+  const double *epsF = a.sh.epsF + kk;                              // [NR][Kp]
+  double *dout = a.sh.diss + size_t(tn * 2 * NR) * Kp + kk;         // [4][2][NR][Kp]
 #pragma unroll 1
   for (int dir = 0; dir < 2; ++dir) {
     double dof[NpFlux];
@@ -861,7 +884,7 @@ __global__ void __launch_bounds__(kThreads) k_shock_grad(StageArgs a) {
       double acc = 0;
 #pragma unroll
       for (int j = 0; j < NpFlux; ++j) acc += Div[r * NpFlux + j] * dof[j];
-      dout[dir * NR + r] = acc * epsF[r];
+      dout[size_t(dir * NR + r) * Kp] = acc * epsF[size_t(r) * Kp];
     }
   }
   if (a.stage < 0) return;
@@ -877,7 +900,7 @@ __global__ void __launch_bounds__(kThreads) k_shock_grad(StageArgs a) {
     if (k < a.K && (code & FC_WAVE)) {
       const int kind = code & 15;
       const double oohK = a.g.oohK[k];
-      const double *ef = a.sh.epsF + size_t(k) * NR + NpInt + f * NpEdge;
+      const double *ef = a.sh.epsF + size_t(NpInt + f * NpEdge) * Kp + k;
       double fwl = 0.0, fwv = 0.0;
       for (int pp = 0; pp < NpEdge; ++pp) {
         double q[4] = {sTr[0][f * NpEdge + pp][e], sTr[1][f * NpEdge + pp][e], sTr[2][f * NpEdge + pp][e],
@@ -889,7 +912,7 @@ __global__ void __launch_bounds__(kThreads) k_shock_grad(StageArgs a) {
         const double s = point_speed(a.ph.gamma, q, oohK);
         if (!(s == s)) bad = 1;
         fwl = fmax(fwl, s);
-        fwv = fmax(fwv, oohK * oohK * ef[pp]);
+        fwv = fmax(fwv, oohK * oohK * ef[size_t(pp) * Kp]);
       }
       if (a.lts.on) {
         a.lts.fw[size_t(a.par2) * 3 * Kp + fi] = fwl;
@@ -926,29 +949,35 @@ size_t lds_bytes_n(size_t csize, int Hmax) {
   return csize * (size_t(Sh::NV) * (kTile + 1) + 4 * (size_t(Sh::NF) * kTile + size_t(Sh::NpEdge) * Hmax));
 }
 
-template <int N, class S, class C, int FLUX, bool LAST, bool SHOCK>
+template <int N, class S, class SIN, class C, int FLUX, bool LAST, bool SHOCK>
 hipError_t launch_stage_t(const StageArgs &a, hipStream_t s) {
   const size_t lds = lds_bytes_n<N>(sizeof(C), a.Hmax);
   static size_t attr_bytes = 0;  // dynamic LDS the kernel is currently allowed
   if (lds > 64 * 1024 && lds > attr_bytes) {
-    hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void *>(&k_stage<N, S, C, FLUX, LAST, SHOCK>),
+    hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void *>(&k_stage<N, S, SIN, C, FLUX, LAST, SHOCK>),
                                        hipFuncAttributeMaxDynamicSharedMemorySize, int(lds));
     if (e != hipSuccess) return e;
     attr_bytes = lds;
   }
   const int q = (a.ntiles + 7) / 8;
-  hipLaunchKernelGGL((k_stage<N, S, C, FLUX, LAST, SHOCK>), dim3(8 * q), dim3(kThreads), lds, s, a);
+  hipLaunchKernelGGL((k_stage<N, S, SIN, C, FLUX, LAST, SHOCK>), dim3(8 * q), dim3(kThreads), lds, s, a);
   return hipGetLastError();
+}
+
+template <int N, class S, class C, int FLUX, bool SHOCK>
+hipError_t launch_stage_l(const StageArgs &a, hipStream_t s) {
+  if (a.stage == 4) return launch_stage_t<N, S, S, C, FLUX, true, SHOCK>(a, s);
+  if (a.stage <= 0) return launch_stage_t<N, S, double, C, FLUX, false, SHOCK>(a, s);  // input q0
+  return launch_stage_t<N, S, S, C, FLUX, false, SHOCK>(a, s);
 }
 
 template <int N, class S, class C, bool SHOCK>
 hipError_t launch_stage_f(const StageArgs &a, hipStream_t s) {
-  const bool last = a.stage == 4;
   switch (a.ph.flux) {
-    case 0: return last ? launch_stage_t<N, S, C, 0, true, SHOCK>(a, s) : launch_stage_t<N, S, C, 0, false, SHOCK>(a, s);
-    case 1: return last ? launch_stage_t<N, S, C, 1, true, SHOCK>(a, s) : launch_stage_t<N, S, C, 1, false, SHOCK>(a, s);
-    case 2: return last ? launch_stage_t<N, S, C, 2, true, SHOCK>(a, s) : launch_stage_t<N, S, C, 2, false, SHOCK>(a, s);
-    case 3: return last ? launch_stage_t<N, S, C, 3, true, SHOCK>(a, s) : launch_stage_t<N, S, C, 3, false, SHOCK>(a, s);
+    case 0: return launch_stage_l<N, S, C, 0, SHOCK>(a, s);
+    case 1: return launch_stage_l<N, S, C, 1, SHOCK>(a, s);
+    case 2: return launch_stage_l<N, S, C, 2, SHOCK>(a, s);
+    case 3: return launch_stage_l<N, S, C, 3, SHOCK>(a, s);
   }
   return hipErrorInvalidValue;
 }
@@ -966,8 +995,8 @@ template <int N>
 hipError_t launch_prologue_n(int prec, const StageArgs &a, hipStream_t s) {
   const long items = long(a.K) * 3;
   const int blocks = int((items + 255) / 256);
-  if (prec == 0) hipLaunchKernelGGL((k_prologue<N, double>), dim3(blocks), dim3(256), 0, s, a);
-  else hipLaunchKernelGGL((k_prologue<N, float>), dim3(blocks), dim3(256), 0, s, a);
+  (void)prec;  // the prologue reads q0, which is FP64 in every precision
+  hipLaunchKernelGGL((k_prologue<N, double>), dim3(blocks), dim3(256), 0, s, a);
   return hipGetLastError();
 }
 
