@@ -14,8 +14,9 @@ with the density L2/Linf error of the FP32 run against the FP64 run and of
 both against the analytic convected vortex.
 
 The same run measures the shock-capturing path (PerssonC0) on a Mach 2
-wedge-channel mesh of about 250k triangles at P=2 (FP64, 2000 steps) and
-reports it under "shock_path".
+wedge-channel mesh of about 250k triangles at P=2 (FP64, 2000 steps) under
+"shock_path", and a batched ensemble of 64 independent cases (2k..40k
+triangles, orders 1..4, FP32, 1000 steps) under "ensemble".
 
 python bench.py [--gpus 1] [--steps 500] [--warmup 10] [--no-cpu] [--no-shock]
 """
@@ -114,6 +115,71 @@ def run_gpu_shock(mesh, steps, warmup):
                 NpEdge=e.NpEdge, flagged=int((sigma > 0).sum()), sigma_max=float(sigma.max()))
 
 
+def ensemble_cases(ncases: int, seed: int = 7):
+    """The batched workload: ncases independent cases, K log-uniform in
+    [2k, 40k] triangles (jittered quads, n = sqrt(K/2) per side), order drawn
+    from {1,2,3,4}, Mach from U[0.3,0.8], angle from U[-5,5] deg; even cases
+    start from the isentropic vortex, odd ones from the freestream. Roe flux,
+    CFL 1."""
+    from gocfd_amd import InputParameters2D, Mesh, meshgen
+    u = meshgen.splitmix64(seed, 4 * ncases).reshape(ncases, 4)
+    cases = []
+    for c in range(ncases):
+        K = 2000.0 * (20.0 ** u[c, 0])
+        n = max(4, int(round(math.sqrt(K / 2))))
+        N = 1 + int(u[c, 1] * 4) % 4
+        M = 0.3 + 0.5 * u[c, 2]
+        a = -5.0 + 10.0 * u[c, 3]
+        VX, VY, EToV, tags = meshgen.jittered_quads(n, n, seed=1000 + c)
+        ip = InputParameters2D(Title=f"case{c}", CFL=1.0, FluxType="Roe",
+                               InitType="IVortex" if c % 2 == 0 else "Freestream",
+                               PolynomialOrder=N, Minf=M, Alpha=a, FinalTime=1e9)
+        cases.append((ip, Mesh.from_arrays(VX, VY, EToV, tags)))
+    return cases
+
+
+def run_ensemble(ncases, steps, warmup):
+    """The ensemble in FP32 (one launch per order group and stage, graph
+    replays), the same cases run one after another as single solvers, and the
+    FP32 ensemble against single-case FP64 runs for the first cases."""
+    from gocfd_amd import Ensemble, Euler
+    cases = ensemble_cases(ncases)
+    ens = Ensemble(cases, precision="fp32")
+    ens.step(warmup)
+    t0 = time.perf_counter()
+    ens.step(steps)
+    elapsed = time.perf_counter() - t0
+    ms, _ = ens.kernel_time()
+    dof = sum(K * Np for K, Np in ens.dims)
+    # sequential single-case runs of the same cases (a bounded sample of steps)
+    seq_steps = max(6, steps // 20)
+    seq = 0.0
+    for ip, m in cases:
+        e = Euler(ip, m, precision="fp32")
+        e.step(1)
+        e.sync()
+        t1 = time.perf_counter()
+        e.step(seq_steps)
+        seq += time.perf_counter() - t1
+    seq_rate = dof * 5 * seq_steps / seq
+    # the first case of each order against its own single-case FP64 run
+    errs = {}
+    for c, (ip, m) in enumerate(cases):
+        if ip.PolynomialOrder in errs:
+            continue
+        e = Euler(ip, m, precision="fp64")
+        e.step(warmup + steps)
+        q64 = e.Q
+        q32 = ens.state(c)
+        errs[ip.PolynomialOrder] = {"case": c, "K": ens.dims[c][0],
+                                   "rho_l2": float(np.linalg.norm(q32[0] - q64[0]) / np.linalg.norm(q64[0]))}
+    orders = [ip.PolynomialOrder for ip, _ in cases]
+    return dict(value=dof * 5 * steps / elapsed, elapsed=elapsed, ms=ms, dof=dof,
+                K_total=sum(K for K, _ in ens.dims), K_min=min(K for K, _ in ens.dims),
+                K_max=max(K for K, _ in ens.dims), orders={n: orders.count(n) for n in sorted(set(orders))},
+                seq_rate=seq_rate, errs=errs)
+
+
 def cpu_baseline(mesh, target_seconds=15.0):
     """The CPU oracle (plain C, FP64, OpenMP over the host cores) on the same mesh and order:
     whole SSP54 steps until about `target_seconds` of CPU work."""
@@ -150,12 +216,15 @@ def main():
     ap.add_argument("--n", type=int, default=200, help="quads per side (2n^2 triangles)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
-    ap.add_argument("--only", choices=["fp32", "fp64", "shock"], default=None,
+    ap.add_argument("--only", choices=["fp32", "fp64", "shock", "ensemble"], default=None,
                     help="run one leg only (for profiler runs)")
     ap.add_argument("--shock-points", type=int, default=125_000,
                     help="interior points of the wedge mesh (about twice as many triangles)")
     ap.add_argument("--shock-steps", type=int, default=2000)
     ap.add_argument("--no-shock", action="store_true", help="skip the shock-capturing leg")
+    ap.add_argument("--ensemble-cases", type=int, default=64)
+    ap.add_argument("--ensemble-steps", type=int, default=1000)
+    ap.add_argument("--no-ensemble", action="store_true", help="skip the batched-ensemble leg")
     ap.add_argument("--copy-probe", type=float, default=0.0, metavar="MB",
                     help="run a plain device copy of MB megabytes (counter calibration) and exit")
     args = ap.parse_args()
@@ -181,6 +250,11 @@ def main():
         v = r["K"] * r["NpInt"] * 5 * args.shock_steps / r["elapsed"]
         print(json.dumps({"leg": "shock", "value": v, "stage_ms": r["stage_ms"],
                           "ms_per_step": r["elapsed"] / args.shock_steps * 1e3, "K": r["K"]}))
+        return
+    if args.only == "ensemble":
+        r = run_ensemble(args.ensemble_cases, args.ensemble_steps, args.warmup)
+        print(json.dumps({"leg": "ensemble", "value": r["value"], "ms_per_step": r["elapsed"] / args.ensemble_steps * 1e3,
+                          "dof": r["dof"], "seq_rate": r["seq_rate"], "errs": r["errs"]}))
         return
     if args.only:
         r = run_gpu(mesh, args.only, args.steps, args.warmup)
@@ -229,6 +303,18 @@ def main():
                  "time": rs["time"], "dt": rs["dt"], "sensor_flagged_elements": rs["flagged"],
                  "sigma_max": rs["sigma_max"]}
 
+    ensemble = None
+    if not args.no_ensemble:
+        re_ = run_ensemble(args.ensemble_cases, args.ensemble_steps, args.warmup)
+        ensemble = {"workload": f"{args.ensemble_cases} independent cases, K {re_['K_min']}..{re_['K_max']} "
+                                f"(total {re_['K_total']}), orders {re_['orders']}, Mach U[0.3,0.8], "
+                                f"AoA U[-5,5] deg, Roe, FP32 bulk, {args.ensemble_steps} steps",
+                    "value": re_["value"], "unit": "DOF-stage/s",
+                    "ms_per_step": re_["elapsed"] / args.ensemble_steps * 1e3,
+                    "sequential_single_case_value": re_["seq_rate"],
+                    "speedup_vs_sequential": re_["value"] / re_["seq_rate"],
+                    "rho_l2_fp32_vs_fp64_single_case_by_order": re_["errs"]}
+
     cpu = None
     if not args.no_cpu:
         c = cpu_baseline(mesh, args.cpu_seconds)
@@ -266,6 +352,7 @@ def main():
                      "alg_bytes_per_launch": b32},
         "cpu_baseline": cpu,
         "shock_path": shock,
+        "ensemble": ensemble,
     }
     print(json.dumps(line))
 

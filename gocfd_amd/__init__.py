@@ -7,5 +7,6 @@ step running in hand-written gfx950 HIP kernels behind a C ABI
 """
 from .input_parameters import InputParameters2D
 from .euler2d import Euler, Mesh, NewEuler, RungeKutta5SSP
+from .ensemble import Ensemble
 
-__all__ = ["InputParameters2D", "Euler", "Mesh", "NewEuler", "RungeKutta5SSP"]
+__all__ = ["InputParameters2D", "Euler", "Ensemble", "Mesh", "NewEuler", "RungeKutta5SSP"]

@@ -23,6 +23,8 @@ EXPORTS = [
     "gocfd_euler2d_get_edges", "gocfd_euler2d_get_connectivity", "gocfd_euler2d_get_operators",
     "gocfd_euler2d_get_geometry", "gocfd_euler2d_get_coords", "gocfd_euler2d_kernel_time",
     "gocfd_euler2d_tiling", "gocfd_euler2d_get_shock", "gocfd_euler2d_get_element_dt",
+    "gocfd_ensemble_create", "gocfd_ensemble_destroy", "gocfd_ensemble_case_dims", "gocfd_ensemble_step",
+    "gocfd_ensemble_get_state", "gocfd_ensemble_time", "gocfd_ensemble_kernel_time",
 ]
 
 
@@ -90,6 +92,16 @@ def load() -> C.CDLL:
     L.gocfd_euler2d_get_coords.argtypes = [_vp, _dp, _dp, _dp, _dp]
     L.gocfd_euler2d_kernel_time.argtypes = [_vp, _dp, _i32p]
     L.gocfd_euler2d_tiling.argtypes = [_vp, _i32p, _i32p, _i32p]
+    L.gocfd_euler2d_get_shock.argtypes = [_vp, _dp, _dp, _dp]
+    L.gocfd_euler2d_get_element_dt.argtypes = [_vp, _dp]
+    L.gocfd_ensemble_create.argtypes = [C.c_int32, C.POINTER(_vp), C.POINTER(Params), C.POINTER(_vp)]
+    L.gocfd_ensemble_destroy.argtypes = [_vp]
+    L.gocfd_ensemble_destroy.restype = None
+    L.gocfd_ensemble_case_dims.argtypes = [_vp, C.c_int32, _i32p, _i32p]
+    L.gocfd_ensemble_step.argtypes = [_vp, C.c_int32]
+    L.gocfd_ensemble_get_state.argtypes = [_vp, C.c_int32, _dp]
+    L.gocfd_ensemble_time.argtypes = [_vp, C.c_int32, _dp, _dp, _i32p]
+    L.gocfd_ensemble_kernel_time.argtypes = [_vp, _dp, _i32p]
     for name in EXPORTS:  # ctypes default restype is int: every other entry returns a status
         getattr(L, name)
     _lib = L

@@ -17,7 +17,7 @@ CSRC = os.path.join(HERE, "csrc")
 BUILD = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libgocfd_euler2d.so")
 
-SOURCES = ["dfr_setup.cpp", "euler2d_capi.cpp", "kernels_common.hip"] + [
+SOURCES = ["dfr_setup.cpp", "capi_common.cpp", "euler2d_capi.cpp", "ensemble_capi.cpp", "kernels_common.hip"] + [
     f"stage_n{n}.hip" for n in range(5)]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")

@@ -1,0 +1,377 @@
+// Batched ensembles: many independent cases (meshes of different sizes,
+// orders, Mach numbers and angles) stepped together. The reference runs one
+// case per process; this has no counterpart there.
+//
+// Cases of the same order form a group whose device layout is the
+// concatenation of the cases' layouts (every case starts on a tile boundary,
+// so a tile belongs to one case); each stage is one launch per group. Every
+// case keeps its own control block (time, dt, wave-speed maxima) and
+// physics constants, selected per tile. A run of 6 steps -- the period of
+// the per-stage ring and time-parity arguments -- is captured once as a HIP
+// graph and replayed.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstring>
+#include <map>
+#include <memory>
+#include <vector>
+
+#include "capi_common.hpp"
+#include "dfr_setup.hpp"
+#include "gocfd_euler2d.h"
+#include "kernels.hpp"
+
+using namespace gocfd;
+using namespace gocfd::capi;
+
+namespace {
+
+constexpr int kGraphSteps = 6;
+
+struct EnsCase {
+  gocfd_params prm{};
+  Discretisation d;
+  DeviceLayout L;
+  FreeStream fs;
+  std::vector<double> Q0;
+  int group = 0, off = 0, boff = 0;
+};
+
+struct EnsGroup {
+  int N = 0, Kp = 0, ntiles = 0, Hmax = 1, Wmax = 1, Fmax = 1, NV = 0;
+  std::vector<int> cases;
+  DevBuf<double> Jdet, Jinv, IInII, Nx, Ny, oohK, bxy, Interp64;
+  DevBuf<int> code, nbr_slot, halo_elem, halo_face, wave_list, wave_cnt, flux_list, flux_cnt, tile_case;
+  void *q[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
+  void *R = nullptr;
+  ~EnsGroup() {
+    for (auto &p : q) if (p) (void)hipFree(p);
+    if (R) (void)hipFree(R);
+  }
+};
+
+}  // namespace
+
+struct gocfd_ensemble {
+  int device = 0, prec = 0, flux = 0;
+  size_t ssize = 8;
+  std::vector<std::unique_ptr<EnsCase>> cases;
+  std::vector<std::unique_ptr<EnsGroup>> groups;
+  DevBuf<dev::CaseInfo> cinfo;
+  dev::Ctl *ctl = nullptr;
+  hipStream_t stream = nullptr;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t gexec = nullptr;
+  long long gstage = 0;
+  int step_index = 0;
+  double last_ms = 0;
+  int last_steps = 0;
+
+  ~gocfd_ensemble() {
+    if (stream) (void)hipStreamSynchronize(stream);
+    if (gexec) (void)hipGraphExecDestroy(gexec);
+    if (graph) (void)hipGraphDestroy(graph);
+    groups.clear();
+    if (ctl) (void)hipFree(ctl);
+    if (ev0) (void)hipEventDestroy(ev0);
+    if (ev1) (void)hipEventDestroy(ev1);
+    if (stream) (void)hipStreamDestroy(stream);
+  }
+
+  dev::StageArgs args(const EnsGroup &g) const {
+    dev::StageArgs a{};
+    a.K = g.Kp; a.Kp = g.Kp; a.ntiles = g.ntiles; a.Hmax = g.Hmax; a.Wmax = g.Wmax; a.Fmax = g.Fmax;
+    a.q0 = g.q[0]; a.q1 = g.q[1]; a.q2 = g.q[2]; a.q3 = g.q[3]; a.q4 = g.q[4]; a.R = g.R;
+    a.ctl = ctl;
+    a.cases = cinfo.p;
+    a.tile_case = g.tile_case.p;
+    a.g.Jdet = g.Jdet.p; a.g.Jinv = g.Jinv.p; a.g.IInII = g.IInII.p; a.g.Nx = g.Nx.p; a.g.Ny = g.Ny.p;
+    a.g.oohK = g.oohK.p; a.g.code = g.code.p; a.g.nbr_slot = g.nbr_slot.p;
+    a.g.halo_elem = g.halo_elem.p; a.g.halo_face = g.halo_face.p; a.g.bxy = g.bxy.p;
+    a.g.wave_list = g.wave_list.p; a.g.wave_cnt = g.wave_cnt.p;
+    a.g.flux_list = g.flux_list.p; a.g.flux_cnt = g.flux_cnt.p;
+    a.Interp64 = g.Interp64.p;
+    const EnsCase &c0 = *cases[g.cases[0]];
+    a.ph = phys_of(c0.fs, c0.prm);
+    return a;
+  }
+
+  void restart() {
+    HIPCHECK(dev::launch_ctl_init(ctl, 0.0, stream, int(cases.size())));
+    gstage = 0;
+    step_index = 0;
+    for (auto &g : groups) {
+      dev::StageArgs a = args(*g);
+      a.qs = g->q[0];
+      a.ring = 0;
+      a.par2 = 0;
+      a.step_index = 0;
+      HIPCHECK(dev::launch_prologue(g->N, prec, a, stream));
+    }
+  }
+
+  void enqueue_step() {
+    for (int s = 0; s < 5; ++s) {
+      for (auto &g : groups) {
+        dev::StageArgs a = args(*g);
+        a.stage = s;
+        a.step_index = step_index;
+        a.ring = int(gstage % 3);
+        a.par2 = int(gstage & 1);
+        a.qs = g->q[s];
+        HIPCHECK(dev::launch_stage(g->N, prec, 0, a, stream));
+      }
+      ++gstage;
+    }
+    ++step_index;
+  }
+
+  // Steps run eagerly until the step counter is 1 mod 6; from there whole
+  // blocks of 6 steps replay the captured graph (the first block is
+  // captured on first use), and the remainder runs eagerly again.
+  void enqueue_steps(int nsteps) {
+    HIPCHECK(hipEventRecord(ev0, stream));
+    int left = nsteps;
+    while (left > 0) {
+      if (step_index % kGraphSteps == 1 && left >= kGraphSteps) {
+        if (!gexec) {
+          HIPCHECK(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
+          const long long g0 = gstage;
+          const int s0 = step_index;
+          for (int j = 0; j < kGraphSteps; ++j) enqueue_step();
+          HIPCHECK(hipStreamEndCapture(stream, &graph));
+          HIPCHECK(hipGraphInstantiate(&gexec, graph, nullptr, nullptr, 0));
+          gstage = g0;
+          step_index = s0;
+        }
+        HIPCHECK(hipGraphLaunch(gexec, stream));
+        gstage += 5LL * kGraphSteps;
+        step_index += kGraphSteps;
+        left -= kGraphSteps;
+      } else {
+        enqueue_step();
+        --left;
+      }
+    }
+    HIPCHECK(hipEventRecord(ev1, stream));
+    last_steps = nsteps;
+  }
+
+  int finish() {
+    HIPCHECK(hipEventSynchronize(ev1));
+    float ms = 0;
+    HIPCHECK(hipEventElapsedTime(&ms, ev0, ev1));
+    last_ms = ms;
+    std::vector<dev::Ctl> c(cases.size());
+    HIPCHECK(hipMemcpy(c.data(), ctl, c.size() * sizeof(dev::Ctl), hipMemcpyDeviceToHost));
+    for (size_t i = 0; i < c.size(); ++i)
+      if (c[i].nan) return fail(GOCFD_ERR_NAN, "NAN found in case " + std::to_string(i));
+    return GOCFD_OK;
+  }
+};
+
+namespace {
+
+// Concatenate the cases of one order into a group's device layout.
+void build_group(gocfd_ensemble &h, EnsGroup &g) {
+  const int E = dev::kTile;
+  int off = 0, boff = 0;
+  for (int ci : g.cases) {
+    EnsCase &c = *h.cases[ci];
+    c.off = off;
+    c.boff = boff;
+    off += c.L.Kp;
+    boff += c.L.nbfaces;
+    g.Hmax = std::max(g.Hmax, c.L.Hmax);
+    g.Wmax = std::max(g.Wmax, c.L.Wmax);
+    g.Fmax = std::max(g.Fmax, c.L.Fmax);
+  }
+  g.Kp = off;
+  g.ntiles = off / E;
+  const int N = g.N, NpEdge = N + 2, NpInt = (N + 1) * (N + 2) / 2;
+  g.NV = 4 * NpInt;
+  const size_t Kp = g.Kp;
+  std::vector<double> Jdet(Kp), Jinv(4 * Kp), IInII(3 * Kp), Nx(3 * Kp), Ny(3 * Kp), oohK(Kp), bxy;
+  std::vector<int> code(3 * Kp), nbr_slot(3 * Kp), tile_case(g.ntiles);
+  std::vector<int> halo_elem(size_t(g.ntiles) * g.Hmax, -1), halo_face(size_t(g.ntiles) * g.Hmax, 0);
+  std::vector<int> wave_list(size_t(g.ntiles) * g.Wmax, 0), wave_cnt(g.ntiles, 0);
+  std::vector<int> flux_list(size_t(g.ntiles) * g.Fmax, 0), flux_cnt(g.ntiles, 0);
+  std::vector<double> q0(Kp * g.NV);
+  for (int ci : g.cases) {
+    const EnsCase &c = *h.cases[ci];
+    const DeviceLayout &L = c.L;
+    const size_t o = c.off, kp = L.Kp;
+    for (size_t kk = 0; kk < kp; ++kk) {
+      Jdet[o + kk] = L.Jdet[kk];
+      oohK[o + kk] = L.oohK[kk];
+      for (int j = 0; j < 4; ++j) Jinv[j * Kp + o + kk] = L.Jinv[j * kp + kk];
+      for (int f = 0; f < 3; ++f) {
+        const size_t src = f * kp + kk, dst = f * Kp + o + kk;
+        IInII[dst] = L.IInII[src];
+        Nx[dst] = L.Nx[src];
+        Ny[dst] = L.Ny[src];
+        nbr_slot[dst] = L.nbr_slot[src];  // tile-relative
+        int cd = L.face_code[src];
+        if ((cd & 15) == FK_IVortex) cd = (cd & 0xff) | (((cd >> 8) + c.boff) << 8);
+        code[dst] = cd;
+      }
+    }
+    bxy.insert(bxy.end(), L.bpoint_xy.begin(), L.bpoint_xy.end());
+    const int t0 = c.off / E;
+    for (int t = 0; t < L.ntiles; ++t) {
+      tile_case[t0 + t] = ci;
+      for (int i = 0; i < L.Hmax; ++i) {
+        const int e = L.halo_elem[size_t(t) * L.Hmax + i];
+        halo_elem[size_t(t0 + t) * g.Hmax + i] = e < 0 ? -1 : e + c.off;
+        halo_face[size_t(t0 + t) * g.Hmax + i] = L.halo_face[size_t(t) * L.Hmax + i];
+      }
+      wave_cnt[t0 + t] = L.wave_cnt[t];
+      for (int i = 0; i < L.wave_cnt[t]; ++i) wave_list[size_t(t0 + t) * g.Wmax + i] = L.wave_list[size_t(t) * L.Wmax + i];
+      flux_cnt[t0 + t] = L.flux_cnt[t];
+      for (int i = 0; i < L.flux_cnt[t]; ++i) flux_list[size_t(t0 + t) * g.Fmax + i] = L.flux_list[size_t(t) * L.Fmax + i];
+    }
+    fill_element_major(L, NpInt, c.Q0.data(), q0.data() + o * g.NV);
+  }
+  g.Jdet.upload(Jdet); g.Jinv.upload(Jinv); g.IInII.upload(IInII); g.Nx.upload(Nx); g.Ny.upload(Ny);
+  g.oohK.upload(oohK); g.bxy.upload(bxy); g.code.upload(code); g.nbr_slot.upload(nbr_slot);
+  g.halo_elem.upload(halo_elem); g.halo_face.upload(halo_face);
+  g.wave_list.upload(wave_list); g.wave_cnt.upload(wave_cnt);
+  g.flux_list.upload(flux_list); g.flux_cnt.upload(flux_cnt); g.tile_case.upload(tile_case);
+  const EnsCase &c0 = *h.cases[g.cases[0]];
+  g.Interp64.upload(c0.d.ops.Interp);
+  std::vector<double> divT(c0.d.ops.DivInt.size());
+  for (int i = 0; i < NpInt; ++i)
+    for (int j = 0; j < c0.d.ops.NpFlux; ++j)
+      divT[size_t(j) * NpInt + i] = c0.d.ops.DivInt[size_t(i) * c0.d.ops.NpFlux + j];
+  HIPCHECK(dev::upload_operators(N, divT.data(), c0.d.ops.Interp.data()));
+  const size_t nval = Kp * g.NV;
+  HIPCHECK(hipMalloc(&g.q[0], nval * sizeof(double)));
+  HIPCHECK(hipMemcpy(g.q[0], q0.data(), nval * sizeof(double), hipMemcpyHostToDevice));
+  for (int s = 1; s < 5; ++s) {
+    HIPCHECK(hipMalloc(&g.q[s], nval * h.ssize));
+    HIPCHECK(hipMemset(g.q[s], 0, nval * h.ssize));
+  }
+  HIPCHECK(hipMalloc(&g.R, nval * sizeof(double)));
+  HIPCHECK(hipMemset(g.R, 0, nval * sizeof(double)));
+}
+
+}  // namespace
+
+extern "C" {
+
+int gocfd_ensemble_create(int32_t ncases, const gocfd_mesh *const *meshes, const gocfd_params *params,
+                          gocfd_ensemble **out) {
+  GUARD_BEGIN
+  if (ncases <= 0 || !meshes || !params || !out) return fail(GOCFD_ERR_ARG, "bad argument");
+  *out = nullptr;
+  auto h = std::make_unique<gocfd_ensemble>();
+  h->device = params[0].device;
+  h->prec = params[0].precision;
+  h->flux = params[0].flux_type;
+  h->ssize = h->prec == 0 ? sizeof(double) : sizeof(float);
+  for (int i = 0; i < ncases; ++i) {
+    const gocfd_params &p = params[i];
+    if (!meshes[i]) return fail(GOCFD_ERR_ARG, "null mesh");
+    if (validate_params(&p) != GOCFD_OK) return GOCFD_ERR_ARG;
+    if (p.precision != h->prec || p.flux_type != h->flux || p.device != h->device)
+      return fail(GOCFD_ERR_ARG, "every case of an ensemble needs the same precision, flux type and device");
+    if (p.local_time_stepping || p.limiter != 0)
+      return fail(GOCFD_ERR_ARG, "ensembles run the plain SSP54 step (no LocalTimeStepping, no limiter)");
+    auto c = std::make_unique<EnsCase>();
+    c->prm = p;
+    c->d = build_discretisation(meshes[i]->m, p.polynomial_order);
+    init_case(c->d, p, c->fs, c->Q0);
+    c->L = build_device_layout(c->d, dev::kTile);
+    h->cases.push_back(std::move(c));
+  }
+  std::map<int, std::vector<int>> by_order;
+  for (int i = 0; i < ncases; ++i) by_order[h->cases[i]->prm.polynomial_order].push_back(i);
+  size_t lds = 0;
+  for (auto &kv : by_order) {
+    auto g = std::make_unique<EnsGroup>();
+    g->N = kv.first;
+    g->cases = kv.second;
+    for (int ci : g->cases) {
+      h->cases[ci]->group = int(h->groups.size());
+      lds = std::max(lds, dev::stage_lds_bytes(g->N, h->prec, h->cases[ci]->L.Hmax));
+    }
+    h->groups.push_back(std::move(g));
+  }
+  check_device(h->device, lds + 4096);
+  HIPCHECK(hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking));
+  HIPCHECK(hipEventCreate(&h->ev0));
+  HIPCHECK(hipEventCreate(&h->ev1));
+  for (auto &g : h->groups) build_group(*h, *g);
+  std::vector<dev::CaseInfo> ci(ncases);
+  for (int i = 0; i < ncases; ++i) {
+    const EnsCase &c = *h->cases[i];
+    ci[i].ph = phys_of(c.fs, c.prm);
+    ci[i].k_end = c.off + c.L.K;
+    ci[i].tile0 = c.off / dev::kTile;
+  }
+  h->cinfo.upload(ci);
+  HIPCHECK(hipMalloc(&h->ctl, sizeof(dev::Ctl) * ncases));
+  h->restart();
+  HIPCHECK(hipStreamSynchronize(h->stream));
+  *out = h.release();
+  return GOCFD_OK;
+  GUARD_END
+}
+
+void gocfd_ensemble_destroy(gocfd_ensemble *h) { delete h; }
+
+int gocfd_ensemble_case_dims(gocfd_ensemble *h, int32_t c, int32_t *K, int32_t *NpInt) {
+  if (!h || c < 0 || c >= int(h->cases.size())) return fail(GOCFD_ERR_ARG, "bad case index");
+  if (K) *K = h->cases[c]->d.K;
+  if (NpInt) *NpInt = h->cases[c]->d.ops.NpInt;
+  return GOCFD_OK;
+}
+
+int gocfd_ensemble_step(gocfd_ensemble *h, int32_t nsteps) {
+  GUARD_BEGIN
+  if (!h || nsteps < 0) return fail(GOCFD_ERR_ARG, "bad argument");
+  HIPCHECK(hipSetDevice(h->device));
+  h->enqueue_steps(nsteps);
+  return h->finish();
+  GUARD_END
+}
+
+int gocfd_ensemble_get_state(gocfd_ensemble *h, int32_t c, double *Q) {
+  GUARD_BEGIN
+  if (!h || !Q || c < 0 || c >= int(h->cases.size())) return fail(GOCFD_ERR_ARG, "bad argument");
+  HIPCHECK(hipSetDevice(h->device));
+  HIPCHECK(hipStreamSynchronize(h->stream));
+  const EnsCase &cs = *h->cases[c];
+  const EnsGroup &g = *h->groups[cs.group];
+  std::vector<double> buf(size_t(cs.L.Kp) * g.NV);
+  HIPCHECK(hipMemcpy(buf.data(), static_cast<double *>(g.q[0]) + size_t(cs.off) * g.NV,
+                     buf.size() * sizeof(double), hipMemcpyDeviceToHost));
+  read_element_major(cs.L, cs.d.ops.NpInt, buf.data(), Q);
+  return GOCFD_OK;
+  GUARD_END
+}
+
+int gocfd_ensemble_time(gocfd_ensemble *h, int32_t c, double *time, double *global_dt, int32_t *steps) {
+  GUARD_BEGIN
+  if (!h || c < 0 || c >= int(h->cases.size())) return fail(GOCFD_ERR_ARG, "bad argument");
+  HIPCHECK(hipSetDevice(h->device));
+  HIPCHECK(hipStreamSynchronize(h->stream));
+  dev::Ctl k;
+  HIPCHECK(hipMemcpy(&k, h->ctl + c, sizeof(k), hipMemcpyDeviceToHost));
+  if (time) *time = k.time[k.steps & 1];
+  if (global_dt) *global_dt = k.dt[4];
+  if (steps) *steps = k.steps;
+  return GOCFD_OK;
+  GUARD_END
+}
+
+int gocfd_ensemble_kernel_time(gocfd_ensemble *h, double *ms, int32_t *steps) {
+  if (!h) return fail(GOCFD_ERR_ARG, "null handle");
+  if (ms) *ms = h->last_ms;
+  if (steps) *steps = h->last_steps;
+  return GOCFD_OK;
+}
+
+}  // extern "C"

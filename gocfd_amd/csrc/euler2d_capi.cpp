@@ -12,84 +12,11 @@
 #include "dfr_setup.hpp"
 #include "gocfd_euler2d.h"
 #include "kernels.hpp"
+#include "capi_common.hpp"
 
 using namespace gocfd;
 
-namespace {
-thread_local std::string g_last_error;
-
-int fail(int code, const std::string &msg) {
-  g_last_error = msg;
-  return code;
-}
-
-#define HIPCHECK(expr)                                                                     \
-  do {                                                                                     \
-    hipError_t e_ = (expr);                                                                \
-    if (e_ != hipSuccess)                                                                  \
-      throw SetupError(GOCFD_ERR_DEVICE, std::string(#expr) + ": " + hipGetErrorString(e_)); \
-  } while (0)
-
-template <class T>
-struct DevBuf {
-  T *p = nullptr;
-  size_t n = 0;
-  void alloc(size_t count) {
-    free();
-    if (count == 0) count = 1;
-    HIPCHECK(hipMalloc(&p, count * sizeof(T)));
-    n = count;
-  }
-  void upload(const std::vector<T> &v) {
-    alloc(v.size());
-    if (!v.empty()) HIPCHECK(hipMemcpy(p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice));
-  }
-  void free() {
-    if (p) (void)hipFree(p);
-    p = nullptr;
-    n = 0;
-  }
-  ~DevBuf() { free(); }
-};
-
-// Freestream (model_problems/Euler2D/fluids.go:245-273)
-struct FreeStream {
-  double gamma = 1.4, minf = 0, alpha = 0, q[4] = {1, 0, 0, 0}, pinf = 0, cinf = 0;
-  void from_qinf(double g, const double qq[4]) {
-    gamma = g;
-    for (int i = 0; i < 4; ++i) q[i] = qq[i];
-    const double oorho = 1. / q[0], u = q[1] * oorho, v = q[2] * oorho;
-    pinf = (g - 1.) * (q[3] - 0.5 * q[0] * (u * u + v * v));
-    cinf = std::sqrt(std::fabs(g * pinf * oorho));
-  }
-  void from_mach(double M, double g, double a) {
-    const double ooggm1 = 1. / (g * (g - 1.));
-    const double qq[4] = {1, M * std::cos(a * M_PI / 180.), M * std::sin(a * M_PI / 180.), ooggm1 + 0.5 * M * M};
-    from_qinf(g, qq);
-    minf = M;
-    alpha = a;
-  }
-};
-
-// IVortex.GetStateC (isentropic_vortex/analytic_vortex.go:31-82), host copy for initialisation
-void ivortex_host(double t, double x, double y, double q[4]) {
-  const double beta = 5., X0 = 5., Y0 = 0., G = 1.4;
-  const double oo2pi = 0.5 * (1. / M_PI), GM1 = G - 1, OOGM1 = 1. / GM1, fac = 16 * G * (M_PI * M_PI);
-  double u = 1.0, v = 0.;
-  const double xm = x - u * t, ym = y - v * t;
-  const double r2 = (xm - X0) * (xm - X0) + (ym - Y0) * (ym - Y0);
-  const double ex1r = std::exp(1 - r2);
-  const double tv1 = 1.0 - (GM1 * beta * beta * std::exp(2.0 * (1.0 - r2)) / fac);
-  u -= beta * ex1r * (ym - Y0) * oo2pi;
-  v += beta * ex1r * (xm - X0) * oo2pi;
-  const double rho = std::pow(tv1, OOGM1), p = std::pow(rho, G);
-  q[0] = rho; q[1] = rho * u; q[2] = rho * v; q[3] = p * (1. / (G - 1.)) + 0.5 * rho * (u * u + v * v);
-}
-}  // namespace
-
-struct gocfd_mesh {
-  Mesh m;
-};
+using namespace gocfd::capi;
 
 struct gocfd_euler2d {
   gocfd_params prm{};
@@ -118,6 +45,9 @@ struct gocfd_euler2d {
   DevBuf<double> sh_ops, hK, Se, sigma, eps, epsF, diss, Nxo, Nyo, vooL;
   DevBuf<int> evert, nbr_elem, nbr_face;
   DevBuf<unsigned long long> sigV, epsV;
+  // a single solver is a batch of one case
+  DevBuf<dev::CaseInfo> cases;
+  DevBuf<int> tile_case;
   dev::Ctl *ctl = nullptr;
   long long gstage = 0;  // stage launches since the control block was initialised
   int step_index = 0;    // steps since the control block was initialised
@@ -161,53 +91,24 @@ struct gocfd_euler2d {
       a.sh.cdiff = 1. / (np1 * np1);
       a.sh.omega = 1. * double(d.N * d.N);
     }
-    a.ph.gamma = fs.gamma; a.ph.minf = fs.minf; a.ph.pinf = fs.pinf; a.ph.cinf = fs.cinf;
-    for (int i = 0; i < 4; ++i) a.ph.qinf[i] = fs.q[i];
-    a.ph.cfl = prm.cfl; a.ph.final_time = prm.final_time; a.ph.flux = prm.flux_type;
+    a.ph = phys_of(fs, prm);
+    a.cases = cases.p;
+    a.tile_case = tile_case.p;
     return a;
   }
 
   // reference layout [4][NpInt][K] (double) <-> element-major FP64 device
   // buffer (q0 and the RHS-evaluation input)
   void to_device(const double *Q, void *dst) {
-    const int K = L.K, NpInt = d.ops.NpInt;
-    const size_t tot = size_t(L.Kp) * NV;
-    std::vector<double> h(tot);
-    fill_host(Q, h.data(), K, NpInt);
-    HIPCHECK(hipMemcpy(dst, h.data(), tot * sizeof(double), hipMemcpyHostToDevice));
+    std::vector<double> h(size_t(L.Kp) * NV);
+    fill_element_major(L, d.ops.NpInt, Q, h.data());
+    HIPCHECK(hipMemcpy(dst, h.data(), h.size() * sizeof(double), hipMemcpyHostToDevice));
   }
-  template <class T>
-  void fill_host(const double *Q, T *h, int K, int NpInt) {
-    const double pad[4] = {1.0, 0.0, 0.0, 2.5};
-    for (int kk = 0; kk < L.Kp; ++kk) {
-      const int k = L.perm[kk];
-      for (int n = 0; n < 4; ++n)
-        for (int i = 0; i < NpInt; ++i)
-          h[size_t(kk) * NV + n * NpInt + i] =
-              T(k < 0 ? pad[n] : Q[size_t(n) * NpInt * K + size_t(i) * K + k]);
-    }
-  }
-  void from_device(const void *src, double *Q, bool is_double) {
-    const int K = L.K, NpInt = d.ops.NpInt;
-    const size_t tot = size_t(L.Kp) * NV;
-    if (is_double) {
-      std::vector<double> h(tot);
-      HIPCHECK(hipMemcpy(h.data(), src, tot * sizeof(double), hipMemcpyDeviceToHost));
-      read_host(h.data(), Q, K, NpInt);
-    } else {
-      std::vector<float> h(tot);
-      HIPCHECK(hipMemcpy(h.data(), src, tot * sizeof(float), hipMemcpyDeviceToHost));
-      read_host(h.data(), Q, K, NpInt);
-    }
-  }
-  template <class T>
-  void read_host(const T *h, double *Q, int K, int NpInt) {
-    for (int kk = 0; kk < L.K; ++kk) {
-      const int k = L.perm[kk];
-      for (int n = 0; n < 4; ++n)
-        for (int i = 0; i < NpInt; ++i)
-          Q[size_t(n) * NpInt * K + size_t(i) * K + k] = double(h[size_t(kk) * NV + n * NpInt + i]);
-    }
+  // element-major FP64 device buffer -> reference layout
+  void from_device(const void *src, double *Q) {
+    std::vector<double> h(size_t(L.Kp) * NV);
+    HIPCHECK(hipMemcpy(h.data(), src, h.size() * sizeof(double), hipMemcpyDeviceToHost));
+    read_element_major(L, d.ops.NpInt, h.data(), Q);
   }
 
   dev::Ctl read_ctl() {
@@ -278,17 +179,12 @@ struct gocfd_euler2d {
   }
 };
 
-#define GUARD_BEGIN try {
 #define NEED_DEVICE(h) \
   if ((h)->device < 0) return fail(GOCFD_ERR_DEVICE, "handle was created without a device");
-#define GUARD_END                                         \
-  }                                                       \
-  catch (const SetupError &e) { return fail(e.code, e.what()); } \
-  catch (const std::exception &e) { return fail(GOCFD_ERR_ARG, e.what()); }
 
 extern "C" {
 
-const char *gocfd_last_error(void) { return g_last_error.c_str(); }
+const char *gocfd_last_error(void) { return last_error(); }
 
 int gocfd_mesh_read(const char *path, gocfd_mesh **out) {
   GUARD_BEGIN
@@ -346,16 +242,7 @@ int gocfd_euler2d_create(const gocfd_mesh *mesh, const gocfd_params *p, gocfd_eu
   GUARD_BEGIN
   if (!mesh || !p || !out) return fail(GOCFD_ERR_ARG, "null argument");
   *out = nullptr;
-  if (p->polynomial_order < 0 || p->polynomial_order > 4)
-    return fail(GOCFD_ERR_ARG, "polynomial order must be 0..4 on the GPU path");
-  if (p->flux_type < 0 || p->flux_type > 3) return fail(GOCFD_ERR_ARG, "unable to use flux type");
-  if (p->init_type != GOCFD_INIT_FREESTREAM && p->init_type != GOCFD_INIT_IVORTEX)
-    return fail(GOCFD_ERR_ARG, "init type not supported on the GPU path (freestream, ivortex)");
-  if (p->limiter != 0 && p->limiter != 1) return fail(GOCFD_ERR_ARG, "unable to use limiter");
-  if (p->precision != GOCFD_PREC_FP64 && p->precision != GOCFD_PREC_FP32)
-    return fail(GOCFD_ERR_ARG, "unknown precision");
-  if (p->limiter == 1 && p->precision != GOCFD_PREC_FP64)
-    return fail(GOCFD_ERR_ARG, "the PerssonC0 limiter runs in FP64 only");
+  if (validate_params(p) != GOCFD_OK) return GOCFD_ERR_ARG;
   auto h = std::make_unique<gocfd_euler2d>();
   h->prm = *p;
   h->lts = p->local_time_stepping != 0;
@@ -367,24 +254,8 @@ int gocfd_euler2d_create(const gocfd_mesh *mesh, const gocfd_params *p, gocfd_eu
   h->prec = p->precision;
   h->ssize = h->prec == 0 ? sizeof(double) : sizeof(float);
   h->d = build_discretisation(mesh->m, p->polynomial_order);
-  // InitializeSolution (euler.go:728-794)
-  std::vector<double> Q0(size_t(4) * h->d.ops.NpInt * h->d.K);
-  const size_t npts = size_t(h->d.ops.NpInt) * h->d.K;
-  if (p->init_type == GOCFD_INIT_IVORTEX) {
-    const double qq[4] = {1, 1, 0, 3};
-    h->fs.from_qinf(1.4, qq);
-    for (auto &E : h->d.edges)
-      if (E.bc == BC_Wall) E.bc = BC_IVortex;
-    for (size_t i = 0; i < npts; ++i) {
-      double q[4];
-      ivortex_host(0, h->d.SolX[i], h->d.SolY[i], q);
-      for (int n = 0; n < 4; ++n) Q0[n * npts + i] = q[n];
-    }
-  } else {
-    h->fs.from_mach(p->minf, p->gamma, p->alpha);
-    for (int n = 0; n < 4; ++n)
-      for (size_t i = 0; i < npts; ++i) Q0[n * npts + i] = 0.0 + h->fs.q[n];
-  }
+  std::vector<double> Q0;
+  init_case(h->d, *p, h->fs, Q0);
   h->L = build_device_layout(h->d, dev::kTile);
   h->NV = 4 * h->d.ops.NpInt;
   if (p->device < 0) {  // host-only handle: setup and introspection, no device work
@@ -393,23 +264,8 @@ int gocfd_euler2d_create(const gocfd_mesh *mesh, const gocfd_params *p, gocfd_eu
     return GOCFD_OK;
   }
   // Device
-  int ndev = 0;
-  HIPCHECK(hipGetDeviceCount(&ndev));
-  if (p->device < 0 || p->device >= ndev) return fail(GOCFD_ERR_DEVICE, "no such HIP device");
+  check_device(p->device, dev::stage_lds_bytes(p->polynomial_order, h->prec, h->L.Hmax) + 4096);
   h->device = p->device;
-  HIPCHECK(hipSetDevice(h->device));
-  hipDeviceProp_t prop;
-  HIPCHECK(hipGetDeviceProperties(&prop, h->device));
-  if (std::strncmp(prop.gcnArchName, "gfx950", 6) != 0)
-    return fail(GOCFD_ERR_DEVICE, std::string("device is not gfx950: ") + prop.gcnArchName);
-  const size_t lds = dev::stage_lds_bytes(p->polynomial_order, h->prec, h->L.Hmax) + 4096;
-  int lds_max = 0, lds_optin = 0;
-  HIPCHECK(hipDeviceGetAttribute(&lds_max, hipDeviceAttributeMaxSharedMemoryPerBlock, h->device));
-  if (hipDeviceGetAttribute(&lds_optin, hipDeviceAttributeSharedMemPerBlockOptin, h->device) == hipSuccess)
-    lds_max = std::max(lds_max, lds_optin);
-  if (lds > size_t(lds_max))
-    return fail(GOCFD_ERR_SETUP, "stage kernel needs " + std::to_string(lds) + " B of LDS, device allows " +
-                                     std::to_string(lds_max));
   HIPCHECK(hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking));
   HIPCHECK(hipEventCreate(&h->ev0));
   HIPCHECK(hipEventCreate(&h->ev1));
@@ -430,6 +286,14 @@ int gocfd_euler2d_create(const gocfd_mesh *mesh, const gocfd_params *p, gocfd_eu
   h->halo_elem.upload(L.halo_elem); h->halo_face.upload(L.halo_face);
   h->wave_list.upload(L.wave_list); h->wave_cnt.upload(L.wave_cnt);
   h->flux_list.upload(L.flux_list); h->flux_cnt.upload(L.flux_cnt);
+  {
+    dev::CaseInfo ci{};
+    ci.ph = phys_of(h->fs, h->prm);
+    ci.k_end = L.K;
+    ci.tile0 = 0;
+    h->cases.upload(std::vector<dev::CaseInfo>{ci});
+    h->tile_case.upload(std::vector<int>(L.ntiles, 0));
+  }
   if (h->lts) {
     h->fw.alloc(size_t(2) * 3 * L.Kp);
     h->fwv.alloc(size_t(2) * 3 * L.Kp);
@@ -496,7 +360,7 @@ int gocfd_euler2d_get_state(gocfd_euler2d *h, double *Q) {
   NEED_DEVICE(h);
   HIPCHECK(hipSetDevice(h->device));
   HIPCHECK(hipStreamSynchronize(h->stream));
-  h->from_device(h->q[0], Q, true);
+  h->from_device(h->q[0], Q);
   return GOCFD_OK;
   GUARD_END
 }
@@ -507,7 +371,7 @@ int gocfd_euler2d_get_residual(gocfd_euler2d *h, double *R) {
   NEED_DEVICE(h);
   HIPCHECK(hipSetDevice(h->device));
   HIPCHECK(hipStreamSynchronize(h->stream));
-  h->from_device(h->R, R, true);
+  h->from_device(h->R, R);
   return GOCFD_OK;
   GUARD_END
 }
@@ -535,7 +399,7 @@ int gocfd_euler2d_rhs(gocfd_euler2d *h, const double *Q, double t, double *RHS) 
     HIPCHECK(dev::launch_fill_u64(h->epsV.p, h->epsV.n, 0ull, h->stream));
   }
   HIPCHECK(hipStreamSynchronize(h->stream));
-  h->from_device(h->rhs_out.p, RHS, true);
+  h->from_device(h->rhs_out.p, RHS);
   return GOCFD_OK;
   GUARD_END
 }

@@ -45,6 +45,15 @@ struct Phys {
   int flux;
 };
 
+// One case of a batch (a single solver is a batch of one): its physics
+// constants, the end of its element range and its first tile. Tiles never
+// straddle cases.
+struct CaseInfo {
+  Phys ph;
+  int k_end;   // one past the case's last element
+  int tile0;   // the case's first tile (does the case's control bookkeeping)
+};
+
 // Local time stepping (CalculateLocalDT, euler.go:973-1002)
 struct LtsArgs {
   int on;
@@ -85,10 +94,12 @@ struct StageArgs {
   void *qs;            // stage input state (element-major)
   void *q0, *q1, *q2, *q3, *q4, *R;
   double *rhs_out;     // RHS-only output (element-major, double)
-  Ctl *ctl;
+  Ctl *ctl;            // control blocks, one per case
+  const CaseInfo *cases;
+  const int *tile_case;  // [ntiles] case of each tile
   Geom g;
   const double *Interp64;       // FluxEdgeInterp in FP64
-  Phys ph;
+  Phys ph;             // case 0's constants (launch dispatch; shock-capturing passes)
   LtsArgs lts;
   ShockArgs sh;
 };
@@ -103,7 +114,7 @@ hipError_t launch_prologue(int N, int prec, const StageArgs &a, hipStream_t s);
 // Shock-capturing passes of one stage: 0 sensor + vertex max, 1 element
 // fields (+ stage-state filter), 2 gradient, viscosity and wave speeds.
 hipError_t launch_shock(int N, int pass, const StageArgs &a, hipStream_t s);
-hipError_t launch_ctl_init(Ctl *ctl, double time, hipStream_t s);
+hipError_t launch_ctl_init(Ctl *ctl, double time, hipStream_t s, int ncases = 1);
 hipError_t launch_fill_u64(unsigned long long *p, size_t n, unsigned long long v, hipStream_t s);
 hipError_t launch_fill_f64(double *p, size_t n, double v, hipStream_t s);
 // Copy the order-N operators (DivInt^T [NpFlux][NpInt], FluxEdgeInterp

@@ -51,7 +51,10 @@ size_t stage_lds_bytes(int N, int prec, int Hmax) {
   return cs * (4 * NpInt * (kTile + 1) + 4 * (3 * NpEdge * kTile + NpEdge * size_t(Hmax)));
 }
 
-__global__ void k_ctl_init(Ctl *ctl, double time) {
+__global__ void k_ctl_init(Ctl *ctls, double time, int ncases) {
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= ncases) return;
+  Ctl *ctl = ctls + c;
   ctl->time[0] = time;
   ctl->time[1] = time;
   for (int i = 0; i < 5; ++i) ctl->dt[i] = 0.0;
@@ -73,8 +76,8 @@ __global__ void k_fill_f64(double *p, size_t n, double v) {
     p[i] = v;
 }
 
-hipError_t launch_ctl_init(Ctl *ctl, double time, hipStream_t s) {
-  hipLaunchKernelGGL(k_ctl_init, dim3(1), dim3(1), 0, s, ctl, time);
+hipError_t launch_ctl_init(Ctl *ctl, double time, hipStream_t s, int ncases) {
+  hipLaunchKernelGGL(k_ctl_init, dim3((ncases + 63) / 64), dim3(64), 0, s, ctl, time, ncases);
   return hipGetLastError();
 }
 

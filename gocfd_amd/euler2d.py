@@ -111,6 +111,27 @@ class RungeKutta5SSP:
         return self._c.residual()
 
 
+def params_of(ip: InputParameters2D, precision: str = "fp64", device: int = 0) -> "_lib.Params":
+    """The C-ABI parameter block of an InputParameters2D; unknown names raise
+    as the reference panics (fluxes.go:18-33, initialization.go:18-31,
+    filter.go:15-23)."""
+    ft = ip.FluxType.lower()
+    if ft not in FLUX_TYPES:
+        raise ValueError(f"unable to use flux named {ft}")
+    it = ip.InitType.lower()
+    if not it:
+        raise ValueError(f"empty init type, must be one of {list(INIT_TYPES)}")
+    if it not in INIT_TYPES:
+        raise ValueError(f"unable to use init type named {it}")
+    lim = ip.Limiter.strip().lower()
+    if lim not in LIMITERS:
+        raise ValueError(f"unable to use limiter named [{lim}]")
+    return _lib.Params(ip.PolynomialOrder, FLUX_TYPES[ft], INIT_TYPES[it], ip.MaxIterations,
+                       ip.CFL, ip.FinalTime, ip.Minf, ip.Gamma, ip.Alpha,
+                       int(ip.LocalTimeStepping), LIMITERS[lim], ip.Kappa,
+                       PRECISIONS[precision], device)
+
+
 class Euler:
     """The device-resident 2D Euler DFR solver (reference: euler.go:31-136)."""
 
@@ -118,26 +139,13 @@ class Euler:
                  verbose: bool = False, profile: bool = False, precision: str = "fp64",
                  device: int = 0):
         L = _lib.load()
-        ft = ip.FluxType.lower()
-        if ft not in FLUX_TYPES:
-            raise ValueError(f"unable to use flux named {ft}")
+        p = params_of(ip, precision, device)
         it = ip.InitType.lower()
-        if not it:
-            raise ValueError(f"empty init type, must be one of {list(INIT_TYPES)}")
-        if it not in INIT_TYPES:
-            raise ValueError(f"unable to use init type named {it}")
-        lim = ip.Limiter.strip().lower()
-        if lim not in LIMITERS:
-            raise ValueError(f"unable to use limiter named [{lim}]")
         self.ip = ip
         self.MeshFile = meshFile if isinstance(meshFile, str) else "<arrays>"
         self.CFL, self.FinalTime, self.MaxIterations = ip.CFL, ip.FinalTime, ip.MaxIterations
         self.precision = precision
         mesh = Mesh.read(meshFile) if isinstance(meshFile, str) else meshFile
-        p = _lib.Params(ip.PolynomialOrder, FLUX_TYPES[ft], INIT_TYPES[it], ip.MaxIterations,
-                        ip.CFL, ip.FinalTime, ip.Minf, ip.Gamma, ip.Alpha,
-                        int(ip.LocalTimeStepping), LIMITERS[lim], ip.Kappa,
-                        PRECISIONS[precision], device)
         h = C.c_void_p()
         _lib.check(L.gocfd_euler2d_create(mesh._h, C.byref(p), C.byref(h)))
         self._h = h

@@ -153,6 +153,27 @@ int gocfd_euler2d_get_shock(gocfd_euler2d *h, double *Se, double *Sigma, double 
  * CalculateLocalDT, euler.go:973-1002) [K] */
 int gocfd_euler2d_get_element_dt(gocfd_euler2d *h, double *DT);
 
+/* ---- Batched ensembles (no counterpart in the reference, which runs one
+ *      case per process). ncases independent cases -- each its own mesh,
+ *      order, initial state, Mach number, angle, CFL and FinalTime -- stepped
+ *      together: cases of one order share a launch per stage, and runs of 6
+ *      steps replay a captured HIP graph. Every case needs the same
+ *      precision, flux type and device; no LocalTimeStepping or limiter. ---- */
+typedef struct gocfd_ensemble gocfd_ensemble;
+int gocfd_ensemble_create(int32_t ncases, const gocfd_mesh *const *meshes, const gocfd_params *params,
+                          gocfd_ensemble **out);
+void gocfd_ensemble_destroy(gocfd_ensemble *h);
+int gocfd_ensemble_case_dims(gocfd_ensemble *h, int32_t c, int32_t *K, int32_t *NpInt);
+/* nsteps SSP54 steps of every case (each case stops at its own FinalTime) */
+int gocfd_ensemble_step(gocfd_ensemble *h, int32_t nsteps);
+/* State of case c in the reference layout [4][NpInt][K] */
+int gocfd_ensemble_get_state(gocfd_ensemble *h, int32_t c, double *Q);
+/* RK.Time, RK.GlobalDT and RK.StepCount of case c */
+int gocfd_ensemble_time(gocfd_ensemble *h, int32_t c, double *time, double *global_dt, int32_t *steps);
+/* Wall time in ms of the last gocfd_ensemble_step call measured with HIP
+ * events on the ensemble's stream, and its number of steps */
+int gocfd_ensemble_kernel_time(gocfd_ensemble *h, double *ms, int32_t *steps);
+
 /* ---- Device-level figures for measurement (no reference counterpart) ---- */
 /* Average duration in ms of the last timed batch of stage kernels, measured
  * with HIP events on the handle's stream, and the number of launches. */

@@ -1,0 +1,84 @@
+"""GPU checks of batched ensembles: each case of an FP64 or FP32 ensemble is
+bit-identical to its own single-case run in the same precision (same
+tiling, same arithmetic; 25 steps cover the eager steps and four replays of
+the captured 6-step graph); FP32 cases stay within 1e-5 relative L2 of the
+FP64 oracle; cases stop at their own FinalTime."""
+import numpy as np
+import pytest
+
+import oracle
+from gocfd_amd import Ensemble, Euler, InputParameters2D, Mesh, meshgen
+
+pytestmark = pytest.mark.gpu
+
+
+def _mesh(n, seed):
+    VX, VY, EToV, tags = meshgen.jittered_quads(n, n, seed=seed)
+    return Mesh.from_arrays(VX, VY, EToV, tags)
+
+
+def _cases(tmp_path):
+    spec = [  # (quads per side, order, init, Minf, Alpha, flux CFL, FinalTime)
+        (10, 2, "IVortex", 0.5, 0.0, 1.0, 1e9),
+        (7, 2, "Freestream", 0.4, 3.0, 0.8, 1e9),
+        (12, 3, "IVortex", 0.5, 0.0, 1.0, 1e9),
+        (9, 1, "Freestream", 0.7, -4.0, 1.0, 1e9),
+        (6, 4, "Freestream", 0.3, 1.0, 0.9, 1e9),
+    ]
+    out = []
+    for i, (n, N, init, M, a, cfl, ft) in enumerate(spec):
+        path = str(tmp_path / f"case{i}.su2")
+        VX, VY, EToV, tags = meshgen.jittered_quads(n, n, seed=100 + i)
+        meshgen.write_su2(path, VX, VY, EToV, tags)
+        ip = InputParameters2D(CFL=cfl, FluxType="Roe", InitType=init, PolynomialOrder=N, Minf=M, Alpha=a,
+                               FinalTime=ft, MaxIterations=1000)
+        out.append((ip, path))
+    return out
+
+
+def test_ensemble_fp64_is_bitwise_the_single_case_run(tmp_path):
+    cases = _cases(tmp_path)
+    ens = Ensemble(cases, precision="fp64")
+    ens.step(25)
+    for c, (ip, path) in enumerate(cases):
+        single = Euler(ip, path, precision="fp64")
+        single.step(25)
+        assert np.array_equal(ens.state(c), single.Q), c
+        assert ens.time_info(c) == single.time_info(), c
+
+
+def test_ensemble_fp32_against_single_case_and_oracle(tmp_path):
+    """FP32 ensemble: bit-identical to each case's single-case FP32 run, and
+    within 1e-5 relative L2 of the FP64 oracle. (At N<=1 the FP32 path keeps
+    FP32 stage states but computes in FP64: its RT edge points sit 0.056
+    apart and |DivInt| reaches 1.2e3.)"""
+    cases = _cases(tmp_path)
+    ens = Ensemble(cases, precision="fp32")
+    ens.step(20)
+    for c, (ip, path) in enumerate(cases):
+        single = Euler(ip, path, precision="fp32")
+        single.step(20)
+        assert np.array_equal(ens.state(c), single.Q), c
+        o = oracle.OracleSolver(path, ip.PolynomialOrder, flux="roe", init=ip.InitType.lower(), CFL=ip.CFL,
+                                FinalTime=ip.FinalTime, Minf=ip.Minf, Alpha=ip.Alpha)
+        for _ in range(20):
+            o.step()
+        qo = o.state()
+        l2 = np.linalg.norm(ens.state(c)[0] - qo[0]) / np.linalg.norm(qo[0])
+        assert l2 < 1e-5, (c, l2)
+        t, dt, steps = ens.time_info(c)
+        assert steps == 20 and abs(t - o.time) < 1e-6 * o.time
+
+
+def test_ensemble_cases_stop_at_their_final_time(tmp_path):
+    cases = _cases(tmp_path)[:3]
+    cases[1][0].FinalTime = 0.02
+    ens = Ensemble(cases, precision="fp64")
+    ens.step(40)
+    t1, _, s1 = ens.time_info(1)
+    assert t1 == pytest.approx(0.02, rel=1e-14) and s1 < 40
+    t0, _, s0 = ens.time_info(0)
+    assert s0 == 40 and t0 > 0.02
+    single = Euler(cases[1][0], cases[1][1], precision="fp64")
+    single.step(40)
+    assert np.array_equal(ens.state(1), single.Q)
