@@ -48,7 +48,8 @@ size_t stage_lds_bytes(int N, int prec, int Hmax) {
   if (N < 0 || N > 4) return 0;
   const size_t cs = prec == 0 ? sizeof(double) : sizeof(float);
   const size_t NpInt = size_t(N + 1) * (N + 2) / 2, NpEdge = size_t(N) + 2;
-  return cs * (4 * NpInt * (kTile + 1) + 4 * (3 * NpEdge * kTile + NpEdge * size_t(Hmax)));
+  return (cs * (4 * NpInt * (kTile + 1) + 4 * (3 * NpEdge * kTile + NpEdge * size_t(Hmax))) + 15) / 16 * 16 +
+         sizeof(double) * kTile + sizeof(unsigned long long) * 3 * kTile;  // with the LTS buffers
 }
 
 __global__ void k_ctl_init(Ctl *ctls, double time, int ncases) {

@@ -112,6 +112,16 @@ __device__ __forceinline__ double point_speed(double g, const double q[4], doubl
   return oohK * (U + c);
 }
 
+// Dynamic LDS of the stage kernel: the tile state [NV][kTile+1] and the
+// trace/flux buffer [4][3 NpEdge kTile + NpEdge Hmax], rounded to 16 bytes;
+// local time stepping adds the element dt and face-speed buffers.
+template <int N>
+__host__ __device__ constexpr size_t lds_tile_bytes(size_t csize, int Hmax) {
+  return (csize * (size_t(Shape<N>::NV) * (kTile + 1) +
+                   4 * (size_t(Shape<N>::NF) * kTile + size_t(Shape<N>::NpEdge) * Hmax)) + 15) / 16 * 16;
+}
+constexpr size_t lds_lts_bytes = sizeof(double) * kTile + sizeof(unsigned long long) * 3 * kTile;
+
 // CheckIfFinished (euler.go:796-801): no further steps once Time >= FinalTime
 __device__ __forceinline__ bool step_skipped(const StageArgs &a, const Ctl *ctl, double final_time) {
   return a.stage >= 0 && a.step_index > 0 && ctl->time[a.step_index & 1] >= final_time;
@@ -151,9 +161,11 @@ k_stage(StageArgs a) {
   const int TS = NF * E + NpEdge * a.Hmax;
   C *sT = sQ + NV * P;                       // [4][TS]: traces, then fluxes
   C *sR = sT;                                // [NV][P]: RHS (aliases sT)
+  // local time stepping only: element dt and per-face speed maxima, after
+  // the tile buffers (the launch adds their bytes only when LTS is on)
+  double *sDT = reinterpret_cast<double *>(smem_raw + lds_tile_bytes<N>(sizeof(C), a.Hmax));
+  unsigned long long *sWf = reinterpret_cast<unsigned long long *>(sDT + kTile);
   __shared__ double red[kThreads / 64];
-  __shared__ double sDT[kTile];
-  __shared__ unsigned long long sWf[3 * kTile];
 
   const int tile = tile_of_block();
   if (tile >= a.ntiles) return;
@@ -176,6 +188,24 @@ k_stage(StageArgs a) {
   const C *DvT = ops_base<C>();                         // DivInt^T [NpFlux][NpInt]
   const C *Itp = DvT + Sh::NpFlux * NpInt;              // FluxEdgeInterp [NF][NpInt]
   const C g = C(ph.gamma);
+
+  // The tile's work lists do not depend on the state: their loads go out
+  // first, so that their latency overlaps the state load.
+  constexpr int ITB = (E * NF + kThreads - 1) / kThreads;
+  const int nfl = a.g.flux_cnt[tile] * NpEdge;
+  const int *flist = a.g.flux_list + size_t(tile) * a.Fmax;
+  int fpk[ITB];
+#pragma unroll
+  for (int j = 0; j < ITB; ++j) {
+    const int it = tid + j * kThreads;
+    fpk[j] = it < nfl ? flist[it / NpEdge] : 0;
+  }
+  int hk0 = -1, hf0 = 0;
+  if (tid < a.Hmax * 4) {
+    const int h = tid % a.Hmax;
+    hk0 = a.g.halo_elem[size_t(tile) * a.Hmax + h];
+    hf0 = a.g.halo_face[size_t(tile) * a.Hmax + h];
+  }
 
   // ---- A: tile state -> LDS (coalesced, contiguous chunk) ----
   // Every load is issued before the first LDS store, so a thread's LPT
@@ -214,9 +244,9 @@ k_stage(StageArgs a) {
   }
   for (int it = tid; it < a.Hmax * 4; it += kThreads) {
     const int h = it % a.Hmax, n = it / a.Hmax;
-    const int kh = a.g.halo_elem[size_t(tile) * a.Hmax + h];
+    const int kh = it == tid ? hk0 : a.g.halo_elem[size_t(tile) * a.Hmax + h];
     if (kh < 0) continue;
-    const int f = a.g.halo_face[size_t(tile) * a.Hmax + h];
+    const int f = it == tid ? hf0 : a.g.halo_face[size_t(tile) * a.Hmax + h];
     const SIN *qn = qs + size_t(kh) * NV + n * NpInt;
     C q[NpInt];
 #pragma unroll
@@ -241,16 +271,13 @@ k_stage(StageArgs a) {
   // Results stay in registers until every trace has been read. Boundary
   // faces follow in a rolled loop: a boundary face point reads and writes
   // only its own LDS slot.
-  constexpr int ITB = (E * NF + kThreads - 1) / kThreads;
   C fl[ITB][4];
-  const int nfl = a.g.flux_cnt[tile] * NpEdge;
-  const int *flist = a.g.flux_list + size_t(tile) * a.Fmax;
 #pragma unroll
   for (int j = 0; j < ITB; ++j) {
     const int it = tid + j * kThreads;
     if (it < nfl) {
       const int w = it / NpEdge, pp = it - w * NpEdge;
-      const int packed = flist[w];
+      const int packed = fpk[j];
       const int e = packed & 255, f = (packed >> 8) & 3, p = f * NpEdge + pp;
       const size_t fi = size_t(f) * Kp + k0 + e;
       const int code = a.g.code[fi];
@@ -302,7 +329,7 @@ k_stage(StageArgs a) {
     const int it = tid + j * kThreads;
     if (it < nfl) {
       const int w = it / NpEdge, pp = it - w * NpEdge;
-      const int packed = flist[w];
+      const int packed = fpk[j];
       const int e = packed & 255, f = (packed >> 8) & 3, p = f * NpEdge + pp;
       const size_t fi = size_t(f) * Kp + k0 + e;
       const C wn = C(a.g.IInII[fi]);
@@ -437,6 +464,20 @@ k_stage(StageArgs a) {
       out[idx] = double(sR[r * P + e]);
     }
     return;
+  }
+
+  // the owned-face list of phase G, fetched under phase E's loads
+  constexpr int ITG = (3 * E * NpEdge + kThreads - 1) / kThreads;
+  int wpk[ITG];
+  int nw = 0;
+  if constexpr (!SHOCK) {
+    nw = a.g.wave_cnt[tile] * NpEdge;
+    const int *wlist = a.g.wave_list + size_t(tile) * a.Wmax;
+#pragma unroll
+    for (int j = 0; j < ITG; ++j) {
+      const int it = tid + j * kThreads;
+      wpk[j] = it < nw ? wlist[it / NpEdge] : 0;
+    }
   }
 
   // ---- E: SSP54 stage combination (euler.go:511-565), FP64 arithmetic ----
@@ -589,13 +630,12 @@ k_stage(StageArgs a) {
     __syncthreads();
 
     // ---- G: owner-side wave speed of the next stage, max over the tile ----
-    {
-      const int nw = a.g.wave_cnt[tile] * NpEdge;
-      const int *wlist = a.g.wave_list + size_t(tile) * a.Wmax;
-#pragma unroll 1
-      for (int it = tid; it < nw; it += kThreads) {
+#pragma unroll
+    for (int j = 0; j < ITG; ++j) {
+      const int it = tid + j * kThreads;
+      if (it < nw) {
         const int w = it / NpEdge, pp = it - w * NpEdge;
-        const int packed = wlist[w];
+        const int packed = wpk[j];
         const int e = packed & 255, f = packed >> 8, p = f * NpEdge + pp;
         const int kk = k0 + e;
         const size_t fi = size_t(f) * Kp + kk;
@@ -945,14 +985,13 @@ __global__ void __launch_bounds__(kThreads) k_shock_grad(StageArgs a) {
 // ---- launchers ----
 
 template <int N>
-size_t lds_bytes_n(size_t csize, int Hmax) {
-  using Sh = Shape<N>;
-  return csize * (size_t(Sh::NV) * (kTile + 1) + 4 * (size_t(Sh::NF) * kTile + size_t(Sh::NpEdge) * Hmax));
+size_t lds_bytes_n(size_t csize, int Hmax, bool lts) {
+  return lds_tile_bytes<N>(csize, Hmax) + (lts ? lds_lts_bytes : 0);
 }
 
 template <int N, class S, class SIN, class C, int FLUX, bool LAST, bool SHOCK>
 hipError_t launch_stage_t(const StageArgs &a, hipStream_t s) {
-  const size_t lds = lds_bytes_n<N>(sizeof(C), a.Hmax);
+  const size_t lds = lds_bytes_n<N>(sizeof(C), a.Hmax, a.lts.on != 0);
   static size_t attr_bytes = 0;  // dynamic LDS the kernel is currently allowed
   if (lds > 64 * 1024 && lds > attr_bytes) {
     hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void *>(&k_stage<N, S, SIN, C, FLUX, LAST, SHOCK>),
