@@ -9,10 +9,14 @@
 namespace gocfd {
 namespace dev {
 
-// Division in the bulk arithmetic type: IEEE division in FP64 (the drop-in
-// path), the hardware reciprocal in FP32 (the FP32-bulk path).
+// Division and square root in the bulk arithmetic type: IEEE in FP64 (the
+// drop-in path); in FP32 (the FP32-bulk path) the hardware reciprocal and
+// square root (v_rcp_f32, v_sqrt_f32, about 1 ulp) instead of the
+// correctly rounded sequences.
 __device__ __forceinline__ double fdiv(double a, double b) { return a / b; }
-__device__ __forceinline__ float fdiv(float a, float b) { return __fdividef(a, b); }
+__device__ __forceinline__ float fdiv(float a, float b) { return a * __builtin_amdgcn_rcpf(b); }
+__device__ __forceinline__ double fsqrt(double x) { return sqrt(x); }
+__device__ __forceinline__ float fsqrt(float x) { return __builtin_amdgcn_sqrtf(x); }
 
 // GetFlowFunctionBase StaticPressure (model_problems/Euler2D/fluids.go:289-336)
 template <class C>
@@ -50,7 +54,7 @@ __device__ __forceinline__ void flux_component(int n, C g, C rho, C rhoU, C rhoV
 template <class C>
 __device__ __forceinline__ C sound_speed(C g, const C q[4]) {
   const C p = pressure(g, q[0], q[1], q[2], q[3]);
-  return sqrt(fabs(g * p * (fdiv(C(1), q[0]))));
+  return fsqrt(fabs(g * p * (fdiv(C(1), q[0]))));
 }
 
 // AvgFlux (fluxes.go:135-159)
@@ -69,7 +73,7 @@ __device__ __forceinline__ void lax_flux(C g, const C L[4], const C R[4], C nx, 
   const C uL = fdiv(L[1], L[0]), vL = fdiv(L[2], L[0]), uR = fdiv(R[1], R[0]), vR = fdiv(R[2], R[0]);
   const C pL = pressure(g, L[0], L[1], L[2], L[3]), pR = pressure(g, R[0], R[1], R[2], R[3]);
   const C CL = sound_speed(g, L), CR = sound_speed(g, R);
-  const C maxV = fmax(sqrt(uL * uL + vL * vL) + CL, sqrt(uR * uR + vR * vR) + CR);
+  const C maxV = fmax(fsqrt(uL * uL + vL * vL) + CL, fsqrt(uR * uR + vR * vR) + CR);
   F[0] = C(0.5) * (nx * (L[1] + R[1]) + ny * (L[2] + R[2]));
   F[1] = C(0.5) * (nx * (L[1] * uL + R[1] * uR + pL + pR) + ny * (L[1] * vL + R[1] * vR));
   F[2] = C(0.5) * (nx * (L[2] * uL + R[2] * uR) + ny * (L[2] * vL + R[2] * vR + pL + pR));
@@ -88,13 +92,13 @@ __device__ __forceinline__ void roe_flux(C g, const C L[4], const C R[4], C nx, 
   const C rhoR = R[0], uR = fdiv(rURr, R[0]), vR = fdiv(rVRr, R[0]);
   const C pL = pressure(g, L[0], L[1], L[2], L[3]), pR = pressure(g, R[0], R[1], R[2], R[3]);
   const C hL = fdiv(L[3] + pL, rhoL), hR = fdiv(R[3] + pR, rhoR);
-  const C sL = sqrt(rhoL), sR = sqrt(rhoR), sLR = sL + sR;
+  const C sL = fsqrt(rhoL), sR = fsqrt(rhoR), sLR = sL + sR;
   const C rho = sL * sR;
   const C u = fdiv(sL * uL + sR * uR, sLR);
   const C v = fdiv(sL * vL + sR * vR, sLR);
   const C h = fdiv(sL * hL + sR * hR, sLR);
   const C c2 = GM1 * (h - C(0.5) * (u * u + v * v));
-  const C cc = sqrt(c2);
+  const C cc = fsqrt(c2);
   C dW1 = fdiv(-C(0.5) * (rho * (uR - uL)), cc) + fdiv(C(0.5) * (pR - pL), c2);
   C dW2 = (rhoR - rhoL) - fdiv(pR - pL, c2);
   C dW3 = rho * (vR - vL);
@@ -123,7 +127,7 @@ __device__ __forceinline__ void roeer_flux(C g, const C L[4], const C R[4], C nx
   const C GM1 = g - C(1);
   const C rhoL = L[0], rhoR = R[0];
   const C ooL = fdiv(C(1), rhoL), ooR = fdiv(C(1), rhoR);
-  const C sL = sqrt(rhoL), sR = sqrt(rhoR);
+  const C sL = fsqrt(rhoL), sR = fsqrt(rhoR);
   const C uL = L[1] * ooL, vL = L[2] * ooL, uR = R[1] * ooR, vR = R[2] * ooR;
   const C EL = L[3], ER = R[3];
   const C UL = nx * uL + ny * vL, UR = nx * uR + ny * vR;
@@ -133,20 +137,20 @@ __device__ __forceinline__ void roeer_flux(C g, const C L[4], const C R[4], C nx
   const C u = (sL * uL + sR * uR) * ooRs, v = (sL * vL + sR * vR) * ooRs, h = (HL + HR) * ooRs;
   const C rho = sL * sR, H = h * rho, U = nx * u + ny * v;
   const C C2 = GM1 * (h - C(0.5) * (u * u + v * v));
-  const C cc = sqrt(C2), ooC = fdiv(C(1), cc), Uabs = fabs(U);
+  const C cc = fsqrt(C2), ooC = fdiv(C(1), cc), Uabs = fabs(U);
   F[0] = C(0.5) * (UL * rhoL + UR * rhoR);
   F[1] = C(0.5) * (UL * rhoL * uL + pL * nx + UR * rhoR * uR + pR * nx);
   F[2] = C(0.5) * (UL * rhoL * vL + pL * ny + UR * rhoR * vR + pR * ny);
   F[3] = C(0.5) * (UL * HL + UR * HR);
   const C Uef = C(0.05) * cc;
   const C du = uR - uL, dv = vR - vL, dV2 = du * du + dv * dv;
-  const C ooVmag = fdiv(C(1), sqrt(u * u + v * v));
+  const C ooVmag = fdiv(C(1), fsqrt(u * u + v * v));
   C n1x, n1y;
   if (dV2 < C(0.01) * C2) { n1x = nx; n1y = ny; } else { n1x = ooVmag * du; n1y = ooVmag * dv; }
   const C n2x = n1y * (nx * n1y - n1x * ny), n2y = -n1x * (nx * n1y - n1x * ny);
   const C alp1 = nx * n1x + ny * n1y, alp2 = nx * n2x + ny * n2y;
   const C U1x = n1x * u, U1y = n1y * v, U2x = n2x * u, U2y = n2y * v;
-  const C Urot = sqrt(alp1 * alp1 * (U1x * U1x + U1y * U1y)) + sqrt(alp2 * alp2 * (U2x * U2x + U2y * U2y));
+  const C Urot = fsqrt(alp1 * alp1 * (U1x * U1x + U1y * U1y)) + fsqrt(alp2 * alp2 * (U2x * U2x + U2y * U2y));
   const C sigma = fmax(Uabs, fmin(Uef, Urot));
   const C UabsP = Uabs - C(0.25) * fmax(C(0), UR - UL) * (copysign(C(1), U + cc) - copysign(C(1), U - cc));
   const C dU = UR - UL, dP = pR - pL, dRho = rhoR - rhoL, dRU = rhoR * uR - rhoL * uL;
