@@ -35,22 +35,22 @@ int validate_params(const gocfd_params *p) {
   if (p->polynomial_order < 0 || p->polynomial_order > 4)
     return fail(GOCFD_ERR_ARG, "polynomial order must be 0..4 on the GPU path");
   if (p->flux_type < 0 || p->flux_type > 3) return fail(GOCFD_ERR_ARG, "unable to use flux type");
-  if (p->init_type != GOCFD_INIT_FREESTREAM && p->init_type != GOCFD_INIT_IVORTEX)
-    return fail(GOCFD_ERR_ARG, "init type not supported on the GPU path (freestream, ivortex)");
+  if (p->init_type != GOCFD_INIT_FREESTREAM && p->init_type != GOCFD_INIT_IVORTEX &&
+      p->init_type != GOCFD_INIT_SHOCKTUBE)
+    return fail(GOCFD_ERR_ARG, "unknown init type");
   if (p->limiter != 0 && p->limiter != 1) return fail(GOCFD_ERR_ARG, "unable to use limiter");
   if (p->precision != GOCFD_PREC_FP64 && p->precision != GOCFD_PREC_FP32)
     return fail(GOCFD_ERR_ARG, "unknown precision");
-  if (p->limiter == 1 && p->precision != GOCFD_PREC_FP64)
-    return fail(GOCFD_ERR_ARG, "the PerssonC0 limiter runs in FP64 only");
   return GOCFD_OK;
 }
 
-void init_case(Discretisation &d, const gocfd_params &p, FreeStream &fs, std::vector<double> &Q0) {
+void init_case(Discretisation &d, const gocfd_params &p, CaseStreams &fs, std::vector<double> &Q0) {
   Q0.assign(size_t(4) * d.ops.NpInt * d.K, 0.0);
   const size_t npts = size_t(d.ops.NpInt) * d.K;
   if (p.init_type == GOCFD_INIT_IVORTEX) {
     const double qq[4] = {1, 1, 0, 3};
-    fs.from_qinf(1.4, qq);
+    fs.far.from_qinf(1.4, qq);
+    fs.in = fs.out = fs.far;
     for (auto &E : d.edges)
       if (E.bc == BC_Wall) E.bc = BC_IVortex;
     for (size_t i = 0; i < npts; ++i) {
@@ -58,10 +58,22 @@ void init_case(Discretisation &d, const gocfd_params &p, FreeStream &fs, std::ve
       ivortex_host(0, d.SolX[i], d.SolY[i], q);
       for (int n = 0; n < 4; ++n) Q0[n * npts + i] = q[n];
     }
+  } else if (p.init_type == GOCFD_INIT_SHOCKTUBE) {
+    // Sod's states left and right of x = 0.5, which are also the In and Out
+    // boundary states (euler.go:738-771)
+    const double g = 1.4, qin[4] = {1, 0, 0, 1 / (g - 1)}, qout[4] = {0.125, 0, 0, 0.1 / (g - 1)};
+    fs.far.from_mach(p.minf, p.gamma, p.alpha);
+    fs.in.from_qinf(g, qin);
+    fs.out.from_qinf(g, qout);
+    for (size_t i = 0; i < npts; ++i) {
+      const double *q = d.SolX[i] < 0.5 ? fs.in.q : fs.out.q;
+      for (int n = 0; n < 4; ++n) Q0[n * npts + i] = q[n];
+    }
   } else {
-    fs.from_mach(p.minf, p.gamma, p.alpha);
+    fs.far.from_mach(p.minf, p.gamma, p.alpha);
+    fs.in = fs.out = fs.far;
     for (int n = 0; n < 4; ++n)
-      for (size_t i = 0; i < npts; ++i) Q0[n * npts + i] = 0.0 + fs.q[n];
+      for (size_t i = 0; i < npts; ++i) Q0[n * npts + i] = 0.0 + fs.far.q[n];
   }
 }
 
@@ -83,10 +95,16 @@ void check_device(int device, size_t lds) {
                                           std::to_string(lds_max));
 }
 
-dev::Phys phys_of(const FreeStream &fs, const gocfd_params &p) {
+dev::Phys phys_of(const CaseStreams &fs, const gocfd_params &p) {
   dev::Phys ph{};
-  ph.gamma = fs.gamma; ph.minf = fs.minf; ph.pinf = fs.pinf; ph.cinf = fs.cinf;
-  for (int i = 0; i < 4; ++i) ph.qinf[i] = fs.q[i];
+  ph.gamma = fs.far.gamma; ph.minf = fs.far.minf; ph.pinf = fs.far.pinf; ph.cinf = fs.far.cinf;
+  for (int i = 0; i < 4; ++i) ph.qinf[i] = fs.far.q[i];
+  const FreeStream *io[2] = {&fs.in, &fs.out};
+  for (int j = 0; j < 2; ++j) {
+    dev::FarState &s = ph.io[j];
+    for (int i = 0; i < 4; ++i) s.q[i] = io[j]->q[i];
+    s.gamma = io[j]->gamma; s.minf = io[j]->minf; s.pinf = io[j]->pinf; s.cinf = io[j]->cinf;
+  }
   ph.cfl = p.cfl; ph.final_time = p.final_time; ph.flux = p.flux_type;
   return ph;
 }

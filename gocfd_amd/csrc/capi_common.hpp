@@ -77,19 +77,26 @@ struct FreeStream {
   }
 };
 
+// The far-field states of a case: FSFar (NewEuler, euler.go:70) and the
+// FSIn / FSOut of the shock tube (euler.go:742-743), which equal FSFar in
+// the other cases
+struct CaseStreams {
+  FreeStream far, in, out;
+};
+
 // Parameter checks shared by every solver kind; returns GOCFD_OK or an error
 int validate_params(const gocfd_params *p);
 
-// InitializeSolution (euler.go:728-794): the freestream, the initial state
+// InitializeSolution (euler.go:728-794): the freestreams, the initial state
 // Q0 [4][NpInt][K] in the reference layout, and the IVortex boundary
 // conversion of Wall edges (initialization.go:61-83).
-void init_case(Discretisation &d, const gocfd_params &p, FreeStream &fs, std::vector<double> &Q0);
+void init_case(Discretisation &d, const gocfd_params &p, CaseStreams &fs, std::vector<double> &Q0);
 
 // Device checks: gfx950 and enough LDS for the stage kernel
 void check_device(int device, size_t lds_bytes);
 
 // Physics constants of a case for the kernels
-dev::Phys phys_of(const FreeStream &fs, const gocfd_params &p);
+dev::Phys phys_of(const CaseStreams &fs, const gocfd_params &p);
 
 // Element-major FP64 device layout of a reference-layout state, padding
 // elements set to the inert state

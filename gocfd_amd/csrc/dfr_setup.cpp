@@ -820,7 +820,10 @@ DeviceLayout build_device_layout(const Discretisation &d, int E) {
         code |= FC_FLUX_OWNER;
         switch (Ed.bc) {
           case BC_None: code |= FK_Open; break;
-          case BC_Far: case BC_In: case BC_Out: code |= FK_Far; break;
+          // bits 8.. select the far state: FSFar, FSIn, FSOut
+          case BC_Far: code |= FK_Far; break;
+          case BC_In: code |= FK_Far | (1 << 8); break;
+          case BC_Out: code |= FK_Far | (2 << 8); break;
           case BC_IVortex: {
             code |= FK_IVortex;
             code |= (L.nbfaces << 8);

@@ -82,7 +82,7 @@ Discretisation build_discretisation(const Mesh &mesh, int N);
 enum FaceKind : int {
   FK_Shared = 0,   // flux against a neighbour trace (interior or periodic)
   FK_Open = 1,     // unconnected, no BC tag: physical flux of the own trace
-  FK_Far = 2,      // Riemann invariant far field (also In/Out)
+  FK_Far = 2,      // Riemann invariant far field; code >> 8: 0 Far, 1 In, 2 Out
   FK_IVortex = 3,  // analytic isentropic vortex + Riemann BC
   FK_Wall = 4,     // pressure-only wall flux (Wall, Cyl)
   FK_Inert = 5     // padding element face

@@ -33,7 +33,7 @@ struct EnsCase {
   gocfd_params prm{};
   Discretisation d;
   DeviceLayout L;
-  FreeStream fs;
+  CaseStreams fs;
   std::vector<double> Q0;
   int group = 0, off = 0, boff = 0;
 };

@@ -22,7 +22,7 @@ struct gocfd_euler2d {
   gocfd_params prm{};
   Discretisation d;
   DeviceLayout L;
-  FreeStream fs;
+  CaseStreams fs;
   int device = 0;
   int prec = 0;
   size_t ssize = 8;  // bytes per stored stage-state value (q1..q4); q0 and R are FP64
@@ -43,6 +43,7 @@ struct gocfd_euler2d {
   bool shock = false;
   double kappa = 0, eps0 = 0;
   DevBuf<double> sh_ops, hK, Se, sigma, eps, epsF, diss, Nxo, Nyo, vooL;
+  DevBuf<float> sh_opsF;
   DevBuf<int> evert, nbr_elem, nbr_face;
   DevBuf<unsigned long long> sigV, epsV;
   // a single solver is a batch of one case
@@ -81,7 +82,7 @@ struct gocfd_euler2d {
     a.lts.on = lts ? 1 : 0;
     a.lts.fw = fw.p; a.lts.fwv = fwv.p; a.lts.wsrc = wsrc.p; a.lts.DTk = DTk.p; a.lts.DTv = DTv.p;
     if (shock) {
-      a.sh.ops = sh_ops.p; a.sh.evert = evert.p; a.sh.hK = hK.p;
+      a.sh.ops = sh_ops.p; a.sh.opsF = sh_opsF.p; a.sh.evert = evert.p; a.sh.hK = hK.p;
       a.sh.sigV = sigV.p; a.sh.epsV = epsV.p; a.sh.Nv = d.Nv;
       a.sh.Se = Se.p; a.sh.sigma = sigma.p; a.sh.eps = eps.p; a.sh.epsF = epsF.p; a.sh.diss = diss.p;
       a.sh.nbr_elem = nbr_elem.p; a.sh.nbr_face = nbr_face.p;
@@ -144,7 +145,7 @@ struct gocfd_euler2d {
     if (shock) {
       a.sh.vbuf = int(gstage & 1);
       a.sh.filter_state = a.stage == 2 ? 1 : 0;
-      for (int pass = 0; pass < 3; ++pass) HIPCHECK(dev::launch_shock(N, pass, a, stream));
+      for (int pass = 0; pass < 3; ++pass) HIPCHECK(dev::launch_shock(N, prec, pass, a, stream));
     }
     HIPCHECK(dev::launch_stage(N, prec, shock ? 1 : 0, a, stream));
   }
@@ -303,7 +304,9 @@ int gocfd_euler2d_create(const gocfd_mesh *mesh, const gocfd_params *p, gocfd_eu
   }
   if (h->shock) {
     const int NR = h->d.ops.NpInt + 3 * h->d.ops.NpEdge;
-    h->sh_ops.upload(build_shock_operators(h->d.ops));
+    const std::vector<double> sops = build_shock_operators(h->d.ops);
+    h->sh_ops.upload(sops);
+    h->sh_opsF.upload(std::vector<float>(sops.begin(), sops.end()));
     h->evert.upload(L.evert); h->hK.upload(L.hK);
     h->nbr_elem.upload(L.nbr_elem); h->nbr_face.upload(L.nbr_face);
     h->Nxo.upload(L.Nxo); h->Nyo.upload(L.Nyo); h->vooL.upload(L.vooL);
@@ -311,7 +314,7 @@ int gocfd_euler2d_create(const gocfd_mesh *mesh, const gocfd_params *p, gocfd_eu
     h->epsV.alloc(size_t(2) * std::max(h->d.Nv, 1));
     h->Se.alloc(L.Kp); h->sigma.alloc(L.Kp); h->eps.alloc(L.Kp);
     h->epsF.alloc(size_t(L.Kp) * NR);
-    h->diss.alloc(size_t(L.Kp) * 4 * 2 * NR);
+    h->diss.alloc(size_t(L.Kp) * 4 * 2 * NR);  // FP64 or FP32 values (the larger size)
   }
   std::vector<double> divT(h->d.ops.DivInt.size());  // [NpFlux][NpInt] for contiguous per-DOF rows
   for (int i = 0; i < h->d.ops.NpInt; ++i)

@@ -38,10 +38,18 @@ struct Geom {
   const double *bxy;                                   // IVortex boundary points
 };
 
-struct Phys {
+// A far-field state of the Riemann-invariant boundary (FreeStream,
+// fluids.go:245-273)
+struct FarState {
+  double q[4];
   double gamma, minf, pinf, cinf;
+};
+
+struct Phys {
+  double gamma, minf, pinf, cinf;  // FSFar
   double qinf[4];
   double cfl, final_time;
+  FarState io[2];                  // FSIn, FSOut (In / Out boundary edges)
   int flux;
 };
 
@@ -65,18 +73,21 @@ struct LtsArgs {
 };
 
 // Shock capturing (PerssonC0): per-order operators and per-element fields.
-// Operator block (double): Pm, Mass, Dm, V, Vinv [NpInt^2] each, mf [NpInt],
+// Operator block: Pm, Mass, Dm, V, Vinv [NpInt^2] each, mf [NpInt],
 // DivR [NR][NpFlux] (rows 0..NpInt-1 and 2NpInt.. of Div), Bary [NR][3],
-// with NR = NpInt + 3 NpEdge.
+// with NR = NpInt + 3 NpEdge; in FP64 (ops) and FP32 (opsF).
+// The dissipation eps*grad Q (diss) is stored in the stage kernel's
+// arithmetic type: FP64, or FP32 for the FP32-bulk precision from N=2 on.
 struct ShockArgs {
   const double *ops;
+  const float *opsF;
   const int *evert;                  // [3][Kp] vertex ids (original numbering)
   const double *hK;                  // [Kp] EdgeLenMax / (N+1)^2
   unsigned long long *sigV, *epsV;   // [2][Nv] vertex maxima (IEEE bits)
   int Nv, vbuf;                      // vertex buffer used by this stage
   double *Se, *sigma, *eps;          // [Kp]
   double *epsF;                      // [NR][Kp] viscosity at the used RT points
-  double *diss;                      // [4][2][NR][Kp] eps * grad at the used RT points
+  void *diss;                        // [4][2][NR][Kp] eps * grad at the used RT points
   const int *nbr_elem, *nbr_face;    // [3][Kp]
   const double *Nxo, *Nyo;           // [3][Kp] own outward normals
   const double *vooL;                // [3][Kp] 1 / owner edge length (shared faces)
@@ -108,12 +119,12 @@ struct StageArgs {
 // the stage states q1..q4 and FP32 bulk arithmetic (fluxes, divergence),
 // with FP64 geometry, dt reduction, SSP combination and FP64 storage of the
 // accumulated state q0 and the stage-3 RHS R.
-// shock != 0 selects the PerssonC0 stage kernel (FP64 only).
+// shock != 0 selects the PerssonC0 stage kernel.
 hipError_t launch_stage(int N, int prec, int shock, const StageArgs &a, hipStream_t s);
 hipError_t launch_prologue(int N, int prec, const StageArgs &a, hipStream_t s);
 // Shock-capturing passes of one stage: 0 sensor + vertex max, 1 element
 // fields (+ stage-state filter), 2 gradient, viscosity and wave speeds.
-hipError_t launch_shock(int N, int pass, const StageArgs &a, hipStream_t s);
+hipError_t launch_shock(int N, int prec, int pass, const StageArgs &a, hipStream_t s);
 hipError_t launch_ctl_init(Ctl *ctl, double time, hipStream_t s, int ncases = 1);
 hipError_t launch_fill_u64(unsigned long long *p, size_t n, unsigned long long v, hipStream_t s);
 hipError_t launch_fill_f64(double *p, size_t n, double v, hipStream_t s);
@@ -125,7 +136,7 @@ size_t stage_lds_bytes(int N, int prec, int Hmax);
 // Per-order entry points (one translation unit per order)
 template <int N> hipError_t launch_stage_n(int prec, int shock, const StageArgs &a, hipStream_t s);
 template <int N> hipError_t launch_prologue_n(int prec, const StageArgs &a, hipStream_t s);
-template <int N> hipError_t launch_shock_n(int pass, const StageArgs &a, hipStream_t s);
+template <int N> hipError_t launch_shock_n(int prec, int pass, const StageArgs &a, hipStream_t s);
 template <int N> hipError_t upload_operators_n(const double *divT, const double *interp);
 
 }  // namespace dev

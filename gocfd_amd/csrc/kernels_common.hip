@@ -30,8 +30,8 @@ hipError_t launch_prologue(int N, int prec, const StageArgs &a, hipStream_t s) {
 #undef CALL
 }
 
-hipError_t launch_shock(int N, int pass, const StageArgs &a, hipStream_t s) {
-#define CALL(n) launch_shock_n<n>(pass, a, s)
+hipError_t launch_shock(int N, int prec, int pass, const StageArgs &a, hipStream_t s) {
+#define CALL(n) launch_shock_n<n>(prec, pass, a, s)
   GOCFD_DISPATCH_ORDER(N, CALL)
 #undef CALL
 }

@@ -33,6 +33,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <type_traits>
 #include <vector>
 
 #include "dfr_setup.hpp"
@@ -55,7 +56,24 @@ struct Shape {
                        oVi = 4 * NpInt * NpInt, omf = 5 * NpInt * NpInt, oDiv = omf + NpInt,
                        oBary = oDiv + NR * NpFlux, shock_ops = oBary + 3 * NR;
   static constexpr int ops = (NpFlux + NF) * NpInt;
+  // MFMA form of the DivInt contraction (16x16x4 FP32 tiles): the interior
+  // DOFs in G groups of 4 points (one K-step for the r and one for the s
+  // components of a group), then the face DOFs in KE steps of 4
+  static constexpr int G = (NpInt + 3) / 4, KE = (NF + 3) / 4, KS = 2 * G + KE;
+  // MFMA form of the FluxEdgeInterp contraction: RB row blocks of 16 face
+  // points, KI K-steps of 4 solution points
+  static constexpr int RB = (NF + 15) / 16, KI = (NpInt + 3) / 4;
 };
+
+// The DivInt and FluxEdgeInterp contractions run on the matrix cores for the
+// FP32 arithmetic from N=3 on (NpInt = 10, 15 rows of a 16-row tile) without
+// shock capturing (whose RHS filter needs a whole element column in one
+// thread).
+template <int N, class C, bool SHOCK>
+struct UseMfma {
+  static constexpr bool value = std::is_same<C, float>::value && N >= 3 && !SHOCK;
+};
+typedef float floatx4 __attribute__((ext_vector_type(4)));
 
 // The operators of this translation unit's order in constant memory, so that
 // every coefficient is a uniform scalar load: DivInt^T [NpFlux][NpInt]
@@ -63,6 +81,51 @@ struct Shape {
 constexpr int kOpsMax = ((4 + 2) * (4 + 4) + 3 * (4 + 2)) * 15;
 static __constant__ double cOpsD[kOpsMax];
 static __constant__ float cOpsF[kOpsMax];
+// DivInt as the A operand of the MFMA contraction: K-step s, lane l holds
+// DivInt[row l%16][DOF of K-row 4s + l/16] (zero outside the matrix)
+constexpr int kMfMax = 13 * 64;
+static __constant__ float cMfA[kMfMax];
+// FluxEdgeInterp as the A operand: row block rb, K-step s, lane l holds
+// Interp[16 rb + l%16][4 s + l/16]
+constexpr int kMfIMax = 8 * 64;
+static __constant__ float cMfI[kMfIMax];
+
+// Face traces sT[var][p][e] = sum_i FluxEdgeInterp[p][i] sQ[var][i][e] of
+// variable tn (= the wave) for the tile's 64 elements on the matrix cores:
+// four 16-element column blocks, lane (col, krow) supplies the solution
+// point 4s + krow of element 16b + col as the B operand of K-step s.
+template <int N>
+__device__ __forceinline__ void traces_mfma(const float *sQ, float *sT, int tn, int te, int TS) {
+  using Sh = Shape<N>;
+  constexpr int NpInt = Sh::NpInt, NF = Sh::NF, RB = Sh::RB, KI = Sh::KI, E = kTile, P = kTile + 1;
+  const int col = te & 15, krow = te >> 4;
+  float ai[RB][KI];
+#pragma unroll
+  for (int rb = 0; rb < RB; ++rb)
+#pragma unroll
+    for (int s = 0; s < KI; ++s) ai[rb][s] = cMfI[(rb * KI + s) * 64 + te];
+#pragma unroll
+  for (int b = 0; b < 4; ++b) {
+    const int e = 16 * b + col;
+    floatx4 acc[RB];
+#pragma unroll
+    for (int rb = 0; rb < RB; ++rb) acc[rb] = floatx4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int s = 0; s < KI; ++s) {
+      // past NpInt the A column is zero: any finite point value will do
+      const float qv = sQ[(tn * NpInt + min(4 * s + krow, NpInt - 1)) * P + e];
+#pragma unroll
+      for (int rb = 0; rb < RB; ++rb) acc[rb] = __builtin_amdgcn_mfma_f32_16x16x4f32(ai[rb][s], qv, acc[rb], 0, 0, 0);
+    }
+#pragma unroll
+    for (int rb = 0; rb < RB; ++rb)
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const int p = 16 * rb + 4 * krow + v;
+        if (p < NF) sT[tn * TS + p * E + e] = acc[rb][v];
+      }
+  }
+}
 template <class C>
 __device__ __forceinline__ const C *ops_base();
 template <>
@@ -95,13 +158,20 @@ __device__ __forceinline__ void boundary_state(int kind, const Phys &ph, const d
                                                double nx, double ny, double qb[4]) {
   double qi[4] = {double(own[0]), double(own[1]), double(own[2]), double(own[3])};
   double qf[4];
+  double gam = ph.gamma, minf = ph.minf, pinf = ph.pinf, cinf = ph.cinf;
   if (kind == FK_Far) {
-    qf[0] = ph.qinf[0]; qf[1] = ph.qinf[1]; qf[2] = ph.qinf[2]; qf[3] = ph.qinf[3];
+    // bidx selects FSFar (0), FSIn (1) or FSOut (2) (edges.go:425-431)
+    const double *qs = ph.qinf;
+    if (bidx > 0) {
+      const FarState &fs = ph.io[bidx - 1];
+      qs = fs.q; gam = fs.gamma; minf = fs.minf; pinf = fs.pinf; cinf = fs.cinf;
+    }
+    qf[0] = qs[0]; qf[1] = qs[1]; qf[2] = qs[2]; qf[3] = qs[3];
   } else {
     const double *xy = bxy + 2 * (size_t(bidx) * NpEdge + pp);
     ivortex_state(t, xy[0], xy[1], qf);
   }
-  riemann_bc(ph.gamma, ph.minf, ph.pinf, ph.cinf, qi, qf, nx, ny, qb);
+  riemann_bc(gam, minf, pinf, cinf, qi, qf, nx, ny, qb);
 }
 
 // Wave speed (|U| + c) / hK of a face point (StoreEdgeAggregates, edges.go:264-272)
@@ -230,7 +300,9 @@ k_stage(StageArgs a) {
   }
   __syncthreads();
   const int te = tid % E, tn = tid / E;  // (element, variable) of this thread
-  {
+  if constexpr (UseMfma<N, C, SHOCK>::value) {
+    traces_mfma<N>(sQ, sT, tn, te, TS);
+  } else {
     C q[NpInt];
 #pragma unroll
     for (int i = 0; i < NpInt; ++i) q[i] = sQ[(tn * NpInt + i) * P + te];
@@ -305,18 +377,18 @@ k_stage(StageArgs a) {
         const int rowL = NpInt + (owner ? f * NpEdge + pp : nf * NpEdge + rp);
         const int rowR = NpInt + (owner ? nf * NpEdge + rp : f * NpEdge + pp);
         const double lam = 0.5 * (a.sh.epsF[size_t(rowL) * Kp + kL] + a.sh.epsF[size_t(rowR) * Kp + kR]);
-        const double coef = (a.sh.omega * lam) * a.sh.vooL[fi];
+        const C coef = C((a.sh.omega * lam) * a.sh.vooL[fi]);
+        const C *diss = static_cast<const C *>(a.sh.diss);
 #pragma unroll
         for (int c = 0; c < 4; ++c) {
           // owner traces at owner points jo and NpEdge-1-jo
-          const double EQj = owner ? double(own[c]) : double(oth[c]);
-          const double EQr = owner ? double(sT[c * TS + (f * NpEdge + rp) * E + e])
-                                   : double(sT[c * TS + base + pp * stride]);
-          const double *dX = a.sh.diss + size_t(c * 2 * NR) * Kp, *dY = dX + size_t(NR) * Kp;
-          const double vFL = double(nx) * dX[size_t(rowL) * Kp + kL] + double(ny) * dY[size_t(rowL) * Kp + kL];
-          const double vFR = double(nx) * dX[size_t(rowR) * Kp + kR] + double(ny) * dY[size_t(rowR) * Kp + kR];
-          const double VF = 0.5 * (vFL + vFR) - coef * (EQj - EQr);
-          F[c] = C(double(F[c]) - VF);
+          const C EQj = owner ? own[c] : oth[c];
+          const C EQr = owner ? sT[c * TS + (f * NpEdge + rp) * E + e] : sT[c * TS + base + pp * stride];
+          const C *dX = diss + size_t(c * 2 * NR) * Kp, *dY = dX + size_t(NR) * Kp;
+          const C vFL = nx * dX[size_t(rowL) * Kp + kL] + ny * dY[size_t(rowL) * Kp + kL];
+          const C vFR = nx * dX[size_t(rowR) * Kp + kR] + ny * dY[size_t(rowR) * Kp + kR];
+          const C VF = C(0.5) * (vFL + vFR) - coef * (EQj - EQr);
+          F[c] = F[c] - VF;
         }
       }
 #pragma unroll
@@ -387,8 +459,8 @@ k_stage(StageArgs a) {
       const int row = NpInt + f * NpEdge + pp;
 #pragma unroll
       for (int c = 0; c < 4; ++c) {
-        const double *dX = a.sh.diss + size_t(c * 2 * NR + row) * Kp + k0 + e;
-        F[c] = C(double(F[c]) - (nxd * dX[0] + nyd * dX[size_t(NR) * Kp]));
+        const C *dX = static_cast<const C *>(a.sh.diss) + size_t(c * 2 * NR + row) * Kp + k0 + e;
+        F[c] = F[c] - (nx * dX[0] + ny * dX[size_t(NR) * Kp]);
       }
     }
     const C w = C(a.g.IInII[fi]);
@@ -397,8 +469,56 @@ k_stage(StageArgs a) {
   }
   __syncthreads();
 
-  // ---- C: interior flux + divergence, thread (element te, variable tn) ----
-  {
+  // ---- C: interior flux + divergence ----
+  if constexpr (UseMfma<N, C, SHOCK>::value) {
+    // Matrix cores: wave tn forms variable tn of the tile's 64 elements in
+    // four 16-element column blocks; lane (col, krow) evaluates the interior
+    // flux at the points 4g + krow of element 16b + col and feeds the r and
+    // s components as the B operands of two K-steps, then the face fluxes.
+    static_assert(E == 64, "one wave per variable");
+    const int col = te & 15, krow = te >> 4;
+    float af[Sh::KS];
+#pragma unroll
+    for (int s2 = 0; s2 < Sh::KS; ++s2) af[s2] = cMfA[s2 * 64 + te];
+    floatx4 out[4];
+#pragma unroll
+    for (int b = 0; b < 4; ++b) {
+      const int e = 16 * b + col, kk = k0 + e;
+      const double Jd = a.g.Jdet[kk];
+      const C Jdc = C(Jd);
+      const C J0 = C(a.g.Jinv[kk]), J1 = C(a.g.Jinv[size_t(Kp) + kk]);
+      const C J2 = C(a.g.Jinv[2 * size_t(Kp) + kk]), J3 = C(a.g.Jinv[3 * size_t(Kp) + kk]);
+      floatx4 acc = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+      for (int g2 = 0; g2 < Sh::G; ++g2) {
+        // past NpInt the A column is zero: any finite point value will do
+        const int j = min(4 * g2 + krow, NpInt - 1);
+        const C r0 = sQ[(0 * NpInt + j) * P + e], r1 = sQ[(1 * NpInt + j) * P + e];
+        const C r2 = sQ[(2 * NpInt + j) * P + e], r3 = sQ[(3 * NpInt + j) * P + e];
+        C Fx, Fy;
+        flux_component(tn, g, r0, r1, r2, r3, Fx, Fy);
+        const C Fr = Jdc * (J0 * Fx + J1 * Fy), Fs = Jdc * (J2 * Fx + J3 * Fy);
+        acc = __builtin_amdgcn_mfma_f32_16x16x4f32(af[2 * g2], Fr, acc, 0, 0, 0);
+        acc = __builtin_amdgcn_mfma_f32_16x16x4f32(af[2 * g2 + 1], Fs, acc, 0, 0, 0);
+      }
+#pragma unroll
+      for (int s2 = 0; s2 < Sh::KE; ++s2) {
+        const int p = min(4 * s2 + krow, NF - 1);
+        acc = __builtin_amdgcn_mfma_f32_16x16x4f32(af[2 * Sh::G + s2], sT[tn * TS + p * E + e], acc, 0, 0, 0);
+      }
+      out[b] = acc * (-C(1. / Jd));
+    }
+    __syncthreads();  // every wave has read its fluxes; sR may now overwrite sT
+#pragma unroll
+    for (int b = 0; b < 4; ++b) {
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const int i = 4 * krow + v;
+        if (i < NpInt) sR[(tn * NpInt + i) * P + 16 * b + col] = out[b][v];
+      }
+    }
+  } else {
+    // VALU: thread (element te, variable tn)
     const int kk = k0 + te;
     const double Jd = a.g.Jdet[kk];
     const C Jdc = C(Jd);
@@ -414,7 +534,7 @@ k_stage(StageArgs a) {
       C Fx, Fy;
       flux_component(tn, g, r0, r1, r2, r3, Fx, Fy);
       if constexpr (SHOCK) {
-        const double *dX = a.sh.diss + size_t(tn * 2 * NR + j) * Kp + kk;
+        const C *dX = static_cast<const C *>(a.sh.diss) + size_t(tn * 2 * NR + j) * Kp + kk;
         Fx -= dX[0];
         Fy -= dX[size_t(NR) * Kp];
       }
@@ -615,7 +735,9 @@ k_stage(StageArgs a) {
     __syncthreads();
 
     // ---- F: traces of the new state (own faces) ----
-    {
+    if constexpr (UseMfma<N, C, SHOCK>::value) {
+      traces_mfma<N>(sQ, sT, tn, te, TS);
+    } else {
       C q[NpInt];
 #pragma unroll
       for (int i = 0; i < NpInt; ++i) q[i] = sQ[(tn * NpInt + i) * P + te];
@@ -732,7 +854,16 @@ __global__ void __launch_bounds__(256) k_prologue(StageArgs a) {
   if (bad) atomicOr(&ctl->nan, 1);
 }
 
-// ---- Shock capturing (FP64) ----
+// ---- Shock capturing ----
+// SIN: storage of the stage state the passes read; the sensor, the element
+// fields and the state filter compute in FP64; the gradient pass computes
+// and stores in D, the stage kernel's arithmetic type.
+template <class D>
+__device__ __forceinline__ const D *shock_ops(const ShockArgs &s);
+template <>
+__device__ __forceinline__ const double *shock_ops<double>(const ShockArgs &s) { return s.ops; }
+template <>
+__device__ __forceinline__ const float *shock_ops<float>(const ShockArgs &s) { return s.opsF; }
 
 // Pass 0: Persson sensor on the density (UpdateSeMoment,
 // DG2D/dfr_shock_capturing.go:142-166), sigma (UpdateShockFinderSigma,
@@ -740,14 +871,14 @@ __global__ void __launch_bounds__(256) k_prologue(StageArgs a) {
 // (CalculateElementViscosity :397-412), and the max-merge of both to the
 // vertices (MergeElementScalarToVertices euler.go:1048-1065) by atomics on
 // the IEEE bits of the non-negative values. One thread per element.
-template <int N>
+template <int N, class SIN>
 __global__ void __launch_bounds__(256) k_shock_sensor(StageArgs a) {
   using Sh = Shape<N>;
   constexpr int NpInt = Sh::NpInt, NV = Sh::NV;
   if (step_skipped(a)) return;
   const int kk = blockIdx.x * blockDim.x + threadIdx.x;
   if (kk >= a.K) return;
-  const double *rho = static_cast<const double *>(a.qs) + size_t(kk) * NV;
+  const SIN *rho = static_cast<const SIN *>(a.qs) + size_t(kk) * NV;
   const double *Pm = a.sh.ops + Sh::oP, *Mm = a.sh.ops + Sh::oM, *Dm = a.sh.ops + Sh::oD;
   double r[NpInt];
 #pragma unroll
@@ -790,7 +921,7 @@ __global__ void __launch_bounds__(256) k_shock_sensor(StageArgs a) {
 // (InterpolateEpsilonSigma dissipation.go:219-242), the modal filter of the
 // stage state at rkStep 2 (euler.go:603-608), and clearing of the other
 // vertex buffer for the next stage. One thread per element (padded range).
-template <int N>
+template <int N, class SIN>
 __global__ void __launch_bounds__(256) k_shock_prep(StageArgs a) {
   using Sh = Shape<N>;
   constexpr int NpInt = Sh::NpInt, NV = Sh::NV, NR = Sh::NR;
@@ -824,7 +955,7 @@ __global__ void __launch_bounds__(256) k_shock_prep(StageArgs a) {
   if (a.sh.filter_state) {
     const double *V = a.sh.ops + Sh::oV, *Vi = a.sh.ops + Sh::oVi, *mf = a.sh.ops + Sh::omf;
     const double fac = 1. - sin(0.5 * M_PI * sg);
-    double *q = static_cast<double *>(a.qs) + size_t(kk) * NV;
+    SIN *q = static_cast<SIN *>(a.qs) + size_t(kk) * NV;
     for (int n = 0; n < 4; ++n) {
       double u[NpInt], m[NpInt];
 #pragma unroll
@@ -841,7 +972,7 @@ __global__ void __launch_bounds__(256) k_shock_prep(StageArgs a) {
         double t = 0;
 #pragma unroll
         for (int j = 0; j < NpInt; ++j) t += V[i * NpInt + j] * m[j];
-        q[n * NpInt + i] = t;
+        q[n * NpInt + i] = SIN(t);
       }
     }
   }
@@ -854,24 +985,24 @@ __global__ void __launch_bounds__(256) k_shock_prep(StageArgs a) {
 // by the non-owner); then the stage's owner-side wave speeds and viscous
 // speeds eps/hK^2 (StoreEdgeAggregates edges.go:246-289). Workgroup = tile of
 // 64 elements x 4 variables.
-template <int N>
+template <int N, class SIN, class D>
 __global__ void __launch_bounds__(kThreads) k_shock_grad(StageArgs a) {
   using Sh = Shape<N>;
   constexpr int NpInt = Sh::NpInt, NpEdge = Sh::NpEdge, NpFlux = Sh::NpFlux, NF = Sh::NF, NV = Sh::NV,
                 NR = Sh::NR, E = kTile;
-  __shared__ double sTr[4][NF][E];
+  __shared__ D sTr[4][NF][E];
   __shared__ double red[2][kThreads / 64];
   if (step_skipped(a)) return;
   const int tile = blockIdx.x, tid = threadIdx.x;
   const int te = tid % E, tn = tid / E, k0 = tile * E, kk = k0 + te, Kp = a.Kp;
-  const double *Q = static_cast<const double *>(a.qs);
-  const double *Itp = ops_base<double>() + NpFlux * NpInt;
-  const double *Div = a.sh.ops + Sh::oDiv;
-  double u[NpInt];
+  const SIN *Q = static_cast<const SIN *>(a.qs);
+  const D *Itp = ops_base<D>() + NpFlux * NpInt;
+  const D *Div = shock_ops<D>(a.sh) + Sh::oDiv;
+  D u[NpInt];
 #pragma unroll
-  for (int i = 0; i < NpInt; ++i) u[i] = Q[size_t(kk) * NV + tn * NpInt + i];
+  for (int i = 0; i < NpInt; ++i) u[i] = D(Q[size_t(kk) * NV + tn * NpInt + i]);
   // edge values: the owner's trace at this face's points, in this face's order
-  double ue[NF];
+  D ue[NF];
 #pragma unroll
   for (int f = 0; f < 3; ++f) {
     const size_t fi = size_t(f) * Kp + kk;
@@ -881,32 +1012,32 @@ __global__ void __launch_bounds__(kThreads) k_shock_grad(StageArgs a) {
     const int nb = a.sh.nbr_elem[fi], nf = a.sh.nbr_face[fi];
 #pragma unroll
     for (int i = 0; i < NpEdge; ++i) {
-      double t = 0;
+      D t = 0;
 #pragma unroll
       for (int j = 0; j < NpInt; ++j) t += Itp[(f * NpEdge + i) * NpInt + j] * u[j];
       sTr[tn][f * NpEdge + i][te] = t;
       ue[f * NpEdge + i] = t;
     }
     if (!owner) {
-      const double *qn = Q + size_t(nb) * NV + tn * NpInt;
+      const SIN *qn = Q + size_t(nb) * NV + tn * NpInt;
 #pragma unroll
       for (int i = 0; i < NpEdge; ++i) {
         const int p = nf * NpEdge + (NpEdge - 1 - i);
-        double t = 0;
+        D t = 0;
 #pragma unroll
-        for (int j = 0; j < NpInt; ++j) t += Itp[p * NpInt + j] * qn[j];
+        for (int j = 0; j < NpInt; ++j) t += Itp[p * NpInt + j] * D(qn[j]);
         ue[f * NpEdge + i] = t;
       }
     }
   }
   const double Jd = a.g.Jdet[kk], ooJd = 1. / Jd;
-  const double Ji[4] = {a.g.Jinv[kk], a.g.Jinv[size_t(Kp) + kk], a.g.Jinv[2 * size_t(Kp) + kk],
-                        a.g.Jinv[3 * size_t(Kp) + kk]};
-  const double *epsF = a.sh.epsF + kk;                              // [NR][Kp]
-  double *dout = a.sh.diss + size_t(tn * 2 * NR) * Kp + kk;         // [4][2][NR][Kp]
+  const D Ji[4] = {D(a.g.Jinv[kk]), D(a.g.Jinv[size_t(Kp) + kk]), D(a.g.Jinv[2 * size_t(Kp) + kk]),
+                   D(a.g.Jinv[3 * size_t(Kp) + kk])};
+  const double *epsF = a.sh.epsF + kk;                                        // [NR][Kp]
+  D *dout = static_cast<D *>(a.sh.diss) + size_t(tn * 2 * NR) * Kp + kk;      // [4][2][NR][Kp]
 #pragma unroll 1
   for (int dir = 0; dir < 2; ++dir) {
-    double dof[NpFlux];
+    D dof[NpFlux];
 #pragma unroll
     for (int i = 0; i < NpInt; ++i) {
       dof[i] = Ji[dir] * u[i];
@@ -916,16 +1047,16 @@ __global__ void __launch_bounds__(kThreads) k_shock_grad(StageArgs a) {
     for (int f = 0; f < 3; ++f) {
       const size_t fi = size_t(f) * Kp + kk;
       const double nrm = dir == 0 ? a.sh.Nxo[fi] : a.sh.Nyo[fi];
-      const double w = a.g.IInII[fi];
+      const D w = D(ooJd * nrm * a.g.IInII[fi]);
 #pragma unroll
-      for (int i = 0; i < NpEdge; ++i) dof[2 * NpInt + f * NpEdge + i] = ooJd * nrm * w * ue[f * NpEdge + i];
+      for (int i = 0; i < NpEdge; ++i) dof[2 * NpInt + f * NpEdge + i] = w * ue[f * NpEdge + i];
     }
 #pragma unroll 3
     for (int r = 0; r < NR; ++r) {
-      double acc = 0;
+      D acc = 0;
 #pragma unroll
       for (int j = 0; j < NpFlux; ++j) acc += Div[r * NpFlux + j] * dof[j];
-      dout[size_t(dir * NR + r) * Kp] = acc * epsF[size_t(r) * Kp];
+      dout[size_t(dir * NR + r) * Kp] = acc * D(epsF[size_t(r) * Kp]);
     }
   }
   if (a.stage < 0) return;
@@ -944,8 +1075,8 @@ __global__ void __launch_bounds__(kThreads) k_shock_grad(StageArgs a) {
       const double *ef = a.sh.epsF + size_t(NpInt + f * NpEdge) * Kp + k;
       double fwl = 0.0, fwv = 0.0;
       for (int pp = 0; pp < NpEdge; ++pp) {
-        double q[4] = {sTr[0][f * NpEdge + pp][e], sTr[1][f * NpEdge + pp][e], sTr[2][f * NpEdge + pp][e],
-                       sTr[3][f * NpEdge + pp][e]};
+        double q[4] = {double(sTr[0][f * NpEdge + pp][e]), double(sTr[1][f * NpEdge + pp][e]),
+                       double(sTr[2][f * NpEdge + pp][e]), double(sTr[3][f * NpEdge + pp][e])};
         if (kind == FK_Far || kind == FK_IVortex) {
           const double own[4] = {q[0], q[1], q[2], q[3]};
           boundary_state(kind, a.ph, a.g.bxy, code >> 8, NpEdge, pp, time, own, a.g.Nx[fi], a.g.Ny[fi], q);
@@ -1022,18 +1153,21 @@ hipError_t launch_stage_f(const StageArgs &a, hipStream_t s) {
   return hipErrorInvalidValue;
 }
 
+// Arithmetic type of the FP32-bulk precision: FP32 from N=2 on. At N<=1 the
+// RT edge points sit 0.056 apart and |DivInt| reaches 1.2e3, so the
+// divergence cancels three digits and runs in FP64 (the elements are small
+// there).
+template <int N>
+using BulkC = typename std::conditional<(N <= 1), double, float>::type;
+
 template <int N>
 hipError_t launch_stage_n(int prec, int shock, const StageArgs &a, hipStream_t s) {
-  if (shock) {
-    if (prec != 0) return hipErrorInvalidValue;
-    return launch_stage_f<N, double, double, true>(a, s);
+  if (prec == 0) {
+    if (shock) return launch_stage_f<N, double, double, true>(a, s);
+    return launch_stage_f<N, double, double, false>(a, s);
   }
-  if (prec == 0) return launch_stage_f<N, double, double, false>(a, s);
-  // FP32 stage states; the arithmetic is FP32 from N=2 on. At N<=1 the RT
-  // edge points sit 0.056 apart and |DivInt| reaches 1.2e3, so the divergence
-  // cancels three digits and runs in FP64 (the elements are small there).
-  if constexpr (N <= 1) return launch_stage_f<N, float, double, false>(a, s);
-  else return launch_stage_f<N, float, float, false>(a, s);
+  if (shock) return launch_stage_f<N, float, BulkC<N>, true>(a, s);
+  return launch_stage_f<N, float, BulkC<N>, false>(a, s);
 }
 
 template <int N>
@@ -1045,17 +1179,17 @@ hipError_t launch_prologue_n(int prec, const StageArgs &a, hipStream_t s) {
   return hipGetLastError();
 }
 
-template <int N>
-hipError_t launch_shock_n(int pass, const StageArgs &a, hipStream_t s) {
+template <int N, class SIN, class D>
+hipError_t launch_shock_t(int pass, const StageArgs &a, hipStream_t s) {
   switch (pass) {
     case 0:
-      hipLaunchKernelGGL((k_shock_sensor<N>), dim3((a.K + 255) / 256), dim3(256), 0, s, a);
+      hipLaunchKernelGGL((k_shock_sensor<N, SIN>), dim3((a.K + 255) / 256), dim3(256), 0, s, a);
       break;
     case 1:
-      hipLaunchKernelGGL((k_shock_prep<N>), dim3((a.Kp + 255) / 256), dim3(256), 0, s, a);
+      hipLaunchKernelGGL((k_shock_prep<N, SIN>), dim3((a.Kp + 255) / 256), dim3(256), 0, s, a);
       break;
     case 2:
-      hipLaunchKernelGGL((k_shock_grad<N>), dim3(a.ntiles), dim3(kThreads), 0, s, a);
+      hipLaunchKernelGGL((k_shock_grad<N, SIN, D>), dim3(a.ntiles), dim3(kThreads), 0, s, a);
       break;
     default:
       return hipErrorInvalidValue;
@@ -1063,9 +1197,53 @@ hipError_t launch_shock_n(int pass, const StageArgs &a, hipStream_t s) {
   return hipGetLastError();
 }
 
+// The passes read the stage input: q0 (FP64) at stage 0 and for the RHS
+// evaluation, else the stage state in the precision's storage.
+template <int N>
+hipError_t launch_shock_n(int prec, int pass, const StageArgs &a, hipStream_t s) {
+  if (prec == 0) return launch_shock_t<N, double, double>(pass, a, s);
+  if (a.stage <= 0) return launch_shock_t<N, double, BulkC<N>>(pass, a, s);
+  return launch_shock_t<N, float, BulkC<N>>(pass, a, s);
+}
+
 template <int N>
 hipError_t upload_operators_n(const double *divT, const double *interp) {
   using Sh = Shape<N>;
+  static_assert(Sh::KS * 64 <= kMfMax && Sh::RB * Sh::KI * 64 <= kMfIMax, "MFMA operand tables");
+  {
+    // A fragments of the MFMA FluxEdgeInterp contraction
+    std::vector<float> ai(size_t(Sh::RB) * Sh::KI * 64, 0.f);
+    for (int rb = 0; rb < Sh::RB; ++rb)
+      for (int s = 0; s < Sh::KI; ++s)
+        for (int l = 0; l < 64; ++l) {
+          const int p = 16 * rb + l % 16, i = 4 * s + l / 16;
+          if (p < Sh::NF && i < Sh::NpInt)
+            ai[(size_t(rb) * Sh::KI + s) * 64 + l] = float(interp[size_t(p) * Sh::NpInt + i]);
+        }
+    hipError_t e = hipMemcpyToSymbol(HIP_SYMBOL(cMfI), ai.data(), ai.size() * sizeof(float), 0,
+                                     hipMemcpyHostToDevice);
+    if (e != hipSuccess) return e;
+  }
+  {
+    // A fragments of the MFMA DivInt contraction (the K-row order of MfmaDiv)
+    std::vector<float> af(size_t(Sh::KS) * 64, 0.f);
+    for (int s = 0; s < Sh::KS; ++s)
+      for (int l = 0; l < 64; ++l) {
+        const int row = l % 16, kr = l / 16;
+        int j = -1;
+        if (s < 2 * Sh::G) {
+          const int i = 4 * (s / 2) + kr;
+          if (i < Sh::NpInt) j = (s % 2) * Sh::NpInt + i;
+        } else {
+          const int p = 4 * (s - 2 * Sh::G) + kr;
+          if (p < Sh::NF) j = 2 * Sh::NpInt + p;
+        }
+        if (row < Sh::NpInt && j >= 0) af[size_t(s) * 64 + l] = float(divT[size_t(j) * Sh::NpInt + row]);
+      }
+    hipError_t e = hipMemcpyToSymbol(HIP_SYMBOL(cMfA), af.data(), af.size() * sizeof(float), 0,
+                                     hipMemcpyHostToDevice);
+    if (e != hipSuccess) return e;
+  }
   std::vector<double> d(static_cast<size_t>(Sh::ops));
   std::copy(divT, divT + size_t(Sh::NpFlux) * Sh::NpInt, d.begin());
   std::copy(interp, interp + size_t(Sh::NF) * Sh::NpInt, d.begin() + size_t(Sh::NpFlux) * Sh::NpInt);
@@ -1085,7 +1263,7 @@ hipError_t upload_operators_n(const double *divT, const double *interp) {
   namespace dev {                                                                                        \
   template hipError_t launch_stage_n<NN>(int, int, const StageArgs &, hipStream_t);                      \
   template hipError_t launch_prologue_n<NN>(int, const StageArgs &, hipStream_t);                        \
-  template hipError_t launch_shock_n<NN>(int, const StageArgs &, hipStream_t);                           \
+  template hipError_t launch_shock_n<NN>(int, int, const StageArgs &, hipStream_t);                      \
   template hipError_t upload_operators_n<NN>(const double *, const double *);                           \
   }                                                                                                      \
   }

@@ -108,6 +108,27 @@ def vortex_mesh(path: str, n: int, seed: int = 1, jitter: float = 0.1) -> str:
     return path
 
 
+def shocktube_mesh(path: str, nx: int, ny: int = 2, height: float = 0.1, jitter: float = 0.0,
+                   seed: int = 1) -> str:
+    """Sod shock-tube channel on [0,1] x [0,height]: nx x ny quads split into
+    2 nx ny triangles, "in" on x=0, "out" on x=1 and "wall" on y=0 and
+    y=height (the marker names readfiles/readSU2Grid.go maps to BC_In,
+    BC_Out, BC_Wall). With an even nx and no jitter the initial jump at
+    x = 0.5 lies on element edges."""
+    VX, VY, EToV, _ = jittered_quads(nx, ny, x0=0.0, x1=1.0, y0=0.0, y1=height, jitter=jitter, seed=seed)
+
+    def v(i, j):
+        return j * (nx + 1) + i
+
+    tags = {
+        "in": [(v(0, j), v(0, j + 1)) for j in range(ny)],
+        "out": [(v(nx, j), v(nx, j + 1)) for j in range(ny)],
+        "wall": [(v(i, 0), v(i + 1, 0)) for i in range(nx)] + [(v(i, ny), v(i + 1, ny)) for i in range(nx)],
+    }
+    write_su2(path, VX, VY, EToV, tags)
+    return path
+
+
 def wedge_mesh(path: str, n_points: int = 125_000, seed: int = 1, x_ramp: float = 0.5,
                angle_deg: float = 15.0, jitter: float = 0.2) -> str:
     """Mach-2 wedge channel: [0,3] x [0,1] with a ramp rising at angle_deg from

@@ -58,7 +58,7 @@ typedef struct gocfd_params {
   double final_time;            /* FinalTime */
   double minf, gamma, alpha;    /* Minf, Gamma, Alpha (degrees) */
   int32_t local_time_stepping;  /* LocalTimeStepping */
-  int32_t limiter;              /* 0 none, 1 PerssonC0 (Limiter; FP64 precision only) */
+  int32_t limiter;              /* 0 none, 1 PerssonC0 (Limiter) */
   double kappa;                 /* Kappa */
   int32_t precision;            /* GOCFD_PREC_* */
   int32_t device;               /* HIP device ordinal; -1 = host-only handle (setup and

@@ -160,6 +160,10 @@ class OracleSolver:
     def dt(self):
         return lib().or_dt(self._h)
 
+    @property
+    def step_count(self):
+        return lib().or_step_count(self._h)
+
     def residual(self):
         r = np.empty((4, self.NpInt, self.K))
         lib().or_get_residual(self._h, _p(r))

@@ -749,7 +749,8 @@ struct or_solver {
   int *FaceEdge;  /* [3][K]: EdgeNumber as edge index */
   double *Jdet, *Jinv, *FNx, *FNy, *IInII, *EdgeLenMax;
   double *SolX, *SolY, *FluxX, *FluxY;
-  fs_t fs;
+  fs_t fs;          /* FSFar */
+  fs_t fsin, fsout; /* FSIn, FSOut: set by the shock-tube case, else FSFar */
   int ivortex;
   /* state */
   double *Q, *Q1, *Q2, *Q3, *Q4, *RHSQ, *Resid, *DT, *DTVisc, *QFace, *FRT;
@@ -1193,6 +1194,8 @@ or_solver *or_create(const char *meshfile, const or_params *p, char *err, int er
   s->EdgeAgg = calloc(s->ne, sizeof(double)); s->EdgeAggVisc = calloc(s->ne, sizeof(double));
   /* InitializeSolution (euler.go:728-794) */
   fs_new(&s->fs, p->Minf, p->Gamma, p->Alpha);
+  s->fsin = s->fs;
+  s->fsout = s->fs;
   if (p->init == OR_INIT_FREESTREAM) {
     for (int n = 0; n < 4; n++)
       for (size_t i = 0; i < (size_t)NpInt * K; i++) s->Q[n * (size_t)NpInt * K + i] = 0.0 + s->fs.Qinf[n];
@@ -1207,6 +1210,18 @@ or_solver *or_create(const char *meshfile, const or_params *p, char *err, int er
       for (int n = 0; n < 4; n++) s->Q[n * (size_t)NpInt * K + i] = qq[n];
     }
     for (int e = 0; e < s->ne; e++) if (s->E[e].bc == OR_BC_WALL) s->E[e].bc = OR_BC_IVORTEX;
+  } else if (p->init == OR_INIT_SHOCKTUBE) {
+    /* InitializeSolution SHOCKTUBE (euler.go:738-771): Sod states left and
+     * right of x = 0.5, which are also the In and Out boundary states */
+    double gam = 1.4, qin[4] = {1, 0, 0, 1 / (gam - 1)}, qout[4] = {0.125, 0, 0, 0.1 / (gam - 1)};
+    fs_from_qinf(&s->fsin, gam, qin);
+    s->fsin.Minf = 0; s->fsin.Alpha = 0;
+    fs_from_qinf(&s->fsout, gam, qout);
+    s->fsout.Minf = 0; s->fsout.Alpha = 0;
+    for (size_t i = 0; i < (size_t)NpInt * K; i++) {
+      const double *qq = s->SolX[i] < 0.5 ? s->fsin.Qinf : s->fsout.Qinf;
+      for (int n = 0; n < 4; n++) s->Q[n * (size_t)NpInt * K + i] = qq[n];
+    }
   } else {
     seterr(err, errlen, "init type %d not supported by the oracle", p->init);
     goto fail;
@@ -1352,16 +1367,19 @@ static void edge_fluxes(or_solver *s, double t) {
     int normal_flux = 1;
     switch (E->bc) {
     case OR_BC_NONE: break;
-    case OR_BC_FAR: case OR_BC_IN: case OR_BC_OUT:
-      /* FSIn/FSOut only exist for the shocktube case; In/Out use FSFar here
-       * where the reference would dereference nil (edges.go:427-430) */
+    case OR_BC_FAR: case OR_BC_IN: case OR_BC_OUT: {
+      /* FarBC with FSFar, FSIn or FSOut (edges.go:425-431); FSIn/FSOut only
+       * exist for the shocktube case, elsewhere In/Out use FSFar where the
+       * reference would dereference nil */
+      const fs_t *FS = E->bc == OR_BC_IN ? &s->fsin : E->bc == OR_BC_OUT ? &s->fsout : &s->fs;
       for (int i = 0; i < NpEdge; i++) {
         double Qi[4], Qb[4];
         for (int n = 0; n < 4; n++) Qi[n] = *QF(s, n, i + shiftL, kL);
-        riemann_bc(&s->fs, Qi, s->fs.Qinf, nrm, Qb);
+        riemann_bc(FS, Qi, FS->Qinf, nrm, Qb);
         for (int n = 0; n < 4; n++) *QF(s, n, i + shiftL, kL) = Qb[n];
       }
       break;
+    }
     case OR_BC_IVORTEX:
       for (int i = 0; i < NpEdge; i++) {
         int indFlux = kL + (2 * NpInt + i + shiftL) * K;

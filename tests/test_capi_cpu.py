@@ -125,8 +125,6 @@ def test_host_handle_rejects_device_calls(vortex_meshes):
 
 @pytest.mark.parametrize("kw,msg,prec", [
     (dict(PolynomialOrder=5), "polynomial order", "fp64"),
-    (dict(InitType="shocktube"), "init type", "fp64"),
-    (dict(Limiter="PerssonC0"), "FP64 only", "fp32"),
 ])
 def test_parameter_errors(vortex_meshes, kw, msg, prec):
     base = dict(CFL=1.0, FluxType="Lax", InitType="ivortex", PolynomialOrder=2, FinalTime=1.0)
@@ -136,12 +134,25 @@ def test_parameter_errors(vortex_meshes, kw, msg, prec):
     assert ei.value.code == _lib.ERR_ARG and msg.lower() in str(ei.value).lower()
 
 
+def test_shocktube_host_setup(tmp_path):
+    """InitType shocktube builds on the host: In/Out edges of the channel
+    mesh map to the In/Out boundary kinds like the oracle's."""
+    from gocfd_amd import meshgen
+    mesh = meshgen.shocktube_mesh(str(tmp_path / "st.su2"), 20, 2, height=0.02)
+    ip = InputParameters2D(CFL=0.5, FluxType="Roe", InitType="shocktube", PolynomialOrder=1, FinalTime=0.1,
+                           Limiter="PerssonC0", Kappa=3.0)
+    h = Euler(ip, mesh, device=-1)
+    o = oracle.OracleSolver(mesh, 1, flux="roe", init="shocktube")
+    assert np.array_equal(h.edges(), o.edges())
+
+
 def test_shock_and_lts_handles_build_on_host(vortex_meshes):
-    """PerssonC0 and LocalTimeStepping are accepted (FP64); host-only handles
+    """PerssonC0 and LocalTimeStepping are accepted (FP64 and FP32); host-only handles
     refuse the device-only introspection."""
     ip = InputParameters2D(CFL=1.0, FluxType="Roe", InitType="ivortex", PolynomialOrder=2, FinalTime=1.0,
                            Limiter="PerssonC0", Kappa=3.0, LocalTimeStepping=True)
     h = Euler(ip, vortex_meshes[4], device=-1)
+    Euler(ip, vortex_meshes[4], precision="fp32", device=-1)
     with pytest.raises(GocfdError):
         h.shock()
     with pytest.raises(GocfdError):

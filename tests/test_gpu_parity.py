@@ -120,15 +120,32 @@ def test_final_time_stop(vortex_meshes):
     assert _rel(gpu.Q, ora.state()) < 1e-11
 
 
-def test_fp32_bulk_precision(vortex_meshes):
-    """FP32 storage and bulk arithmetic: density L2 error <= 1e-5 vs the FP64 oracle."""
-    gpu, ora = _pair(vortex_meshes[20], 2, "Lax", "ivortex", precision="fp32")
+@pytest.mark.parametrize("N,flux", [(1, "Roe"), (2, "Lax"), (3, "Roe"), (4, "Roe"), (4, "Lax")])
+def test_fp32_bulk_precision(vortex_meshes, N, flux):
+    """FP32 storage and bulk arithmetic: density L2 error <= 1e-5 vs the FP64
+    oracle after 100 steps (N >= 3: the DivInt contraction on the matrix
+    cores)."""
+    gpu, ora = _pair(vortex_meshes[20], N, flux, "ivortex", precision="fp32")
     gpu.step(100)
     for _ in range(100):
         ora.step()
     qg, qo = gpu.Q, ora.state()
     l2 = np.linalg.norm(qg[0] - qo[0]) / np.linalg.norm(qo[0])
     assert l2 < 1e-5, l2
+
+
+@pytest.mark.parametrize("N", [0, 1, 2, 3, 4])
+def test_fp32_rhs(vortex_meshes, N):
+    """FP32 RHS on a perturbed state within 5e-6 of the summed magnitude of
+    the divergence terms (about 80 FP32 ulps of the largest term: the
+    componentwise bound of an FP32 contraction; the RHS itself is a 1e3-fold
+    cancellation of those terms); N <= 1 computes in FP64 (1e-12)."""
+    gpu, ora = _pair(vortex_meshes[20], N, "Roe", "ivortex", precision="fp32")
+    q = _perturbed(ora.state(), 30 + N)
+    r_gpu = gpu.rhs(q, 0.0)
+    ora.rhs_last = ora.rhs(q, 0.0)
+    bar = 1e-12 if N <= 1 else 5e-6
+    assert _rhs_err(r_gpu, ora) < bar, _rhs_err(r_gpu, ora)
 
 
 def test_set_state_round_trip(vortex_meshes):
