@@ -256,10 +256,13 @@ def main():
     if args.only == "shock" or (args.only is None and not args.no_shock):
         wedge = meshgen.wedge_mesh(os.path.join(tmpdir, "wedge.su2"), args.shock_points, seed=1)
     if args.only == "shock":
-        r = run_gpu_shock(wedge, args.shock_steps, args.warmup)
-        v = r["K"] * r["NpInt"] * 5 * args.shock_steps / r["elapsed"]
-        print(json.dumps({"leg": "shock", "value": v, "stage_ms": r["stage_ms"],
-                          "ms_per_step": r["elapsed"] / args.shock_steps * 1e3, "K": r["K"]}))
+        out = {"leg": "shock"}
+        for prec in ("fp64", "fp32"):
+            r = run_gpu_shock(wedge, args.shock_steps, args.warmup, prec)
+            out[prec] = {"value": r["K"] * r["NpInt"] * 5 * args.shock_steps / r["elapsed"],
+                         "stage_ms": r["stage_ms"], "ms_per_step": r["elapsed"] / args.shock_steps * 1e3,
+                         "K": r["K"], "sensor_flagged_elements": r["flagged"]}
+        print(json.dumps(out))
         return
     if args.only == "ensemble":
         r = run_ensemble(args.ensemble_cases, args.ensemble_steps, args.warmup)

@@ -25,6 +25,8 @@
 //      and the owner-side CFL wave-speed maximum (edges.go:246-323), folded
 //      into one atomic per workgroup for the next stage's dt
 //      (calculateGlobalDT euler.go:945-971)
+// With FP32 arithmetic from N=3 on, the FluxEdgeInterp (A, F) and DivInt (C)
+// contractions run on the matrix cores as 16x16x4 FP32 MFMA tiles.
 // With shock capturing three passes precede each stage launch: the sensor
 // with the vertex max-merge, the element viscosity fields (and the stage-2
 // state filter), and the RT gradient with the stage's wave speeds.
@@ -654,35 +656,43 @@ k_stage(StageArgs a) {
     double *Rr = static_cast<double *>(a.R) + base;
     const int nreal = min(E, ci.k_end - k0);
     if constexpr (LAST) {
-      // stage 4: R = -q0 + a q2 + b q3 + c q4 + d dt R3 + e dt RHS, q0 += R
-      double v0[LPT], vR[LPT];
-      S v2[LPT], v3[LPT];
+      // stage 4: R = -q0 + a q2 + b q3 + c q4 + d dt R3 + e dt RHS, q0 += R;
+      // with FP32 stage states the four operand arrays are loaded in two
+      // halves (the register budget of three waves per SIMD)
+      constexpr int CH = sizeof(S) == 4 ? (LPT + 1) / 2 : LPT;
 #pragma unroll
-      for (int j = 0; j < LPT; ++j) {
-        const int idx = tid + j * kThreads;
-        const bool ok = idx < E * NV;
-        v0[j] = ok ? q0[idx] : 0.0;
-        v2[j] = ok ? q2[idx] : S(0);
-        v3[j] = ok ? q3[idx] : S(0);
-        vR[j] = ok ? Rr[idx] : 0.0;
+      for (int j0 = 0; j0 < LPT; j0 += CH) {
+      double v0[CH], vR[CH];
+      S v2[CH], v3[CH];
+#pragma unroll
+      for (int jj = 0; jj < CH; ++jj) {
+        const int idx = tid + (j0 + jj) * kThreads;
+        const bool ok = j0 + jj < LPT && idx < E * NV;
+        v0[jj] = ok ? q0[idx] : 0.0;
+        v2[jj] = ok ? q2[idx] : S(0);
+        v3[jj] = ok ? q3[idx] : S(0);
+        vR[jj] = ok ? Rr[idx] : 0.0;
       }
 #pragma unroll
-      for (int j = 0; j < LPT; ++j) {
+      for (int jj = 0; jj < CH; ++jj) {
+        const int j = j0 + jj;
         const int idx = tid + j * kThreads;
+        if (j >= LPT) continue;
         if (idx >= E * NV) continue;
         const int e = idx / NV, r = idx - e * NV;
         const double dte = lts ? sDT[e] : dt;
         const double dtR = dte * double(sR[r * P + e]);
         const double qsv = double(sQ[r * P + e]);
-        const double q0v = double(v0[j]);
-        const double dtR3 = dte * double(vR[j]);
-        const double Rn = -q0v + 0.517231671970585 * double(v2[j]) + 0.096059710526146 * double(v3[j]) +
+        const double q0v = double(v0[jj]);
+        const double dtR3 = dte * double(vR[jj]);
+        const double Rn = -q0v + 0.517231671970585 * double(v2[jj]) + 0.096059710526146 * double(v3[jj]) +
                           0.386708617503269 * qsv + 0.063692468666290 * dtR3 + 0.226007483236906 * dtR;
         Rr[idx] = Rn;
         const double nv = q0v + Rn;
         q0[idx] = nv;
         sQ[r * P + e] = C(nv);
         if (e < nreal && !(nv == nv)) bad = 1;
+      }
       }
     } else {
       double v0[LPT];
