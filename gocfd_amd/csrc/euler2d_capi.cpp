@@ -44,7 +44,7 @@ struct gocfd_euler2d {
   double kappa = 0, eps0 = 0;
   DevBuf<double> sh_ops, hK, Se, sigma, eps, epsF, diss, Nxo, Nyo, vooL;
   DevBuf<float> sh_opsF;
-  DevBuf<int> evert, nbr_elem, nbr_face;
+  DevBuf<int> evert, nbr_elem, nbr_face, active;
   DevBuf<unsigned long long> sigV, epsV;
   // a single solver is a batch of one case
   DevBuf<dev::CaseInfo> cases;
@@ -85,6 +85,7 @@ struct gocfd_euler2d {
       a.sh.ops = sh_ops.p; a.sh.opsF = sh_opsF.p; a.sh.evert = evert.p; a.sh.hK = hK.p;
       a.sh.sigV = sigV.p; a.sh.epsV = epsV.p; a.sh.Nv = d.Nv;
       a.sh.Se = Se.p; a.sh.sigma = sigma.p; a.sh.eps = eps.p; a.sh.epsF = epsF.p; a.sh.diss = diss.p;
+      a.sh.active = active.p;
       a.sh.nbr_elem = nbr_elem.p; a.sh.nbr_face = nbr_face.p;
       a.sh.Nxo = Nxo.p; a.sh.Nyo = Nyo.p; a.sh.vooL = vooL.p;
       a.sh.kappa = kappa; a.sh.eps0 = eps0;
@@ -315,6 +316,7 @@ int gocfd_euler2d_create(const gocfd_mesh *mesh, const gocfd_params *p, gocfd_eu
     h->Se.alloc(L.Kp); h->sigma.alloc(L.Kp); h->eps.alloc(L.Kp);
     h->epsF.alloc(size_t(L.Kp) * NR);
     h->diss.alloc(size_t(L.Kp) * 4 * 2 * NR);  // FP64 or FP32 values (the larger size)
+    h->active.alloc(L.Kp);
   }
   std::vector<double> divT(h->d.ops.DivInt.size());  // [NpFlux][NpInt] for contiguous per-DOF rows
   for (int i = 0; i < h->d.ops.NpInt; ++i)

@@ -88,6 +88,9 @@ struct ShockArgs {
   double *Se, *sigma, *eps;          // [Kp]
   double *epsF;                      // [NR][Kp] viscosity at the used RT points
   void *diss;                        // [4][2][NR][Kp] eps * grad at the used RT points
+  int *active;                       // [Kp] 1 if a vertex of the element carries eps > 0;
+                                     // else eps = 0 at all its RT points and diss is
+                                     // neither written nor read (it is zero)
   const int *nbr_elem, *nbr_face;    // [3][Kp]
   const double *Nxo, *Nyo;           // [3][Kp] own outward normals
   const double *vooL;                // [3][Kp] 1 / owner edge length (shared faces)

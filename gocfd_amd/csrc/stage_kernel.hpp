@@ -378,19 +378,23 @@ k_stage(StageArgs a) {
         const int kL = owner ? kk : nb, kR = owner ? nb : kk;
         const int rowL = NpInt + (owner ? f * NpEdge + pp : nf * NpEdge + rp);
         const int rowR = NpInt + (owner ? nf * NpEdge + rp : f * NpEdge + pp);
-        const double lam = 0.5 * (a.sh.epsF[size_t(rowL) * Kp + kL] + a.sh.epsF[size_t(rowR) * Kp + kR]);
-        const C coef = C((a.sh.omega * lam) * a.sh.vooL[fi]);
-        const C *diss = static_cast<const C *>(a.sh.diss);
+        // with both sides inactive eps is zero on the edge and VF vanishes
+        const bool aL = a.sh.active[kL] != 0, aR = a.sh.active[kR] != 0;
+        if (aL || aR) {
+          const double lam = 0.5 * (a.sh.epsF[size_t(rowL) * Kp + kL] + a.sh.epsF[size_t(rowR) * Kp + kR]);
+          const C coef = C((a.sh.omega * lam) * a.sh.vooL[fi]);
+          const C *diss = static_cast<const C *>(a.sh.diss);
 #pragma unroll
-        for (int c = 0; c < 4; ++c) {
-          // owner traces at owner points jo and NpEdge-1-jo
-          const C EQj = owner ? own[c] : oth[c];
-          const C EQr = owner ? sT[c * TS + (f * NpEdge + rp) * E + e] : sT[c * TS + base + pp * stride];
-          const C *dX = diss + size_t(c * 2 * NR) * Kp, *dY = dX + size_t(NR) * Kp;
-          const C vFL = nx * dX[size_t(rowL) * Kp + kL] + ny * dY[size_t(rowL) * Kp + kL];
-          const C vFR = nx * dX[size_t(rowR) * Kp + kR] + ny * dY[size_t(rowR) * Kp + kR];
-          const C VF = C(0.5) * (vFL + vFR) - coef * (EQj - EQr);
-          F[c] = F[c] - VF;
+          for (int c = 0; c < 4; ++c) {
+            // owner traces at owner points jo and NpEdge-1-jo
+            const C EQj = owner ? own[c] : oth[c];
+            const C EQr = owner ? sT[c * TS + (f * NpEdge + rp) * E + e] : sT[c * TS + base + pp * stride];
+            const C *dX = diss + size_t(c * 2 * NR) * Kp, *dY = dX + size_t(NR) * Kp;
+            const C vFL = aL ? nx * dX[size_t(rowL) * Kp + kL] + ny * dY[size_t(rowL) * Kp + kL] : C(0);
+            const C vFR = aR ? nx * dX[size_t(rowR) * Kp + kR] + ny * dY[size_t(rowR) * Kp + kR] : C(0);
+            const C VF = C(0.5) * (vFL + vFR) - coef * (EQj - EQr);
+            F[c] = F[c] - VF;
+          }
         }
       }
 #pragma unroll
@@ -459,10 +463,12 @@ k_stage(StageArgs a) {
     if constexpr (SHOCK) {
       // unshared edge: the own side's eps*grad Q along its normal
       const int row = NpInt + f * NpEdge + pp;
+      if (a.sh.active[k0 + e] != 0) {
 #pragma unroll
-      for (int c = 0; c < 4; ++c) {
-        const C *dX = static_cast<const C *>(a.sh.diss) + size_t(c * 2 * NR + row) * Kp + k0 + e;
-        F[c] = F[c] - (nx * dX[0] + ny * dX[size_t(NR) * Kp]);
+        for (int c = 0; c < 4; ++c) {
+          const C *dX = static_cast<const C *>(a.sh.diss) + size_t(c * 2 * NR + row) * Kp + k0 + e;
+          F[c] = F[c] - (nx * dX[0] + ny * dX[size_t(NR) * Kp]);
+        }
       }
     }
     const C w = C(a.g.IInII[fi]);
@@ -522,6 +528,8 @@ k_stage(StageArgs a) {
   } else {
     // VALU: thread (element te, variable tn)
     const int kk = k0 + te;
+    bool act = false;  // shock capturing: eps*grad Q is nonzero somewhere in the element
+    if constexpr (SHOCK) act = a.sh.active[kk] != 0;
     const double Jd = a.g.Jdet[kk];
     const C Jdc = C(Jd);
     const C J0 = C(a.g.Jinv[kk]), J1 = C(a.g.Jinv[size_t(Kp) + kk]);
@@ -536,9 +544,11 @@ k_stage(StageArgs a) {
       C Fx, Fy;
       flux_component(tn, g, r0, r1, r2, r3, Fx, Fy);
       if constexpr (SHOCK) {
-        const C *dX = static_cast<const C *>(a.sh.diss) + size_t(tn * 2 * NR + j) * Kp + kk;
-        Fx -= dX[0];
-        Fy -= dX[size_t(NR) * Kp];
+        if (act) {
+          const C *dX = static_cast<const C *>(a.sh.diss) + size_t(tn * 2 * NR + j) * Kp + kk;
+          Fx -= dX[0];
+          Fy -= dX[size_t(NR) * Kp];
+        }
       }
       const C Fr = Jdc * (J0 * Fx + J1 * Fy), Fs = Jdc * (J2 * Fx + J3 * Fy);
 #pragma unroll
@@ -950,6 +960,7 @@ __global__ void __launch_bounds__(256) k_shock_prep(StageArgs a) {
   double *epsF = a.sh.epsF + kk;  // [NR][Kp]
   if (kk >= a.K) {
     a.sh.sigma[kk] = 0.;
+    a.sh.active[kk] = 0;
     for (int r = 0; r < NR; ++r) epsF[size_t(r) * a.Kp] = 0.;
     return;
   }
@@ -959,6 +970,7 @@ __global__ void __launch_bounds__(256) k_shock_prep(StageArgs a) {
   const double sg = ((from_bits(sv[v0]) + from_bits(sv[v1])) + from_bits(sv[v2])) / 3.;
   a.sh.sigma[kk] = sg;
   const double e0 = from_bits(ev[v0]), e1 = from_bits(ev[v1]), e2 = from_bits(ev[v2]);
+  a.sh.active[kk] = (e0 > 0. || e1 > 0. || e2 > 0.) ? 1 : 0;
   const double *B = a.sh.ops + Sh::oBary;
 #pragma unroll
   for (int r = 0; r < NR; ++r) epsF[size_t(r) * a.Kp] = B[3 * r] * e0 + B[3 * r + 1] * e1 + B[3 * r + 2] * e2;
@@ -1045,8 +1057,11 @@ __global__ void __launch_bounds__(kThreads) k_shock_grad(StageArgs a) {
                    D(a.g.Jinv[3 * size_t(Kp) + kk])};
   const double *epsF = a.sh.epsF + kk;                                        // [NR][Kp]
   D *dout = static_cast<D *>(a.sh.diss) + size_t(tn * 2 * NR) * Kp + kk;      // [4][2][NR][Kp]
+  // eps is zero at every RT point of an inactive element: its eps*grad is
+  // zero, and the stage kernel does not read it
+  const bool act = a.sh.active[kk] != 0;
 #pragma unroll 1
-  for (int dir = 0; dir < 2; ++dir) {
+  for (int dir = 0; dir < (act ? 2 : 0); ++dir) {
     D dof[NpFlux];
 #pragma unroll
     for (int i = 0; i < NpInt; ++i) {
