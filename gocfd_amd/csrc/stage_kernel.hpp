@@ -67,15 +67,39 @@ struct Shape {
   static constexpr int RB = (NF + 15) / 16, KI = (NpInt + 3) / 4;
 };
 
-// The DivInt and FluxEdgeInterp contractions run on the matrix cores for the
-// FP32 arithmetic from N=3 on (NpInt = 10, 15 rows of a 16-row tile) without
-// shock capturing (whose RHS filter needs a whole element column in one
-// thread).
+// The DivInt and FluxEdgeInterp contractions run on the matrix cores from
+// N=3 on (NpInt = 10, 15 rows of a 16-row tile) without shock capturing
+// (whose RHS filter needs a whole element column in one thread).
 template <int N, class C, bool SHOCK>
 struct UseMfma {
-  static constexpr bool value = std::is_same<C, float>::value && N >= 3 && !SHOCK;
+  static constexpr bool value = N >= 3 && !SHOCK;
 };
 typedef float floatx4 __attribute__((ext_vector_type(4)));
+typedef double doublex4 __attribute__((ext_vector_type(4)));
+
+// 16x16x4 MFMA tiles in FP32 and FP64. Operands: lane l holds A[l%16][l/16]
+// and B[l/16][l%16]; result element v of lane l is D[row(l/16, v)][l%16],
+// row = 4 krow + v in FP32 and krow + 4 v in FP64 (both checked on gfx950).
+template <class C>
+struct Mma;
+template <>
+struct Mma<float> {
+  using vec = floatx4;
+  static __device__ __forceinline__ vec zero() { return vec{0.f, 0.f, 0.f, 0.f}; }
+  static __device__ __forceinline__ vec mma(float a, float b, vec c) {
+    return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+  }
+  static __device__ __forceinline__ int row(int krow, int v) { return 4 * krow + v; }
+};
+template <>
+struct Mma<double> {
+  using vec = doublex4;
+  static __device__ __forceinline__ vec zero() { return vec{0., 0., 0., 0.}; }
+  static __device__ __forceinline__ vec mma(double a, double b, vec c) {
+    return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+  }
+  static __device__ __forceinline__ int row(int krow, int v) { return krow + 4 * v; }
+};
 
 // The operators of this translation unit's order in constant memory, so that
 // every coefficient is a uniform scalar load: DivInt^T [NpFlux][NpInt]
@@ -87,43 +111,59 @@ static __constant__ float cOpsF[kOpsMax];
 // DivInt[row l%16][DOF of K-row 4s + l/16] (zero outside the matrix)
 constexpr int kMfMax = 13 * 64;
 static __constant__ float cMfA[kMfMax];
+static __constant__ double cMfAD[kMfMax];
 // FluxEdgeInterp as the A operand: row block rb, K-step s, lane l holds
 // Interp[16 rb + l%16][4 s + l/16]
 constexpr int kMfIMax = 8 * 64;
 static __constant__ float cMfI[kMfIMax];
+static __constant__ double cMfID[kMfIMax];
+template <class C>
+__device__ __forceinline__ const C *mfa_div();
+template <>
+__device__ __forceinline__ const float *mfa_div<float>() { return cMfA; }
+template <>
+__device__ __forceinline__ const double *mfa_div<double>() { return cMfAD; }
+template <class C>
+__device__ __forceinline__ const C *mfa_interp();
+template <>
+__device__ __forceinline__ const float *mfa_interp<float>() { return cMfI; }
+template <>
+__device__ __forceinline__ const double *mfa_interp<double>() { return cMfID; }
 
 // Face traces sT[var][p][e] = sum_i FluxEdgeInterp[p][i] sQ[var][i][e] of
 // variable tn (= the wave) for the tile's 64 elements on the matrix cores:
 // four 16-element column blocks, lane (col, krow) supplies the solution
 // point 4s + krow of element 16b + col as the B operand of K-step s.
-template <int N>
-__device__ __forceinline__ void traces_mfma(const float *sQ, float *sT, int tn, int te, int TS) {
+template <int N, class C>
+__device__ __forceinline__ void traces_mfma(const C *sQ, C *sT, int tn, int te, int TS) {
   using Sh = Shape<N>;
+  using M = Mma<C>;
   constexpr int NpInt = Sh::NpInt, NF = Sh::NF, RB = Sh::RB, KI = Sh::KI, E = kTile, P = kTile + 1;
   const int col = te & 15, krow = te >> 4;
-  float ai[RB][KI];
+  const C *tab = mfa_interp<C>();
+  C ai[RB][KI];
 #pragma unroll
   for (int rb = 0; rb < RB; ++rb)
 #pragma unroll
-    for (int s = 0; s < KI; ++s) ai[rb][s] = cMfI[(rb * KI + s) * 64 + te];
+    for (int s = 0; s < KI; ++s) ai[rb][s] = tab[(rb * KI + s) * 64 + te];
 #pragma unroll
   for (int b = 0; b < 4; ++b) {
     const int e = 16 * b + col;
-    floatx4 acc[RB];
+    typename M::vec acc[RB];
 #pragma unroll
-    for (int rb = 0; rb < RB; ++rb) acc[rb] = floatx4{0.f, 0.f, 0.f, 0.f};
+    for (int rb = 0; rb < RB; ++rb) acc[rb] = M::zero();
 #pragma unroll
     for (int s = 0; s < KI; ++s) {
       // past NpInt the A column is zero: any finite point value will do
-      const float qv = sQ[(tn * NpInt + min(4 * s + krow, NpInt - 1)) * P + e];
+      const C qv = sQ[(tn * NpInt + min(4 * s + krow, NpInt - 1)) * P + e];
 #pragma unroll
-      for (int rb = 0; rb < RB; ++rb) acc[rb] = __builtin_amdgcn_mfma_f32_16x16x4f32(ai[rb][s], qv, acc[rb], 0, 0, 0);
+      for (int rb = 0; rb < RB; ++rb) acc[rb] = M::mma(ai[rb][s], qv, acc[rb]);
     }
 #pragma unroll
     for (int rb = 0; rb < RB; ++rb)
 #pragma unroll
       for (int v = 0; v < 4; ++v) {
-        const int p = 16 * rb + 4 * krow + v;
+        const int p = 16 * rb + M::row(krow, v);
         if (p < NF) sT[tn * TS + p * E + e] = acc[rb][v];
       }
   }
@@ -303,7 +343,7 @@ k_stage(StageArgs a) {
   __syncthreads();
   const int te = tid % E, tn = tid / E;  // (element, variable) of this thread
   if constexpr (UseMfma<N, C, SHOCK>::value) {
-    traces_mfma<N>(sQ, sT, tn, te, TS);
+    traces_mfma<N, C>(sQ, sT, tn, te, TS);
   } else {
     C q[NpInt];
 #pragma unroll
@@ -484,11 +524,12 @@ k_stage(StageArgs a) {
     // flux at the points 4g + krow of element 16b + col and feeds the r and
     // s components as the B operands of two K-steps, then the face fluxes.
     static_assert(E == 64, "one wave per variable");
+    using M = Mma<C>;
     const int col = te & 15, krow = te >> 4;
-    float af[Sh::KS];
+    C af[Sh::KS];
 #pragma unroll
-    for (int s2 = 0; s2 < Sh::KS; ++s2) af[s2] = cMfA[s2 * 64 + te];
-    floatx4 out[4];
+    for (int s2 = 0; s2 < Sh::KS; ++s2) af[s2] = mfa_div<C>()[s2 * 64 + te];
+    typename M::vec out[4];
 #pragma unroll
     for (int b = 0; b < 4; ++b) {
       const int e = 16 * b + col, kk = k0 + e;
@@ -496,7 +537,7 @@ k_stage(StageArgs a) {
       const C Jdc = C(Jd);
       const C J0 = C(a.g.Jinv[kk]), J1 = C(a.g.Jinv[size_t(Kp) + kk]);
       const C J2 = C(a.g.Jinv[2 * size_t(Kp) + kk]), J3 = C(a.g.Jinv[3 * size_t(Kp) + kk]);
-      floatx4 acc = {0.f, 0.f, 0.f, 0.f};
+      typename M::vec acc = M::zero();
 #pragma unroll
       for (int g2 = 0; g2 < Sh::G; ++g2) {
         // past NpInt the A column is zero: any finite point value will do
@@ -506,13 +547,13 @@ k_stage(StageArgs a) {
         C Fx, Fy;
         flux_component(tn, g, r0, r1, r2, r3, Fx, Fy);
         const C Fr = Jdc * (J0 * Fx + J1 * Fy), Fs = Jdc * (J2 * Fx + J3 * Fy);
-        acc = __builtin_amdgcn_mfma_f32_16x16x4f32(af[2 * g2], Fr, acc, 0, 0, 0);
-        acc = __builtin_amdgcn_mfma_f32_16x16x4f32(af[2 * g2 + 1], Fs, acc, 0, 0, 0);
+        acc = M::mma(af[2 * g2], Fr, acc);
+        acc = M::mma(af[2 * g2 + 1], Fs, acc);
       }
 #pragma unroll
       for (int s2 = 0; s2 < Sh::KE; ++s2) {
         const int p = min(4 * s2 + krow, NF - 1);
-        acc = __builtin_amdgcn_mfma_f32_16x16x4f32(af[2 * Sh::G + s2], sT[tn * TS + p * E + e], acc, 0, 0, 0);
+        acc = M::mma(af[2 * Sh::G + s2], sT[tn * TS + p * E + e], acc);
       }
       out[b] = acc * (-C(1. / Jd));
     }
@@ -521,7 +562,7 @@ k_stage(StageArgs a) {
     for (int b = 0; b < 4; ++b) {
 #pragma unroll
       for (int v = 0; v < 4; ++v) {
-        const int i = 4 * krow + v;
+        const int i = M::row(krow, v);
         if (i < NpInt) sR[(tn * NpInt + i) * P + 16 * b + col] = out[b][v];
       }
     }
@@ -756,7 +797,7 @@ k_stage(StageArgs a) {
 
     // ---- F: traces of the new state (own faces) ----
     if constexpr (UseMfma<N, C, SHOCK>::value) {
-      traces_mfma<N>(sQ, sT, tn, te, TS);
+      traces_mfma<N, C>(sQ, sT, tn, te, TS);
     } else {
       C q[NpInt];
 #pragma unroll
@@ -1237,21 +1278,23 @@ hipError_t upload_operators_n(const double *divT, const double *interp) {
   static_assert(Sh::KS * 64 <= kMfMax && Sh::RB * Sh::KI * 64 <= kMfIMax, "MFMA operand tables");
   {
     // A fragments of the MFMA FluxEdgeInterp contraction
-    std::vector<float> ai(size_t(Sh::RB) * Sh::KI * 64, 0.f);
+    std::vector<double> ai(size_t(Sh::RB) * Sh::KI * 64, 0.);
     for (int rb = 0; rb < Sh::RB; ++rb)
       for (int s = 0; s < Sh::KI; ++s)
         for (int l = 0; l < 64; ++l) {
           const int p = 16 * rb + l % 16, i = 4 * s + l / 16;
-          if (p < Sh::NF && i < Sh::NpInt)
-            ai[(size_t(rb) * Sh::KI + s) * 64 + l] = float(interp[size_t(p) * Sh::NpInt + i]);
+          if (p < Sh::NF && i < Sh::NpInt) ai[(size_t(rb) * Sh::KI + s) * 64 + l] = interp[size_t(p) * Sh::NpInt + i];
         }
-    hipError_t e = hipMemcpyToSymbol(HIP_SYMBOL(cMfI), ai.data(), ai.size() * sizeof(float), 0,
+    const std::vector<float> aif(ai.begin(), ai.end());
+    hipError_t e = hipMemcpyToSymbol(HIP_SYMBOL(cMfI), aif.data(), aif.size() * sizeof(float), 0,
                                      hipMemcpyHostToDevice);
+    if (e != hipSuccess) return e;
+    e = hipMemcpyToSymbol(HIP_SYMBOL(cMfID), ai.data(), ai.size() * sizeof(double), 0, hipMemcpyHostToDevice);
     if (e != hipSuccess) return e;
   }
   {
-    // A fragments of the MFMA DivInt contraction (the K-row order of MfmaDiv)
-    std::vector<float> af(size_t(Sh::KS) * 64, 0.f);
+    // A fragments of the MFMA DivInt contraction (the K-row order of phase C)
+    std::vector<double> af(size_t(Sh::KS) * 64, 0.);
     for (int s = 0; s < Sh::KS; ++s)
       for (int l = 0; l < 64; ++l) {
         const int row = l % 16, kr = l / 16;
@@ -1263,10 +1306,13 @@ hipError_t upload_operators_n(const double *divT, const double *interp) {
           const int p = 4 * (s - 2 * Sh::G) + kr;
           if (p < Sh::NF) j = 2 * Sh::NpInt + p;
         }
-        if (row < Sh::NpInt && j >= 0) af[size_t(s) * 64 + l] = float(divT[size_t(j) * Sh::NpInt + row]);
+        if (row < Sh::NpInt && j >= 0) af[size_t(s) * 64 + l] = divT[size_t(j) * Sh::NpInt + row];
       }
-    hipError_t e = hipMemcpyToSymbol(HIP_SYMBOL(cMfA), af.data(), af.size() * sizeof(float), 0,
+    const std::vector<float> aff(af.begin(), af.end());
+    hipError_t e = hipMemcpyToSymbol(HIP_SYMBOL(cMfA), aff.data(), aff.size() * sizeof(float), 0,
                                      hipMemcpyHostToDevice);
+    if (e != hipSuccess) return e;
+    e = hipMemcpyToSymbol(HIP_SYMBOL(cMfAD), af.data(), af.size() * sizeof(double), 0, hipMemcpyHostToDevice);
     if (e != hipSuccess) return e;
   }
   std::vector<double> d(static_cast<size_t>(Sh::ops));
