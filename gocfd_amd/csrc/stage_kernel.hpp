@@ -101,6 +101,11 @@ struct Mma<double> {
   static __device__ __forceinline__ int row(int krow, int v) { return krow + 4 * v; }
 };
 
+// Element of column col of MFMA column block b. Pairing the blocks
+// (e = 2 col + (b & 1) + 32 (b >> 1)) puts the two K-rows of a half-wave
+// operand read (row pitch 65 words) on even and odd LDS banks.
+__device__ __forceinline__ int mfma_elem(int b, int col) { return 2 * col + (b & 1) + 32 * (b >> 1); }
+
 // The operators of this translation unit's order in constant memory, so that
 // every coefficient is a uniform scalar load: DivInt^T [NpFlux][NpInt]
 // followed by FluxEdgeInterp [3 NpEdge][NpInt], in FP64 and FP32.
@@ -133,7 +138,7 @@ __device__ __forceinline__ const double *mfa_interp<double>() { return cMfID; }
 // Face traces sT[var][p][e] = sum_i FluxEdgeInterp[p][i] sQ[var][i][e] of
 // variable tn (= the wave) for the tile's 64 elements on the matrix cores:
 // four 16-element column blocks, lane (col, krow) supplies the solution
-// point 4s + krow of element 16b + col as the B operand of K-step s.
+// point 4s + krow of element mfma_elem(b, col) as the B operand of K-step s.
 template <int N, class C>
 __device__ __forceinline__ void traces_mfma(const C *sQ, C *sT, int tn, int te, int TS) {
   using Sh = Shape<N>;
@@ -148,7 +153,7 @@ __device__ __forceinline__ void traces_mfma(const C *sQ, C *sT, int tn, int te, 
     for (int s = 0; s < KI; ++s) ai[rb][s] = tab[(rb * KI + s) * 64 + te];
 #pragma unroll
   for (int b = 0; b < 4; ++b) {
-    const int e = 16 * b + col;
+    const int e = mfma_elem(b, col);
     typename M::vec acc[RB];
 #pragma unroll
     for (int rb = 0; rb < RB; ++rb) acc[rb] = M::zero();
@@ -521,7 +526,7 @@ k_stage(StageArgs a) {
   if constexpr (UseMfma<N, C, SHOCK>::value) {
     // Matrix cores: wave tn forms variable tn of the tile's 64 elements in
     // four 16-element column blocks; lane (col, krow) evaluates the interior
-    // flux at the points 4g + krow of element 16b + col and feeds the r and
+    // flux at the points 4g + krow of element mfma_elem(b, col) and feeds the r and
     // s components as the B operands of two K-steps, then the face fluxes.
     static_assert(E == 64, "one wave per variable");
     using M = Mma<C>;
@@ -532,7 +537,7 @@ k_stage(StageArgs a) {
     typename M::vec out[4];
 #pragma unroll
     for (int b = 0; b < 4; ++b) {
-      const int e = 16 * b + col, kk = k0 + e;
+      const int e = mfma_elem(b, col), kk = k0 + e;
       const double Jd = a.g.Jdet[kk];
       const C Jdc = C(Jd);
       const C J0 = C(a.g.Jinv[kk]), J1 = C(a.g.Jinv[size_t(Kp) + kk]);
@@ -563,7 +568,7 @@ k_stage(StageArgs a) {
 #pragma unroll
       for (int v = 0; v < 4; ++v) {
         const int i = M::row(krow, v);
-        if (i < NpInt) sR[(tn * NpInt + i) * P + 16 * b + col] = out[b][v];
+        if (i < NpInt) sR[(tn * NpInt + i) * P + mfma_elem(b, col)] = out[b][v];
       }
     }
   } else {
