@@ -25,6 +25,7 @@ EXPORTS = [
     "gocfd_euler2d_tiling", "gocfd_euler2d_get_shock", "gocfd_euler2d_get_element_dt",
     "gocfd_ensemble_create", "gocfd_ensemble_destroy", "gocfd_ensemble_case_dims", "gocfd_ensemble_step",
     "gocfd_ensemble_get_state", "gocfd_ensemble_time", "gocfd_ensemble_kernel_time",
+    "gocfd_ensemble_set_launch", "gocfd_ensemble_get_launch", "gocfd_ensemble_tune",
 ]
 
 
@@ -102,6 +103,9 @@ def load() -> C.CDLL:
     L.gocfd_ensemble_get_state.argtypes = [_vp, C.c_int32, _dp]
     L.gocfd_ensemble_time.argtypes = [_vp, C.c_int32, _dp, _dp, _i32p]
     L.gocfd_ensemble_kernel_time.argtypes = [_vp, _dp, _i32p]
+    L.gocfd_ensemble_set_launch.argtypes = [_vp, C.c_int32, C.c_int32, C.c_int32]
+    L.gocfd_ensemble_get_launch.argtypes = [_vp, _i32p, _i32p, _i32p]
+    L.gocfd_ensemble_tune.argtypes = [_vp, C.c_int32]
     for name in EXPORTS:  # ctypes default restype is int: every other entry returns a status
         getattr(L, name)
     _lib = L

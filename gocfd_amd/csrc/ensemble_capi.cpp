@@ -7,8 +7,11 @@
 // so a tile belongs to one case); each stage is one launch per group. Every
 // case keeps its own control block (time, dt, wave-speed maxima) and
 // physics constants, selected per tile. A run of 6 steps -- the period of
-// the per-stage ring and time-parity arguments -- is captured once as a HIP
-// graph and replayed.
+// the per-stage ring and time-parity arguments -- or a multiple of it is
+// captured once as a HIP graph and replayed. The launch configuration (the
+// groups serialized on one stream or as parallel graph branches, the steps
+// per graph, the XCD tile order) can be picked by timing the candidates
+// (gocfd_ensemble_tune); it does not change the results.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -27,7 +30,7 @@ using namespace gocfd::capi;
 
 namespace {
 
-constexpr int kGraphSteps = 6;
+constexpr int kGraphPeriod = 6;  // steps after which every per-stage argument repeats
 
 struct EnsCase {
   gocfd_params prm{};
@@ -45,9 +48,14 @@ struct EnsGroup {
   DevBuf<int> code, nbr_slot, halo_elem, halo_face, wave_list, wave_cnt, flux_list, flux_cnt, tile_case;
   void *q[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
   void *R = nullptr;
+  std::vector<double> q0_init;         // initial element-major state (tuning restores it)
+  hipStream_t stream = nullptr;        // the group's branch of a parallel graph
+  hipEvent_t done = nullptr;
   ~EnsGroup() {
     for (auto &p : q) if (p) (void)hipFree(p);
     if (R) (void)hipFree(R);
+    if (done) (void)hipEventDestroy(done);
+    if (stream) (void)hipStreamDestroy(stream);
   }
 };
 
@@ -61,28 +69,40 @@ struct gocfd_ensemble {
   DevBuf<dev::CaseInfo> cinfo;
   dev::Ctl *ctl = nullptr;
   hipStream_t stream = nullptr;
-  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr, fork = nullptr;
   hipGraph_t graph = nullptr;
   hipGraphExec_t gexec = nullptr;
   long long gstage = 0;
   int step_index = 0;
   double last_ms = 0;
   int last_steps = 0;
+  // launch configuration
+  int parallel = 0;              // groups as parallel graph branches
+  int graph_steps = kGraphPeriod;
+  int xcd_tiles = 1;
 
   ~gocfd_ensemble() {
     if (stream) (void)hipStreamSynchronize(stream);
-    if (gexec) (void)hipGraphExecDestroy(gexec);
-    if (graph) (void)hipGraphDestroy(graph);
+    drop_graph();
     groups.clear();
     if (ctl) (void)hipFree(ctl);
     if (ev0) (void)hipEventDestroy(ev0);
     if (ev1) (void)hipEventDestroy(ev1);
+    if (fork) (void)hipEventDestroy(fork);
     if (stream) (void)hipStreamDestroy(stream);
+  }
+
+  void drop_graph() {
+    if (gexec) (void)hipGraphExecDestroy(gexec);
+    if (graph) (void)hipGraphDestroy(graph);
+    gexec = nullptr;
+    graph = nullptr;
   }
 
   dev::StageArgs args(const EnsGroup &g) const {
     dev::StageArgs a{};
     a.K = g.Kp; a.Kp = g.Kp; a.ntiles = g.ntiles; a.Hmax = g.Hmax; a.Wmax = g.Wmax; a.Fmax = g.Fmax;
+    a.xcd_tiles = xcd_tiles;
     a.q0 = g.q[0]; a.q1 = g.q[1]; a.q2 = g.q[2]; a.q3 = g.q[3]; a.q4 = g.q[4]; a.R = g.R;
     a.ctl = ctl;
     a.cases = cinfo.p;
@@ -112,7 +132,9 @@ struct gocfd_ensemble {
     }
   }
 
-  void enqueue_step() {
+  // One SSP54 step of every group; with `branches` each group's stages go
+  // to the group's own stream (inside a graph capture: parallel branches).
+  void enqueue_step(bool branches) {
     for (int s = 0; s < 5; ++s) {
       for (auto &g : groups) {
         dev::StageArgs a = args(*g);
@@ -121,42 +143,109 @@ struct gocfd_ensemble {
         a.ring = int(gstage % 3);
         a.par2 = int(gstage & 1);
         a.qs = g->q[s];
-        HIPCHECK(dev::launch_stage(g->N, prec, 0, a, stream));
+        HIPCHECK(dev::launch_stage(g->N, prec, 0, a, branches ? g->stream : stream));
       }
       ++gstage;
     }
     ++step_index;
   }
 
-  // Steps run eagerly until the step counter is 1 mod 6; from there whole
-  // blocks of 6 steps replay the captured graph (the first block is
-  // captured on first use), and the remainder runs eagerly again.
+  // Steps run eagerly (one stream) until the step counter is 1 mod
+  // graph_steps; from there whole blocks replay the captured graph (captured
+  // on first use; the groups fork from the ensemble's stream and join it
+  // again inside the graph when `parallel`), and the remainder runs eagerly.
   void enqueue_steps(int nsteps) {
     HIPCHECK(hipEventRecord(ev0, stream));
     int left = nsteps;
     while (left > 0) {
-      if (step_index % kGraphSteps == 1 && left >= kGraphSteps) {
+      if (step_index % graph_steps == 1 && left >= graph_steps) {
         if (!gexec) {
           HIPCHECK(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
           const long long g0 = gstage;
           const int s0 = step_index;
-          for (int j = 0; j < kGraphSteps; ++j) enqueue_step();
+          const bool br = parallel && groups.size() > 1;
+          if (br) {
+            HIPCHECK(hipEventRecord(fork, stream));
+            for (auto &g : groups) HIPCHECK(hipStreamWaitEvent(g->stream, fork, 0));
+          }
+          for (int j = 0; j < graph_steps; ++j) enqueue_step(br);
+          if (br) {
+            for (auto &g : groups) {
+              HIPCHECK(hipEventRecord(g->done, g->stream));
+              HIPCHECK(hipStreamWaitEvent(stream, g->done, 0));
+            }
+          }
           HIPCHECK(hipStreamEndCapture(stream, &graph));
           HIPCHECK(hipGraphInstantiate(&gexec, graph, nullptr, nullptr, 0));
           gstage = g0;
           step_index = s0;
         }
         HIPCHECK(hipGraphLaunch(gexec, stream));
-        gstage += 5LL * kGraphSteps;
-        step_index += kGraphSteps;
-        left -= kGraphSteps;
+        gstage += 5LL * graph_steps;
+        step_index += graph_steps;
+        left -= graph_steps;
       } else {
-        enqueue_step();
+        enqueue_step(false);
         --left;
       }
     }
     HIPCHECK(hipEventRecord(ev1, stream));
     last_steps = nsteps;
+  }
+
+  // Back to the initial state and time 0 (the stage states and R are
+  // written before they are read in every step).
+  void reset_state() {
+    HIPCHECK(hipStreamSynchronize(stream));
+    for (auto &g : groups)
+      HIPCHECK(hipMemcpy(g->q[0], g->q0_init.data(), g->q0_init.size() * sizeof(double), hipMemcpyHostToDevice));
+    restart();
+    HIPCHECK(hipStreamSynchronize(stream));
+  }
+
+  int set_launch(int par, int gsteps, int xcd) {
+    if (gsteps < kGraphPeriod || gsteps % kGraphPeriod != 0 || gsteps > 600)
+      return fail(GOCFD_ERR_ARG, "graph_steps must be a positive multiple of 6 (at most 600)");
+    HIPCHECK(hipStreamSynchronize(stream));
+    drop_graph();
+    parallel = par ? 1 : 0;
+    graph_steps = gsteps;
+    xcd_tiles = xcd ? 1 : 0;
+    return GOCFD_OK;
+  }
+
+  // Times every candidate configuration over `blocks` graph replays from
+  // the initial state, keeps the fastest and restores the initial state.
+  int tune(int blocks) {
+    struct Cfg {
+      int par, gs, xcd;
+    };
+    std::vector<Cfg> cand;
+    for (int par = 0; par <= (groups.size() > 1 ? 1 : 0); ++par)
+      for (int gs : {kGraphPeriod, 4 * kGraphPeriod})
+        for (int xcd : {1, 0}) cand.push_back({par, gs, xcd});
+    double best = 1e300;
+    Cfg pick = cand[0];
+    for (const Cfg &c : cand) {
+      int rc = set_launch(c.par, c.gs, c.xcd);
+      if (rc != GOCFD_OK) return rc;
+      reset_state();
+      enqueue_steps(1 + c.gs);  // eager first step, then the block that captures the graph
+      if ((rc = finish()) != GOCFD_OK) return rc;
+      enqueue_steps(blocks * c.gs);
+      if ((rc = finish()) != GOCFD_OK) return rc;
+      const double ms = last_ms / (blocks * c.gs);
+      if (ms < best) {
+        best = ms;
+        pick = c;
+      }
+    }
+    int rc = set_launch(pick.par, pick.gs, pick.xcd);
+    if (rc != GOCFD_OK) return rc;
+    reset_state();
+    last_ms = best;
+    last_steps = 1;
+    return GOCFD_OK;
   }
 
   int finish() {
@@ -249,6 +338,9 @@ void build_group(gocfd_ensemble &h, EnsGroup &g) {
   const size_t nval = Kp * g.NV;
   HIPCHECK(hipMalloc(&g.q[0], nval * sizeof(double)));
   HIPCHECK(hipMemcpy(g.q[0], q0.data(), nval * sizeof(double), hipMemcpyHostToDevice));
+  g.q0_init = std::move(q0);
+  HIPCHECK(hipStreamCreateWithFlags(&g.stream, hipStreamNonBlocking));
+  HIPCHECK(hipEventCreateWithFlags(&g.done, hipEventDisableTiming));
   for (int s = 1; s < 5; ++s) {
     HIPCHECK(hipMalloc(&g.q[s], nval * h.ssize));
     HIPCHECK(hipMemset(g.q[s], 0, nval * h.ssize));
@@ -303,6 +395,7 @@ int gocfd_ensemble_create(int32_t ncases, const gocfd_mesh *const *meshes, const
   HIPCHECK(hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking));
   HIPCHECK(hipEventCreate(&h->ev0));
   HIPCHECK(hipEventCreate(&h->ev1));
+  HIPCHECK(hipEventCreateWithFlags(&h->fork, hipEventDisableTiming));
   for (auto &g : h->groups) build_group(*h, *g);
   std::vector<dev::CaseInfo> ci(ncases);
   for (int i = 0; i < ncases; ++i) {
@@ -364,6 +457,31 @@ int gocfd_ensemble_time(gocfd_ensemble *h, int32_t c, double *time, double *glob
   if (global_dt) *global_dt = k.dt[4];
   if (steps) *steps = k.steps;
   return GOCFD_OK;
+  GUARD_END
+}
+
+int gocfd_ensemble_set_launch(gocfd_ensemble *h, int32_t parallel, int32_t graph_steps, int32_t xcd_tiles) {
+  GUARD_BEGIN
+  if (!h) return fail(GOCFD_ERR_ARG, "null handle");
+  HIPCHECK(hipSetDevice(h->device));
+  return h->set_launch(parallel, graph_steps, xcd_tiles);
+  GUARD_END
+}
+
+int gocfd_ensemble_get_launch(gocfd_ensemble *h, int32_t *parallel, int32_t *graph_steps, int32_t *xcd_tiles) {
+  if (!h) return fail(GOCFD_ERR_ARG, "null handle");
+  if (parallel) *parallel = h->parallel;
+  if (graph_steps) *graph_steps = h->graph_steps;
+  if (xcd_tiles) *xcd_tiles = h->xcd_tiles;
+  return GOCFD_OK;
+}
+
+int gocfd_ensemble_tune(gocfd_ensemble *h, int32_t trial_blocks) {
+  GUARD_BEGIN
+  if (!h || trial_blocks <= 0) return fail(GOCFD_ERR_ARG, "bad argument");
+  if (h->step_index != 0) return fail(GOCFD_ERR_ARG, "tune before the first step (tuning restarts from the initial state)");
+  HIPCHECK(hipSetDevice(h->device));
+  return h->tune(trial_blocks);
   GUARD_END
 }
 

@@ -71,6 +71,7 @@ struct gocfd_euler2d {
   dev::StageArgs args() const {
     dev::StageArgs a{};
     a.K = L.K; a.Kp = L.Kp; a.ntiles = L.ntiles; a.Hmax = L.Hmax; a.Wmax = L.Wmax; a.Fmax = L.Fmax;
+    a.xcd_tiles = 1;
     a.q0 = q[0]; a.q1 = q[1]; a.q2 = q[2]; a.q3 = q[3]; a.q4 = q[4]; a.R = R;
     a.ctl = ctl;
     a.g.Jdet = Jdet.p; a.g.Jinv = Jinv.p; a.g.IInII = IInII.p; a.g.Nx = Nx.p; a.g.Ny = Ny.p;

@@ -100,6 +100,7 @@ struct ShockArgs {
 
 struct StageArgs {
   int K, Kp, ntiles, Hmax, Wmax, Fmax;
+  int xcd_tiles;       // 1: each XCD takes a contiguous tile range; 0: tile = workgroup id
   int stage;           // 0..4, or -1 for the RHS-only evaluation
   int step_index;      // steps since the state was set (selects time parity)
   int ring;            // global stage counter mod 3

@@ -182,8 +182,9 @@ __device__ __forceinline__ const float *ops_base<float>() { return cOpsF; }
 
 // Tile index of a workgroup: with workgroups dealt round-robin to the 8 XCDs,
 // XCD x processes the contiguous tile range [x*q, (x+1)*q), so neighbouring
-// tiles share an L2.
-__device__ __forceinline__ int tile_of_block() {
+// tiles share an L2 (xcd_tiles = 1); otherwise tile = workgroup id.
+__device__ __forceinline__ int tile_of_block(int xcd_tiles) {
+  if (!xcd_tiles) return blockIdx.x;
   const int q = gridDim.x >> 3;
   return (blockIdx.x & 7) * q + (blockIdx.x >> 3);
 }
@@ -284,7 +285,7 @@ k_stage(StageArgs a) {
   unsigned long long *sWf = reinterpret_cast<unsigned long long *>(sDT + kTile);
   __shared__ double red[kThreads / 64];
 
-  const int tile = tile_of_block();
+  const int tile = tile_of_block(a.xcd_tiles);
   if (tile >= a.ntiles) return;
   const int tid = threadIdx.x;
   const bool rhs_only = a.stage < 0;

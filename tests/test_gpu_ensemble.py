@@ -8,6 +8,7 @@ import pytest
 
 import oracle
 from gocfd_amd import Ensemble, Euler, InputParameters2D, Mesh, meshgen
+from gocfd_amd._lib import GocfdError
 
 pytestmark = pytest.mark.gpu
 
@@ -82,3 +83,37 @@ def test_ensemble_cases_stop_at_their_final_time(tmp_path):
     single = Euler(cases[1][0], cases[1][1], precision="fp64")
     single.step(40)
     assert np.array_equal(ens.state(1), single.Q)
+
+
+@pytest.mark.parametrize("prec", ["fp64", "fp32"])
+def test_launch_configurations_are_bitwise_identical(tmp_path, prec):
+    """Order groups as parallel graph branches, 24-step graphs and the plain
+    tile order change the schedule, not the arithmetic."""
+    cases = _cases(tmp_path)
+    ref = Ensemble(cases, precision=prec)
+    assert ref.launch() == {"parallel": False, "graph_steps": 6, "xcd_tiles": True}
+    ref.step(31)
+    for par, gs, xcd in [(True, 6, True), (True, 24, False), (False, 12, False)]:
+        ens = Ensemble(cases, precision=prec)
+        ens.set_launch(par, gs, xcd)
+        ens.step(31)
+        for c in range(len(cases)):
+            assert np.array_equal(ens.state(c), ref.state(c)), (par, gs, xcd, c)
+            assert ens.time_info(c) == ref.time_info(c)
+
+
+def test_tune_restores_the_initial_state(tmp_path):
+    cases = _cases(tmp_path)
+    ens = Ensemble(cases, precision="fp32")
+    choice = ens.tune(1)
+    assert choice["graph_steps"] in (6, 24) and choice["ms_per_step"] > 0
+    for c in range(len(cases)):
+        t, _, steps = ens.time_info(c)
+        assert t == 0.0 and steps == 0
+    ens.step(30)
+    ref = Ensemble(cases, precision="fp32")
+    ref.step(30)
+    for c in range(len(cases)):
+        assert np.array_equal(ens.state(c), ref.state(c)), c
+    with pytest.raises(GocfdError):
+        ens.tune(1)
