@@ -288,27 +288,13 @@ k_stage(StageArgs a) {
   const int tile = tile_of_block(a.xcd_tiles);
   if (tile >= a.ntiles) return;
   const int tid = threadIdx.x;
-  const bool rhs_only = a.stage < 0;
-  const int par = a.step_index & 1;
-  const int cs = a.tile_case[tile];
-  const CaseInfo &ci = a.cases[cs];
-  const Phys &ph = ci.ph;
-  const bool first = tile == ci.tile0;  // does the case's control bookkeeping
-  Ctl *ctl = a.ctl + cs;
-  const double time = rhs_only ? a.t_rhs : ctl->time[par];
-  if (step_skipped(a, ctl, ph.final_time)) {
-    if (a.stage == 4 && first && threadIdx.x == 0) ctl->time[par ^ 1] = time;
-    return;
-  }
   const int k0 = tile * E;
   const int Kp = a.Kp;
   const SIN *qs = static_cast<const SIN *>(a.qs);
-  const C *DvT = ops_base<C>();                         // DivInt^T [NpFlux][NpInt]
-  const C *Itp = DvT + Sh::NpFlux * NpInt;              // FluxEdgeInterp [NF][NpInt]
-  const C g = C(ph.gamma);
 
-  // The tile's work lists do not depend on the state: their loads go out
-  // first, so that their latency overlaps the state load.
+  // The tile's work lists and state do not depend on the control block:
+  // their loads go out first, so that their latency overlaps the chain
+  // tile -> case -> control block and each other.
   constexpr int ITB = (E * NF + kThreads - 1) / kThreads;
   const int nfl = a.g.flux_cnt[tile] * NpEdge;
   const int *flist = a.g.flux_list + size_t(tile) * a.Fmax;
@@ -329,14 +315,32 @@ k_stage(StageArgs a) {
   // Every load is issued before the first LDS store, so a thread's LPT
   // loads are in flight together.
   constexpr int LPT = (E * NV + kThreads - 1) / kThreads;
+  SIN v[LPT];
   {
     const SIN *qt = qs + size_t(k0) * NV;
-    SIN v[LPT];
 #pragma unroll
     for (int j = 0; j < LPT; ++j) {
       const int idx = tid + j * kThreads;
       v[j] = (idx < E * NV) ? qt[idx] : S(0);
     }
+  }
+
+  const bool rhs_only = a.stage < 0;
+  const int par = a.step_index & 1;
+  const int cs = a.tile_case[tile];
+  const CaseInfo &ci = a.cases[cs];
+  const Phys &ph = ci.ph;
+  const bool first = tile == ci.tile0;  // does the case's control bookkeeping
+  Ctl *ctl = a.ctl + cs;
+  const double time = rhs_only ? a.t_rhs : ctl->time[par];
+  if (step_skipped(a, ctl, ph.final_time)) {
+    if (a.stage == 4 && first && threadIdx.x == 0) ctl->time[par ^ 1] = time;
+    return;
+  }
+  const C *DvT = ops_base<C>();                         // DivInt^T [NpFlux][NpInt]
+  const C *Itp = DvT + Sh::NpFlux * NpInt;              // FluxEdgeInterp [NF][NpInt]
+  const C g = C(ph.gamma);
+  {
 #pragma unroll
     for (int j = 0; j < LPT; ++j) {
       const int idx = tid + j * kThreads;
