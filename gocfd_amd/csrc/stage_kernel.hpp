@@ -1074,15 +1074,15 @@ __global__ void __launch_bounds__(kThreads) k_shock_grad(StageArgs a) {
   D u[NpInt];
 #pragma unroll
   for (int i = 0; i < NpInt; ++i) u[i] = D(Q[size_t(kk) * NV + tn * NpInt + i]);
+  // eps is zero at every RT point of an inactive element: its eps*grad is
+  // zero, the stage kernel does not read it, and only the own traces (for
+  // the wave speeds) are formed
+  const bool act = a.sh.active[kk] != 0;
   // edge values: the owner's trace at this face's points, in this face's order
   D ue[NF];
 #pragma unroll
   for (int f = 0; f < 3; ++f) {
     const size_t fi = size_t(f) * Kp + kk;
-    const int code = a.g.code[fi];
-    // unshared faces (boundary, padding) are always their own edge's owner
-    const bool owner = (code & FC_FLUX_OWNER) || (code & 15) != FK_Shared;
-    const int nb = a.sh.nbr_elem[fi], nf = a.sh.nbr_face[fi];
 #pragma unroll
     for (int i = 0; i < NpEdge; ++i) {
       D t = 0;
@@ -1091,7 +1091,12 @@ __global__ void __launch_bounds__(kThreads) k_shock_grad(StageArgs a) {
       sTr[tn][f * NpEdge + i][te] = t;
       ue[f * NpEdge + i] = t;
     }
+    if (!act) continue;
+    const int code = a.g.code[fi];
+    // unshared faces (boundary, padding) are always their own edge's owner
+    const bool owner = (code & FC_FLUX_OWNER) || (code & 15) != FK_Shared;
     if (!owner) {
+      const int nb = a.sh.nbr_elem[fi], nf = a.sh.nbr_face[fi];
       const SIN *qn = Q + size_t(nb) * NV + tn * NpInt;
 #pragma unroll
       for (int i = 0; i < NpEdge; ++i) {
@@ -1103,36 +1108,36 @@ __global__ void __launch_bounds__(kThreads) k_shock_grad(StageArgs a) {
       }
     }
   }
-  const double Jd = a.g.Jdet[kk], ooJd = 1. / Jd;
-  const D Ji[4] = {D(a.g.Jinv[kk]), D(a.g.Jinv[size_t(Kp) + kk]), D(a.g.Jinv[2 * size_t(Kp) + kk]),
-                   D(a.g.Jinv[3 * size_t(Kp) + kk])};
-  const double *epsF = a.sh.epsF + kk;                                        // [NR][Kp]
-  D *dout = static_cast<D *>(a.sh.diss) + size_t(tn * 2 * NR) * Kp + kk;      // [4][2][NR][Kp]
-  // eps is zero at every RT point of an inactive element: its eps*grad is
-  // zero, and the stage kernel does not read it
-  const bool act = a.sh.active[kk] != 0;
+  if (act) {
+    const double Jd = a.g.Jdet[kk], ooJd = 1. / Jd;
+    const D J0 = D(a.g.Jinv[kk]), J1 = D(a.g.Jinv[size_t(Kp) + kk]);
+    const D J2 = D(a.g.Jinv[2 * size_t(Kp) + kk]), J3 = D(a.g.Jinv[3 * size_t(Kp) + kk]);
+    const double *epsF = a.sh.epsF + kk;                                    // [NR][Kp]
+    D *dout = static_cast<D *>(a.sh.diss) + size_t(tn * 2 * NR) * Kp + kk;  // [4][2][NR][Kp]
 #pragma unroll 1
-  for (int dir = 0; dir < (act ? 2 : 0); ++dir) {
-    D dof[NpFlux];
+    for (int dir = 0; dir < 2; ++dir) {
+      const D jr = dir == 0 ? J0 : J1, js = dir == 0 ? J2 : J3;
+      D dof[NpFlux];
 #pragma unroll
-    for (int i = 0; i < NpInt; ++i) {
-      dof[i] = Ji[dir] * u[i];
-      dof[NpInt + i] = Ji[2 + dir] * u[i];
-    }
+      for (int i = 0; i < NpInt; ++i) {
+        dof[i] = jr * u[i];
+        dof[NpInt + i] = js * u[i];
+      }
 #pragma unroll
-    for (int f = 0; f < 3; ++f) {
-      const size_t fi = size_t(f) * Kp + kk;
-      const double nrm = dir == 0 ? a.sh.Nxo[fi] : a.sh.Nyo[fi];
-      const D w = D(ooJd * nrm * a.g.IInII[fi]);
+      for (int f = 0; f < 3; ++f) {
+        const size_t fi = size_t(f) * Kp + kk;
+        const double nrm = dir == 0 ? a.sh.Nxo[fi] : a.sh.Nyo[fi];
+        const D w = D(ooJd * nrm * a.g.IInII[fi]);
 #pragma unroll
-      for (int i = 0; i < NpEdge; ++i) dof[2 * NpInt + f * NpEdge + i] = w * ue[f * NpEdge + i];
-    }
+        for (int i = 0; i < NpEdge; ++i) dof[2 * NpInt + f * NpEdge + i] = w * ue[f * NpEdge + i];
+      }
 #pragma unroll 3
-    for (int r = 0; r < NR; ++r) {
-      D acc = 0;
+      for (int r = 0; r < NR; ++r) {
+        D acc = 0;
 #pragma unroll
-      for (int j = 0; j < NpFlux; ++j) acc += Div[r * NpFlux + j] * dof[j];
-      dout[size_t(dir * NR + r) * Kp] = acc * D(epsF[size_t(r) * Kp]);
+        for (int j = 0; j < NpFlux; ++j) acc += Div[r * NpFlux + j] * dof[j];
+        dout[size_t(dir * NR + r) * Kp] = acc * D(epsF[size_t(r) * Kp]);
+      }
     }
   }
   if (a.stage < 0) return;
