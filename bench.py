@@ -88,15 +88,15 @@ def shock_alg_bytes_per_elem_stage(np_int: int, np_edge: int, state_bytes: int =
     2 x 4 x NR points out, diss_bytes each) and the stage kernel's reads of
     eps and eps*grad; NR = NpInt + 3 NpEdge. The stage input is q0 (FP64) at
     one stage of five and a stage state (state_bytes) at the other four.
-    eps*grad is written and read only for active elements (a vertex carries
-    eps > 0; elsewhere it is identically zero): those terms and the stage
-    kernel's eps reads scale with the active fraction."""
+    eps at the RT points and eps*grad are written and read only for active
+    elements (a vertex carries eps > 0; elsewhere both are identically zero):
+    those terms scale with the active fraction."""
     nr = np_int + 3 * np_edge
     nv = 4 * np_int
     sin = (8 + 4 * state_bytes) / 5.0
     plain = alg_bytes_per_elem_stage(np_int, state_bytes)
     sensor = np_int * sin + 16 + 48
-    prep = 48 + nr * 8 + 8
+    prep = 48 + nr * 8 * active + 8
     grad = nv * sin + nr * 8 + 144 + 8 * nr * diss_bytes * active
     stage_extra = (nr * 8 + 8 * nr * diss_bytes) * active + 8
     return plain + sensor + prep + grad + stage_extra

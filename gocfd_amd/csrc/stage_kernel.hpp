@@ -431,7 +431,8 @@ k_stage(StageArgs a) {
         // with both sides inactive eps is zero on the edge and VF vanishes
         const bool aL = a.sh.active[kL] != 0, aR = a.sh.active[kR] != 0;
         if (aL || aR) {
-          const double lam = 0.5 * (a.sh.epsF[size_t(rowL) * Kp + kL] + a.sh.epsF[size_t(rowR) * Kp + kR]);
+          const double lam = 0.5 * ((aL ? a.sh.epsF[size_t(rowL) * Kp + kL] : 0.) +
+                                    (aR ? a.sh.epsF[size_t(rowR) * Kp + kR] : 0.));
           const C coef = C((a.sh.omega * lam) * a.sh.vooL[fi]);
           const C *diss = static_cast<const C *>(a.sh.diss);
 #pragma unroll
@@ -979,11 +980,14 @@ __global__ void __launch_bounds__(256) k_shock_sensor(StageArgs a) {
   a.sh.eps[kk] = eps;
   unsigned long long *sv = a.sh.sigV + size_t(a.sh.vbuf) * a.sh.Nv;
   unsigned long long *ev = a.sh.epsV + size_t(a.sh.vbuf) * a.sh.Nv;
+  // the buffers start at zero: a zero sigma (and so eps) changes no maximum
+  if (sg > 0.) {
 #pragma unroll
-  for (int c = 0; c < 3; ++c) {
-    const int v = a.sh.evert[size_t(c) * a.Kp + kk];
-    atomicMax(&sv[v], bits(sg));
-    atomicMax(&ev[v], bits(eps));
+    for (int c = 0; c < 3; ++c) {
+      const int v = a.sh.evert[size_t(c) * a.Kp + kk];
+      atomicMax(&sv[v], bits(sg));
+      atomicMax(&ev[v], bits(eps));
+    }
   }
 }
 
@@ -1012,7 +1016,6 @@ __global__ void __launch_bounds__(256) k_shock_prep(StageArgs a) {
   if (kk >= a.K) {
     a.sh.sigma[kk] = 0.;
     a.sh.active[kk] = 0;
-    for (int r = 0; r < NR; ++r) epsF[size_t(r) * a.Kp] = 0.;
     return;
   }
   const unsigned long long *sv = a.sh.sigV + size_t(a.sh.vbuf) * a.sh.Nv;
@@ -1021,10 +1024,15 @@ __global__ void __launch_bounds__(256) k_shock_prep(StageArgs a) {
   const double sg = ((from_bits(sv[v0]) + from_bits(sv[v1])) + from_bits(sv[v2])) / 3.;
   a.sh.sigma[kk] = sg;
   const double e0 = from_bits(ev[v0]), e1 = from_bits(ev[v1]), e2 = from_bits(ev[v2]);
-  a.sh.active[kk] = (e0 > 0. || e1 > 0. || e2 > 0.) ? 1 : 0;
-  const double *B = a.sh.ops + Sh::oBary;
+  const bool act = e0 > 0. || e1 > 0. || e2 > 0.;
+  a.sh.active[kk] = act ? 1 : 0;
+  // an inactive element's eps is zero at every RT point; readers check the
+  // flag instead of reading epsF
+  if (act) {
+    const double *B = a.sh.ops + Sh::oBary;
 #pragma unroll
-  for (int r = 0; r < NR; ++r) epsF[size_t(r) * a.Kp] = B[3 * r] * e0 + B[3 * r + 1] * e1 + B[3 * r + 2] * e2;
+    for (int r = 0; r < NR; ++r) epsF[size_t(r) * a.Kp] = B[3 * r] * e0 + B[3 * r + 1] * e1 + B[3 * r + 2] * e2;
+  }
   if (a.sh.filter_state) {
     const double *V = a.sh.ops + Sh::oV, *Vi = a.sh.ops + Sh::oVi, *mf = a.sh.ops + Sh::omf;
     const double fac = 1. - sin(0.5 * M_PI * sg);
@@ -1153,6 +1161,7 @@ __global__ void __launch_bounds__(kThreads) k_shock_grad(StageArgs a) {
     if (k < a.K && (code & FC_WAVE)) {
       const int kind = code & 15;
       const double oohK = a.g.oohK[k];
+      const bool actk = a.sh.active[k] != 0;  // else eps = 0 on the face
       const double *ef = a.sh.epsF + size_t(NpInt + f * NpEdge) * Kp + k;
       double fwl = 0.0, fwv = 0.0;
       for (int pp = 0; pp < NpEdge; ++pp) {
@@ -1165,7 +1174,7 @@ __global__ void __launch_bounds__(kThreads) k_shock_grad(StageArgs a) {
         const double s = point_speed(a.ph.gamma, q, oohK);
         if (!(s == s)) bad = 1;
         fwl = fmax(fwl, s);
-        fwv = fmax(fwv, oohK * oohK * ef[size_t(pp) * Kp]);
+        if (actk) fwv = fmax(fwv, oohK * oohK * ef[size_t(pp) * Kp]);
       }
       if (a.lts.on) {
         a.lts.fw[size_t(a.par2) * 3 * Kp + fi] = fwl;
