@@ -200,6 +200,16 @@ __device__ __forceinline__ unsigned long long bits(double v) {
 }
 __device__ __forceinline__ double from_bits(unsigned long long b) { return __longlong_as_double((long long)b); }
 
+// Device-wide maximum of non-negative doubles kept as IEEE bits (which order
+// like the values). The atomic is skipped when the stored value is already at
+// least as large: maxima only grow during a launch, so a stale read can only
+// be smaller, and then the atomic runs. This keeps thousands of workgroups
+// from serialising on one address.
+__device__ __forceinline__ void atomic_max_bits(unsigned long long *p, double v) {
+  const unsigned long long b = bits(v);
+  if (b > *reinterpret_cast<volatile unsigned long long *>(p)) atomicMax(p, b);
+}
+
 template <class C>
 __device__ __forceinline__ void boundary_state(int kind, const Phys &ph, const double *bxy, int bidx,
                                                int NpEdge, int pp, double t, const C own[4],
@@ -866,7 +876,7 @@ k_stage(StageArgs a) {
     if constexpr (!SHOCK) {
       double m = red[0];
       for (int w = 1; w < kThreads / 64; ++w) m = fmax(m, red[w]);
-      atomicMax(&ctl->wmax[(a.ring + 1) % 3], bits(m));
+      atomic_max_bits(&ctl->wmax[(a.ring + 1) % 3], m);
     }
     if (bad) atomicOr(&ctl->nan, 1);
     if (first) {
@@ -922,7 +932,7 @@ __global__ void __launch_bounds__(256) k_prologue(StageArgs a) {
     wl = fmax(wl, w);
   }
   if (a.lts.on) a.lts.fw[size_t(a.par2) * 3 * a.Kp + fi] = wl;
-  atomicMax(&ctl->wmax[a.ring], bits(wl));
+  atomic_max_bits(&ctl->wmax[a.ring], wl);
   if (bad) atomicOr(&ctl->nan, 1);
 }
 
@@ -1197,8 +1207,8 @@ __global__ void __launch_bounds__(kThreads) k_shock_grad(StageArgs a) {
       m = fmax(m, red[0][w]);
       mv = fmax(mv, red[1][w]);
     }
-    atomicMax(&a.ctl->wmax[a.ring], bits(m));
-    atomicMax(&a.ctl->wmaxv[a.ring], bits(mv));
+    atomic_max_bits(&a.ctl->wmax[a.ring], m);
+    atomic_max_bits(&a.ctl->wmaxv[a.ring], mv);
     if (bad) atomicOr(&a.ctl->nan, 1);
   }
 }
