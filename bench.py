@@ -226,7 +226,24 @@ def cpu_baseline(mesh, target_seconds=15.0):
         el = time.perf_counter() - t0
         if el >= target_seconds or n >= 50:
             break
-    return dict(value=s.K * s.NpInt * 5 * n / el, steps=n, seconds=el, K=s.K, threads=oracle.threads())
+    return dict(value=s.K * s.NpInt * 5 * n / el, steps=n, seconds=el, K=s.K, threads=oracle.threads(),
+                rho=s.state()[0])
+
+
+def gpu_vs_oracle(mesh, steps, rho_oracle):
+    """FP64 and FP32 GPU runs of the same number of steps from the same
+    initial state, against the oracle's density (relative L2 and Linf)."""
+    from gocfd_amd import Euler, InputParameters2D
+    ip = InputParameters2D(Title="bench", CFL=1.0, FluxType="Roe", InitType="IVortex",
+                           PolynomialOrder=4, FinalTime=1e9, MaxIterations=steps)
+    out = {"steps": steps}
+    for prec in ("fp64", "fp32"):
+        e = Euler(ip, mesh, precision=prec)
+        e.step(steps)
+        d = e.Q[0] - rho_oracle
+        out[prec + "_rho_l2"] = float(np.linalg.norm(d) / np.linalg.norm(rho_oracle))
+        out[prec + "_rho_linf"] = float(np.abs(d).max() / np.abs(rho_oracle).max())
+    return out
 
 
 def copy_probe(mb: float) -> None:
@@ -366,8 +383,10 @@ def main():
                     "rho_l2_fp32_vs_fp64_single_case_by_order": re_["errs"]}
 
     cpu = None
+    vs_oracle = None
     if not args.no_cpu:
         c = cpu_baseline(mesh, args.cpu_seconds)
+        vs_oracle = gpu_vs_oracle(mesh, c["steps"], c["rho"])
         cpu = {"value": c["value"], "unit": "DOF-stage/s", "cores": c["threads"], "kind": "port",
                "sample": f"CPU oracle (plain C, FP64, OpenMP {c['threads']} threads), same {K}-triangle P=4 Roe "
                          f"mesh, {c['steps']} SSP54 steps in {c['seconds']:.1f} s"}
@@ -395,7 +414,7 @@ def main():
         "fp32_over_fp64": v32 / v64,
         "accuracy": {"rho_l2_fp32_vs_fp64": l2_32_64, "rho_linf_fp32_vs_fp64": linf_32_64,
                      "rho_l2_fp64_vs_analytic": l2_64_an, "rho_l2_fp32_vs_analytic": l2_32_an,
-                     "time": r64["time"]},
+                     "time": r64["time"], "vs_oracle": vs_oracle},
         "roofline": {"bound": "hbm", "achieved": ach32, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": ach32 / HBM_PEAK_GBS, "traffic": traffic,
                      "kernel": "k_stage<4,float,float>", "avg_launch_ms": r32["stage_ms"],
