@@ -214,6 +214,85 @@ def run_ensemble(ncases, steps, warmup):
                 seq_rate=seq_rate, errs=errs, tuned=tuned, tune_seconds=t_tune)
 
 
+BC_WALL = 5  # types/cfd.go BCFLAG numbering, as returned by gocfd_euler2d_get_edges
+
+
+def lift_functional(e, alpha_deg: float, gamma: float = 1.4) -> float:
+    """Lift of the wall edges, a functional the reference does not define:
+    J = sum over wall edges of |e| * mean_i p_i * (n . l), with p_i the
+    pressure of the owner's trace at the edge's RT points (FluxEdgeInterp of
+    the element state), n the element's outward normal (into the body, so
+    p n is the pressure force on it) and l = (-sin a, cos a) the lift
+    direction."""
+    Q = e.Q                                                 # [4][NpInt][K]
+    I = e.operators()["Interp"].reshape(3, e.NpEdge, e.NpInt)
+    geo = e.geometry()
+    ed = e.edges()
+    w = ed[ed[:, 9] == BC_WALL]
+    k, f = w[:, 3], w[:, 5]
+    Qf = np.einsum("epi,nie->nep", I[f], Q[:, :, k])        # [4][nwall][NpEdge]
+    rho, ru, rv, E = Qf
+    p = (gamma - 1.0) * (E - 0.5 * (ru * ru + rv * rv) / rho)
+    length = geo["IInII"][f, k] * np.where(f == 1, 2.0 * math.sqrt(2.0), 2.0)
+    a = math.radians(alpha_deg)
+    proj = -geo["FaceNx"][f, k] * math.sin(a) + geo["FaceNy"][f, k] * math.cos(a)
+    return float(np.sum(length * p.mean(axis=1) * proj))
+
+
+def run_airfoil(mesh, steps, warmup, h=1e-6, ndir=4, seed=3):
+    """The adjoint workload's forward part: 4-digit airfoil in an O-domain,
+    P=3, M=0.5, AoA 2 deg, Roe, FP64, `steps` SSP54 steps from the
+    freestream. Times the step and evaluates the lift functional and its
+    central finite differences (step h) with respect to Mach, angle of attack
+    (degrees) and `ndir` seeded directions of the initial state -- the values
+    an adjoint gradient is checked against."""
+    from gocfd_amd import Euler, InputParameters2D
+
+    def make(M, a):
+        ip = InputParameters2D(Title="airfoil", CFL=1.0, FluxType="Roe", InitType="Freestream",
+                               PolynomialOrder=3, Minf=M, Alpha=a, FinalTime=1e9,
+                               MaxIterations=warmup + steps)
+        return Euler(ip, mesh, precision="fp64")
+
+    M0, a0 = 0.5, 2.0
+    e = make(M0, a0)
+    e.step(warmup)
+    e.sync()
+    t0 = time.perf_counter()
+    e.step(steps)
+    elapsed = time.perf_counter() - t0
+    stage_ms, _ = e.kernel_time()
+    t_end, dt, _ = e.time_info()
+    J = lift_functional(e, a0)
+    n = warmup + steps
+
+    def J_of(M, a):
+        g = make(M, a)
+        g.step(n)
+        return lift_functional(g, a)
+
+    t1 = time.perf_counter()
+    dJ_dM = (J_of(M0 + h, a0) - J_of(M0 - h, a0)) / (2 * h)
+    dJ_da = (J_of(M0, a0 + h) - J_of(M0, a0 - h)) / (2 * h)
+    g = make(M0, a0)
+    q0 = g.Q
+    rng = np.random.default_rng(seed)
+
+    def J_from(q):
+        g.Q = q
+        g.set_time(0.0)
+        g.step(n)
+        return lift_functional(g, a0)
+
+    dirs = []
+    for _ in range(ndir):
+        v = rng.standard_normal(q0.shape) * np.abs(q0)
+        dirs.append((J_from(q0 + h * v) - J_from(q0 - h * v)) / (2 * h))
+    fd_seconds = time.perf_counter() - t1
+    return dict(elapsed=elapsed, stage_ms=stage_ms, K=e.K, NpInt=e.NpInt, J=J, time=t_end, dt=dt,
+                dJ_dMinf=dJ_dM, dJ_dalpha_deg=dJ_da, dJ_dq0_dirs=dirs, fd_seconds=fd_seconds)
+
+
 def cpu_baseline(mesh, target_seconds=15.0):
     """The CPU oracle (plain C, FP64, OpenMP over the host cores) on the same mesh and order:
     whole SSP54 steps until about `target_seconds` of CPU work."""
@@ -267,7 +346,7 @@ def main():
     ap.add_argument("--n", type=int, default=200, help="quads per side (2n^2 triangles)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
-    ap.add_argument("--only", choices=["fp32", "fp64", "shock", "ensemble"], default=None,
+    ap.add_argument("--only", choices=["fp32", "fp64", "shock", "ensemble", "airfoil"], default=None,
                     help="run one leg only (for profiler runs)")
     ap.add_argument("--shock-points", type=int, default=125_000,
                     help="interior points of the wedge mesh (about twice as many triangles)")
@@ -276,6 +355,8 @@ def main():
     ap.add_argument("--ensemble-cases", type=int, default=64)
     ap.add_argument("--ensemble-steps", type=int, default=1000)
     ap.add_argument("--no-ensemble", action="store_true", help="skip the batched-ensemble leg")
+    ap.add_argument("--no-airfoil", action="store_true", help="skip the airfoil (adjoint-workload forward) leg")
+    ap.add_argument("--airfoil-steps", type=int, default=200)
     ap.add_argument("--copy-probe", type=float, default=0.0, metavar="MB",
                     help="run a plain device copy of MB megabytes (counter calibration) and exit")
     args = ap.parse_args()
@@ -304,6 +385,14 @@ def main():
                          "stage_ms": r["stage_ms"], "ms_per_step": r["elapsed"] / args.shock_steps * 1e3,
                          "K": r["K"], "sensor_flagged_elements": r["flagged"]}
         print(json.dumps(out))
+        return
+    airfoil_mesh = None
+    if args.only == "airfoil" or (args.only is None and not args.no_airfoil):
+        airfoil_mesh = meshgen.airfoil_mesh(os.path.join(tmpdir, "airfoil.su2"), seed=1)
+    if args.only == "airfoil":
+        r = run_airfoil(airfoil_mesh, args.airfoil_steps, args.warmup)
+        r["value"] = r["K"] * r["NpInt"] * 5 * args.airfoil_steps / r["elapsed"]
+        print(json.dumps({"leg": "airfoil", **r}))
         return
     if args.only == "ensemble":
         r = run_ensemble(args.ensemble_cases, args.ensemble_steps, args.warmup)
@@ -382,6 +471,20 @@ def main():
                     "launch_autotuned": re_["tuned"], "autotune_seconds": re_["tune_seconds"],
                     "rho_l2_fp32_vs_fp64_single_case_by_order": re_["errs"]}
 
+    airfoil = None
+    if airfoil_mesh is not None:
+        ra = run_airfoil(airfoil_mesh, args.airfoil_steps, args.warmup)
+        airfoil = {"workload": f"symmetric 4-digit airfoil (12 %, chord 1) in an O-domain of radius 20, {ra['K']} "
+                               f"triangles (400 x 125 wall-clustered O-mesh, seeded jitter), P=3, M=0.5, AoA 2 deg, "
+                               f"Roe, FP64, {args.airfoil_steps} steps",
+                   "value": ra["K"] * ra["NpInt"] * 5 * args.airfoil_steps / ra["elapsed"], "unit": "DOF-stage/s",
+                   "ms_per_step": ra["elapsed"] / args.airfoil_steps * 1e3, "stage_ms": ra["stage_ms"],
+                   "time": ra["time"], "dt": ra["dt"], "lift_functional": ra["J"],
+                   "fd_central_h1e-6": {"dJ_dMinf": ra["dJ_dMinf"], "dJ_dalpha_deg": ra["dJ_dalpha_deg"],
+                                        "dJ_dq0_random_dirs": ra["dJ_dq0_dirs"], "seconds": ra["fd_seconds"]},
+                   "adjoint": "not built: the finite differences above are the values a discrete adjoint "
+                              "would be checked against"}
+
     cpu = None
     vs_oracle = None
     if not args.no_cpu:
@@ -421,6 +524,7 @@ def main():
                      "alg_bytes_per_launch": b32},
         "cpu_baseline": cpu,
         "shock_path": shock,
+        "airfoil": airfoil,
         "ensemble": ensemble,
     }
     print(json.dumps(line))

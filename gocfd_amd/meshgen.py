@@ -129,6 +129,79 @@ def shocktube_mesh(path: str, nx: int, ny: int = 2, height: float = 0.1, jitter:
     return path
 
 
+def naca4_thickness(x, t=0.12):
+    """Half-thickness of the symmetric 4-digit section with a closed trailing
+    edge: y = 5 t (0.2969 sqrt x - 0.126 x - 0.3516 x^2 + 0.2843 x^3 - 0.1036 x^4)."""
+    return 5.0 * t * (0.2969 * np.sqrt(x) - 0.126 * x - 0.3516 * x ** 2 + 0.2843 * x ** 3 - 0.1036 * x ** 4)
+
+
+def airfoil_mesh(path: str, n_around: int = 400, n_radial: int = 125, radius: float = 20.0, seed: int = 1,
+                 jitter: float = 0.1, stretch: float = 8.0) -> str:
+    """Symmetric 4-digit airfoil (chord 1, 12 % thick, closed trailing edge)
+    in an O-domain of the given radius about (0.5, 0): n_around surface
+    points (cosine-clustered toward the leading edge) by n_radial
+    layers, geometrically stretched toward the wall (g = (e^(s u) - 1)/(e^s - 1));
+    layer points leave the wall along its normal and bend toward the circle
+    point at the surface point's angle about the centre,
+    P = S + n R g (1 - g) + (C - S) g^2. Interior nodes move by up to
+    +-jitter of the local spacing (SplitMix64 stream from `seed`); each quad is
+    split into two counter-clockwise triangles: 2 n_around n_radial
+    triangles (100,000 at the defaults, 50,000 points). Tags: "wall" on the
+    airfoil, "far" on the circle."""
+    assert n_around % 2 == 0
+    m = n_around // 2
+    # clustered toward the leading edge only: a cusp-side clustering would
+    # pinch the cells between the two rays that leave the trailing edge
+    xs = 1.0 - np.cos(0.5 * np.pi * np.linspace(0.0, 1.0, m + 1))  # 0 (LE) .. 1 (TE)
+    up_x = xs[::-1][:-1]                       # TE -> LE, without the LE point
+    lo_x = xs[:-1]                             # LE -> TE, without the TE point
+    sx = np.concatenate([up_x, lo_x])
+    sy = np.concatenate([naca4_thickness(up_x), -naca4_thickness(lo_x)])
+    sy[0] = 0.0                                # the trailing-edge point
+    theta = np.arctan2(sy, sx - 0.5)
+    theta[m:] = np.where(theta[m:] <= 0.0, theta[m:] + 2 * np.pi, theta[m:])  # monotone around the body
+    cx, cy = 0.5 + radius * np.cos(theta), radius * np.sin(theta)
+    # outward wall normals (the surface runs counter-clockwise; at the
+    # trailing-edge cusp the central difference gives the bisector)
+    tx = np.roll(sx, -1) - np.roll(sx, 1)
+    ty = np.roll(sy, -1) - np.roll(sy, 1)
+    tn = np.hypot(tx, ty)
+    nx, ny = ty / tn, -tx / tn
+    u = np.linspace(0.0, 1.0, n_radial + 1)
+    g = (np.expm1(stretch * u) / np.expm1(stretch))[:, None]
+    # near the wall the layers follow the normals, far out the circle:
+    # P = S + n R g (1 - g) + (C - S) g^2
+    X = sx[None, :] + nx[None, :] * radius * g * (1 - g) + (cx - sx)[None, :] * g ** 2   # [n_radial+1][n_around]
+    Y = sy[None, :] + ny[None, :] * radius * g * (1 - g) + (cy - sy)[None, :] * g ** 2
+    g = g[:, 0]
+    # jitter of the interior layers, scaled by the local radial and tangential spacing
+    dr = np.hypot(np.diff(X, axis=0), np.diff(Y, axis=0))
+    ds = np.hypot(np.roll(X, -1, axis=1) - X, np.roll(Y, -1, axis=1) - Y)
+    h = np.minimum(dr[1:], np.minimum(ds[1:-1], np.roll(ds[1:-1], 1, axis=1)))
+    r = splitmix64(seed, 2 * h.size).reshape(2, *h.shape)
+    X[1:-1] += (2 * r[0] - 1) * jitter * h
+    Y[1:-1] += (2 * r[1] - 1) * jitter * h
+    VX, VY = X.ravel(), Y.ravel()
+
+    def v(j, i):
+        return j * n_around + (i % n_around)
+
+    tris = []
+    for j in range(n_radial):
+        for i in range(n_around):
+            a, b, c, d = v(j, i), v(j, i + 1), v(j + 1, i + 1), v(j + 1, i)
+            tris.append((a, b, c))
+            tris.append((a, c, d))
+    EToV = np.array(tris, dtype=np.int64)
+    p0, p1, p2 = (np.stack([VX[EToV[:, k]], VY[EToV[:, k]]], axis=1) for k in range(3))
+    cross = (p1[:, 0] - p0[:, 0]) * (p2[:, 1] - p0[:, 1]) - (p1[:, 1] - p0[:, 1]) * (p2[:, 0] - p0[:, 0])
+    EToV[cross < 0] = EToV[cross < 0][:, [0, 2, 1]]
+    tags = {"wall": [(v(0, i), v(0, i + 1)) for i in range(n_around)],
+            "far": [(v(n_radial, i), v(n_radial, i + 1)) for i in range(n_around)]}
+    write_su2(path, VX, VY, EToV.astype(np.int32), tags)
+    return path
+
+
 def wedge_mesh(path: str, n_points: int = 125_000, seed: int = 1, x_ramp: float = 0.5,
                angle_deg: float = 15.0, jitter: float = 0.2) -> str:
     """Mach-2 wedge channel: [0,3] x [0,1] with a ramp rising at angle_deg from
