@@ -2,9 +2,11 @@
 // orders, Mach numbers and angles) stepped together. The reference runs one
 // case per process; this has no counterpart there.
 //
-// Cases of the same order form a group whose device layout is the
-// concatenation of the cases' layouts (every case starts on a tile boundary,
-// so a tile belongs to one case); each stage is one launch per group. Every
+// Cases of the same order, limiter (PerssonC0 with one Kappa) and
+// time-stepping mode form a group whose device layout is the concatenation
+// of the cases' layouts (every case starts on a tile boundary, so a tile
+// belongs to one case); each stage is one launch per group (after the three
+// shock-capturing passes in a PerssonC0 group). Every
 // case keeps its own control block (time, dt, wave-speed maxima) and
 // physics constants, selected per tile. A run of 6 steps -- the period of
 // the per-stage ring and time-parity arguments -- or a multiple of it is
@@ -17,6 +19,7 @@
 #include <algorithm>
 #include <cstring>
 #include <map>
+#include <tuple>
 #include <memory>
 #include <vector>
 
@@ -43,12 +46,23 @@ struct EnsCase {
 
 struct EnsGroup {
   int N = 0, Kp = 0, ntiles = 0, Hmax = 1, Wmax = 1, Fmax = 1, NV = 0;
+  bool shock = false, lts = false;  // PerssonC0 limiter, LocalTimeStepping (the same for the group)
+  double kappa = 0;
+  int Nv = 0;                       // vertices of the group's cases
   std::vector<int> cases;
   DevBuf<double> Jdet, Jinv, IInII, Nx, Ny, oohK, bxy, Interp64;
   DevBuf<int> code, nbr_slot, halo_elem, halo_face, wave_list, wave_cnt, flux_list, flux_cnt, tile_case;
   void *q[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
   void *R = nullptr;
   std::vector<double> q0_init;         // initial element-major state (tuning restores it)
+  // local time stepping
+  DevBuf<double> fw, fwv, DTk, DTv;
+  DevBuf<int> wsrc;
+  // shock capturing
+  DevBuf<double> sh_ops, hK, Se, sigma, eps, epsF, diss, Nxo, Nyo, vooL;
+  DevBuf<float> sh_opsF;
+  DevBuf<int> evert, nbr_elem, nbr_face, active;
+  DevBuf<unsigned long long> sigV, epsV;
   hipStream_t stream = nullptr;        // the group's branch of a parallel graph
   hipEvent_t done = nullptr;
   ~EnsGroup() {
@@ -115,6 +129,20 @@ struct gocfd_ensemble {
     a.Interp64 = g.Interp64.p;
     const EnsCase &c0 = *cases[g.cases[0]];
     a.ph = phys_of(c0.fs, c0.prm);
+    a.lts.on = g.lts ? 1 : 0;
+    a.lts.fw = g.fw.p; a.lts.fwv = g.fwv.p; a.lts.wsrc = g.wsrc.p; a.lts.DTk = g.DTk.p; a.lts.DTv = g.DTv.p;
+    if (g.shock) {
+      a.sh.ops = g.sh_ops.p; a.sh.opsF = g.sh_opsF.p; a.sh.evert = g.evert.p; a.sh.hK = g.hK.p;
+      a.sh.sigV = g.sigV.p; a.sh.epsV = g.epsV.p; a.sh.Nv = g.Nv;
+      a.sh.Se = g.Se.p; a.sh.sigma = g.sigma.p; a.sh.eps = g.eps.p; a.sh.epsF = g.epsF.p; a.sh.diss = g.diss.p;
+      a.sh.active = g.active.p;
+      a.sh.nbr_elem = g.nbr_elem.p; a.sh.nbr_face = g.nbr_face.p;
+      a.sh.Nxo = g.Nxo.p; a.sh.Nyo = g.Nyo.p; a.sh.vooL = g.vooL.p;
+      a.sh.kappa = g.kappa; a.sh.eps0 = 5. / 1.5;
+      const double np1 = double(g.N + 1);
+      a.sh.cdiff = 1. / (np1 * np1);
+      a.sh.omega = double(g.N * g.N);
+    }
     return a;
   }
 
@@ -123,6 +151,13 @@ struct gocfd_ensemble {
     gstage = 0;
     step_index = 0;
     for (auto &g : groups) {
+      if (g->shock) {
+        HIPCHECK(dev::launch_fill_u64(g->sigV.p, g->sigV.n, 0ull, stream));
+        HIPCHECK(dev::launch_fill_u64(g->epsV.p, g->epsV.n, 0ull, stream));
+      }
+      if (g->lts) HIPCHECK(dev::launch_fill_f64(g->DTv.p, g->DTv.n, 0.0, stream));
+      // with shock capturing each stage's gradient pass measures its wave speeds
+      if (g->shock) continue;
       dev::StageArgs a = args(*g);
       a.qs = g->q[0];
       a.ring = 0;
@@ -143,7 +178,13 @@ struct gocfd_ensemble {
         a.ring = int(gstage % 3);
         a.par2 = int(gstage & 1);
         a.qs = g->q[s];
-        HIPCHECK(dev::launch_stage(g->N, prec, 0, a, branches ? g->stream : stream));
+        hipStream_t st = branches ? g->stream : stream;
+        if (g->shock) {
+          a.sh.vbuf = int(gstage & 1);
+          a.sh.filter_state = s == 2 ? 1 : 0;
+          for (int pass = 0; pass < 3; ++pass) HIPCHECK(dev::launch_shock(g->N, prec, pass, a, st));
+        }
+        HIPCHECK(dev::launch_stage(g->N, prec, g->shock ? 1 : 0, a, st));
       }
       ++gstage;
     }
@@ -339,6 +380,60 @@ void build_group(gocfd_ensemble &h, EnsGroup &g) {
   HIPCHECK(hipMalloc(&g.q[0], nval * sizeof(double)));
   HIPCHECK(hipMemcpy(g.q[0], q0.data(), nval * sizeof(double), hipMemcpyHostToDevice));
   g.q0_init = std::move(q0);
+  if (g.lts || g.shock) {
+    // per-element and per-face arrays of the cases, concatenated like the
+    // geometry; element, face and vertex indices offset per case
+    std::vector<int> wsrc(3 * Kp, 0), evert(3 * Kp, 0), nbr_elem(3 * Kp, -1), nbr_face(3 * Kp, 0);
+    std::vector<double> hK(Kp, 0.0), Nxo(3 * Kp, 0.0), Nyo(3 * Kp, 0.0), vooL(3 * Kp, 0.0);
+    int voff = 0;
+    for (int ci : g.cases) {
+      const EnsCase &c = *h.cases[ci];
+      const DeviceLayout &L = c.L;
+      const size_t o = c.off, kp = L.Kp;
+      for (size_t kk = 0; kk < kp; ++kk) {
+        if (g.shock) hK[o + kk] = L.hK[kk];
+        for (int f = 0; f < 3; ++f) {
+          const size_t src = f * kp + kk, dst = f * Kp + o + kk;
+          if (g.lts) {
+            const int w = L.wsrc[src], wf = int(w / kp), wk = int(w % kp);
+            wsrc[dst] = int(wf * Kp + o + wk);
+          }
+          if (g.shock) {
+            evert[dst] = L.evert[src] + voff;
+            nbr_elem[dst] = L.nbr_elem[src] < 0 ? -1 : L.nbr_elem[src] + int(o);
+            nbr_face[dst] = L.nbr_face[src];
+            Nxo[dst] = L.Nxo[src];
+            Nyo[dst] = L.Nyo[src];
+            vooL[dst] = L.vooL[src];
+          }
+        }
+      }
+      voff += c.d.Nv;
+    }
+    if (g.lts) {
+      g.fw.alloc(2 * 3 * Kp);
+      g.fwv.alloc(2 * 3 * Kp);
+      g.DTk.alloc(Kp);
+      g.DTv.alloc(Kp);
+      g.wsrc.upload(wsrc);
+    }
+    if (g.shock) {
+      g.Nv = voff;
+      const int NR = NpInt + 3 * NpEdge;
+      const std::vector<double> sops = build_shock_operators(c0.d.ops);
+      g.sh_ops.upload(sops);
+      g.sh_opsF.upload(std::vector<float>(sops.begin(), sops.end()));
+      g.evert.upload(evert); g.hK.upload(hK);
+      g.nbr_elem.upload(nbr_elem); g.nbr_face.upload(nbr_face);
+      g.Nxo.upload(Nxo); g.Nyo.upload(Nyo); g.vooL.upload(vooL);
+      g.sigV.alloc(2 * size_t(std::max(voff, 1)));
+      g.epsV.alloc(2 * size_t(std::max(voff, 1)));
+      g.Se.alloc(Kp); g.sigma.alloc(Kp); g.eps.alloc(Kp);
+      g.epsF.alloc(Kp * NR);
+      g.diss.alloc(Kp * 4 * 2 * NR);  // FP64 or FP32 values (the larger size)
+      g.active.alloc(Kp);
+    }
+  }
   HIPCHECK(hipStreamCreateWithFlags(&g.stream, hipStreamNonBlocking));
   HIPCHECK(hipEventCreateWithFlags(&g.done, hipEventDisableTiming));
   for (int s = 1; s < 5; ++s) {
@@ -369,8 +464,6 @@ int gocfd_ensemble_create(int32_t ncases, const gocfd_mesh *const *meshes, const
     if (validate_params(&p) != GOCFD_OK) return GOCFD_ERR_ARG;
     if (p.precision != h->prec || p.flux_type != h->flux || p.device != h->device)
       return fail(GOCFD_ERR_ARG, "every case of an ensemble needs the same precision, flux type and device");
-    if (p.local_time_stepping || p.limiter != 0)
-      return fail(GOCFD_ERR_ARG, "ensembles run the plain SSP54 step (no LocalTimeStepping, no limiter)");
     auto c = std::make_unique<EnsCase>();
     c->prm = p;
     c->d = build_discretisation(meshes[i]->m, p.polynomial_order);
@@ -378,12 +471,22 @@ int gocfd_ensemble_create(int32_t ncases, const gocfd_mesh *const *meshes, const
     c->L = build_device_layout(c->d, dev::kTile);
     h->cases.push_back(std::move(c));
   }
-  std::map<int, std::vector<int>> by_order;
-  for (int i = 0; i < ncases; ++i) by_order[h->cases[i]->prm.polynomial_order].push_back(i);
+  // one group per order, limiter (with its Kappa) and time-stepping mode:
+  // a group shares one kernel variant and one set of shock constants
+  std::map<std::tuple<int, int, int, double>, std::vector<int>> by_kind;
+  for (int i = 0; i < ncases; ++i) {
+    const gocfd_params &p = h->cases[i]->prm;
+    const bool shock = p.limiter == 1 && p.polynomial_order != 0;
+    const double kappa = shock ? (p.kappa != 0. ? p.kappa : 5.) : 0.;
+    by_kind[std::make_tuple(p.polynomial_order, shock ? 1 : 0, p.local_time_stepping ? 1 : 0, kappa)].push_back(i);
+  }
   size_t lds = 0;
-  for (auto &kv : by_order) {
+  for (auto &kv : by_kind) {
     auto g = std::make_unique<EnsGroup>();
-    g->N = kv.first;
+    g->N = std::get<0>(kv.first);
+    g->shock = std::get<1>(kv.first) != 0;
+    g->lts = std::get<2>(kv.first) != 0;
+    g->kappa = std::get<3>(kv.first);
     g->cases = kv.second;
     for (int ci : g->cases) {
       h->cases[ci]->group = int(h->groups.size());

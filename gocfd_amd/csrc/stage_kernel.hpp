@@ -254,7 +254,6 @@ constexpr size_t lds_lts_bytes = sizeof(double) * kTile + sizeof(unsigned long l
 __device__ __forceinline__ bool step_skipped(const StageArgs &a, const Ctl *ctl, double final_time) {
   return a.stage >= 0 && a.step_index > 0 && ctl->time[a.step_index & 1] >= final_time;
 }
-__device__ __forceinline__ bool step_skipped(const StageArgs &a) { return step_skipped(a, a.ctl, a.ph.final_time); }
 
 // Waves per SIMD the register allocator targets. FP64 tiles need 80 KB of
 // LDS at N=4 (two workgroups per CU), so FP64 gains nothing above 2; the FP32
@@ -957,9 +956,12 @@ template <int N, class SIN>
 __global__ void __launch_bounds__(256) k_shock_sensor(StageArgs a) {
   using Sh = Shape<N>;
   constexpr int NpInt = Sh::NpInt, NV = Sh::NV;
-  if (step_skipped(a)) return;
   const int kk = blockIdx.x * blockDim.x + threadIdx.x;
-  if (kk >= a.K) return;
+  if (kk >= a.Kp) return;
+  // the element's case (a batch holds several, each with its own control)
+  const int cs = a.tile_case[kk / kTile];
+  const CaseInfo &ci = a.cases[cs];
+  if (kk >= ci.k_end || step_skipped(a, a.ctl + cs, ci.ph.final_time)) return;
   const SIN *rho = static_cast<const SIN *>(a.qs) + size_t(kk) * NV;
   const double *Pm = a.sh.ops + Sh::oP, *Mm = a.sh.ops + Sh::oM, *Dm = a.sh.ops + Sh::oD;
   double r[NpInt];
@@ -1010,7 +1012,6 @@ template <int N, class SIN>
 __global__ void __launch_bounds__(256) k_shock_prep(StageArgs a) {
   using Sh = Shape<N>;
   constexpr int NpInt = Sh::NpInt, NV = Sh::NV, NR = Sh::NR;
-  if (step_skipped(a)) return;
   const int gid = blockIdx.x * blockDim.x + threadIdx.x;
   {
     unsigned long long *so = a.sh.sigV + size_t(a.sh.vbuf ^ 1) * a.sh.Nv;
@@ -1022,8 +1023,11 @@ __global__ void __launch_bounds__(256) k_shock_prep(StageArgs a) {
   }
   const int kk = gid;
   if (kk >= a.Kp) return;
+  const int cs = a.tile_case[kk / kTile];
+  const CaseInfo &ci = a.cases[cs];
+  if (step_skipped(a, a.ctl + cs, ci.ph.final_time)) return;
   double *epsF = a.sh.epsF + kk;  // [NR][Kp]
-  if (kk >= a.K) {
+  if (kk >= ci.k_end) {  // padding
     a.sh.sigma[kk] = 0.;
     a.sh.active[kk] = 0;
     return;
@@ -1083,8 +1087,12 @@ __global__ void __launch_bounds__(kThreads) k_shock_grad(StageArgs a) {
                 NR = Sh::NR, E = kTile;
   __shared__ D sTr[4][NF][E];
   __shared__ double red[2][kThreads / 64];
-  if (step_skipped(a)) return;
   const int tile = blockIdx.x, tid = threadIdx.x;
+  // the tile's case (a batch holds several, each with its own control)
+  const int cs = a.tile_case[tile];
+  const CaseInfo &ci = a.cases[cs];
+  Ctl *ctl = a.ctl + cs;
+  if (step_skipped(a, ctl, ci.ph.final_time)) return;
   const int te = tid % E, tn = tid / E, k0 = tile * E, kk = k0 + te, Kp = a.Kp;
   const SIN *Q = static_cast<const SIN *>(a.qs);
   const D *Itp = ops_base<D>() + NpFlux * NpInt;
@@ -1163,12 +1171,12 @@ __global__ void __launch_bounds__(kThreads) k_shock_grad(StageArgs a) {
   // wave speeds of the owned faces: thread (element, face)
   double wl = 0.0, wv = 0.0;
   int bad = 0;
-  const double time = a.ctl->time[a.step_index & 1];
+  const double time = ctl->time[a.step_index & 1];
   if (tid < 3 * E) {
     const int e = tid / 3, f = tid - 3 * e, k = k0 + e;
     const size_t fi = size_t(f) * Kp + k;
     const int code = a.g.code[fi];
-    if (k < a.K && (code & FC_WAVE)) {
+    if (k < ci.k_end && (code & FC_WAVE)) {
       const int kind = code & 15;
       const double oohK = a.g.oohK[k];
       const bool actk = a.sh.active[k] != 0;  // else eps = 0 on the face
@@ -1179,9 +1187,9 @@ __global__ void __launch_bounds__(kThreads) k_shock_grad(StageArgs a) {
                        double(sTr[2][f * NpEdge + pp][e]), double(sTr[3][f * NpEdge + pp][e])};
         if (kind == FK_Far || kind == FK_IVortex) {
           const double own[4] = {q[0], q[1], q[2], q[3]};
-          boundary_state(kind, a.ph, a.g.bxy, code >> 8, NpEdge, pp, time, own, a.g.Nx[fi], a.g.Ny[fi], q);
+          boundary_state(kind, ci.ph, a.g.bxy, code >> 8, NpEdge, pp, time, own, a.g.Nx[fi], a.g.Ny[fi], q);
         }
-        const double s = point_speed(a.ph.gamma, q, oohK);
+        const double s = point_speed(ci.ph.gamma, q, oohK);
         if (!(s == s)) bad = 1;
         fwl = fmax(fwl, s);
         if (actk) fwv = fmax(fwv, oohK * oohK * ef[size_t(pp) * Kp]);
@@ -1207,9 +1215,9 @@ __global__ void __launch_bounds__(kThreads) k_shock_grad(StageArgs a) {
       m = fmax(m, red[0][w]);
       mv = fmax(mv, red[1][w]);
     }
-    atomic_max_bits(&a.ctl->wmax[a.ring], m);
-    atomic_max_bits(&a.ctl->wmaxv[a.ring], mv);
-    if (bad) atomicOr(&a.ctl->nan, 1);
+    atomic_max_bits(&ctl->wmax[a.ring], m);
+    atomic_max_bits(&ctl->wmaxv[a.ring], mv);
+    if (bad) atomicOr(&ctl->nan, 1);
   }
 }
 

@@ -1,8 +1,9 @@
 """Batched ensembles of independent cases on one GPU (no counterpart in the
 reference, which runs one case per process): each case has its own mesh,
-order, initial state, Mach number, angle, CFL and FinalTime; cases of one
-order share a kernel launch per stage, and runs of 6 steps replay a
-captured HIP graph."""
+order, initial state, Mach number, angle, CFL, FinalTime, limiter and
+time-stepping mode; cases of one order, limiter (PerssonC0 with its Kappa)
+and time-stepping mode share the kernel launches of a stage, and runs of 6
+steps replay a captured HIP graph."""
 from __future__ import annotations
 
 import ctypes as C

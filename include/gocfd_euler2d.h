@@ -156,9 +156,10 @@ int gocfd_euler2d_get_element_dt(gocfd_euler2d *h, double *DT);
 /* ---- Batched ensembles (no counterpart in the reference, which runs one
  *      case per process). ncases independent cases -- each its own mesh,
  *      order, initial state, Mach number, angle, CFL and FinalTime -- stepped
- *      together: cases of one order share a launch per stage, and runs of 6
- *      steps replay a captured HIP graph. Every case needs the same
- *      precision, flux type and device; no LocalTimeStepping or limiter. ---- */
+ *      together: cases of one order, limiter (PerssonC0 and its Kappa) and
+ *      time-stepping mode share the launches of a stage, and runs of 6 steps
+ *      replay a captured HIP graph. Every case needs the same precision,
+ *      flux type and device. ---- */
 typedef struct gocfd_ensemble gocfd_ensemble;
 int gocfd_ensemble_create(int32_t ncases, const gocfd_mesh *const *meshes, const gocfd_params *params,
                           gocfd_ensemble **out);

@@ -117,3 +117,32 @@ def test_tune_restores_the_initial_state(tmp_path):
         assert np.array_equal(ens.state(c), ref.state(c)), c
     with pytest.raises(GocfdError):
         ens.tune(1)
+
+
+@pytest.mark.parametrize("prec", ["fp64", "fp32"])
+def test_shock_and_lts_cases_in_an_ensemble(tmp_path, prec):
+    """PerssonC0 cases (two Kappa values, so two groups of one order),
+    LocalTimeStepping cases (with and without the limiter) and a plain case
+    step together; each is bit-identical to its single-case run over the
+    eager steps and replays of the captured graph."""
+    tube = meshgen.shocktube_mesh(str(tmp_path / "st.su2"), 60, 2, height=0.02)
+    wedge = meshgen.wedge_mesh(str(tmp_path / "wedge.su2"), 900, seed=3)
+
+    def ip(N, init, lim="", kappa=0.0, lts=False, minf=0.0, cfl=0.5):
+        return InputParameters2D(CFL=cfl, FluxType="Roe", InitType=init, PolynomialOrder=N, Minf=minf,
+                                 FinalTime=1e9, MaxIterations=1000, Limiter=lim, Kappa=kappa,
+                                 LocalTimeStepping=lts)
+
+    cases = [(ip(2, "shocktube", "PerssonC0", 3.0), tube),
+             (ip(2, "Freestream", "PerssonC0", 10.0, minf=2.0, cfl=1.0), wedge),
+             (ip(2, "shocktube", "PerssonC0", 3.0), tube),
+             (ip(1, "Freestream", "", lts=True, minf=2.0, cfl=1.0), wedge),
+             (ip(2, "Freestream", "PerssonC0", 10.0, lts=True, minf=2.0, cfl=1.0), wedge),
+             (ip(3, "shocktube", ""), tube)]
+    ens = Ensemble(cases, precision=prec)
+    ens.step(25)
+    for c, (p, path) in enumerate(cases):
+        single = Euler(p, path, precision=prec)
+        single.step(25)
+        assert np.array_equal(ens.state(c), single.Q), c
+        assert ens.time_info(c) == single.time_info(), c
