@@ -17,8 +17,9 @@ CSRC = os.path.join(HERE, "csrc")
 BUILD = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libgocfd_euler2d.so")
 
-SOURCES = ["dfr_setup.cpp", "capi_common.cpp", "euler2d_capi.cpp", "ensemble_capi.cpp", "kernels_common.hip"] + [
-    f"stage_n{n}.hip" for n in range(5)]
+SOURCES = ["dfr_setup.cpp", "capi_common.cpp", "euler2d_capi.cpp", "ensemble_capi.cpp", "adjoint_runner.cpp",
+           "kernels_common.hip", "adjoint_common.hip"] + [f"stage_n{n}.hip" for n in range(5)] + [
+           f"adjoint_n{n}.hip" for n in range(5)]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"
@@ -48,7 +49,7 @@ def build(verbose: bool = False, out: str = LIB, defines=()) -> str:
         raise RuntimeError("hipcc not found; the HIP toolchain is required to build gocfd_amd")
     build_dir = BUILD if out == LIB else os.path.join(os.path.dirname(os.path.abspath(out)), "_build")
     os.makedirs(build_dir, exist_ok=True)
-    with ThreadPoolExecutor(max_workers=len(SOURCES)) as ex:
+    with ThreadPoolExecutor(max_workers=min(len(SOURCES), os.cpu_count() or 8)) as ex:
         objs = list(ex.map(lambda src: _compile(src, build_dir, defines), SOURCES))
     LIBP = out
     tmp = LIBP + ".tmp"

@@ -1,0 +1,101 @@
+// Tangent-linear and discrete-adjoint kernels of the FP64 SSP54 step
+// (global time stepping, no limiter): interface between the host driver
+// (adjoint_runner.cpp) and the per-order kernel translation units.
+//
+// The differentiated map is one SSP54 step of the reference (rkAdvance,
+// model_problems/Euler2D/euler.go:460-566) with the per-stage global time
+// step dt_s = CFL / max wave speed (calculateGlobalDT euler.go:945-971,
+// StoreEdgeAggregates edges.go:246-289) of the stage input, the DFR
+// right-hand side (RHSInternalPoints euler.go:665-699, SetRTFluxInternal
+// euler.go:701-726, CalculateEdgeEulerFlux edges.go:325-452) and the
+// Riemann-invariant far field (RiemannBC bcs.go:70-133). Its inputs are the
+// step-start state and the far-field state FSFar.Qinf.
+//
+// The forward kernels are templated on the arithmetic type: double (the
+// recomputation of the stages for the adjoint) or ad::Dual (the
+// tangent-linear step). The adjoint kernels apply the transposed pointwise
+// Jacobians (numerical flux, boundary state, physical flux, wave speed),
+// which they form one input direction at a time in dual arithmetic, and the
+// transposed DivInt / FluxEdgeInterp contractions and SSP combinations.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "kernels.hpp"
+
+namespace gocfd {
+namespace dev {
+
+// Device storage of one dual value (ad::Dual in the kernels)
+struct ad_storage {
+  double v, d;
+};
+
+struct AdjArgs {
+  int N, K, Kp;
+  const double *Jdet, *Jinv, *IInII, *Nx, *Ny, *oohK;  // [Kp], [4][Kp], [3][Kp] x3, [Kp]
+  const int *code, *nbr_elem, *nbr_face;               // [3][Kp]
+  const double *Interp;                                // FluxEdgeInterp [3 NpEdge][NpInt]
+  const double *DivInt;                                // [NpInt][NpFlux]
+  Phys ph;
+  int io_tied;     // In/Out faces use the far state (every case but the shock tube)
+  double dfar[4];  // tangent of the far state (tangent-linear step)
+};
+
+// Buffers of one launch. Element arrays are element-major [Kp][4][NpInt];
+// face-point arrays are [3][Kp][NpEdge] records of 4 values.
+struct AdjBufs {
+  int dual;             // forward kernels in dual arithmetic
+  int stage;            // 0..4
+  const void *Us;       // stage input
+  const void *U0, *U2, *U3, *R3;  // SSP operands (step start, stage inputs 2 and 3, stage-3 RHS)
+  const void *Rs;       // this stage's RHS
+  void *out;            // next stage input (or the step's result)
+  void *FE;             // face-point edge fluxes, scaled by ||n|| and side
+  void *R;              // RHS output
+  void *SP;             // face-point wave speeds
+  void *dts;            // [5] stage time steps (double or dual)
+  int *argmax;          // [5] face point of each stage's maximum wave speed
+  double *Wv;           // [5] the maximum wave speeds
+  double *time;         // step-start time (advanced by the last stage's dt)
+  int *flag;            // set when a stage's dt would be clamped to FinalTime
+  double *part_v;       // reduction partials
+  int *part_i;
+  int nparts;
+  // adjoint
+  double *lam;          // cotangent of this stage's output
+  double *lamU0;        // cotangent of the step-start state
+  double *lamQ[5];      // cotangents of the stage inputs 1..4 (index 1..4)
+  double *lamR;         // cotangent of this stage's RHS
+  double *lamR3;        // stage-3 RHS cotangent from the last stage
+  double *lamOwn, *lamOth;  // face-point cotangents of the own and neighbour traces
+  double *lamFarPts;    // face-point cotangents of the far state (accumulated)
+  double *lamFarW;      // [4] far-state cotangent from the dt terms (accumulated)
+  double *lam_dt;       // [5] cotangents of the stage dts
+  double *target;       // cotangent of this stage's input (accumulated)
+  const double *pack_v, *pack_d;  // value and tangent arrays (pack/unpack)
+  double *unpack_v, *unpack_d;
+  size_t n;             // element-array length (Kp * 4 * NpInt) or reduction length
+};
+
+enum AdjOp : int {
+  ADJ_FWD_FACE = 0,  // FE <- edge fluxes of Us
+  ADJ_FWD_RHS,       // R <- RHS(Us, FE)
+  ADJ_FWD_WAVE,      // SP <- owner-side wave speeds of Us
+  ADJ_ARGMAX,        // dts[stage], argmax[stage], Wv[stage] <- from SP; time, flag
+  ADJ_FWD_SSP,       // out <- SSP54 combination of the stage
+  ADJ_ADJ_SSP,       // cotangents through the SSP combination; lam_dt[stage]
+  ADJ_ADJ_FACE,      // lamOwn, lamOth, lamFarPts <- from lamR at Us
+  ADJ_ADJ_ELEM,      // target += interior-flux and trace transposes
+  ADJ_ADJ_WAVE,      // target, lamFarW += the dt term of the stage
+  ADJ_FAR_SUM,       // part_v[0..3] <- sum of lamFarPts + lamFarW
+  ADJ_PACK,          // out (dual) <- (pack_v, pack_d)
+  ADJ_UNPACK,        // (unpack_v, unpack_d) <- Us (dual)
+};
+
+hipError_t adj_launch(int op, const AdjArgs &a, const AdjBufs &b, hipStream_t s);
+template <int N> hipError_t adj_launch_n(int op, const AdjArgs &a, const AdjBufs &b, hipStream_t s);
+hipError_t adj_launch_common(int op, const AdjArgs &a, const AdjBufs &b, hipStream_t s);
+
+}  // namespace dev
+}  // namespace gocfd

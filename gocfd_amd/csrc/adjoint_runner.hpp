@@ -1,0 +1,60 @@
+// Host driver of the tangent-linear and discrete-adjoint step (kernels in
+// adjoint_kernels.hpp / adjoint_common.hip, scope in adjoint.hpp).
+//
+// vjp: one forward sweep stores the state every `every` steps; the reverse
+// sweep recomputes each block of steps from its checkpoint, then each step's
+// five stage inputs, right-hand sides, dts and arg-max points, and applies
+// the transposed stages from the last to the first.
+// jvp: the forward sweep in dual arithmetic.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <vector>
+
+#include "adjoint.hpp"
+#include "capi_common.hpp"
+#include "dfr_setup.hpp"
+
+namespace gocfd {
+namespace capi {
+
+struct AdjointGeometry {
+  const double *Jdet, *Jinv, *IInII, *Nx, *Ny, *oohK, *Interp;
+  const int *code;
+};
+
+class AdjointRunner {
+ public:
+  AdjointRunner(const Discretisation &d, const DeviceLayout &L, const dev::Phys &ph, bool io_tied,
+                const AdjointGeometry &geo, hipStream_t stream);
+  // lam0 = (dq_n/dq_0)^T lamN and far = (dq_n/dqinf)^T lamN for nsteps steps
+  // from q0 (device, element-major FP64) at time t0; every = 0 picks
+  // ceil(sqrt(nsteps)) steps per checkpoint
+  void vjp(const double *q0, double t0, int nsteps, int every, const double *lamN, double *lam0, double far[4]);
+  // qn and dqn = (dq_n/dq_0) dq0 + (dq_n/dqinf) dfar
+  void jvp(const double *q0, const double *dq0, const double dfar[4], double t0, int nsteps, double *qn,
+           double *dqn);
+  int last_every() const { return every_; }
+
+ private:
+  void launch(int op, dev::AdjBufs &b);
+  void forward_step(bool dual, const void *Ustart, void *Uend, double *time);
+  void check_flag();
+  dev::AdjArgs a_{};
+  hipStream_t s_;
+  size_t n_ = 0, nfp_ = 0;  // element-array and face-point counts
+  int every_ = 0;
+  DevBuf<double> divint_;
+  DevBuf<int> nbr_elem_, nbr_face_;
+  // stage record (sized for dual values; the double sweeps use the front half)
+  DevBuf<dev::ad_storage> U_[5], R_[5];
+  DevBuf<dev::ad_storage> FE_, SP_, tmp_;
+  DevBuf<dev::ad_storage> dts_;
+  DevBuf<int> argmax_, flag_, part_i_;
+  DevBuf<double> Wv_, time_, part_v_;
+  int nparts_ = 1024;
+};
+
+}  // namespace capi
+}  // namespace gocfd

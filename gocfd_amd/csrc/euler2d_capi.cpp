@@ -13,6 +13,7 @@
 #include "gocfd_euler2d.h"
 #include "kernels.hpp"
 #include "capi_common.hpp"
+#include "adjoint_runner.hpp"
 
 using namespace gocfd;
 
@@ -56,6 +57,7 @@ struct gocfd_euler2d {
   double last_batch_ms = 0;
   int last_batch_launches = 0;
   int NV = 0;
+  std::unique_ptr<capi::AdjointRunner> adj;  // tangent-linear / adjoint step, built on first use
 
   ~gocfd_euler2d() {
     if (stream) (void)hipStreamSynchronize(stream);
@@ -107,6 +109,14 @@ struct gocfd_euler2d {
     fill_element_major(L, d.ops.NpInt, Q, h.data());
     HIPCHECK(hipMemcpy(dst, h.data(), h.size() * sizeof(double), hipMemcpyHostToDevice));
   }
+  // reference-layout cotangent or tangent -> element-major device buffer,
+  // zero on the padding elements
+  void to_device_zero_pad(const double *Q, void *dst) {
+    std::vector<double> h(size_t(L.Kp) * NV);
+    fill_element_major(L, d.ops.NpInt, Q, h.data());
+    for (int kk = L.K; kk < L.Kp; ++kk) std::fill(h.begin() + size_t(kk) * NV, h.begin() + size_t(kk + 1) * NV, 0.0);
+    HIPCHECK(hipMemcpy(dst, h.data(), h.size() * sizeof(double), hipMemcpyHostToDevice));
+  }
   // element-major FP64 device buffer -> reference layout
   void from_device(const void *src, double *Q) {
     std::vector<double> h(size_t(L.Kp) * NV);
@@ -119,6 +129,24 @@ struct gocfd_euler2d {
     HIPCHECK(hipStreamSynchronize(stream));
     HIPCHECK(hipMemcpy(&c, ctl, sizeof(c), hipMemcpyDeviceToHost));
     return c;
+  }
+
+  // The tangent-linear / adjoint driver (FP64, global time stepping, no
+  // limiter), on the solver's geometry and stream
+  capi::AdjointRunner &adjoint() {
+    if (prec != 0) throw SetupError(GOCFD_ERR_ARG, "the tangent-linear and adjoint steps differentiate the FP64 step");
+    if (lts) throw SetupError(GOCFD_ERR_ARG, "the tangent-linear and adjoint steps use the global time step");
+    if (shock) throw SetupError(GOCFD_ERR_ARG, "the tangent-linear and adjoint steps do not cover the PerssonC0 limiter");
+    if (!adj) {
+      capi::AdjointGeometry geo{Jdet.p, Jinv.p, IInII.p, Nx.p, Ny.p, oohK.p, Interp64.p, code.p};
+      adj = std::make_unique<capi::AdjointRunner>(d, L, phys_of(fs, prm), prm.init_type != GOCFD_INIT_SHOCKTUBE,
+                                                  geo, stream);
+    }
+    return *adj;
+  }
+  double current_time() {
+    dev::Ctl c = read_ctl();
+    return c.time[c.steps & 1];
   }
 
   // (Re)start the step sequence from the state in q[0] at `time`
@@ -548,6 +576,52 @@ int gocfd_euler2d_get_element_dt(gocfd_euler2d *h, double *DT) {
   std::vector<double> tmp(h->L.Kp);
   HIPCHECK(hipMemcpy(tmp.data(), h->DTk.p, tmp.size() * sizeof(double), hipMemcpyDeviceToHost));
   for (int kk = 0; kk < h->L.K; ++kk) DT[h->L.perm[kk]] = tmp[kk];
+  return GOCFD_OK;
+  GUARD_END
+}
+
+int gocfd_euler2d_vjp(gocfd_euler2d *h, int32_t nsteps, int32_t checkpoint_every, const double *dQn, double *dQ0,
+                      double *dQinf) {
+  GUARD_BEGIN
+  if (!h || !dQn || !dQ0) return fail(GOCFD_ERR_ARG, "null argument");
+  NEED_DEVICE(h);
+  if (nsteps < 1 || checkpoint_every < 0) return fail(GOCFD_ERR_ARG, "nsteps must be positive, checkpoint_every >= 0");
+  HIPCHECK(hipSetDevice(h->device));
+  capi::AdjointRunner &r = h->adjoint();
+  HIPCHECK(hipStreamSynchronize(h->stream));
+  const double t0 = h->current_time();
+  const size_t tot = size_t(h->L.Kp) * h->NV;
+  DevBuf<double> lamN, lam0;
+  lamN.alloc(tot);
+  lam0.alloc(tot);
+  h->to_device_zero_pad(dQn, lamN.p);
+  double far[4];
+  r.vjp(static_cast<const double *>(h->q[0]), t0, nsteps, checkpoint_every, lamN.p, lam0.p, far);
+  h->from_device(lam0.p, dQ0);
+  if (dQinf) for (int c = 0; c < 4; ++c) dQinf[c] = far[c];
+  return GOCFD_OK;
+  GUARD_END
+}
+
+int gocfd_euler2d_jvp(gocfd_euler2d *h, int32_t nsteps, const double *dQ0, const double *dQinf, double *Qn,
+                      double *dQn) {
+  GUARD_BEGIN
+  if (!h || !dQ0 || !dQn) return fail(GOCFD_ERR_ARG, "null argument");
+  NEED_DEVICE(h);
+  if (nsteps < 1) return fail(GOCFD_ERR_ARG, "nsteps must be positive");
+  HIPCHECK(hipSetDevice(h->device));
+  capi::AdjointRunner &r = h->adjoint();
+  HIPCHECK(hipStreamSynchronize(h->stream));
+  const double t0 = h->current_time();
+  const size_t tot = size_t(h->L.Kp) * h->NV;
+  DevBuf<double> dq0, qn, dqn;
+  dq0.alloc(tot);
+  qn.alloc(tot);
+  dqn.alloc(tot);
+  h->to_device_zero_pad(dQ0, dq0.p);
+  r.jvp(static_cast<const double *>(h->q[0]), dq0.p, dQinf, t0, nsteps, qn.p, dqn.p);
+  if (Qn) h->from_device(qn.p, Qn);
+  h->from_device(dqn.p, dQn);
   return GOCFD_OK;
   GUARD_END
 }

@@ -174,30 +174,31 @@ __device__ __forceinline__ void numflux(int kind, C g, const C L[4], const C R[4
   }
 }
 
-// RiemannBC (model_problems/Euler2D/bcs.go:70-133), always in FP64.
+// RiemannBC (model_problems/Euler2D/bcs.go:70-133), in FP64 (T = double) or
+// in FP64 dual arithmetic for the tangent-linear and adjoint kernels.
 // pinf/cinf come from the freestream even for the analytic vortex state.
-__device__ __forceinline__ void riemann_bc(double g, double minf, double pinf, double cinf,
-                                           const double qi[4], const double qinf[4], double nx,
-                                           double ny, double qb[4]) {
-  const double rhoI = qi[0], uI = qi[1] / qi[0], vI = qi[2] / qi[0];
-  const double pI = pressure(g, qi[0], qi[1], qi[2], qi[3]);
-  const double cI = sound_speed(g, qi);
+template <class T, class P>
+__device__ __forceinline__ void riemann_bc(double g, double minf, P pinf, P cinf, const T qi[4],
+                                           const T qinf[4], double nx, double ny, T qb[4]) {
+  const T rhoI = qi[0], uI = qi[1] / qi[0], vI = qi[2] / qi[0];
+  const T pI = pressure(T(g), qi[0], qi[1], qi[2], qi[3]);
+  const T cI = sound_speed(T(g), qi);
   const double GM1 = g - 1., OOGM1 = 1. / GM1;
-  const double rhoF = qinf[0], uF = qinf[1] / qinf[0], vF = qinf[2] / qinf[0];
+  const T rhoF = qinf[0], uF = qinf[1] / qinf[0], vF = qinf[2] / qinf[0];
   const double tx = -ny, ty = nx;
-  const double VnI = nx * uI + ny * vI;
+  const T VnI = nx * uI + ny * vI;
   if (minf <= 1.) {
-    const double VnF = nx * uF + ny * vF;
-    const double Rinf = VnF - 2. * cinf * OOGM1;
-    const double Rint = VnI + 2. * cI * OOGM1;
-    const double Vn = 0.5 * (Rint + Rinf);
-    const double c = 0.25 * GM1 * (Rint - Rinf);
-    double Vt, beta;
+    const T VnF = nx * uF + ny * vF;
+    const T Rinf = VnF - 2. * cinf * OOGM1;
+    const T Rint = VnI + 2. * cI * OOGM1;
+    const T Vn = 0.5 * (Rint + Rinf);
+    const T c = 0.25 * GM1 * (Rint - Rinf);
+    T Vt, beta;
     if (VnI < 0) { Vt = tx * uF + ty * vF; beta = pinf / pow(rhoF, g); }
     else { Vt = tx * uI + ty * vI; beta = pI / pow(rhoI, g); }
-    const double u = Vn * nx + Vt * tx, v = Vn * ny + Vt * ty;
-    const double rho = pow(c * c / (g * beta), OOGM1);
-    const double p = beta * pow(rho, g);
+    const T u = Vn * nx + Vt * tx, v = Vn * ny + Vt * ty;
+    const T rho = pow(c * c / (g * beta), OOGM1);
+    const T p = beta * pow(rho, g);
     qb[0] = rho; qb[1] = rho * u; qb[2] = rho * v; qb[3] = p * OOGM1 + 0.5 * rho * (u * u + v * v);
   } else if (VnI < 0) {
     for (int c = 0; c < 4; ++c) qb[c] = qinf[c];

@@ -227,6 +227,30 @@ class Euler:
         _lib.check(_lib.load().gocfd_euler2d_get_element_dt(self._h, _dp(d)))
         return d
 
+    # ---- derivatives (FP64, global time stepping, no limiter) -------------
+    def vjp(self, nsteps: int, dQn_bar, checkpoint_every: int = 0):
+        """Discrete adjoint of `nsteps` steps from the current state:
+        returns (dQ0, dQinf) = the transposed Jacobians of the state after
+        the steps, with respect to the current state and the far-field state
+        FSFar.Qinf, applied to dQn_bar. The solver is left unchanged."""
+        lam = np.ascontiguousarray(dQn_bar, dtype=np.float64).reshape(4, self.NpInt, self.K)
+        d0 = np.empty_like(lam)
+        dfar = np.empty(4)
+        _lib.check(_lib.load().gocfd_euler2d_vjp(self._h, nsteps, checkpoint_every, _dp(lam), _dp(d0),
+                                                 _dp(dfar)))
+        return d0, dfar
+
+    def jvp(self, nsteps: int, dQ0, dQinf=None):
+        """Tangent-linear run of `nsteps` steps from the current state:
+        returns (Qn, dQn), the state after the steps and its directional
+        derivative along (dQ0, dQinf). The solver is left unchanged."""
+        v = np.ascontiguousarray(dQ0, dtype=np.float64).reshape(4, self.NpInt, self.K)
+        qn, dqn = np.empty_like(v), np.empty_like(v)
+        w = None if dQinf is None else np.ascontiguousarray(dQinf, dtype=np.float64).reshape(4)
+        _lib.check(_lib.load().gocfd_euler2d_jvp(self._h, nsteps, _dp(v), None if w is None else _dp(w),
+                                                 _dp(qn), _dp(dqn)))
+        return qn, dqn
+
     def kernel_time(self):
         ms, n = C.c_double(), C.c_int32()
         _lib.check(_lib.load().gocfd_euler2d_kernel_time(self._h, C.byref(ms), C.byref(n)))

@@ -153,6 +153,30 @@ int gocfd_euler2d_get_shock(gocfd_euler2d *h, double *Se, double *Sigma, double 
  * CalculateLocalDT, euler.go:973-1002) [K] */
 int gocfd_euler2d_get_element_dt(gocfd_euler2d *h, double *DT);
 
+/* ---- Tangent-linear and discrete adjoint of the FP64 step (no counterpart
+ *      in the reference, which has no derivative code; the differentiated
+ *      map is its SSP54 step, rkAdvance euler.go:460-566, with the
+ *      per-stage global dt of calculateGlobalDT euler.go:945-971). Both start
+ *      from the solver's current state and time and leave the solver
+ *      unchanged. Inputs of the map: the state Q [4][NpInt][K] and the
+ *      far-field state FSFar.Qinf [4] (rho, rhoU, rhoV, E; FSFar's pressure
+ *      and sound speed follow it); the In/Out states follow it too except in
+ *      the shock tube. Needs an FP64 solver with global time stepping, no
+ *      limiter, no IVortex boundary, and steps that stay short of FinalTime
+ *      (GOCFD_ERR_ARG otherwise). The dt of a stage follows the face point
+ *      of its maximum wave speed (ties: the first in the device order). ---- */
+/* Vector-Jacobian product over nsteps steps: dQ0 = (dQn/dQ)^T dQn_bar and
+ * dQinf = (dQn/dQinf)^T dQn_bar (dQinf may be NULL). The reverse sweep
+ * recomputes the steps from a checkpoint every checkpoint_every steps (0:
+ * ceil(sqrt(nsteps))) and each step's stages from its start. */
+int gocfd_euler2d_vjp(gocfd_euler2d *h, int32_t nsteps, int32_t checkpoint_every, const double *dQn_bar,
+                      double *dQ0, double *dQinf);
+/* Jacobian-vector product over nsteps steps (tangent-linear step in dual
+ * arithmetic): dQn = (dQn/dQ) dQ0 + (dQn/dQinf) dQinf (dQinf may be NULL for
+ * zero); Qn (may be NULL) receives the state after the steps. */
+int gocfd_euler2d_jvp(gocfd_euler2d *h, int32_t nsteps, const double *dQ0, const double *dQinf, double *Qn,
+                      double *dQn);
+
 /* ---- Batched ensembles (no counterpart in the reference, which runs one
  *      case per process). ncases independent cases -- each its own mesh,
  *      order, initial state, Mach number, angle, CFL and FinalTime -- stepped

@@ -1,0 +1,176 @@
+"""GPU checks of the tangent-linear (jvp) and discrete-adjoint (vjp) step.
+
+* The tangent-linear run's primal state equals the solver's own steps to
+  1e-12 (the same map, other kernels).
+* Dot-product test: <s, J v> from jvp equals <J^T s, v> from vjp to 1e-10
+  relative, for random s and v over the state and the far-field state, on
+  an airfoil (far field + wall), a doubly periodic mesh (periodic shared
+  faces, vortex initial state) and a shock tube (In/Out far states, which stay constant there),
+  every flux and order 1..4.
+* Checkpoint schedules (every step, every 2, sqrt, one block) give
+  bit-identical gradients.
+* Lift gradients against central differences (h = 1e-6): of the GPU solver
+  in Minf, angle of attack and two state directions, and of the CPU oracle
+  (the reference's algorithm restated) in Minf and one state direction;
+  within 1e-6 relative.
+* Refusals: FP32, LocalTimeStepping, PerssonC0, IVortex boundaries, a
+  window that reaches FinalTime.
+"""
+import numpy as np
+import pytest
+
+import oracle
+from gocfd_amd import Euler, InputParameters2D, adjoint, meshgen
+from gocfd_amd._lib import GocfdError
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def meshes(tmp_path_factory):
+    d = tmp_path_factory.mktemp("adj")
+    return dict(
+        foil=meshgen.airfoil_mesh(str(d / "foil.su2"), n_around=60, n_radial=12, radius=8.0, seed=2),
+        periodic=meshgen.periodic_mesh(str(d / "per.su2"), 6, seed=4),
+        tube=meshgen.shocktube_mesh(str(d / "st.su2"), 30, 2, height=0.05),
+    )
+
+
+def _ip(N, flux="Roe", init="Freestream", M=0.5, a=2.0, cfl=1.0, final=1e9, **kw):
+    return InputParameters2D(CFL=cfl, FluxType=flux, InitType=init, PolynomialOrder=N, Minf=M, Alpha=a,
+                             FinalTime=final, MaxIterations=100000, **kw)
+
+
+def _developed(mesh, ip, steps=3):
+    """A solver a few steps into the run (a non-uniform state)."""
+    e = Euler(ip, mesh, precision="fp64")
+    e.step(steps)
+    return e
+
+
+def _rel(a, b):
+    return float(np.abs(a - b).max() / max(np.abs(b).max(), 1e-300))
+
+
+@pytest.mark.parametrize("N,flux", [(1, "Lax"), (2, "Roe"), (3, "Roe"), (4, "Lax"), (2, "Average"),
+                                    (3, "Roe-ER")])
+def test_tangent_linear_primal_matches_solver(meshes, N, flux):
+    e = _developed(meshes["foil"], _ip(N, flux))
+    qn, _ = e.jvp(5, np.zeros_like(e.Q))
+    e.step(5)
+    assert _rel(qn, e.Q) < 1e-12
+
+
+@pytest.mark.parametrize("mesh,N,flux,init", [
+    ("foil", 1, "Lax", "Freestream"), ("foil", 2, "Roe", "Freestream"), ("foil", 3, "Roe-ER", "Freestream"),
+    ("foil", 4, "Roe", "Freestream"), ("foil", 2, "Average", "Freestream"),
+    ("periodic", 2, "Roe", "IVortex"), ("periodic", 3, "Lax", "IVortex"),
+    ("tube", 2, "Roe", "shocktube"), ("tube", 1, "Lax", "shocktube")])
+def test_dot_product(meshes, mesh, N, flux, init):
+    ip = _ip(N, flux, init, M=0.0 if init == "shocktube" else 0.5, cfl=0.5 if init == "shocktube" else 1.0)
+    e = _developed(meshes[mesh], ip)
+    rng = np.random.default_rng(N + 10 * len(mesh))
+    q = e.Q
+    v = rng.standard_normal(q.shape) * np.abs(q)
+    w = rng.standard_normal(4)
+    s = rng.standard_normal(q.shape)
+    nsteps = 4
+    _, jv = e.jvp(nsteps, v, w)
+    d0, dfar = e.vjp(nsteps, s)
+    lhs = float(np.sum(s * jv))
+    rhs = float(np.sum(d0 * v) + dfar @ w)
+    assert abs(lhs - rhs) <= 1e-10 * max(abs(lhs), 1e-300), (lhs, rhs)
+    if init == "shocktube":
+        assert np.all(dfar == 0.0)  # the In/Out states do not follow FSFar in the shock tube
+    # the solver is unchanged
+    assert np.array_equal(e.Q, q)
+
+
+def test_checkpoint_schedules_are_bitwise_identical(meshes):
+    e = _developed(meshes["foil"], _ip(2))
+    s = np.random.default_rng(5).standard_normal(e.Q.shape)
+    ref = e.vjp(7, s, 1)
+    for every in (2, 0, 7):
+        d0, dfar = e.vjp(7, s, every)
+        assert np.array_equal(d0, ref[0]) and np.array_equal(dfar, ref[1]), every
+
+
+def _lift_after(mesh, ip, n, q0=None):
+    e = Euler(ip, mesh, precision="fp64")
+    if q0 is not None:
+        e.Q = q0
+        e.set_time(0.0)
+    e.step(n)
+    return adjoint.lift(e, e.Q, ip.Alpha)
+
+
+def test_lift_gradients_against_solver_differences(meshes):
+    mesh, n, h = meshes["foil"], 20, 1e-6
+    ip = _ip(2)
+    e = Euler(ip, mesh, precision="fp64")
+    g = adjoint.lift_gradients(e, n)
+    assert g["J"] == pytest.approx(_lift_after(mesh, ip, n), rel=1e-10)
+    fdM = (_lift_after(mesh, _ip(2, M=0.5 + h), n) - _lift_after(mesh, _ip(2, M=0.5 - h), n)) / (2 * h)
+    fda = (_lift_after(mesh, _ip(2, a=2.0 + h), n) - _lift_after(mesh, _ip(2, a=2.0 - h), n)) / (2 * h)
+    assert g["dJ_dMinf"] == pytest.approx(fdM, rel=1e-6)
+    assert g["dJ_dalpha_deg"] == pytest.approx(fda, rel=1e-6)
+    # state directions from a developed state (not the uniform start, where
+    # every wave-speed candidate of the dt maximum ties)
+    e2 = _developed(mesh, ip, 5)
+    q0 = e2.Q
+    e2.set_time(0.0)
+    g2 = adjoint.lift_gradients(e2, n)
+    rng = np.random.default_rng(7)
+    for _ in range(2):
+        v = rng.standard_normal(q0.shape) * np.abs(q0)
+        fd = (_lift_after(mesh, ip, n, q0 + h * v) - _lift_after(mesh, ip, n, q0 - h * v)) / (2 * h)
+        assert float(np.sum(g2["dJ_dQ0"] * v)) == pytest.approx(fd, rel=1e-6)
+
+
+def test_lift_gradients_against_oracle_differences(meshes):
+    """Central differences of the CPU oracle (FP64, the reference's
+    algorithm) in Minf and in a state direction."""
+    mesh, n, h, N = meshes["foil"], 10, 1e-6, 2
+    ip = _ip(N)
+    e = Euler(ip, mesh, precision="fp64")
+
+    def J_oracle(M, q0=None):
+        o = oracle.OracleSolver(mesh, N, flux="roe", init="freestream", CFL=1.0, Minf=M, Alpha=2.0)
+        if q0 is not None:
+            o.set_state(q0)
+        for _ in range(n):
+            o.step()
+        return adjoint.lift(e, o.state(), 2.0)
+
+    g = adjoint.lift_gradients(e, n)
+    fdM = (J_oracle(0.5 + h) - J_oracle(0.5 - h)) / (2 * h)
+    assert g["dJ_dMinf"] == pytest.approx(fdM, rel=1e-6)
+    e2 = _developed(mesh, ip, 4)
+    q0 = e2.Q
+    e2.set_time(0.0)
+    g2 = adjoint.lift_gradients(e2, n)
+    v = np.random.default_rng(9).standard_normal(q0.shape) * np.abs(q0)
+    fd = (J_oracle(0.5, q0 + h * v) - J_oracle(0.5, q0 - h * v)) / (2 * h)
+    assert float(np.sum(g2["dJ_dQ0"] * v)) == pytest.approx(fd, rel=1e-6)
+
+
+def test_refusals(meshes):
+    z = None
+    cases = [
+        (dict(ip=_ip(2), precision="fp32"), "FP64"),
+        (dict(ip=_ip(2, LocalTimeStepping=True)), "global time step"),
+        (dict(ip=_ip(2, Limiter="PerssonC0", Kappa=3.0)), "PerssonC0"),
+    ]
+    for kw, msg in cases:
+        e = Euler(kw["ip"], meshes["foil"], precision=kw.get("precision", "fp64"))
+        z = np.zeros_like(e.Q)
+        with pytest.raises(GocfdError, match=msg):
+            e.vjp(2, z)
+        with pytest.raises(GocfdError, match=msg):
+            e.jvp(2, z)
+    e = Euler(_ip(2, init="IVortex"), meshes["foil"])  # its wall edges take the vortex state
+    with pytest.raises(GocfdError, match="IVortex"):
+        e.vjp(2, np.zeros_like(e.Q))
+    e = Euler(_ip(2, final=1e-4), meshes["foil"])
+    with pytest.raises(GocfdError, match="FinalTime"):
+        e.vjp(200, np.zeros_like(e.Q))
