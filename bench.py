@@ -16,7 +16,10 @@ both against the analytic convected vortex.
 The same run measures the shock-capturing path (PerssonC0) on a Mach 2
 wedge-channel mesh of about 250k triangles at P=2 (FP64, 2000 steps) under
 "shock_path", and a batched ensemble of 64 independent cases (2k..40k
-triangles, orders 1..4, FP32, 1000 steps) under "ensemble".
+triangles, orders 1..4, FP32, 1000 steps) under "ensemble", and the adjoint
+workload (a 100k-triangle airfoil O-mesh at P=3, FP64) under "airfoil": the
+forward step, and the lift gradients of one discrete-adjoint sweep against
+central finite differences of the solver.
 
 python bench.py [--gpus 1] [--steps 500] [--warmup 10] [--no-cpu] [--no-shock]
 """
@@ -214,44 +217,23 @@ def run_ensemble(ncases, steps, warmup):
                 seq_rate=seq_rate, errs=errs, tuned=tuned, tune_seconds=t_tune)
 
 
-BC_WALL = 5  # types/cfd.go BCFLAG numbering, as returned by gocfd_euler2d_get_edges
-
-
-def lift_functional(e, alpha_deg: float, gamma: float = 1.4) -> float:
-    """Lift of the wall edges, a functional the reference does not define:
-    J = sum over wall edges of |e| * mean_i p_i * (n . l), with p_i the
-    pressure of the owner's trace at the edge's RT points (FluxEdgeInterp of
-    the element state), n the element's outward normal (into the body, so
-    p n is the pressure force on it) and l = (-sin a, cos a) the lift
-    direction."""
-    Q = e.Q                                                 # [4][NpInt][K]
-    I = e.operators()["Interp"].reshape(3, e.NpEdge, e.NpInt)
-    geo = e.geometry()
-    ed = e.edges()
-    w = ed[ed[:, 9] == BC_WALL]
-    k, f = w[:, 3], w[:, 5]
-    Qf = np.einsum("epi,nie->nep", I[f], Q[:, :, k])        # [4][nwall][NpEdge]
-    rho, ru, rv, E = Qf
-    p = (gamma - 1.0) * (E - 0.5 * (ru * ru + rv * rv) / rho)
-    length = geo["IInII"][f, k] * np.where(f == 1, 2.0 * math.sqrt(2.0), 2.0)
-    a = math.radians(alpha_deg)
-    proj = -geo["FaceNx"][f, k] * math.sin(a) + geo["FaceNy"][f, k] * math.cos(a)
-    return float(np.sum(length * p.mean(axis=1) * proj))
-
-
-def run_airfoil(mesh, steps, warmup, h=1e-6, ndir=4, seed=3):
-    """The adjoint workload's forward part: 4-digit airfoil in an O-domain,
-    P=3, M=0.5, AoA 2 deg, Roe, FP64, `steps` SSP54 steps from the
-    freestream. Times the step and evaluates the lift functional and its
-    central finite differences (step h) with respect to Mach, angle of attack
-    (degrees) and `ndir` seeded directions of the initial state -- the values
-    an adjoint gradient is checked against."""
-    from gocfd_amd import Euler, InputParameters2D
+def run_airfoil(mesh, steps, warmup, h=1e-6, ndir=2, seed=3, developed=20):
+    """The adjoint workload: 4-digit airfoil in an O-domain, P=3, M=0.5,
+    AoA 2 deg, Roe, FP64, n = warmup + steps SSP54 steps from the freestream.
+    Times the forward step; computes the lift functional's gradients by one
+    discrete-adjoint sweep (Euler.vjp, checkpointed) with respect to Mach,
+    angle of attack (degrees) and the initial state, and checks them against
+    central finite differences (step h) of the solver: in Mach and angle from
+    the freestream start, and in `ndir` seeded state directions from a state
+    `developed` steps into the run (at the uniform start every wave-speed
+    candidate of the dt maximum ties, so the state derivative is one-sided
+    there). Also the tangent-linear run's dot product against the adjoint."""
+    from gocfd_amd import Euler, InputParameters2D, adjoint
 
     def make(M, a):
         ip = InputParameters2D(Title="airfoil", CFL=1.0, FluxType="Roe", InitType="Freestream",
                                PolynomialOrder=3, Minf=M, Alpha=a, FinalTime=1e9,
-                               MaxIterations=warmup + steps)
+                               MaxIterations=warmup + steps + developed)
         return Euler(ip, mesh, precision="fp64")
 
     M0, a0 = 0.5, 2.0
@@ -263,34 +245,59 @@ def run_airfoil(mesh, steps, warmup, h=1e-6, ndir=4, seed=3):
     elapsed = time.perf_counter() - t0
     stage_ms, _ = e.kernel_time()
     t_end, dt, _ = e.time_info()
-    J = lift_functional(e, a0)
     n = warmup + steps
+    J = adjoint.lift(e, e.Q, a0)
 
-    def J_of(M, a):
+    def J_of(M, a, q0=None):
         g = make(M, a)
+        if q0 is not None:
+            g.Q = q0
+            g.set_time(0.0)
         g.step(n)
-        return lift_functional(g, a)
+        return adjoint.lift(g, g.Q, a)
 
+    # adjoint from the freestream start: Mach and angle
+    e0 = make(M0, a0)
+    ta = time.perf_counter()
+    ga = adjoint.lift_gradients(e0, n)
+    grad_seconds = time.perf_counter() - ta
+    seed_n = adjoint.lift_state_gradient(e0, e.Q, a0)
+    tv = time.perf_counter()
+    e0.vjp(n, seed_n)
+    vjp_seconds = time.perf_counter() - tv
     t1 = time.perf_counter()
-    dJ_dM = (J_of(M0 + h, a0) - J_of(M0 - h, a0)) / (2 * h)
-    dJ_da = (J_of(M0, a0 + h) - J_of(M0, a0 - h)) / (2 * h)
-    g = make(M0, a0)
-    q0 = g.Q
+    fdM = (J_of(M0 + h, a0) - J_of(M0 - h, a0)) / (2 * h)
+    fda = (J_of(M0, a0 + h) - J_of(M0, a0 - h)) / (2 * h)
+    # state directions from a developed state
+    ed = make(M0, a0)
+    ed.step(developed)
+    qd = ed.Q
+    ed.set_time(0.0)
+    gd = adjoint.lift_gradients(ed, n)
     rng = np.random.default_rng(seed)
-
-    def J_from(q):
-        g.Q = q
-        g.set_time(0.0)
-        g.step(n)
-        return lift_functional(g, a0)
-
     dirs = []
     for _ in range(ndir):
-        v = rng.standard_normal(q0.shape) * np.abs(q0)
-        dirs.append((J_from(q0 + h * v) - J_from(q0 - h * v)) / (2 * h))
+        v = rng.standard_normal(qd.shape) * np.abs(qd)
+        fd = (J_of(M0, a0, qd + h * v) - J_of(M0, a0, qd - h * v)) / (2 * h)
+        ad = float(np.sum(gd["dJ_dQ0"] * v))
+        dirs.append({"adjoint": ad, "fd": fd, "rel_err": abs(ad - fd) / abs(fd)})
     fd_seconds = time.perf_counter() - t1
-    return dict(elapsed=elapsed, stage_ms=stage_ms, K=e.K, NpInt=e.NpInt, J=J, time=t_end, dt=dt,
-                dJ_dMinf=dJ_dM, dJ_dalpha_deg=dJ_da, dJ_dq0_dirs=dirs, fd_seconds=fd_seconds)
+    # tangent-linear dot product at full size: <dJ/dQn, J v> = <J^T dJ/dQn, v>
+    v = rng.standard_normal(qd.shape) * np.abs(qd)
+    qn, jv = ed.jvp(n, v)
+    tlm = float(np.sum(adjoint.lift_state_gradient(ed, qn, a0) * jv))
+    adj = float(np.sum(gd["dJ_dQ0"] * v))
+    return dict(elapsed=elapsed, stage_ms=stage_ms, K=e.K, NpInt=e.NpInt, J=J, time=t_end, dt=dt, steps=n,
+                adjoint={"dJ_dMinf": ga["dJ_dMinf"], "fd_dJ_dMinf": fdM,
+                         "rel_err_Minf": abs(ga["dJ_dMinf"] - fdM) / abs(fdM),
+                         "dJ_dalpha_deg": ga["dJ_dalpha_deg"], "fd_dJ_dalpha_deg": fda,
+                         "rel_err_alpha": abs(ga["dJ_dalpha_deg"] - fda) / abs(fda),
+                         "state_dirs_from_developed_state": dirs,
+                         "tangent_linear_dot_rel": abs(tlm - adj) / abs(tlm),
+                         "vjp_seconds": vjp_seconds, "vjp_ms_per_step": vjp_seconds / n * 1e3,
+                         "vjp_over_forward_step": (vjp_seconds / n) / (elapsed / steps),
+                         "gradient_seconds_with_primal": grad_seconds, "fd_seconds": fd_seconds,
+                         "checkpoint_every": int(math.ceil(math.sqrt(n)))})
 
 
 def cpu_baseline(mesh, target_seconds=15.0):
@@ -480,10 +487,7 @@ def main():
                    "value": ra["K"] * ra["NpInt"] * 5 * args.airfoil_steps / ra["elapsed"], "unit": "DOF-stage/s",
                    "ms_per_step": ra["elapsed"] / args.airfoil_steps * 1e3, "stage_ms": ra["stage_ms"],
                    "time": ra["time"], "dt": ra["dt"], "lift_functional": ra["J"],
-                   "fd_central_h1e-6": {"dJ_dMinf": ra["dJ_dMinf"], "dJ_dalpha_deg": ra["dJ_dalpha_deg"],
-                                        "dJ_dq0_random_dirs": ra["dJ_dq0_dirs"], "seconds": ra["fd_seconds"]},
-                   "adjoint": "not built: the finite differences above are the values a discrete adjoint "
-                              "would be checked against"}
+                   "adjoint_vs_central_fd_h1e-6": ra["adjoint"]}
 
     cpu = None
     vs_oracle = None

@@ -59,7 +59,7 @@ struct AdjBufs {
   double *Wv;           // [5] the maximum wave speeds
   double *time;         // step-start time (advanced by the last stage's dt)
   int *flag;            // set when a stage's dt would be clamped to FinalTime
-  double *part_v;       // reduction partials
+  double *part_v;       // reduction partials (at least nparts and one per 256 element values)
   int *part_i;
   int nparts;
   // adjoint

@@ -262,8 +262,8 @@ hipError_t adj_launch_common(int op, const AdjArgs &a, const AdjBufs &b, hipStre
                            static_cast<const double *>(b.dts), static_cast<double *>(b.out));
       break;
     case ADJ_ADJ_SSP: {
+      // part_v holds one partial per block of the element array
       const unsigned nb = blocks_of(b.n);
-      if (int(nb) > b.nparts) return hipErrorInvalidValue;
       hipLaunchKernelGGL(k_adj_ssp, dim3(nb), dim3(256), 0, s, b.stage, b.n, b.lam, b.lamU0, b.lamQ[1], b.lamQ[2],
                          b.lamQ[3], b.lamQ[4], b.lamR, b.lamR3, static_cast<const double *>(b.Rs),
                          static_cast<const double *>(b.R3), static_cast<const double *>(b.dts), b.part_v);

@@ -251,39 +251,38 @@ __global__ void __launch_bounds__(256) k_adj_face(AdjArgs a, const double *U, co
   int f, kk, pp;
   size_t fi;
   face_point<N>(a, t, f, kk, pp, fi);
-  double lo[4] = {0., 0., 0., 0.}, lt[4] = {0., 0., 0., 0.}, lq[4] = {0., 0., 0., 0.};
-  if (kk < a.K) {
-    const int code = a.code[fi], kind = code & 15;
-    double lf[4];
-    edge_cotangent<N>(a, lamR, kk, f * Sh::NpEdge + pp, lf);
-    double own[4], oth[4] = {0., 0., 0., 0.}, qfar[4];
-    trace_at<N>(a, U, kk, f * Sh::NpEdge + pp, own);
-    if (kind == FK_Shared)
-      trace_at<N>(a, U, a.nbr_elem[fi], a.nbr_face[fi] * Sh::NpEdge + (Sh::NpEdge - 1 - pp), oth);
-    far_state(a, qfar);
-    const bool far = kind == FK_Far && ((code >> 8) == 0 || a.io_tied);
-    // input sets: 0 the own trace, 1 the neighbour's trace, 2 the far state
-    for (int which = 0; which < 3; ++which) {
-      if ((which == 1 && kind != FK_Shared) || (which == 2 && !far)) continue;
-      double *dst = which == 0 ? lo : (which == 1 ? lt : lq);
-      for (int d = 0; d < 4; ++d) {
-        Dual o[4], n[4], q[4], F[4];
-        for (int c = 0; c < 4; ++c) {
-          o[c] = Dual(own[c], which == 0 && c == d ? 1. : 0.);
-          n[c] = Dual(oth[c], which == 1 && c == d ? 1. : 0.);
-          q[c] = Dual(qfar[c], which == 2 && c == d ? 1. : 0.);
-        }
-        face_value(a, code, fi, o, n, q, F);
-        double s = 0.;
-        for (int c = 0; c < 4; ++c) s += lf[c] * F[c].d;
-        dst[d] = s;
-      }
-    }
-  }
+  double *lo = lamOwn + 4 * t, *lt = lamOth + 4 * t, *lq = lamFarPts + 4 * t;
   for (int c = 0; c < 4; ++c) {
-    lamOwn[4 * t + c] = lo[c];
-    lamOth[4 * t + c] = lt[c];
-    lamFarPts[4 * t + c] += lq[c];
+    lo[c] = 0.;
+    lt[c] = 0.;
+  }
+  if (kk >= a.K) return;
+  const int code = a.code[fi], kind = code & 15;
+  double lf[4];
+  edge_cotangent<N>(a, lamR, kk, f * Sh::NpEdge + pp, lf);
+  double own[4], oth[4] = {0., 0., 0., 0.}, qfar[4];
+  trace_at<N>(a, U, kk, f * Sh::NpEdge + pp, own);
+  if (kind == FK_Shared)
+    trace_at<N>(a, U, a.nbr_elem[fi], a.nbr_face[fi] * Sh::NpEdge + (Sh::NpEdge - 1 - pp), oth);
+  far_state(a, qfar);
+  const bool far = kind == FK_Far && ((code >> 8) == 0 || a.io_tied);
+  // input directions j = 4 set + component; sets: 0 the own trace, 1 the
+  // neighbour's trace, 2 the far state (its cotangent accumulates)
+  for (int j = 0; j < 12; ++j) {
+    const int which = j >> 2, d = j & 3;
+    if ((which == 1 && kind != FK_Shared) || (which == 2 && !far)) continue;
+    Dual o[4], n[4], q[4], F[4];
+    for (int c = 0; c < 4; ++c) {
+      o[c] = Dual(own[c], which == 0 && c == d ? 1. : 0.);
+      n[c] = Dual(oth[c], which == 1 && c == d ? 1. : 0.);
+      q[c] = Dual(qfar[c], which == 2 && c == d ? 1. : 0.);
+    }
+    face_value(a, code, fi, o, n, q, F);
+    double sum = 0.;
+    for (int c = 0; c < 4; ++c) sum += lf[c] * F[c].d;
+    if (which == 0) lo[d] = sum;
+    else if (which == 1) lt[d] = sum;
+    else lq[d] += sum;
   }
 }
 
@@ -305,11 +304,40 @@ __global__ void __launch_bounds__(256) k_adj_elem(AdjArgs a, const double *U, co
   const double J0 = a.Jinv[kk], J1 = a.Jinv[size_t(a.Kp) + kk];
   const double J2 = a.Jinv[2 * size_t(a.Kp) + kk], J3 = a.Jinv[3 * size_t(a.Kp) + kk];
   const double *D = a.DivInt;
+  double *tg = target + size_t(kk) * Sh::NV;
+  // face points: FluxEdgeInterp^T of the trace cotangents
   double lu[4][NpInt];
+#pragma unroll
   for (int c = 0; c < 4; ++c)
+#pragma unroll
     for (int i = 0; i < NpInt; ++i) lu[c][i] = 0.;
+  for (int p = 0; p < Sh::NF; ++p) {
+    const int f = p / NpEdge, pp = p - f * NpEdge;
+    const size_t fi = size_t(f) * a.Kp + kk;
+    const size_t t = fi * NpEdge + pp;
+    double lq[4];
+#pragma unroll
+    for (int c = 0; c < 4; ++c) lq[c] = lamOwn[4 * t + c];
+    if ((a.code[fi] & 15) == FK_Shared) {
+      const size_t t2 = (size_t(a.nbr_face[fi]) * a.Kp + a.nbr_elem[fi]) * NpEdge + (NpEdge - 1 - pp);
+#pragma unroll
+      for (int c = 0; c < 4; ++c) lq[c] += lamOth[4 * t2 + c];
+    }
+    const double *row = a.Interp + p * NpInt;
+#pragma unroll
+    for (int c = 0; c < 4; ++c)
+#pragma unroll
+      for (int i = 0; i < NpInt; ++i) lu[c][i] += row[i] * lq[c];
+  }
+#pragma unroll
+  for (int c = 0; c < 4; ++c)
+#pragma unroll
+    for (int i = 0; i < NpInt; ++i) tg[c * NpInt + i] += lu[c][i];
+  // solution points: DivInt^T (r and s columns), the contravariant
+  // transform and the physical-flux Jacobian
   for (int j = 0; j < NpInt; ++j) {
     double lFx[4], lFy[4];
+#pragma unroll
     for (int c = 0; c < 4; ++c) {
       double lr_ = 0., ls_ = 0.;
       for (int i = 0; i < NpInt; ++i) {
@@ -321,32 +349,18 @@ __global__ void __launch_bounds__(256) k_adj_elem(AdjArgs a, const double *U, co
       lFx[c] = Jd * (J0 * lr_ + J2 * ls_);
       lFy[c] = Jd * (J1 * lr_ + J3 * ls_);
     }
+#pragma unroll
     for (int d = 0; d < 4; ++d) {
       Dual q[4], Fx[4], Fy[4];
+#pragma unroll
       for (int c = 0; c < 4; ++c) q[c] = Dual(u[c * NpInt + j], c == d ? 1. : 0.);
       flux_xy(Dual(a.ph.gamma), q, Fx, Fy);
-      double s = 0.;
-      for (int c = 0; c < 4; ++c) s += lFx[c] * Fx[c].d + lFy[c] * Fy[c].d;
-      lu[d][j] += s;
+      double sum = 0.;
+#pragma unroll
+      for (int c = 0; c < 4; ++c) sum += lFx[c] * Fx[c].d + lFy[c] * Fy[c].d;
+      tg[d * NpInt + j] += sum;
     }
   }
-  for (int p = 0; p < Sh::NF; ++p) {
-    const int f = p / NpEdge, pp = p - f * NpEdge;
-    const size_t fi = size_t(f) * a.Kp + kk;
-    const size_t t = fi * NpEdge + pp;
-    double lq[4];
-    for (int c = 0; c < 4; ++c) lq[c] = lamOwn[4 * t + c];
-    if ((a.code[fi] & 15) == FK_Shared) {
-      const size_t t2 = (size_t(a.nbr_face[fi]) * a.Kp + a.nbr_elem[fi]) * NpEdge + (NpEdge - 1 - pp);
-      for (int c = 0; c < 4; ++c) lq[c] += lamOth[4 * t2 + c];
-    }
-    const double *row = a.Interp + p * NpInt;
-    for (int c = 0; c < 4; ++c)
-      for (int i = 0; i < NpInt; ++i) lu[c][i] += row[i] * lq[c];
-  }
-  double *tg = target + size_t(kk) * Sh::NV;
-  for (int c = 0; c < 4; ++c)
-    for (int i = 0; i < NpInt; ++i) tg[c * NpInt + i] += lu[c][i];
 }
 
 // The dt term of a stage: dt = CFL / W with W the wave speed at the stage's
