@@ -1,5 +1,5 @@
 """CPU checks of the adjoint-workload harness: the airfoil O-mesh and the lift
-functional of bench.py, evaluated on the oracle (the functional reads only
+functional (gocfd_amd.adjoint.lift), evaluated on the oracle (the functional reads only
 the state, the operators, the geometry and the edge list, which the oracle
 and the GPU handle expose alike).
 
@@ -12,26 +12,17 @@ and the GPU handle expose alike).
   angle: the force is (0, -b A) and the lift direction (-sin a, cos a).
 """
 import math
-import os
-import sys
 
 import numpy as np
 import pytest
 
 import oracle
-from gocfd_amd import Mesh, meshgen
-
-sys.path.insert(0, os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
-import bench  # noqa: E402
+from gocfd_amd import Mesh, adjoint, meshgen
 
 
 class _OracleView:
-    def __init__(self, o, q=None):
-        self.o, self.NpEdge, self.NpInt, self._q = o, o.NpEdge, o.NpInt, q
-
-    @property
-    def Q(self):
-        return self.o.state() if self._q is None else self._q
+    def __init__(self, o):
+        self.o, self.NpEdge, self.NpInt = o, o.NpEdge, o.NpInt
 
     def operators(self):
         return self.o.operators()
@@ -56,13 +47,13 @@ def test_airfoil_mesh(small):
     assert (cross > 0).all()
     o = oracle.OracleSolver(small, 1, flux="roe", init="freestream", Minf=0.5)
     bc = o.edges()[:, 9]
-    assert (bc == bench.BC_WALL).sum() == 64 and (bc == 4).sum() == 64
+    assert (bc == adjoint.BC_WALL).sum() == 64 and (bc == 4).sum() == 64
 
 
 @pytest.mark.parametrize("N", [1, 3])
 def test_lift_functional_closure_and_linear_pressure(small, N):
     o = oracle.OracleSolver(small, N, flux="roe", init="freestream", Minf=0.5, Alpha=2.0)
-    assert abs(bench.lift_functional(_OracleView(o), 2.0)) < 1e-14
+    assert abs(adjoint.lift(_OracleView(o), o.state(), 2.0)) < 1e-14
     Y = o.coords()["SolY"]
     b, c = 0.3, 2.0
     q = np.stack([np.ones_like(Y), 0 * Y, 0 * Y, (b * Y + c) / 0.4])
@@ -71,5 +62,5 @@ def test_lift_functional_closure_and_linear_pressure(small, N):
     x, y = VX[:64], VY[:64]
     area = 0.5 * abs(np.dot(x, np.roll(y, -1)) - np.dot(y, np.roll(x, -1)))
     for a in (0.0, 2.0, 30.0):
-        J = bench.lift_functional(_OracleView(o, q), a, gamma=1.4)
+        J = adjoint.lift(_OracleView(o), q, a, gamma=1.4)
         assert abs(J - (-b * area * math.cos(math.radians(a)))) < 1e-12 * max(1.0, abs(J)), (a, J)
