@@ -31,6 +31,14 @@ struct ad_storage {
   double v, d;
 };
 
+// SSP54 coefficients (rkAdvance, model_problems/Euler2D/euler.go:511-565)
+constexpr double kSspA0 = 0.391752226571890;
+constexpr double kSspB1 = 0.444370493651235, kSspC1 = 0.555629506348765, kSspA1 = 0.368410593050371;
+constexpr double kSspB2 = 0.620101851488403, kSspC2 = 0.379898148511597, kSspA2 = 0.251891774271694;
+constexpr double kSspB3 = 0.178079954393132, kSspC3 = 0.821920045606868, kSspA3 = 0.544974750228521;
+constexpr double kSspD2 = 0.517231671970585, kSspD3 = 0.096059710526146, kSspD4 = 0.386708617503269;
+constexpr double kSspE3 = 0.063692468666290, kSspE4 = 0.226007483236906;
+
 struct AdjArgs {
   int N, K, Kp;
   const double *Jdet, *Jinv, *IInII, *Nx, *Ny, *oohK;  // [Kp], [4][Kp], [3][Kp] x3, [Kp]
@@ -59,7 +67,7 @@ struct AdjBufs {
   double *Wv;           // [5] the maximum wave speeds
   double *time;         // step-start time (advanced by the last stage's dt)
   int *flag;            // set when a stage's dt would be clamped to FinalTime
-  double *part_v;       // reduction partials (at least nparts and one per 256 element values)
+  double *part_v;       // reduction partials (4 nparts)
   int *part_i;
   int nparts;
   // adjoint
@@ -68,7 +76,7 @@ struct AdjBufs {
   double *lamQ[5];      // cotangents of the stage inputs 1..4 (index 1..4)
   double *lamR;         // cotangent of this stage's RHS
   double *lamR3;        // stage-3 RHS cotangent from the last stage
-  double *lamOwn, *lamOth;  // face-point cotangents of the own and neighbour traces
+  double *lamOwn;       // face-point trace cotangents
   double *lamFarPts;    // face-point cotangents of the far state (accumulated)
   double *lamFarW;      // [4] far-state cotangent from the dt terms (accumulated)
   double *lam_dt;       // [5] cotangents of the stage dts
@@ -79,16 +87,14 @@ struct AdjBufs {
 };
 
 enum AdjOp : int {
-  ADJ_FWD_FACE = 0,  // FE <- edge fluxes of Us
-  ADJ_FWD_RHS,       // R <- RHS(Us, FE)
-  ADJ_FWD_WAVE,      // SP <- owner-side wave speeds of Us
+  ADJ_FWD_FACE = 0,  // FE <- edge values of Us; SP <- owner-side wave speeds of Us
   ADJ_ARGMAX,        // dts[stage], argmax[stage], Wv[stage] <- from SP; time, flag
-  ADJ_FWD_SSP,       // out <- SSP54 combination of the stage
+  ADJ_FWD_RHS,       // R <- RHS(Us, FE); out <- the stage's SSP54 combination
   ADJ_ADJ_SSP,       // cotangents through the SSP combination; lam_dt[stage]
-  ADJ_ADJ_FACE,      // lamOwn, lamOth, lamFarPts <- from lamR at Us
+  ADJ_ADJ_FACE,      // lamOwn, lamFarPts <- from lamR at Us
   ADJ_ADJ_ELEM,      // target += interior-flux and trace transposes
   ADJ_ADJ_WAVE,      // target, lamFarW += the dt term of the stage
-  ADJ_FAR_SUM,       // part_v[0..3] <- sum of lamFarPts + lamFarW
+  ADJ_FAR_SUM,       // target[0..3] <- sum of lamFarPts + lamFarW
   ADJ_PACK,          // out (dual) <- (pack_v, pack_d)
   ADJ_UNPACK,        // (unpack_v, unpack_d) <- Us (dual)
 };

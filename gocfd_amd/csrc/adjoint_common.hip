@@ -1,6 +1,6 @@
 // Order-independent kernels of the tangent-linear and adjoint step: the
-// SSP54 combinations and their transposes, the wave-speed arg-max and dt,
-// the far-state cotangent sum, dual packing; and the order dispatch.
+// transposed SSP54 combinations, the wave-speed arg-max and dt, the
+// far-state cotangent sum, dual packing; and the order dispatch.
 #include <hip/hip_runtime.h>
 
 #include "adjoint.hpp"
@@ -10,37 +10,6 @@ namespace gocfd {
 namespace dev {
 
 using ad::Dual;
-
-// SSP54 coefficients (rkAdvance, model_problems/Euler2D/euler.go:511-565)
-constexpr double kA0 = 0.391752226571890;
-constexpr double kB1 = 0.444370493651235, kC1 = 0.555629506348765, kA1 = 0.368410593050371;
-constexpr double kB2 = 0.620101851488403, kC2 = 0.379898148511597, kA2 = 0.251891774271694;
-constexpr double kB3 = 0.178079954393132, kC3 = 0.821920045606868, kA3 = 0.544974750228521;
-constexpr double kD2 = 0.517231671970585, kD3 = 0.096059710526146, kD4 = 0.386708617503269;
-constexpr double kE3 = 0.063692468666290, kE4 = 0.226007483236906;
-
-// One SSP54 stage in the order of the solver's stage kernel: stage s reads
-// the step start U0, its input Us, its RHS Rs and dt_s; the last stage also
-// U2, U3 and the stage-3 RHS.
-template <class T>
-__global__ void __launch_bounds__(256) k_adj_fwd_ssp(int stage, size_t n, const T *U0, const T *Us, const T *Rs,
-                                                     const T *U2, const T *U3, const T *R3, const T *dts, T *out) {
-  const size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  const T dt = dts[stage];
-  const T dtR = dt * Rs[i];
-  switch (stage) {
-    case 0: out[i] = Us[i] + kA0 * dtR; break;
-    case 1: out[i] = kB1 * U0[i] + kC1 * Us[i] + kA1 * dtR; break;
-    case 2: out[i] = kB2 * U0[i] + kC2 * Us[i] + kA2 * dtR; break;
-    case 3: out[i] = kB3 * U0[i] + kC3 * Us[i] + kA3 * dtR; break;
-    default: {
-      const T dtR3 = dt * R3[i];
-      const T Rn = -U0[i] + kD2 * U2[i] + kD3 * U3[i] + kD4 * Us[i] + kE3 * dtR3 + kE4 * dtR;
-      out[i] = U0[i] + Rn;
-    }
-  }
-}
 
 __device__ __forceinline__ double val_of(double a) { return a; }
 __device__ __forceinline__ double val_of(const Dual &a) { return a.v; }
@@ -134,42 +103,42 @@ __global__ void __launch_bounds__(256) k_adj_ssp(int stage, size_t n, const doub
                                                  double *lamR, double *lamR3, const double *Rs, const double *R3,
                                                  const double *dts, double *part) {
   __shared__ double red[256];
-  const size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  const double dt = dts[stage];
   double dot = 0.;
-  if (i < n) {
-    const double l = lam[i], dt = dts[stage];
+  for (size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += size_t(gridDim.x) * blockDim.x) {
+    const double l = lam[i];
     switch (stage) {
       case 4:
         lamU0[i] = 0.;
-        lamQ2[i] = kD2 * l;
-        lamQ3[i] = kD3 * l;
-        lamQ4[i] = kD4 * l;
-        lamR3[i] = kE3 * dt * l;
-        lamR[i] = kE4 * dt * l;
-        dot = l * (kE3 * R3[i] + kE4 * Rs[i]);
+        lamQ2[i] = kSspD2 * l;
+        lamQ3[i] = kSspD3 * l;
+        lamQ4[i] = kSspD4 * l;
+        lamR3[i] = kSspE3 * dt * l;
+        lamR[i] = kSspE4 * dt * l;
+        dot += l * (kSspE3 * R3[i] + kSspE4 * Rs[i]);
         break;
       case 3:
-        lamU0[i] += kB3 * l;
-        lamQ3[i] += kC3 * l;
-        lamR[i] = lamR3[i] + kA3 * dt * l;
-        dot = kA3 * l * Rs[i];
+        lamU0[i] += kSspB3 * l;
+        lamQ3[i] += kSspC3 * l;
+        lamR[i] = lamR3[i] + kSspA3 * dt * l;
+        dot += kSspA3 * l * Rs[i];
         break;
       case 2:
-        lamU0[i] += kB2 * l;
-        lamQ2[i] += kC2 * l;
-        lamR[i] = kA2 * dt * l;
-        dot = kA2 * l * Rs[i];
+        lamU0[i] += kSspB2 * l;
+        lamQ2[i] += kSspC2 * l;
+        lamR[i] = kSspA2 * dt * l;
+        dot += kSspA2 * l * Rs[i];
         break;
       case 1:
-        lamU0[i] += kB1 * l;
-        lamQ1[i] = kC1 * l;
-        lamR[i] = kA1 * dt * l;
-        dot = kA1 * l * Rs[i];
+        lamU0[i] += kSspB1 * l;
+        lamQ1[i] = kSspC1 * l;
+        lamR[i] = kSspA1 * dt * l;
+        dot += kSspA1 * l * Rs[i];
         break;
       default:
         lamU0[i] += l;
-        lamR[i] = kA0 * dt * l;
-        dot = kA0 * l * Rs[i];
+        lamR[i] = kSspA0 * dt * l;
+        dot += kSspA0 * l * Rs[i];
     }
   }
   red[threadIdx.x] = dot;
@@ -195,14 +164,30 @@ __global__ void __launch_bounds__(256) k_adj_sum(const double *part, int nparts,
   if (threadIdx.x == 0) out[slot] = red[0];
 }
 
-// Far-state cotangent: the face-point records summed per component, plus
-// the dt terms' part (one block, fixed order)
-__global__ void __launch_bounds__(256) k_adj_far_sum(const double *pts, size_t npts, const double *farW,
-                                                     double *out) {
+// Far-state cotangent: the face-point records summed per component (block
+// partials, then one block in a fixed order), plus the dt terms' part
+__global__ void __launch_bounds__(256) k_adj_far_part(const double *pts, size_t npts, double *part) {
   __shared__ double red[4][256];
   double s[4] = {0., 0., 0., 0.};
-  for (size_t i = threadIdx.x; i < npts; i += blockDim.x)
+  for (size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x; i < npts; i += size_t(gridDim.x) * blockDim.x)
     for (int c = 0; c < 4; ++c) s[c] += pts[4 * i + c];
+  for (int c = 0; c < 4; ++c) red[c][threadIdx.x] = s[c];
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if (int(threadIdx.x) < w)
+      for (int c = 0; c < 4; ++c) red[c][threadIdx.x] += red[c][threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0)
+    for (int c = 0; c < 4; ++c) part[4 * blockIdx.x + c] = red[c][0];
+}
+
+__global__ void __launch_bounds__(256) k_adj_far_final(const double *part, int nparts, const double *farW,
+                                                       double *out) {
+  __shared__ double red[4][256];
+  double s[4] = {0., 0., 0., 0.};
+  for (int i = threadIdx.x; i < nparts; i += blockDim.x)
+    for (int c = 0; c < 4; ++c) s[c] += part[4 * i + c];
   for (int c = 0; c < 4; ++c) red[c][threadIdx.x] = s[c];
   __syncthreads();
   for (int w = 128; w > 0; w >>= 1) {
@@ -247,32 +232,18 @@ hipError_t adj_launch_common(int op, const AdjArgs &a, const AdjBufs &b, hipStre
       }
       break;
     }
-    case ADJ_FWD_SSP:
-      if (b.dual)
-        hipLaunchKernelGGL((k_adj_fwd_ssp<Dual>), dim3(blocks_of(b.n)), dim3(256), 0, s, b.stage, b.n,
-                           static_cast<const Dual *>(b.U0), static_cast<const Dual *>(b.Us),
-                           static_cast<const Dual *>(b.Rs), static_cast<const Dual *>(b.U2),
-                           static_cast<const Dual *>(b.U3), static_cast<const Dual *>(b.R3),
-                           static_cast<const Dual *>(b.dts), static_cast<Dual *>(b.out));
-      else
-        hipLaunchKernelGGL((k_adj_fwd_ssp<double>), dim3(blocks_of(b.n)), dim3(256), 0, s, b.stage, b.n,
-                           static_cast<const double *>(b.U0), static_cast<const double *>(b.Us),
-                           static_cast<const double *>(b.Rs), static_cast<const double *>(b.U2),
-                           static_cast<const double *>(b.U3), static_cast<const double *>(b.R3),
-                           static_cast<const double *>(b.dts), static_cast<double *>(b.out));
-      break;
-    case ADJ_ADJ_SSP: {
-      // part_v holds one partial per block of the element array
-      const unsigned nb = blocks_of(b.n);
-      hipLaunchKernelGGL(k_adj_ssp, dim3(nb), dim3(256), 0, s, b.stage, b.n, b.lam, b.lamU0, b.lamQ[1], b.lamQ[2],
-                         b.lamQ[3], b.lamQ[4], b.lamR, b.lamR3, static_cast<const double *>(b.Rs),
+    case ADJ_ADJ_SSP:
+      // nparts blocks stride over the element array; one partial each
+      hipLaunchKernelGGL(k_adj_ssp, dim3(b.nparts), dim3(256), 0, s, b.stage, b.n, b.lam, b.lamU0, b.lamQ[1],
+                         b.lamQ[2], b.lamQ[3], b.lamQ[4], b.lamR, b.lamR3, static_cast<const double *>(b.Rs),
                          static_cast<const double *>(b.R3), static_cast<const double *>(b.dts), b.part_v);
-      hipLaunchKernelGGL(k_adj_sum, dim3(1), dim3(256), 0, s, b.part_v, int(nb), b.lam_dt, b.stage);
+      hipLaunchKernelGGL(k_adj_sum, dim3(1), dim3(256), 0, s, b.part_v, b.nparts, b.lam_dt, b.stage);
       break;
-    }
     case ADJ_FAR_SUM:
-      hipLaunchKernelGGL(k_adj_far_sum, dim3(1), dim3(256), 0, s, b.lamFarPts, size_t(3) * a.Kp * (a.N + 2),
-                         b.lamFarW, b.part_v);
+      // 4 partials per block; the result goes to b.target[0..3]
+      hipLaunchKernelGGL(k_adj_far_part, dim3(b.nparts), dim3(256), 0, s, b.lamFarPts,
+                         size_t(3) * a.Kp * (a.N + 2), b.part_v);
+      hipLaunchKernelGGL(k_adj_far_final, dim3(1), dim3(256), 0, s, b.part_v, b.nparts, b.lamFarW, b.target);
       break;
     case ADJ_PACK:
       hipLaunchKernelGGL(k_adj_pack, dim3(blocks_of(b.n)), dim3(256), 0, s, b.n, b.pack_v, b.pack_d,
@@ -289,8 +260,7 @@ hipError_t adj_launch_common(int op, const AdjArgs &a, const AdjBufs &b, hipStre
 }
 
 hipError_t adj_launch(int op, const AdjArgs &a, const AdjBufs &b, hipStream_t s) {
-  if (op == ADJ_ARGMAX || op == ADJ_FWD_SSP || op == ADJ_ADJ_SSP || op == ADJ_FAR_SUM || op == ADJ_PACK ||
-      op == ADJ_UNPACK)
+  if (op == ADJ_ARGMAX || op == ADJ_ADJ_SSP || op == ADJ_FAR_SUM || op == ADJ_PACK || op == ADJ_UNPACK)
     return adj_launch_common(op, a, b, s);
   switch (a.N) {
     case 0: return adj_launch_n<0>(op, a, b, s);

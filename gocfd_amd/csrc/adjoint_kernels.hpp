@@ -74,22 +74,16 @@ __device__ __forceinline__ void far_bc(const AdjArgs &a, int bidx, const T qfar[
   }
 }
 
-// Edge RT DOF values of face point (fi = f Kp + kk, pp) from its own trace,
-// the neighbour's trace at the reversed point and the far state
+// Edge RT DOF values of an unshared face point (fi = f Kp + kk) from its own
+// trace and the far state: the boundary flux times ||n|| (zero for an
+// uncoupled periodic face)
 template <class T>
-__device__ __forceinline__ void face_value(const AdjArgs &a, int code, size_t fi, const T own[4], const T oth[4],
-                                           const T qfar[4], T F[4]) {
+__device__ __forceinline__ void boundary_value(const AdjArgs &a, int code, size_t fi, const T own[4],
+                                               const T qfar[4], T F[4]) {
   const int kind = code & 15;
   const double nxd = a.Nx[fi], nyd = a.Ny[fi], w = a.IInII[fi];
   const T g = T(a.ph.gamma), nx = T(nxd), ny = T(nyd);
   for (int c = 0; c < 4; ++c) F[c] = T(0.);
-  if (kind == FK_Shared) {
-    const bool owner = code & FC_FLUX_OWNER;
-    if (owner) numflux(a.ph.flux, g, own, oth, nx, ny, F);
-    else numflux(a.ph.flux, g, oth, own, nx, ny, F);
-    for (int c = 0; c < 4; ++c) F[c] = owner ? F[c] * w : -F[c] * w;
-    return;
-  }
   if (kind == FK_Far) {
     T qb[4];
     far_bc(a, code >> 8, qfar, own, nxd, nyd, qb);
@@ -142,85 +136,109 @@ __device__ __forceinline__ void face_point(const AdjArgs &a, size_t t, int &f, i
   kk = int(fi - size_t(f) * a.Kp);
 }
 
-template <int N, class T>
-__global__ void __launch_bounds__(256) k_adj_fwd_face(AdjArgs a, const T *U, T *FE) {
+// A shared edge point's flux is evaluated once, by the owner's face point,
+// which writes both sides' edge values: w_own F* and -w_nbr F* at the
+// partner's reversed point (the value the reference stores once per edge
+// and reads from both sides). The same pass stores each face point's wave
+// speed (|U| + c) / hK on its post-BC trace where the face bounds the time
+// step (the edge's owner side), -1 elsewhere.
+template <int N, class T, int FLUX>
+__global__ void __launch_bounds__(256) k_adj_fwd_face(AdjArgs a, const T *U, T *FE, T *SP) {
   using Sh = AdjShape<N>;
+  constexpr int NpEdge = Sh::NpEdge;
   const size_t t = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (t >= size_t(3) * a.Kp * Sh::NpEdge) return;
+  if (t >= size_t(3) * a.Kp * NpEdge) return;
   int f, kk, pp;
   size_t fi;
   face_point<N>(a, t, f, kk, pp, fi);
-  T F[4] = {T(0.), T(0.), T(0.), T(0.)};
-  if (kk < a.K) {
-    const int code = a.code[fi];
-    T own[4], oth[4] = {T(0.), T(0.), T(0.), T(0.)}, qfar[4];
-    trace_at<N>(a, U, kk, f * Sh::NpEdge + pp, own);
-    if ((code & 15) == FK_Shared)
-      trace_at<N>(a, U, a.nbr_elem[fi], a.nbr_face[fi] * Sh::NpEdge + (Sh::NpEdge - 1 - pp), oth);
-    far_state(a, qfar);
-    face_value(a, code, fi, own, oth, qfar, F);
+  if (kk >= a.K) {
+    for (int c = 0; c < 4; ++c) FE[4 * t + c] = T(0.);
+    SP[t] = T(-1.);
+    return;
   }
+  const int code = a.code[fi], kind = code & 15;
+  T own[4], qfar[4], F[4];
+  trace_at<N>(a, U, kk, f * NpEdge + pp, own);
+  far_state(a, qfar);
+  SP[t] = (code & FC_WAVE) ? wave_value(a, code, fi, a.oohK[kk], own, qfar) : T(-1.);
+  if (kind == FK_Shared) {
+    if (!(code & FC_FLUX_OWNER)) return;  // written by the owner
+    const int nb = a.nbr_elem[fi], nf = a.nbr_face[fi], rp = NpEdge - 1 - pp;
+    T oth[4];
+    trace_at<N>(a, U, nb, nf * NpEdge + rp, oth);
+    numflux(FLUX, T(a.ph.gamma), own, oth, T(a.Nx[fi]), T(a.Ny[fi]), F);
+    const size_t fn = size_t(nf) * a.Kp + nb, t2 = fn * NpEdge + rp;
+    const double wo = a.IInII[fi], wn = a.IInII[fn];
+    for (int c = 0; c < 4; ++c) {
+      FE[4 * t + c] = F[c] * wo;
+      FE[4 * t2 + c] = -F[c] * wn;
+    }
+    return;
+  }
+  boundary_value(a, code, fi, own, qfar, F);
   for (int c = 0; c < 4; ++c) FE[4 * t + c] = F[c];
 }
 
+// Per (element, solution point i): the RHS row i of the four variables
+// (the interior flux at every solution point of the element, its
+// contravariant transform, the DivInt row and the edge values), stored in R,
+// then the stage's SSP54 combination at those four values in the order of
+// the solver's stage kernel (stage s reads the step start U0, its input,
+// its RHS and dt_s; the last stage also U2, U3 and the stage-3 RHS).
 template <int N, class T>
-__global__ void __launch_bounds__(256) k_adj_fwd_rhs(AdjArgs a, const T *U, const T *FE, T *R) {
+__global__ void __launch_bounds__(256) k_adj_fwd_rhs(AdjArgs a, int stage, const T *U, const T *FE, T *R,
+                                                     const T *U0, const T *U2, const T *U3, const T *R3,
+                                                     const T *dts, T *out) {
   using Sh = AdjShape<N>;
   constexpr int NpInt = Sh::NpInt, NpEdge = Sh::NpEdge, NpFlux = Sh::NpFlux;
-  const int kk = blockIdx.x * blockDim.x + threadIdx.x;
-  if (kk >= a.Kp) return;
-  T *r = R + size_t(kk) * Sh::NV;
-  if (kk >= a.K) {
-    for (int i = 0; i < Sh::NV; ++i) r[i] = T(0.);
-    return;
-  }
+  const size_t gid = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (gid >= size_t(a.Kp) * NpInt) return;
+  const int kk = int(gid / NpInt), i = int(gid - size_t(kk) * NpInt);
   const T *u = U + size_t(kk) * Sh::NV;
-  const double Jd = a.Jdet[kk];
-  const double J0 = a.Jinv[kk], J1 = a.Jinv[size_t(a.Kp) + kk];
-  const double J2 = a.Jinv[2 * size_t(a.Kp) + kk], J3 = a.Jinv[3 * size_t(a.Kp) + kk];
-  const double *D = a.DivInt;
-  T acc[4][NpInt];
-  for (int c = 0; c < 4; ++c)
-    for (int i = 0; i < NpInt; ++i) acc[c][i] = T(0.);
-  for (int j = 0; j < NpInt; ++j) {
-    const T q[4] = {u[j], u[NpInt + j], u[2 * NpInt + j], u[3 * NpInt + j]};
-    T Fx[4], Fy[4];
-    flux_xy(T(a.ph.gamma), q, Fx, Fy);
-    for (int c = 0; c < 4; ++c) {
-      const T Fr = Jd * (J0 * Fx[c] + J1 * Fy[c]), Fs = Jd * (J2 * Fx[c] + J3 * Fy[c]);
-      for (int i = 0; i < NpInt; ++i) acc[c][i] += D[i * NpFlux + j] * Fr + D[i * NpFlux + NpInt + j] * Fs;
-    }
-  }
-  for (int p = 0; p < Sh::NF; ++p) {
-    const int f = p / NpEdge, pp = p - f * NpEdge;
-    const T *fe = FE + 4 * ((size_t(f) * a.Kp + kk) * NpEdge + pp);
-    for (int c = 0; c < 4; ++c)
-      for (int i = 0; i < NpInt; ++i) acc[c][i] += D[i * NpFlux + 2 * NpInt + p] * fe[c];
-  }
-  const double m = -(1. / Jd);
-  for (int c = 0; c < 4; ++c)
-    for (int i = 0; i < NpInt; ++i) r[c * NpInt + i] = acc[c][i] * m;
-}
-
-template <int N, class T>
-__global__ void __launch_bounds__(256) k_adj_fwd_wave(AdjArgs a, const T *U, T *SP) {
-  using Sh = AdjShape<N>;
-  const size_t t = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (t >= size_t(3) * a.Kp * Sh::NpEdge) return;
-  int f, kk, pp;
-  size_t fi;
-  face_point<N>(a, t, f, kk, pp, fi);
-  T s = T(-1.);
+  T acc[4] = {T(0.), T(0.), T(0.), T(0.)};
   if (kk < a.K) {
-    const int code = a.code[fi];
-    if (code & FC_WAVE) {
-      T own[4], qfar[4];
-      trace_at<N>(a, U, kk, f * Sh::NpEdge + pp, own);
-      far_state(a, qfar);
-      s = wave_value(a, code, fi, a.oohK[kk], own, qfar);
+    const double Jd = a.Jdet[kk];
+    const double J0 = a.Jinv[kk], J1 = a.Jinv[size_t(a.Kp) + kk];
+    const double J2 = a.Jinv[2 * size_t(a.Kp) + kk], J3 = a.Jinv[3 * size_t(a.Kp) + kk];
+    const double *D = a.DivInt + i * NpFlux;
+    for (int j = 0; j < NpInt; ++j) {
+      const T q[4] = {u[j], u[NpInt + j], u[2 * NpInt + j], u[3 * NpInt + j]};
+      T Fx[4], Fy[4];
+      flux_xy(T(a.ph.gamma), q, Fx, Fy);
+#pragma unroll
+      for (int c = 0; c < 4; ++c) {
+        const T Fr = Jd * (J0 * Fx[c] + J1 * Fy[c]), Fs = Jd * (J2 * Fx[c] + J3 * Fy[c]);
+        acc[c] += D[j] * Fr + D[NpInt + j] * Fs;
+      }
+    }
+    for (int p = 0; p < Sh::NF; ++p) {
+      const int f = p / NpEdge, pp = p - f * NpEdge;
+      const T *fe = FE + 4 * ((size_t(f) * a.Kp + kk) * NpEdge + pp);
+#pragma unroll
+      for (int c = 0; c < 4; ++c) acc[c] += D[2 * NpInt + p] * fe[c];
+    }
+    const double m = -(1. / Jd);
+#pragma unroll
+    for (int c = 0; c < 4; ++c) acc[c] = acc[c] * m;
+  }
+  const T dt = dts[stage];
+#pragma unroll
+  for (int c = 0; c < 4; ++c) {
+    const size_t x = size_t(kk) * Sh::NV + c * NpInt + i;
+    R[x] = acc[c];
+    const T dtR = dt * acc[c];
+    switch (stage) {
+      case 0: out[x] = U[x] + kSspA0 * dtR; break;
+      case 1: out[x] = kSspB1 * U0[x] + kSspC1 * U[x] + kSspA1 * dtR; break;
+      case 2: out[x] = kSspB2 * U0[x] + kSspC2 * U[x] + kSspA2 * dtR; break;
+      case 3: out[x] = kSspB3 * U0[x] + kSspC3 * U[x] + kSspA3 * dtR; break;
+      default: {
+        const T dtR3 = dt * R3[x];
+        const T Rn = -U0[x] + kSspD2 * U2[x] + kSspD3 * U3[x] + kSspD4 * U[x] + kSspE3 * dtR3 + kSspE4 * dtR;
+        out[x] = U0[x] + Rn;
+      }
     }
   }
-  SP[t] = s;
 }
 
 // ---- adjoint ----
@@ -239,128 +257,247 @@ __device__ __forceinline__ void edge_cotangent(const AdjArgs &a, const double *l
   }
 }
 
+// Reverse-mode derivative of roe_flux (RoeFlux, fluxes.go:284-413), by
+// hand: given the cotangent bF of F*(L, R, n), adds (dF*/dL)^T bF to bL and
+// (dF*/dR)^T bF to bR. Statement by statement the transpose of the forward
+// evaluation in physics.hpp; |x| takes the derivative sign(x) with +1 at 0.
+__device__ __forceinline__ void roe_flux_vjp(double g, const double L[4], const double R[4], double nx, double ny,
+                                             const double bF[4], double bL[4], double bR[4]) {
+  const double GM1 = g - 1.;
+  // forward
+  const double rULr = L[1] * nx + L[2] * ny, rVLr = L[1] * (-ny) + L[2] * nx;
+  const double rURr = R[1] * nx + R[2] * ny, rVRr = R[1] * (-ny) + R[2] * nx;
+  const double rhoL = L[0], uL = rULr / L[0], vL = rVLr / L[0];
+  const double rhoR = R[0], uR = rURr / R[0], vR = rVRr / R[0];
+  const double cuL = L[1] / L[0], cvL = L[2] / L[0], cuR = R[1] / R[0], cvR = R[2] / R[0];
+  const double pL = GM1 * (L[3] - 0.5 * rhoL * (cuL * cuL + cvL * cvL));
+  const double pR = GM1 * (R[3] - 0.5 * rhoR * (cuR * cuR + cvR * cvR));
+  const double hL = (L[3] + pL) / rhoL, hR = (R[3] + pR) / rhoR;
+  const double sL = sqrt(rhoL), sR = sqrt(rhoR), sLR = sL + sR;
+  const double rho = sL * sR;
+  const double u = (sL * uL + sR * uR) / sLR;
+  const double v = (sL * vL + sR * vR) / sLR;
+  const double h = (sL * hL + sR * hR) / sLR;
+  const double c2 = GM1 * (h - 0.5 * (u * u + v * v));
+  const double cc = sqrt(c2);
+  const double du = uR - uL, dv = vR - vL, dp = pR - pL;
+  const double X = rho * du / cc, Y = dp / c2;
+  const double dW1 = -0.5 * X + 0.5 * Y, dW2 = (rhoR - rhoL) - Y, dW3 = rho * dv, dW4 = 0.5 * X + 0.5 * Y;
+  const double s1 = (u - cc) < 0. ? -1. : 1., s2 = u < 0. ? -1. : 1., s4 = (u + cc) < 0. ? -1. : 1.;
+  const double a1 = s1 * (u - cc), a2 = s2 * u, a4 = s4 * (u + cc);
+  const double W1 = a1 * dW1, W2 = a2 * dW2, W3 = a2 * dW3, W4 = a4 * dW4;
+  // reverse: the rotation back to x/y
+  const double bf0 = bF[0], bf1 = nx * bF[1] + ny * bF[2], bf2 = -ny * bF[1] + nx * bF[2], bf3 = bF[3];
+  double b_rULr = 0.5 * bf0 + 0.5 * bf1 * uL, b_rURr = 0.5 * bf0 + 0.5 * bf1 * uR;
+  double b_rVLr = 0.5 * bf2 * uL, b_rVRr = 0.5 * bf2 * uR;
+  double b_uL = 0.5 * bf1 * rULr + 0.5 * bf2 * rVLr + 0.5 * bf3 * (pL + L[3]);
+  double b_uR = 0.5 * bf1 * rURr + 0.5 * bf2 * rVRr + 0.5 * bf3 * (pR + R[3]);
+  double b_pL = 0.5 * bf1 + 0.5 * bf3 * uL, b_pR = 0.5 * bf1 + 0.5 * bf3 * uR;
+  double bL3 = 0.5 * bf3 * uL, bR3 = 0.5 * bf3 * uR;
+  const double bW1 = -0.5 * (bf0 + bf1 * (u - cc) + bf2 * v + bf3 * (h - u * cc));
+  const double bW2 = -0.5 * (bf0 + bf1 * u + bf2 * v + bf3 * 0.5 * (u * u + v * v));
+  const double bW3 = -0.5 * (bf2 + bf3 * v);
+  const double bW4 = -0.5 * (bf0 + bf1 * (u + cc) + bf2 * v + bf3 * (h + u * cc));
+  double b_u = -0.5 * bf1 * (W1 + W2 + W4) - 0.5 * bf3 * (cc * (W4 - W1) + W2 * u);
+  double b_v = -0.5 * bf2 * (W1 + W2 + W4) - 0.5 * bf3 * (W2 * v + W3);
+  double b_h = -0.5 * bf3 * (W1 + W4);
+  double b_cc = -0.5 * bf1 * (W4 - W1) - 0.5 * bf3 * u * (W4 - W1);
+  // the wave strengths and their speeds
+  const double b_a1 = bW1 * dW1, b_a2 = bW2 * dW2 + bW3 * dW3, b_a4 = bW4 * dW4;
+  const double b_dW1 = bW1 * a1, b_dW2 = bW2 * a2, b_dW3 = bW3 * a2, b_dW4 = bW4 * a4;
+  b_u += b_a1 * s1 + b_a2 * s2 + b_a4 * s4;
+  b_cc += -b_a1 * s1 + b_a4 * s4;
+  const double b_X = 0.5 * (b_dW4 - b_dW1), b_Y = 0.5 * (b_dW1 + b_dW4) - b_dW2;
+  double b_rhoL = -b_dW2, b_rhoR = b_dW2;
+  double b_rho = b_dW3 * dv + b_X * du / cc;
+  double b_vL = -b_dW3 * rho, b_vR = b_dW3 * rho;
+  const double b_du = b_X * rho / cc;
+  b_uR += b_du;
+  b_uL -= b_du;
+  b_cc += -b_X * X / cc;
+  const double b_dp = b_Y / c2;
+  b_pR += b_dp;
+  b_pL -= b_dp;
+  double b_c2 = -b_Y * Y / c2 + b_cc / (2. * cc);
+  b_h += GM1 * b_c2;
+  b_u += -GM1 * u * b_c2;
+  b_v += -GM1 * v * b_c2;
+  // Roe averages
+  double b_sL = b_rho * sR + (b_u * (uL - u) + b_v * (vL - v) + b_h * (hL - h)) / sLR;
+  double b_sR = b_rho * sL + (b_u * (uR - u) + b_v * (vR - v) + b_h * (hR - h)) / sLR;
+  b_uL += b_u * sL / sLR;
+  b_uR += b_u * sR / sLR;
+  b_vL += b_v * sL / sLR;
+  b_vR += b_v * sR / sLR;
+  const double b_hL = b_h * sL / sLR, b_hR = b_h * sR / sLR;
+  b_rhoL += b_sL / (2. * sL);
+  b_rhoR += b_sR / (2. * sR);
+  // enthalpies
+  bL3 += b_hL / rhoL;
+  b_pL += b_hL / rhoL;
+  b_rhoL += -b_hL * hL / rhoL;
+  bR3 += b_hR / rhoR;
+  b_pR += b_hR / rhoR;
+  b_rhoR += -b_hR * hR / rhoR;
+  // normal-frame velocities
+  b_rULr += b_uL / L[0];
+  b_rVLr += b_vL / L[0];
+  b_rhoL += -(b_uL * uL + b_vL * vL) / L[0];
+  b_rURr += b_uR / R[0];
+  b_rVRr += b_vR / R[0];
+  b_rhoR += -(b_uR * uR + b_vR * vR) / R[0];
+  // pressures
+  bL[0] += b_rhoL + b_pL * GM1 * 0.5 * (cuL * cuL + cvL * cvL);
+  bL[1] += -b_pL * GM1 * cuL + b_rULr * nx - b_rVLr * ny;
+  bL[2] += -b_pL * GM1 * cvL + b_rULr * ny + b_rVLr * nx;
+  bL[3] += b_pL * GM1 + bL3;
+  bR[0] += b_rhoR + b_pR * GM1 * 0.5 * (cuR * cuR + cvR * cvR);
+  bR[1] += -b_pR * GM1 * cuR + b_rURr * nx - b_rVRr * ny;
+  bR[2] += -b_pR * GM1 * cvR + b_rURr * ny + b_rVRr * nx;
+  bR[3] += b_pR * GM1 + bR3;
+}
+
 // Per face point: the transposed Jacobians of its edge value with respect
-// to the own trace, the neighbour's trace and the far state, applied to the
-// edge value's cotangent.
-template <int N>
+// to its inputs, applied to the edge value's cotangent, into the trace
+// cotangent of each face point (lamOwn, written exactly once per launch)
+// and the far state's (lamFarPts, accumulated). A shared edge point is
+// handled by the owner's face point: the cotangent of F* is
+// w_own lambda_own - w_nbr lambda_nbr, and the transposed Jacobians with
+// respect to the two traces go to the owner's and the partner's records.
+// The Jacobians are formed one input direction at a time in dual
+// arithmetic.
+// (Capping its registers for occupancy spills and measured slower.)
+template <int N, int FLUX>
 __global__ void __launch_bounds__(256) k_adj_face(AdjArgs a, const double *U, const double *lamR, double *lamOwn,
-                                                  double *lamOth, double *lamFarPts) {
+                                                  double *lamFarPts) {
   using Sh = AdjShape<N>;
+  constexpr int NpEdge = Sh::NpEdge;
   const size_t t = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (t >= size_t(3) * a.Kp * Sh::NpEdge) return;
+  if (t >= size_t(3) * a.Kp * NpEdge) return;
   int f, kk, pp;
   size_t fi;
   face_point<N>(a, t, f, kk, pp, fi);
-  double *lo = lamOwn + 4 * t, *lt = lamOth + 4 * t, *lq = lamFarPts + 4 * t;
-  for (int c = 0; c < 4; ++c) {
-    lo[c] = 0.;
-    lt[c] = 0.;
+  double *lo = lamOwn + 4 * t;
+  if (kk >= a.K) {
+    for (int c = 0; c < 4; ++c) lo[c] = 0.;
+    return;
   }
-  if (kk >= a.K) return;
   const int code = a.code[fi], kind = code & 15;
-  double lf[4];
-  edge_cotangent<N>(a, lamR, kk, f * Sh::NpEdge + pp, lf);
-  double own[4], oth[4] = {0., 0., 0., 0.}, qfar[4];
-  trace_at<N>(a, U, kk, f * Sh::NpEdge + pp, own);
-  if (kind == FK_Shared)
-    trace_at<N>(a, U, a.nbr_elem[fi], a.nbr_face[fi] * Sh::NpEdge + (Sh::NpEdge - 1 - pp), oth);
+  if (kind == FK_Shared && !(code & FC_FLUX_OWNER)) return;  // written by the owner
+  double own[4];
+  trace_at<N>(a, U, kk, f * NpEdge + pp, own);
+  if (kind == FK_Shared) {
+    const int nb = a.nbr_elem[fi], nf = a.nbr_face[fi], rp = NpEdge - 1 - pp;
+    const size_t fn = size_t(nf) * a.Kp + nb;
+    double oth[4], lfo[4], lfn[4], lF[4];
+    trace_at<N>(a, U, nb, nf * NpEdge + rp, oth);
+    edge_cotangent<N>(a, lamR, kk, f * NpEdge + pp, lfo);
+    edge_cotangent<N>(a, lamR, nb, nf * NpEdge + rp, lfn);
+    const double wo = a.IInII[fi], wn = a.IInII[fn];
+    for (int c = 0; c < 4; ++c) lF[c] = wo * lfo[c] - wn * lfn[c];
+    double *ln = lamOwn + 4 * (fn * NpEdge + rp);
+    if constexpr (FLUX == 2) {
+      double bL[4] = {0., 0., 0., 0.}, bR[4] = {0., 0., 0., 0.};
+      roe_flux_vjp(a.ph.gamma, own, oth, a.Nx[fi], a.Ny[fi], lF, bL, bR);
+      for (int c = 0; c < 4; ++c) {
+        lo[c] = bL[c];
+        ln[c] = bR[c];
+      }
+      return;
+    }
+    const Dual g(a.ph.gamma), nx(a.Nx[fi]), ny(a.Ny[fi]);
+#pragma unroll 1
+    for (int j = 0; j < 8; ++j) {
+      const int side = j >> 2, d = j & 3;
+      Dual L[4], R[4], F[4];
+      for (int c = 0; c < 4; ++c) {
+        L[c] = Dual(own[c], side == 0 && c == d ? 1. : 0.);
+        R[c] = Dual(oth[c], side == 1 && c == d ? 1. : 0.);
+      }
+      numflux(FLUX, g, L, R, nx, ny, F);
+      double sum = 0.;
+      for (int c = 0; c < 4; ++c) sum += lF[c] * F[c].d;
+      if (side == 0) lo[d] = sum;
+      else ln[d] = sum;
+    }
+    return;
+  }
+  double lf[4], qfar[4];
+  edge_cotangent<N>(a, lamR, kk, f * NpEdge + pp, lf);
   far_state(a, qfar);
   const bool far = kind == FK_Far && ((code >> 8) == 0 || a.io_tied);
-  // input directions j = 4 set + component; sets: 0 the own trace, 1 the
-  // neighbour's trace, 2 the far state (its cotangent accumulates)
-  for (int j = 0; j < 12; ++j) {
-    const int which = j >> 2, d = j & 3;
-    if ((which == 1 && kind != FK_Shared) || (which == 2 && !far)) continue;
-    Dual o[4], n[4], q[4], F[4];
+  double *lq = lamFarPts + 4 * t;
+  // input directions: 0..3 the own trace, 4..7 the far state
+#pragma unroll 1
+  for (int j = 0; j < 8; ++j) {
+    const int side = j >> 2, d = j & 3;
+    if (side == 1 && !far) break;
+    Dual o[4], q[4], F[4];
     for (int c = 0; c < 4; ++c) {
-      o[c] = Dual(own[c], which == 0 && c == d ? 1. : 0.);
-      n[c] = Dual(oth[c], which == 1 && c == d ? 1. : 0.);
-      q[c] = Dual(qfar[c], which == 2 && c == d ? 1. : 0.);
+      o[c] = Dual(own[c], side == 0 && c == d ? 1. : 0.);
+      q[c] = Dual(qfar[c], side == 1 && c == d ? 1. : 0.);
     }
-    face_value(a, code, fi, o, n, q, F);
+    boundary_value(a, code, fi, o, q, F);
     double sum = 0.;
     for (int c = 0; c < 4; ++c) sum += lf[c] * F[c].d;
-    if (which == 0) lo[d] = sum;
-    else if (which == 1) lt[d] = sum;
+    if (side == 0) lo[d] = sum;
     else lq[d] += sum;
   }
 }
 
-// Per element: the interior-flux transpose (DivInt columns of the r and s
-// components, the contravariant transform and the physical-flux Jacobian
-// at each solution point) and the FluxEdgeInterp transpose of the face-point
-// trace cotangents (own side, plus the neighbour's cotangent of this
-// element's trace at the reversed point), added to target.
+// Per (element, solution point j): column j of the FluxEdgeInterp transpose
+// of the element's face-point trace cotangents, and the interior-flux
+// transpose at point j (the DivInt columns of its r and s components, the
+// contravariant transform and the physical-flux Jacobian), added to target.
 template <int N>
 __global__ void __launch_bounds__(256) k_adj_elem(AdjArgs a, const double *U, const double *lamR,
-                                                  const double *lamOwn, const double *lamOth, double *target) {
+                                                  const double *lamOwn, double *target) {
   using Sh = AdjShape<N>;
   constexpr int NpInt = Sh::NpInt, NpEdge = Sh::NpEdge, NpFlux = Sh::NpFlux;
-  const int kk = blockIdx.x * blockDim.x + threadIdx.x;
-  if (kk >= a.K) return;
+  const size_t gid = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (gid >= size_t(a.K) * NpInt) return;
+  const int kk = int(gid / NpInt), j = int(gid - size_t(kk) * NpInt);
   const double *u = U + size_t(kk) * Sh::NV;
   const double *lr = lamR + size_t(kk) * Sh::NV;
   const double Jd = a.Jdet[kk], m = -(1. / Jd);
   const double J0 = a.Jinv[kk], J1 = a.Jinv[size_t(a.Kp) + kk];
   const double J2 = a.Jinv[2 * size_t(a.Kp) + kk], J3 = a.Jinv[3 * size_t(a.Kp) + kk];
   const double *D = a.DivInt;
-  double *tg = target + size_t(kk) * Sh::NV;
-  // face points: FluxEdgeInterp^T of the trace cotangents
-  double lu[4][NpInt];
-#pragma unroll
-  for (int c = 0; c < 4; ++c)
-#pragma unroll
-    for (int i = 0; i < NpInt; ++i) lu[c][i] = 0.;
+  double acc[4] = {0., 0., 0., 0.};
   for (int p = 0; p < Sh::NF; ++p) {
     const int f = p / NpEdge, pp = p - f * NpEdge;
-    const size_t fi = size_t(f) * a.Kp + kk;
-    const size_t t = fi * NpEdge + pp;
-    double lq[4];
+    const double Ij = a.Interp[p * NpInt + j];
+    const double *lq = lamOwn + 4 * ((size_t(f) * a.Kp + kk) * NpEdge + pp);
 #pragma unroll
-    for (int c = 0; c < 4; ++c) lq[c] = lamOwn[4 * t + c];
-    if ((a.code[fi] & 15) == FK_Shared) {
-      const size_t t2 = (size_t(a.nbr_face[fi]) * a.Kp + a.nbr_elem[fi]) * NpEdge + (NpEdge - 1 - pp);
+    for (int c = 0; c < 4; ++c) acc[c] += Ij * lq[c];
+  }
+  double lFx[4], lFy[4];
 #pragma unroll
-      for (int c = 0; c < 4; ++c) lq[c] += lamOth[4 * t2 + c];
+  for (int c = 0; c < 4; ++c) {
+    double lr_ = 0., ls_ = 0.;
+    for (int i = 0; i < NpInt; ++i) {
+      lr_ += D[i * NpFlux + j] * lr[c * NpInt + i];
+      ls_ += D[i * NpFlux + NpInt + j] * lr[c * NpInt + i];
     }
-    const double *row = a.Interp + p * NpInt;
-#pragma unroll
-    for (int c = 0; c < 4; ++c)
-#pragma unroll
-      for (int i = 0; i < NpInt; ++i) lu[c][i] += row[i] * lq[c];
+    lr_ *= m;
+    ls_ *= m;
+    lFx[c] = Jd * (J0 * lr_ + J2 * ls_);
+    lFy[c] = Jd * (J1 * lr_ + J3 * ls_);
   }
 #pragma unroll
-  for (int c = 0; c < 4; ++c)
+  for (int d = 0; d < 4; ++d) {
+    Dual q[4], Fx[4], Fy[4];
 #pragma unroll
-    for (int i = 0; i < NpInt; ++i) tg[c * NpInt + i] += lu[c][i];
-  // solution points: DivInt^T (r and s columns), the contravariant
-  // transform and the physical-flux Jacobian
-  for (int j = 0; j < NpInt; ++j) {
-    double lFx[4], lFy[4];
+    for (int c = 0; c < 4; ++c) q[c] = Dual(u[c * NpInt + j], c == d ? 1. : 0.);
+    flux_xy(Dual(a.ph.gamma), q, Fx, Fy);
+    double sum = 0.;
 #pragma unroll
-    for (int c = 0; c < 4; ++c) {
-      double lr_ = 0., ls_ = 0.;
-      for (int i = 0; i < NpInt; ++i) {
-        lr_ += D[i * NpFlux + j] * lr[c * NpInt + i];
-        ls_ += D[i * NpFlux + NpInt + j] * lr[c * NpInt + i];
-      }
-      lr_ *= m;
-      ls_ *= m;
-      lFx[c] = Jd * (J0 * lr_ + J2 * ls_);
-      lFy[c] = Jd * (J1 * lr_ + J3 * ls_);
-    }
-#pragma unroll
-    for (int d = 0; d < 4; ++d) {
-      Dual q[4], Fx[4], Fy[4];
-#pragma unroll
-      for (int c = 0; c < 4; ++c) q[c] = Dual(u[c * NpInt + j], c == d ? 1. : 0.);
-      flux_xy(Dual(a.ph.gamma), q, Fx, Fy);
-      double sum = 0.;
-#pragma unroll
-      for (int c = 0; c < 4; ++c) sum += lFx[c] * Fx[c].d + lFy[c] * Fy[c].d;
-      tg[d * NpInt + j] += sum;
-    }
+    for (int c = 0; c < 4; ++c) sum += lFx[c] * Fx[c].d + lFy[c] * Fy[c].d;
+    acc[d] += sum;
   }
+  double *tg = target + size_t(kk) * Sh::NV;
+#pragma unroll
+  for (int c = 0; c < 4; ++c) tg[c * NpInt + j] += acc[c];
 }
 
 // The dt term of a stage: dt = CFL / W with W the wave speed at the stage's
@@ -413,38 +550,55 @@ hipError_t adj_launch_n(int op, const AdjArgs &a, const AdjBufs &b, hipStream_t 
   const size_t nfp = size_t(3) * a.Kp * Sh::NpEdge;
   switch (op) {
     case ADJ_FWD_FACE:
-      if (b.dual)
-        hipLaunchKernelGGL((k_adj_fwd_face<N, Dual>), dim3(adj_blocks(nfp)), dim3(256), 0, s, a,
-                           static_cast<const Dual *>(b.Us), static_cast<Dual *>(b.FE));
-      else
-        hipLaunchKernelGGL((k_adj_fwd_face<N, double>), dim3(adj_blocks(nfp)), dim3(256), 0, s, a,
-                           static_cast<const double *>(b.Us), static_cast<double *>(b.FE));
+#define GOCFD_FWD_FACE(FL)                                                                                 \
+  if (b.dual)                                                                                              \
+    hipLaunchKernelGGL((k_adj_fwd_face<N, Dual, FL>), dim3(adj_blocks(nfp)), dim3(256), 0, s, a,           \
+                       static_cast<const Dual *>(b.Us), static_cast<Dual *>(b.FE), static_cast<Dual *>(b.SP)); \
+  else                                                                                                     \
+    hipLaunchKernelGGL((k_adj_fwd_face<N, double, FL>), dim3(adj_blocks(nfp)), dim3(256), 0, s, a,         \
+                       static_cast<const double *>(b.Us), static_cast<double *>(b.FE),                     \
+                       static_cast<double *>(b.SP));
+      switch (a.ph.flux) {
+        case 0: GOCFD_FWD_FACE(0) break;
+        case 1: GOCFD_FWD_FACE(1) break;
+        case 2: GOCFD_FWD_FACE(2) break;
+        default: GOCFD_FWD_FACE(3) break;
+      }
+#undef GOCFD_FWD_FACE
       break;
-    case ADJ_FWD_RHS:
+    case ADJ_FWD_RHS: {
+      const unsigned nb = adj_blocks(size_t(a.Kp) * Sh::NpInt);
       if (b.dual)
-        hipLaunchKernelGGL((k_adj_fwd_rhs<N, Dual>), dim3(adj_blocks(a.Kp)), dim3(256), 0, s, a,
+        hipLaunchKernelGGL((k_adj_fwd_rhs<N, Dual>), dim3(nb), dim3(256), 0, s, a, b.stage,
                            static_cast<const Dual *>(b.Us), static_cast<const Dual *>(b.FE),
-                           static_cast<Dual *>(b.R));
+                           static_cast<Dual *>(b.R), static_cast<const Dual *>(b.U0),
+                           static_cast<const Dual *>(b.U2), static_cast<const Dual *>(b.U3),
+                           static_cast<const Dual *>(b.R3), static_cast<const Dual *>(b.dts),
+                           static_cast<Dual *>(b.out));
       else
-        hipLaunchKernelGGL((k_adj_fwd_rhs<N, double>), dim3(adj_blocks(a.Kp)), dim3(256), 0, s, a,
+        hipLaunchKernelGGL((k_adj_fwd_rhs<N, double>), dim3(nb), dim3(256), 0, s, a, b.stage,
                            static_cast<const double *>(b.Us), static_cast<const double *>(b.FE),
-                           static_cast<double *>(b.R));
+                           static_cast<double *>(b.R), static_cast<const double *>(b.U0),
+                           static_cast<const double *>(b.U2), static_cast<const double *>(b.U3),
+                           static_cast<const double *>(b.R3), static_cast<const double *>(b.dts),
+                           static_cast<double *>(b.out));
       break;
-    case ADJ_FWD_WAVE:
-      if (b.dual)
-        hipLaunchKernelGGL((k_adj_fwd_wave<N, Dual>), dim3(adj_blocks(nfp)), dim3(256), 0, s, a,
-                           static_cast<const Dual *>(b.Us), static_cast<Dual *>(b.SP));
-      else
-        hipLaunchKernelGGL((k_adj_fwd_wave<N, double>), dim3(adj_blocks(nfp)), dim3(256), 0, s, a,
-                           static_cast<const double *>(b.Us), static_cast<double *>(b.SP));
-      break;
+    }
     case ADJ_ADJ_FACE:
-      hipLaunchKernelGGL((k_adj_face<N>), dim3(adj_blocks(nfp)), dim3(256), 0, s, a,
-                         static_cast<const double *>(b.Us), b.lamR, b.lamOwn, b.lamOth, b.lamFarPts);
+#define GOCFD_ADJ_FACE(FL)                                                                                 \
+  hipLaunchKernelGGL((k_adj_face<N, FL>), dim3(adj_blocks(nfp)), dim3(256), 0, s, a,                        \
+                     static_cast<const double *>(b.Us), b.lamR, b.lamOwn, b.lamFarPts);
+      switch (a.ph.flux) {
+        case 0: GOCFD_ADJ_FACE(0) break;
+        case 1: GOCFD_ADJ_FACE(1) break;
+        case 2: GOCFD_ADJ_FACE(2) break;
+        default: GOCFD_ADJ_FACE(3) break;
+      }
+#undef GOCFD_ADJ_FACE
       break;
     case ADJ_ADJ_ELEM:
-      hipLaunchKernelGGL((k_adj_elem<N>), dim3(adj_blocks(a.K)), dim3(256), 0, s, a,
-                         static_cast<const double *>(b.Us), b.lamR, b.lamOwn, b.lamOth, b.target);
+      hipLaunchKernelGGL((k_adj_elem<N>), dim3(adj_blocks(size_t(a.K) * Sh::NpInt)), dim3(256), 0, s, a,
+                         static_cast<const double *>(b.Us), b.lamR, b.lamOwn, b.target);
       break;
     case ADJ_ADJ_WAVE:
       hipLaunchKernelGGL((k_adj_wave<N>), dim3(1), dim3(64), 0, s, a, b.stage, static_cast<const double *>(b.Us),

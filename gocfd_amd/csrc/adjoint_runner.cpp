@@ -3,6 +3,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 
 namespace gocfd {
 namespace capi {
@@ -42,7 +43,7 @@ AdjointRunner::AdjointRunner(const Discretisation &d, const DeviceLayout &L, con
   flag_.alloc(1);
   Wv_.alloc(5);
   time_.alloc(1);
-  part_v_.alloc(std::max<size_t>(size_t(nparts_), (n_ + 255) / 256));
+  part_v_.alloc(4 * size_t(nparts_));
   part_i_.alloc(size_t(nparts_));
 }
 
@@ -51,38 +52,66 @@ void AdjointRunner::launch(int op, dev::AdjBufs &b) {
   HIPCHECK(dev::adj_launch(op, a_, b, s_));
 }
 
-// One SSP54 step from Ustart into Uend, leaving the stage inputs (U_[1..4]),
-// right-hand sides (R_[0..4]), dts, arg-max points and wave maxima in the
-// stage record. With `time`, the FinalTime check and the time advance.
-void AdjointRunner::forward_step(bool dual, const void *Ustart, void *Uend, double *time) {
+AdjointRunner::StageRecord AdjointRunner::scratch_record() {
+  StageRecord r{};
+  for (int s = 0; s < 5; ++s) {
+    r.U[s] = U_[s].p;
+    r.R[s] = R_[s].p;
+  }
+  r.dts = dts_.p;
+  r.argmax = argmax_.p;
+  r.Wv = Wv_.p;
+  return r;
+}
+
+// One SSP54 step from Ustart into Uend, leaving the stage record in `rec`.
+// With `time`, the FinalTime check and the time advance.
+void AdjointRunner::forward_step(bool dual, const void *Ustart, void *Uend, double *time, const StageRecord &rec) {
   dev::AdjBufs b{};
   b.dual = dual ? 1 : 0;
-  b.dts = dts_.p;
-  b.argmax = argmax_.p;
-  b.Wv = Wv_.p;
+  b.dts = rec.dts;
+  b.argmax = rec.argmax;
+  b.Wv = rec.Wv;
   b.time = time;
   b.flag = flag_.p;
   b.part_v = part_v_.p;
   b.part_i = part_i_.p;
   b.n = n_;
   for (int s = 0; s < 5; ++s) {
-    const void *Us = s == 0 ? Ustart : static_cast<const void *>(U_[s].p);
+    const void *Us = s == 0 ? Ustart : static_cast<const void *>(rec.U[s]);
     b.stage = s;
     b.Us = Us;
     b.FE = FE_.p;
-    b.R = R_[s].p;
+    b.R = rec.R[s];
     b.SP = SP_.p;
-    launch(dev::ADJ_FWD_FACE, b);
-    launch(dev::ADJ_FWD_RHS, b);
-    launch(dev::ADJ_FWD_WAVE, b);
-    launch(dev::ADJ_ARGMAX, b);
     b.U0 = Ustart;
-    b.Rs = R_[s].p;
-    b.U2 = U_[2].p;
-    b.U3 = U_[3].p;
-    b.R3 = R_[3].p;
-    b.out = s < 4 ? static_cast<void *>(U_[s + 1].p) : Uend;
-    launch(dev::ADJ_FWD_SSP, b);
+    b.U2 = rec.U[2];
+    b.U3 = rec.U[3];
+    b.R3 = rec.R[3];
+    b.out = s < 4 ? rec.U[s + 1] : Uend;
+    launch(dev::ADJ_FWD_FACE, b);
+    launch(dev::ADJ_ARGMAX, b);
+    launch(dev::ADJ_FWD_RHS, b);
+  }
+}
+
+// The transposed stages of one step, last to first, from its stage record:
+// the cotangent of the step's result in b.lamU0 becomes that of its start
+void AdjointRunner::adjoint_step(const double *Ustart, const StageRecord &rec, dev::AdjBufs &b) {
+  b.dts = rec.dts;
+  b.argmax = rec.argmax;
+  b.Wv = rec.Wv;
+  for (int s = 4; s >= 0; --s) {
+    b.stage = s;
+    b.lam = s == 4 ? b.lamU0 : b.lamQ[s + 1];
+    b.Rs = rec.R[s];
+    b.R3 = rec.R[3];
+    launch(dev::ADJ_ADJ_SSP, b);
+    b.Us = s == 0 ? static_cast<const void *>(Ustart) : static_cast<const void *>(rec.U[s]);
+    b.target = s == 0 ? b.lamU0 : b.lamQ[s];
+    launch(dev::ADJ_ADJ_FACE, b);
+    launch(dev::ADJ_ADJ_ELEM, b);
+    launch(dev::ADJ_ADJ_WAVE, b);
   }
 }
 
@@ -113,24 +142,57 @@ void AdjointRunner::vjp(const double *q0, double t0, int nsteps, int every, cons
   HIPCHECK(hipMemcpyAsync(time_.p, &t0, sizeof(double), hipMemcpyHostToDevice, s_));
   HIPCHECK(hipMemcpyAsync(ck[0].p, q0, bytes, hipMemcpyDeviceToDevice, s_));
   // forward sweep: checkpoints at the block starts
+  const StageRecord scratch = scratch_record();
   for (int bk = 0; bk < nblocks; ++bk) {
     const int j0 = bk * every_, cnt = std::min(every_, nsteps - j0);
     const double *src = ck[bk].p;
     for (int i = 0; i < cnt; ++i) {
       double *dst = i + 1 < cnt ? S[i + 1].p : (bk + 1 < nblocks ? ck[bk + 1].p : end.p);
-      forward_step(false, src, dst, time_.p);
+      forward_step(false, src, dst, time_.p, scratch);
       src = dst;
     }
   }
   check_flag();
+  // The stage records of a whole block are kept when they fit in a quarter
+  // of the free device memory; otherwise each step's record is recomputed
+  // from its start just before its transposed stages.
+  size_t free_b = 0, total_b = 0;
+  HIPCHECK(hipMemGetInfo(&free_b, &total_b));
+  stored_records_ = size_t(every_) * 9 * bytes + size_t(every_) * 64 <= free_b / 4;
+  // GOCFD_ADJOINT_KEEP_RECORDS=0 forces the per-step recomputation
+  if (const char *e = std::getenv("GOCFD_ADJOINT_KEEP_RECORDS")) stored_records_ = stored_records_ && e[0] != '0';
+  const size_t nrec = stored_records_ ? size_t(every_) : 0;
+  std::vector<DevBuf<double>> recU(nrec * 5), recR(nrec * 5), recW(nrec);
+  std::vector<DevBuf<dev::ad_storage>> recDt(nrec);
+  std::vector<DevBuf<int>> recArg(nrec);
+  std::vector<StageRecord> recs(nrec);
+  if (stored_records_) {
+    for (int i = 0; i < every_; ++i) {
+      StageRecord &r = recs[i];
+      r.U[0] = nullptr;
+      for (int s = 1; s < 5; ++s) {
+        recU[size_t(i) * 5 + s].alloc(n_);
+        r.U[s] = recU[size_t(i) * 5 + s].p;
+      }
+      for (int s = 0; s < 5; ++s) {
+        recR[size_t(i) * 5 + s].alloc(n_);
+        r.R[s] = recR[size_t(i) * 5 + s].p;
+      }
+      recDt[i].alloc(5);
+      recArg[i].alloc(5);
+      recW[i].alloc(5);
+      r.dts = recDt[i].p;
+      r.argmax = recArg[i].p;
+      r.Wv = recW[i].p;
+    }
+  }
   // reverse sweep
-  DevBuf<double> lamU0, lamQ[5], lamR, lamR3, lamOwn, lamOth, lamFarPts, lamFarW, lam_dt;
+  DevBuf<double> lamU0, lamQ[5], lamR, lamR3, lamOwn, lamFarPts, lamFarW, lam_dt;
   lamU0.alloc(n_);
   for (int s = 1; s < 5; ++s) lamQ[s].alloc(n_);
   lamR.alloc(n_);
   lamR3.alloc(n_);
   lamOwn.alloc(4 * nfp_);
-  lamOth.alloc(4 * nfp_);
   lamFarPts.alloc(4 * nfp_);
   lamFarW.alloc(4);
   lam_dt.alloc(5);
@@ -139,9 +201,6 @@ void AdjointRunner::vjp(const double *q0, double t0, int nsteps, int every, cons
   HIPCHECK(hipMemsetAsync(lamFarW.p, 0, 4 * sizeof(double), s_));
   dev::AdjBufs b{};
   b.dual = 0;
-  b.dts = dts_.p;
-  b.argmax = argmax_.p;
-  b.Wv = Wv_.p;
   b.part_v = part_v_.p;
   b.part_i = part_i_.p;
   b.n = n_;
@@ -150,7 +209,6 @@ void AdjointRunner::vjp(const double *q0, double t0, int nsteps, int every, cons
   b.lamR = lamR.p;
   b.lamR3 = lamR3.p;
   b.lamOwn = lamOwn.p;
-  b.lamOth = lamOth.p;
   b.lamFarPts = lamFarPts.p;
   b.lamFarW = lamFarW.p;
   b.lam_dt = lam_dt.p;
@@ -158,29 +216,27 @@ void AdjointRunner::vjp(const double *q0, double t0, int nsteps, int every, cons
     const int j0 = bk * every_, cnt = std::min(every_, nsteps - j0);
     std::vector<const double *> starts(cnt);
     starts[0] = ck[bk].p;
-    for (int i = 1; i < cnt; ++i) {
-      forward_step(false, starts[i - 1], S[i].p, nullptr);
-      starts[i] = S[i].p;
+    for (int i = 0; i < cnt; ++i) {
+      void *next = i + 1 < cnt ? static_cast<void *>(S[i + 1].p) : static_cast<void *>(tmp_.p);
+      if (stored_records_) forward_step(false, starts[i], next, nullptr, recs[i]);
+      else if (i + 1 < cnt) forward_step(false, starts[i], next, nullptr, scratch);
+      if (i + 1 < cnt) starts[i + 1] = S[i + 1].p;
     }
     for (int i = cnt - 1; i >= 0; --i) {
-      forward_step(false, starts[i], tmp_.p, nullptr);  // the stage record of this step
-      for (int s = 4; s >= 0; --s) {
-        b.stage = s;
-        b.lam = s == 4 ? lamU0.p : lamQ[s + 1].p;
-        b.Rs = R_[s].p;
-        b.R3 = R_[3].p;
-        launch(dev::ADJ_ADJ_SSP, b);
-        b.Us = s == 0 ? static_cast<const void *>(starts[i]) : static_cast<const void *>(U_[s].p);
-        b.target = s == 0 ? lamU0.p : lamQ[s].p;
-        launch(dev::ADJ_ADJ_FACE, b);
-        launch(dev::ADJ_ADJ_ELEM, b);
-        launch(dev::ADJ_ADJ_WAVE, b);
+      if (stored_records_) {
+        adjoint_step(starts[i], recs[i], b);
+      } else {
+        forward_step(false, starts[i], tmp_.p, nullptr, scratch);
+        adjoint_step(starts[i], scratch, b);
       }
     }
   }
+  DevBuf<double> far_d;
+  far_d.alloc(4);
+  b.target = far_d.p;
   launch(dev::ADJ_FAR_SUM, b);
   HIPCHECK(hipMemcpyAsync(lam0, lamU0.p, bytes, hipMemcpyDeviceToDevice, s_));
-  HIPCHECK(hipMemcpyAsync(far, part_v_.p, 4 * sizeof(double), hipMemcpyDeviceToHost, s_));
+  HIPCHECK(hipMemcpyAsync(far, far_d.p, 4 * sizeof(double), hipMemcpyDeviceToHost, s_));
   HIPCHECK(hipStreamSynchronize(s_));
 }
 
@@ -199,8 +255,9 @@ void AdjointRunner::jvp(const double *q0, const double *dq0, const double dfar[4
   b.pack_d = dq0;
   b.out = A.p;
   launch(dev::ADJ_PACK, b);
+  const StageRecord scratch = scratch_record();
   for (int j = 0; j < nsteps; ++j) {
-    forward_step(true, A.p, B.p, time_.p);
+    forward_step(true, A.p, B.p, time_.p, scratch);
     std::swap(A.p, B.p);
   }
   b.Us = A.p;

@@ -36,15 +36,28 @@ class AdjointRunner {
   void jvp(const double *q0, const double *dq0, const double dfar[4], double t0, int nsteps, double *qn,
            double *dqn);
   int last_every() const { return every_; }
+  bool last_stored_records() const { return stored_records_; }
 
  private:
+  // Where a step leaves its stage inputs U[1..4], right-hand sides R[0..4],
+  // dts, arg-max points and wave maxima
+  struct StageRecord {
+    void *U[5];
+    void *R[5];
+    void *dts;
+    int *argmax;
+    double *Wv;
+  };
   void launch(int op, dev::AdjBufs &b);
-  void forward_step(bool dual, const void *Ustart, void *Uend, double *time);
+  void forward_step(bool dual, const void *Ustart, void *Uend, double *time, const StageRecord &rec);
+  void adjoint_step(const double *Ustart, const StageRecord &rec, dev::AdjBufs &b);
+  StageRecord scratch_record();
   void check_flag();
   dev::AdjArgs a_{};
   hipStream_t s_;
   size_t n_ = 0, nfp_ = 0;  // element-array and face-point counts
   int every_ = 0;
+  bool stored_records_ = false;
   DevBuf<double> divint_;
   DevBuf<int> nbr_elem_, nbr_face_;
   // stage record (sized for dual values; the double sweeps use the front half)

@@ -7,8 +7,8 @@
   an airfoil (far field + wall), a doubly periodic mesh (periodic shared
   faces, vortex initial state) and a shock tube (In/Out far states, which stay constant there),
   every flux and order 1..4.
-* Checkpoint schedules (every step, every 2, sqrt, one block) give
-  bit-identical gradients.
+* Checkpoint schedules (every step, every 2, sqrt, one block; stage records
+  kept per block or recomputed per step) give bit-identical gradients.
 * Lift gradients against central differences (h = 1e-6): of the GPU solver
   in Minf, angle of attack and two state directions, and of the CPU oracle
   (the reference's algorithm restated) in Minf and one state direction;
@@ -86,13 +86,19 @@ def test_dot_product(meshes, mesh, N, flux, init):
     assert np.array_equal(e.Q, q)
 
 
-def test_checkpoint_schedules_are_bitwise_identical(meshes):
+def test_checkpoint_schedules_are_bitwise_identical(meshes, monkeypatch):
+    """Every checkpoint interval, with the block's stage records kept or
+    recomputed step by step, gives the same bits."""
     e = _developed(meshes["foil"], _ip(2))
     s = np.random.default_rng(5).standard_normal(e.Q.shape)
     ref = e.vjp(7, s, 1)
     for every in (2, 0, 7):
         d0, dfar = e.vjp(7, s, every)
         assert np.array_equal(d0, ref[0]) and np.array_equal(dfar, ref[1]), every
+    monkeypatch.setenv("GOCFD_ADJOINT_KEEP_RECORDS", "0")
+    for every in (1, 3):
+        d0, dfar = e.vjp(7, s, every)
+        assert np.array_equal(d0, ref[0]) and np.array_equal(dfar, ref[1]), ("recomputed", every)
 
 
 def _lift_after(mesh, ip, n, q0=None):
