@@ -47,6 +47,24 @@ AdjointRunner::AdjointRunner(const Discretisation &d, const DeviceLayout &L, con
   part_i_.alloc(size_t(nparts_));
 }
 
+double *AdjointRunner::state(size_t i) {
+  while (states_.size() <= i) {
+    states_.push_back(std::make_unique<DevBuf<double>>());
+    states_.back()->alloc(n_);
+  }
+  return states_[i]->p;
+}
+
+AdjointRunner::SmallRecord &AdjointRunner::small(size_t i) {
+  while (smalls_.size() <= i) {
+    smalls_.push_back(std::make_unique<SmallRecord>());
+    smalls_.back()->dts.alloc(5);
+    smalls_.back()->argmax.alloc(5);
+    smalls_.back()->Wv.alloc(5);
+  }
+  return *smalls_[i];
+}
+
 void AdjointRunner::launch(int op, dev::AdjBufs &b) {
   b.nparts = nparts_;
   HIPCHECK(dev::adj_launch(op, a_, b, s_));
@@ -133,69 +151,61 @@ void AdjointRunner::vjp(const double *q0, double t0, int nsteps, int every, cons
   every_ = std::min(every_, nsteps);
   const int nblocks = (nsteps + every_ - 1) / every_;
   const size_t bytes = n_ * sizeof(double);
-  std::vector<DevBuf<double>> ck(nblocks), S(every_);
-  for (auto &c : ck) c.alloc(n_);
-  for (int i = 1; i < every_; ++i) S[i].alloc(n_);
-  DevBuf<double> end;
-  end.alloc(n_);
+  // pooled states: checkpoints [0, nblocks), block starts, the result
+  std::vector<double *> ck(nblocks), S(every_, nullptr);
+  for (int i = 0; i < nblocks; ++i) ck[i] = state(size_t(i));
+  for (int i = 1; i < every_; ++i) S[i] = state(size_t(nblocks + i));
+  double *const end = state(size_t(nblocks + every_));
+  const size_t rec0 = size_t(nblocks + every_ + 1);  // first stage-record state
   HIPCHECK(hipMemsetAsync(flag_.p, 0, sizeof(int), s_));
   HIPCHECK(hipMemcpyAsync(time_.p, &t0, sizeof(double), hipMemcpyHostToDevice, s_));
-  HIPCHECK(hipMemcpyAsync(ck[0].p, q0, bytes, hipMemcpyDeviceToDevice, s_));
+  HIPCHECK(hipMemcpyAsync(ck[0], q0, bytes, hipMemcpyDeviceToDevice, s_));
   // forward sweep: checkpoints at the block starts
   const StageRecord scratch = scratch_record();
   for (int bk = 0; bk < nblocks; ++bk) {
     const int j0 = bk * every_, cnt = std::min(every_, nsteps - j0);
-    const double *src = ck[bk].p;
+    const double *src = ck[bk];
     for (int i = 0; i < cnt; ++i) {
-      double *dst = i + 1 < cnt ? S[i + 1].p : (bk + 1 < nblocks ? ck[bk + 1].p : end.p);
+      double *dst = i + 1 < cnt ? S[i + 1] : (bk + 1 < nblocks ? ck[bk + 1] : end);
       forward_step(false, src, dst, time_.p, scratch);
       src = dst;
     }
   }
   check_flag();
-  // The stage records of a whole block are kept when they fit in a quarter
-  // of the free device memory; otherwise each step's record is recomputed
-  // from its start just before its transposed stages.
+  // The stage records of a whole block are kept when the states they still
+  // need fit in a quarter of the free device memory; otherwise each step's
+  // record is recomputed from its start just before its transposed stages.
   size_t free_b = 0, total_b = 0;
   HIPCHECK(hipMemGetInfo(&free_b, &total_b));
-  stored_records_ = size_t(every_) * 9 * bytes + size_t(every_) * 64 <= free_b / 4;
+  const size_t want = rec0 + 9 * size_t(every_), have = states_.size();
+  stored_records_ = (want > have ? (want - have) * bytes : 0) <= free_b / 4;
   // GOCFD_ADJOINT_KEEP_RECORDS=0 forces the per-step recomputation
   if (const char *e = std::getenv("GOCFD_ADJOINT_KEEP_RECORDS")) stored_records_ = stored_records_ && e[0] != '0';
-  const size_t nrec = stored_records_ ? size_t(every_) : 0;
-  std::vector<DevBuf<double>> recU(nrec * 5), recR(nrec * 5), recW(nrec);
-  std::vector<DevBuf<dev::ad_storage>> recDt(nrec);
-  std::vector<DevBuf<int>> recArg(nrec);
-  std::vector<StageRecord> recs(nrec);
-  if (stored_records_) {
-    for (int i = 0; i < every_; ++i) {
-      StageRecord &r = recs[i];
-      r.U[0] = nullptr;
-      for (int s = 1; s < 5; ++s) {
-        recU[size_t(i) * 5 + s].alloc(n_);
-        r.U[s] = recU[size_t(i) * 5 + s].p;
-      }
-      for (int s = 0; s < 5; ++s) {
-        recR[size_t(i) * 5 + s].alloc(n_);
-        r.R[s] = recR[size_t(i) * 5 + s].p;
-      }
-      recDt[i].alloc(5);
-      recArg[i].alloc(5);
-      recW[i].alloc(5);
-      r.dts = recDt[i].p;
-      r.argmax = recArg[i].p;
-      r.Wv = recW[i].p;
-    }
+  std::vector<StageRecord> recs(stored_records_ ? size_t(every_) : 0);
+  for (size_t i = 0; i < recs.size(); ++i) {
+    StageRecord &r = recs[i];
+    r.U[0] = nullptr;
+    for (int s = 1; s < 5; ++s) r.U[s] = state(rec0 + 9 * i + size_t(s - 1));
+    for (int s = 0; s < 5; ++s) r.R[s] = state(rec0 + 9 * i + 4 + size_t(s));
+    SmallRecord &sm = small(i);
+    r.dts = sm.dts.p;
+    r.argmax = sm.argmax.p;
+    r.Wv = sm.Wv.p;
   }
   // reverse sweep
-  DevBuf<double> lamU0, lamQ[5], lamR, lamR3, lamOwn, lamFarPts, lamFarW, lam_dt;
-  lamU0.alloc(n_);
-  for (int s = 1; s < 5; ++s) lamQ[s].alloc(n_);
-  lamR.alloc(n_);
-  lamR3.alloc(n_);
-  lamOwn.alloc(4 * nfp_);
-  lamFarPts.alloc(4 * nfp_);
-  lamFarW.alloc(4);
-  lam_dt.alloc(5);
+  if (!lamU0_.p) {
+    lamU0_.alloc(n_);
+    for (int s = 1; s < 5; ++s) lamQ_[s].alloc(n_);
+    lamR_.alloc(n_);
+    lamR3_.alloc(n_);
+    lamOwn_.alloc(4 * nfp_);
+    lamFarPts_.alloc(4 * nfp_);
+    lamFarW_.alloc(4);
+    lam_dt_.alloc(5);
+    far_d_.alloc(4);
+  }
+  DevBuf<double> &lamU0 = lamU0_, *lamQ = lamQ_, &lamR = lamR_, &lamR3 = lamR3_, &lamOwn = lamOwn_,
+                 &lamFarPts = lamFarPts_, &lamFarW = lamFarW_, &lam_dt = lam_dt_;
   HIPCHECK(hipMemcpyAsync(lamU0.p, lamN, bytes, hipMemcpyDeviceToDevice, s_));
   HIPCHECK(hipMemsetAsync(lamFarPts.p, 0, 4 * nfp_ * sizeof(double), s_));
   HIPCHECK(hipMemsetAsync(lamFarW.p, 0, 4 * sizeof(double), s_));
@@ -215,12 +225,12 @@ void AdjointRunner::vjp(const double *q0, double t0, int nsteps, int every, cons
   for (int bk = nblocks - 1; bk >= 0; --bk) {
     const int j0 = bk * every_, cnt = std::min(every_, nsteps - j0);
     std::vector<const double *> starts(cnt);
-    starts[0] = ck[bk].p;
+    starts[0] = ck[bk];
     for (int i = 0; i < cnt; ++i) {
-      void *next = i + 1 < cnt ? static_cast<void *>(S[i + 1].p) : static_cast<void *>(tmp_.p);
+      void *next = i + 1 < cnt ? static_cast<void *>(S[i + 1]) : static_cast<void *>(tmp_.p);
       if (stored_records_) forward_step(false, starts[i], next, nullptr, recs[i]);
       else if (i + 1 < cnt) forward_step(false, starts[i], next, nullptr, scratch);
-      if (i + 1 < cnt) starts[i + 1] = S[i + 1].p;
+      if (i + 1 < cnt) starts[i + 1] = S[i + 1];
     }
     for (int i = cnt - 1; i >= 0; --i) {
       if (stored_records_) {
@@ -231,12 +241,10 @@ void AdjointRunner::vjp(const double *q0, double t0, int nsteps, int every, cons
       }
     }
   }
-  DevBuf<double> far_d;
-  far_d.alloc(4);
-  b.target = far_d.p;
+  b.target = far_d_.p;
   launch(dev::ADJ_FAR_SUM, b);
   HIPCHECK(hipMemcpyAsync(lam0, lamU0.p, bytes, hipMemcpyDeviceToDevice, s_));
-  HIPCHECK(hipMemcpyAsync(far, far_d.p, 4 * sizeof(double), hipMemcpyDeviceToHost, s_));
+  HIPCHECK(hipMemcpyAsync(far, far_d_.p, 4 * sizeof(double), hipMemcpyDeviceToHost, s_));
   HIPCHECK(hipStreamSynchronize(s_));
 }
 
@@ -244,23 +252,25 @@ void AdjointRunner::jvp(const double *q0, const double *dq0, const double dfar[4
                         double *qn, double *dqn) {
   if (nsteps < 1) throw SetupError(GOCFD_ERR_ARG, "nsteps must be positive");
   for (int c = 0; c < 4; ++c) a_.dfar[c] = dfar ? dfar[c] : 0.;
-  DevBuf<dev::ad_storage> A, B;
-  A.alloc(n_);
-  B.alloc(n_);
+  if (!dualA_.p) {
+    dualA_.alloc(n_);
+    dualB_.alloc(n_);
+  }
+  dev::ad_storage *A = dualA_.p, *B = dualB_.p;
   HIPCHECK(hipMemsetAsync(flag_.p, 0, sizeof(int), s_));
   HIPCHECK(hipMemcpyAsync(time_.p, &t0, sizeof(double), hipMemcpyHostToDevice, s_));
   dev::AdjBufs b{};
   b.n = n_;
   b.pack_v = q0;
   b.pack_d = dq0;
-  b.out = A.p;
+  b.out = A;
   launch(dev::ADJ_PACK, b);
   const StageRecord scratch = scratch_record();
   for (int j = 0; j < nsteps; ++j) {
-    forward_step(true, A.p, B.p, time_.p, scratch);
-    std::swap(A.p, B.p);
+    forward_step(true, A, B, time_.p, scratch);
+    std::swap(A, B);
   }
-  b.Us = A.p;
+  b.Us = A;
   b.unpack_v = qn;
   b.unpack_d = dqn;
   launch(dev::ADJ_UNPACK, b);

@@ -10,6 +10,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <memory>
 #include <vector>
 
 #include "adjoint.hpp"
@@ -53,6 +54,20 @@ class AdjointRunner {
   void adjoint_step(const double *Ustart, const StageRecord &rec, dev::AdjBufs &b);
   StageRecord scratch_record();
   void check_flag();
+  // Buffers kept from call to call (grown on demand, freed with the solver):
+  // element-array states (checkpoints, block starts, stage records) and the
+  // small per-step record arrays
+  double *state(size_t i);
+  struct SmallRecord {
+    DevBuf<dev::ad_storage> dts;
+    DevBuf<int> argmax;
+    DevBuf<double> Wv;
+  };
+  SmallRecord &small(size_t i);
+  std::vector<std::unique_ptr<DevBuf<double>>> states_;
+  std::vector<std::unique_ptr<SmallRecord>> smalls_;
+  DevBuf<double> lamU0_, lamQ_[5], lamR_, lamR3_, lamOwn_, lamFarPts_, lamFarW_, lam_dt_, far_d_;
+  DevBuf<dev::ad_storage> dualA_, dualB_;
   dev::AdjArgs a_{};
   hipStream_t s_;
   size_t n_ = 0, nfp_ = 0;  // element-array and face-point counts
