@@ -185,31 +185,52 @@ __global__ void __launch_bounds__(256) k_adj_fwd_face(AdjArgs a, const T *U, T *
 // then the stage's SSP54 combination at those four values in the order of
 // the solver's stage kernel (stage s reads the step start U0, its input,
 // its RHS and dt_s; the last stage also U2, U3 and the stage-3 RHS).
+// A workgroup holds RhsElems<N> whole elements; thread (e, i) first forms
+// the transformed flux at its own point i into LDS for its element's rows.
+template <int N>
+struct RhsElems {
+  static constexpr int value = 256 / AdjShape<N>::NpInt;
+};
+
 template <int N, class T>
 __global__ void __launch_bounds__(256) k_adj_fwd_rhs(AdjArgs a, int stage, const T *U, const T *FE, T *R,
                                                      const T *U0, const T *U2, const T *U3, const T *R3,
                                                      const T *dts, T *out) {
   using Sh = AdjShape<N>;
-  constexpr int NpInt = Sh::NpInt, NpEdge = Sh::NpEdge, NpFlux = Sh::NpFlux;
-  const size_t gid = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (gid >= size_t(a.Kp) * NpInt) return;
-  const int kk = int(gid / NpInt), i = int(gid - size_t(kk) * NpInt);
-  const T *u = U + size_t(kk) * Sh::NV;
+  constexpr int NpInt = Sh::NpInt, NpEdge = Sh::NpEdge, NpFlux = Sh::NpFlux, EB = RhsElems<N>::value;
+  // Fr of the 4 variables, then Fs, per element: [EB][8][NpInt] (raw
+  // storage: the dual type has a constructor)
+  __shared__ double sraw[EB * 8 * NpInt * (sizeof(T) / sizeof(double))];
+  T(*sF)[8][NpInt] = reinterpret_cast<T(*)[8][NpInt]>(sraw);
+  const int el = threadIdx.x / NpInt, i = threadIdx.x - el * NpInt;
+  const int kk = blockIdx.x * EB + el;
+  const bool live = el < EB && kk < a.Kp;
+  const bool real = live && kk < a.K;
+  double Jd = 1., J0 = 0., J1 = 0., J2 = 0., J3 = 0.;
+  const T *u = U + size_t(live ? kk : 0) * Sh::NV;
+  if (real) {
+    Jd = a.Jdet[kk];
+    J0 = a.Jinv[kk];
+    J1 = a.Jinv[size_t(a.Kp) + kk];
+    J2 = a.Jinv[2 * size_t(a.Kp) + kk];
+    J3 = a.Jinv[3 * size_t(a.Kp) + kk];
+    const T q[4] = {u[i], u[NpInt + i], u[2 * NpInt + i], u[3 * NpInt + i]};
+    T Fx[4], Fy[4];
+    flux_xy(T(a.ph.gamma), q, Fx, Fy);
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+      sF[el][c][i] = Jd * (J0 * Fx[c] + J1 * Fy[c]);
+      sF[el][4 + c][i] = Jd * (J2 * Fx[c] + J3 * Fy[c]);
+    }
+  }
+  __syncthreads();
+  if (!live) return;
   T acc[4] = {T(0.), T(0.), T(0.), T(0.)};
-  if (kk < a.K) {
-    const double Jd = a.Jdet[kk];
-    const double J0 = a.Jinv[kk], J1 = a.Jinv[size_t(a.Kp) + kk];
-    const double J2 = a.Jinv[2 * size_t(a.Kp) + kk], J3 = a.Jinv[3 * size_t(a.Kp) + kk];
+  if (real) {
     const double *D = a.DivInt + i * NpFlux;
     for (int j = 0; j < NpInt; ++j) {
-      const T q[4] = {u[j], u[NpInt + j], u[2 * NpInt + j], u[3 * NpInt + j]};
-      T Fx[4], Fy[4];
-      flux_xy(T(a.ph.gamma), q, Fx, Fy);
 #pragma unroll
-      for (int c = 0; c < 4; ++c) {
-        const T Fr = Jd * (J0 * Fx[c] + J1 * Fy[c]), Fs = Jd * (J2 * Fx[c] + J3 * Fy[c]);
-        acc[c] += D[j] * Fr + D[NpInt + j] * Fs;
-      }
+      for (int c = 0; c < 4; ++c) acc[c] += D[j] * sF[el][c][j] + D[NpInt + j] * sF[el][4 + c][j];
     }
     for (int p = 0; p < Sh::NF; ++p) {
       const int f = p / NpEdge, pp = p - f * NpEdge;
@@ -357,6 +378,64 @@ __device__ __forceinline__ void roe_flux_vjp(double g, const double L[4], const 
   bR[3] += b_pR * GM1 + bR3;
 }
 
+// Reverse-mode derivative of lax_flux (LaxFlux, fluxes.go:161-190), by
+// hand: adds (dF*/dL)^T bF to bL and (dF*/dR)^T bF to bR. The maximum of
+// the two sides' |V| + c takes the left side's branch on a tie, as the
+// dual-number fmax does.
+__device__ __forceinline__ void lax_flux_vjp(double g, const double L[4], const double R[4], double nx, double ny,
+                                             const double bF[4], double bL[4], double bR[4]) {
+  const double GM1 = g - 1.;
+  const double *Q[2] = {L, R};
+  double *B[2] = {bL, bR};
+  double uu[2], vv[2], pp[2], cc[2], zz[2], ss[2], VV[2];
+  for (int k = 0; k < 2; ++k) {
+    const double *q = Q[k];
+    uu[k] = q[1] / q[0];
+    vv[k] = q[2] / q[0];
+    pp[k] = GM1 * (q[3] - 0.5 * q[0] * (uu[k] * uu[k] + vv[k] * vv[k]));
+    zz[k] = g * pp[k] / q[0];
+    cc[k] = sqrt(fabs(zz[k]));
+    ss[k] = sqrt(uu[k] * uu[k] + vv[k] * vv[k]);
+    VV[k] = ss[k] + cc[k];
+  }
+  const bool left = VV[0] >= VV[1];
+  const double maxV = left ? VV[0] : VV[1];
+  double b_maxV = 0.;
+  for (int c = 0; c < 4; ++c) {
+    bL[c] += 0.5 * maxV * bF[c];
+    bR[c] -= 0.5 * maxV * bF[c];
+    b_maxV += 0.5 * bF[c] * (L[c] - R[c]);
+  }
+  for (int k = 0; k < 2; ++k) {
+    const double *q = Q[k];
+    double *b = B[k];
+    const double u = uu[k], v = vv[k], p = pp[k], H = p + q[3];
+    // the central part: each side's physical flux along n, halved
+    b[1] += 0.5 * nx * bF[0] + 0.5 * bF[1] * (nx * u + ny * v);
+    b[2] += 0.5 * ny * bF[0] + 0.5 * bF[2] * (nx * u + ny * v);
+    double b_u = 0.5 * (bF[1] * nx * q[1] + bF[2] * nx * q[2] + bF[3] * nx * H);
+    double b_v = 0.5 * (bF[1] * ny * q[1] + bF[2] * ny * q[2] + bF[3] * ny * H);
+    const double b_H = 0.5 * bF[3] * (nx * u + ny * v);
+    double b_p = 0.5 * (bF[1] * nx + bF[2] * ny) + b_H;
+    b[3] += b_H;
+    // |V| + c of the side that attains the maximum
+    if ((k == 0) == left) {
+      if (ss[k] > 0.) {
+        b_u += b_maxV * u / ss[k];
+        b_v += b_maxV * v / ss[k];
+      }
+      const double b_z = b_maxV * (zz[k] < 0. ? -1. : 1.) / (2. * cc[k]);
+      b_p += b_z * g / q[0];
+      b[0] += -b_z * zz[k] / q[0];
+    }
+    // velocities and pressure
+    b[1] += b_u / q[0] - GM1 * u * b_p;
+    b[2] += b_v / q[0] - GM1 * v * b_p;
+    b[0] += -(b_u * u + b_v * v) / q[0] + GM1 * 0.5 * (u * u + v * v) * b_p;
+    b[3] += GM1 * b_p;
+  }
+}
+
 // Per face point: the transposed Jacobians of its edge value with respect
 // to its inputs, applied to the edge value's cotangent, into the trace
 // cotangent of each face point (lamOwn, written exactly once per launch)
@@ -396,9 +475,10 @@ __global__ void __launch_bounds__(256) k_adj_face(AdjArgs a, const double *U, co
     const double wo = a.IInII[fi], wn = a.IInII[fn];
     for (int c = 0; c < 4; ++c) lF[c] = wo * lfo[c] - wn * lfn[c];
     double *ln = lamOwn + 4 * (fn * NpEdge + rp);
-    if constexpr (FLUX == 2) {
+    if constexpr (FLUX == 1 || FLUX == 2) {
       double bL[4] = {0., 0., 0., 0.}, bR[4] = {0., 0., 0., 0.};
-      roe_flux_vjp(a.ph.gamma, own, oth, a.Nx[fi], a.Ny[fi], lF, bL, bR);
+      if constexpr (FLUX == 2) roe_flux_vjp(a.ph.gamma, own, oth, a.Nx[fi], a.Ny[fi], lF, bL, bR);
+      else lax_flux_vjp(a.ph.gamma, own, oth, a.Nx[fi], a.Ny[fi], lF, bL, bR);
       for (int c = 0; c < 4; ++c) {
         lo[c] = bL[c];
         ln[c] = bR[c];
@@ -567,7 +647,7 @@ hipError_t adj_launch_n(int op, const AdjArgs &a, const AdjBufs &b, hipStream_t 
 #undef GOCFD_FWD_FACE
       break;
     case ADJ_FWD_RHS: {
-      const unsigned nb = adj_blocks(size_t(a.Kp) * Sh::NpInt);
+      const unsigned nb = unsigned((a.Kp + RhsElems<N>::value - 1) / RhsElems<N>::value);
       if (b.dual)
         hipLaunchKernelGGL((k_adj_fwd_rhs<N, Dual>), dim3(nb), dim3(256), 0, s, a, b.stage,
                            static_cast<const Dual *>(b.Us), static_cast<const Dual *>(b.FE),
