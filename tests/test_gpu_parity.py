@@ -163,3 +163,46 @@ def test_nan_is_reported(vortex_meshes):
     gpu.Q = q
     with pytest.raises(FloatingPointError):
         gpu.step(1)
+
+
+def _tiny_meshes(tmp_path):
+    """A single triangle and a two-triangle square with far-field edges, and
+    a 130-triangle mesh (tiles of 64, 64 and 2 elements, with wall and far
+    edges)."""
+    from gocfd_amd import meshgen
+    out = {}
+    p = str(tmp_path / "one.su2")
+    meshgen.write_su2(p, [0.0, 1.0, 0.0], [0.0, 0.0, 1.0], [(0, 1, 2)], {"far": [(0, 1), (1, 2), (2, 0)]})
+    out["one"] = p
+    p = str(tmp_path / "two.su2")
+    meshgen.write_su2(p, [0.0, 1.0, 1.0, 0.0], [0.0, 0.0, 1.0, 1.0], [(0, 1, 2), (0, 2, 3)],
+                      {"far": [(0, 1), (1, 2)], "wall": [(2, 3), (3, 0)]})
+    out["two"] = p
+    VX, VY, EToV, tags = meshgen.jittered_quads(5, 13, seed=7, periodic_x=False)
+    tags = {"wall": tags["far"][:5], "far": tags["far"][5:]}
+    p = str(tmp_path / "ragged.su2")
+    meshgen.write_su2(p, VX, VY, EToV, tags)
+    out["ragged"] = p
+    return out
+
+
+@pytest.mark.parametrize("name,N,flux", [("one", 0, "Roe"), ("one", 2, "Roe"), ("one", 4, "Lax"),
+                                         ("two", 1, "Lax"), ("two", 3, "Roe"), ("ragged", 2, "Roe"),
+                                         ("ragged", 4, "Roe-ER")])
+def test_smallest_and_ragged_meshes(tmp_path, name, N, flux):
+    """Edge cases of the tiling: K = 1 and 2 (one tile, mostly padding) and
+    K = 130 (a last tile of 2 elements): RHS of a perturbed state within
+    1e-11 of the oracle, 10 steps within 1e-10."""
+    mesh = _tiny_meshes(tmp_path)[name]
+    gpu, ora = _pair(mesh, N, flux, "freestream", Minf=0.4, Alpha=10.0)
+    assert gpu.K == {"one": 1, "two": 2, "ragged": 130}[name]
+    q = _perturbed(ora.state(), 3, amp=2e-2)
+    assert _rel(gpu.rhs(q), ora.rhs(q)) < 1e-11
+    gpu.Q = q
+    gpu.set_time(0.0)
+    ora.set_state(q)
+    for _ in range(10):
+        ora.step()
+    gpu.step(10)
+    qg, qo = gpu.Q, ora.state()
+    assert np.linalg.norm(qg[0] - qo[0]) / np.linalg.norm(qo[0]) < 1e-10
