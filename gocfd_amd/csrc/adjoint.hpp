@@ -1,11 +1,12 @@
 // Tangent-linear and discrete-adjoint kernels of the FP64 SSP54 step
-// (global time stepping, no limiter): interface between the host driver
+// (global or local time stepping, no limiter): interface between the host driver
 // (adjoint_runner.cpp) and the per-order kernel translation units.
 //
 // The differentiated map is one SSP54 step of the reference (rkAdvance,
 // model_problems/Euler2D/euler.go:460-566) with the per-stage global time
 // step dt_s = CFL / max wave speed (calculateGlobalDT euler.go:945-971,
-// StoreEdgeAggregates edges.go:246-289) of the stage input, the DFR
+// StoreEdgeAggregates edges.go:246-289) of the stage input, or the element
+// dts of local time stepping (CalculateLocalDT euler.go:973-1002), the DFR
 // right-hand side (RHSInternalPoints euler.go:665-699, SetRTFluxInternal
 // euler.go:701-726, CalculateEdgeEulerFlux edges.go:325-452) and the
 // Riemann-invariant far field (RiemannBC bcs.go:70-133). Its inputs are the
@@ -48,6 +49,11 @@ struct AdjArgs {
   Phys ph;
   int io_tied;     // In/Out faces use the far state (every case but the shock tube)
   double dfar[4];  // tangent of the far state (tangent-linear step)
+  // local time stepping (CalculateLocalDT, euler.go:973-1002): element dt
+  // = CFL / max(previous stage's element dt, speeds of its faces), the face
+  // (f Kp + kk) speed read from face wsrc[f Kp + kk] (the edge's owner side)
+  int lts;
+  const int *wsrc;
 };
 
 // Buffers of one launch. Element arrays are element-major [Kp][4][NpInt];
@@ -80,6 +86,13 @@ struct AdjBufs {
   double *lamFarPts;    // face-point cotangents of the far state (accumulated)
   double *lamFarW;      // [4] far-state cotangent from the dt terms (accumulated)
   double *lam_dt;       // [5] cotangents of the stage dts
+  // local time stepping: per element of this stage
+  void *dtk;            // element dt (double or dual) [Kp]
+  const void *dtk_prev; // the previous stage's element dt (stage >= 1)
+  int *srck;            // face point whose speed sets the dt, -1: the previous dt
+  double *wk;           // the maximum (CFL / dt)
+  double *carry;        // cotangent of the previous stage's element dt [Kp]
+  double *lamSpeed;     // cotangents of the face-point speeds [3 Kp NpEdge]
   double *target;       // cotangent of this stage's input (accumulated)
   const double *pack_v, *pack_d;  // value and tangent arrays (pack/unpack)
   double *unpack_v, *unpack_d;
@@ -95,6 +108,9 @@ enum AdjOp : int {
   ADJ_ADJ_ELEM,      // target += interior-flux and trace transposes
   ADJ_ADJ_WAVE,      // target, lamFarW += the dt term of the stage
   ADJ_FAR_SUM,       // target[0..3] <- sum of lamFarPts + lamFarW
+  ADJ_FWD_LTS,       // dtk, srck, wk <- local dt of each element from SP and dtk_prev
+  ADJ_ADJ_SSP_LTS,   // the SSP transpose with element dts; lamSpeed, carry <- the dt terms
+  ADJ_ADJ_LTS_WAVE,  // lamOwn, lamFarPts += the face-point speed cotangents (then cleared)
   ADJ_PACK,          // out (dual) <- (pack_v, pack_d)
   ADJ_UNPACK,        // (unpack_v, unpack_d) <- Us (dual)
 };

@@ -195,7 +195,7 @@ struct RhsElems {
 template <int N, class T>
 __global__ void __launch_bounds__(256) k_adj_fwd_rhs(AdjArgs a, int stage, const T *U, const T *FE, T *R,
                                                      const T *U0, const T *U2, const T *U3, const T *R3,
-                                                     const T *dts, T *out) {
+                                                     const T *dts, const T *dtk, T *out) {
   using Sh = AdjShape<N>;
   constexpr int NpInt = Sh::NpInt, NpEdge = Sh::NpEdge, NpFlux = Sh::NpFlux, EB = RhsElems<N>::value;
   // Fr of the 4 variables, then Fs, per element: [EB][8][NpInt] (raw
@@ -242,7 +242,7 @@ __global__ void __launch_bounds__(256) k_adj_fwd_rhs(AdjArgs a, int stage, const
 #pragma unroll
     for (int c = 0; c < 4; ++c) acc[c] = acc[c] * m;
   }
-  const T dt = dts[stage];
+  const T dt = a.lts ? dtk[kk] : dts[stage];  // the element's dt with local time stepping
 #pragma unroll
   for (int c = 0; c < 4; ++c) {
     const size_t x = size_t(kk) * Sh::NV + c * NpInt + i;
@@ -260,6 +260,42 @@ __global__ void __launch_bounds__(256) k_adj_fwd_rhs(AdjArgs a, int stage, const
       }
     }
   }
+}
+
+// Local time stepping (CalculateElementMaxWaveSpeed edges.go:291-323,
+// CalculateLocalDT euler.go:973-1002): per element, w = the previous
+// stage's element dt (-100 at stage 0, the reference's unit-mixing max),
+// then the max with the speeds of the points of its three faces (read from
+// the edge owner's face); dt = CFL / w. Records the point that sets w (-1:
+// the previous dt) for the adjoint. Ties keep the earlier candidate.
+template <int N, class T>
+__global__ void __launch_bounds__(256) k_adj_fwd_lts(AdjArgs a, int stage, const T *SP, const T *dtk_prev, T *dtk,
+                                                     int *srck, double *wk) {
+  constexpr int NpEdge = AdjShape<N>::NpEdge;
+  const int kk = blockIdx.x * blockDim.x + threadIdx.x;
+  if (kk >= a.Kp) return;
+  if (kk >= a.K) {
+    dtk[kk] = T(0.);
+    srck[kk] = -1;
+    wk[kk] = 0.;
+    return;
+  }
+  T w = stage == 0 ? T(-100.) : dtk_prev[kk];
+  int src = -1;
+  for (int f = 0; f < 3; ++f) {
+    const size_t fs = size_t(a.wsrc[size_t(f) * a.Kp + kk]);
+    for (int pp = 0; pp < NpEdge; ++pp) {
+      const size_t t = fs * NpEdge + pp;
+      const T sp = SP[t];
+      if (ad::val(sp) > ad::val(w)) {
+        w = sp;
+        src = int(t);
+      }
+    }
+  }
+  dtk[kk] = a.ph.cfl / w;
+  srck[kk] = src;
+  wk[kk] = ad::val(w);
 }
 
 // ---- adjoint ----
@@ -580,6 +616,112 @@ __global__ void __launch_bounds__(256) k_adj_elem(AdjArgs a, const double *U, co
   for (int c = 0; c < 4; ++c) tg[c * NpInt + j] += acc[c];
 }
 
+// Transpose of one SSP54 stage with element dts (one thread per element):
+// the updates of k_adj_ssp, and the element's dt cotangent (its dot product
+// plus what the next stage's dt passed back through the previous-dt term),
+// routed through dt = CFL / w to the point that set w (a face-point speed
+// cotangent; at most the two elements of an edge add to one point, so the
+// sum does not depend on their order) or, if w was the previous stage's dt,
+// carried to that stage.
+template <int N>
+__global__ void __launch_bounds__(256) k_adj_ssp_lts(AdjArgs a, int stage, const double *lam, double *lamU0,
+                                                     double *lamQ1, double *lamQ2, double *lamQ3, double *lamQ4,
+                                                     double *lamR, double *lamR3, const double *Rs, const double *R3,
+                                                     const double *dtk, const int *srck, const double *wk,
+                                                     double *carry, double *lamSpeed) {
+  constexpr int NV = AdjShape<N>::NV;
+  const int kk = blockIdx.x * blockDim.x + threadIdx.x;
+  if (kk >= a.Kp) return;
+  const double dt = dtk[kk];
+  double dot = 0.;
+  for (size_t i = size_t(kk) * NV; i < size_t(kk + 1) * NV; ++i) {
+    const double l = lam[i];
+    switch (stage) {
+      case 4:
+        lamU0[i] = 0.;
+        lamQ2[i] = kSspD2 * l;
+        lamQ3[i] = kSspD3 * l;
+        lamQ4[i] = kSspD4 * l;
+        lamR3[i] = kSspE3 * dt * l;
+        lamR[i] = kSspE4 * dt * l;
+        dot += l * (kSspE3 * R3[i] + kSspE4 * Rs[i]);
+        break;
+      case 3:
+        lamU0[i] += kSspB3 * l;
+        lamQ3[i] += kSspC3 * l;
+        lamR[i] = lamR3[i] + kSspA3 * dt * l;
+        dot += kSspA3 * l * Rs[i];
+        break;
+      case 2:
+        lamU0[i] += kSspB2 * l;
+        lamQ2[i] += kSspC2 * l;
+        lamR[i] = kSspA2 * dt * l;
+        dot += kSspA2 * l * Rs[i];
+        break;
+      case 1:
+        lamU0[i] += kSspB1 * l;
+        lamQ1[i] = kSspC1 * l;
+        lamR[i] = kSspA1 * dt * l;
+        dot += kSspA1 * l * Rs[i];
+        break;
+      default:
+        lamU0[i] += l;
+        lamR[i] = kSspA0 * dt * l;
+        dot += kSspA0 * l * Rs[i];
+    }
+  }
+  if (kk >= a.K) {
+    carry[kk] = 0.;
+    return;
+  }
+  const double lam_d = dot + carry[kk];
+  const double W = wk[kk];
+  const double lw = -(a.ph.cfl / (W * W)) * lam_d;
+  const int src = srck[kk];
+  if (src < 0) {
+    carry[kk] = lw;
+  } else {
+    atomicAdd(&lamSpeed[src], lw);
+    carry[kk] = 0.;
+  }
+}
+
+// The face-point speed cotangents of local time stepping into the trace
+// (and far-state) cotangents of the point; clears them for the next stage
+template <int N>
+__global__ void __launch_bounds__(256) k_adj_lts_wave(AdjArgs a, const double *U, double *lamSpeed, double *lamOwn,
+                                                      double *lamFarPts) {
+  using Sh = AdjShape<N>;
+  const size_t t = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (t >= size_t(3) * a.Kp * Sh::NpEdge) return;
+  const double ls = lamSpeed[t];
+  if (ls == 0.) return;
+  lamSpeed[t] = 0.;
+  int f, kk, pp;
+  size_t fi;
+  face_point<N>(a, t, f, kk, pp, fi);
+  const int code = a.code[fi];
+  double own[4], qfar[4];
+  trace_at<N>(a, U, kk, f * Sh::NpEdge + pp, own);
+  far_state(a, qfar);
+  const bool far = (code & 15) == FK_Far && ((code >> 8) == 0 || a.io_tied);
+  for (int d = 0; d < 4; ++d) {
+    Dual o[4], q[4];
+    for (int c = 0; c < 4; ++c) {
+      o[c] = Dual(own[c], c == d ? 1. : 0.);
+      q[c] = Dual(qfar[c]);
+    }
+    lamOwn[4 * t + d] += ls * wave_value(a, code, fi, a.oohK[kk], o, q).d;
+    if (far) {
+      for (int c = 0; c < 4; ++c) {
+        o[c] = Dual(own[c]);
+        q[c] = Dual(qfar[c], c == d ? 1. : 0.);
+      }
+      lamFarPts[4 * t + d] += ls * wave_value(a, code, fi, a.oohK[kk], o, q).d;
+    }
+  }
+}
+
 // The dt term of a stage: dt = CFL / W with W the wave speed at the stage's
 // arg-max face point; its cotangent goes to that point's trace (and the far
 // state for a far-field face). One thread.
@@ -654,16 +796,36 @@ hipError_t adj_launch_n(int op, const AdjArgs &a, const AdjBufs &b, hipStream_t 
                            static_cast<Dual *>(b.R), static_cast<const Dual *>(b.U0),
                            static_cast<const Dual *>(b.U2), static_cast<const Dual *>(b.U3),
                            static_cast<const Dual *>(b.R3), static_cast<const Dual *>(b.dts),
-                           static_cast<Dual *>(b.out));
+                           static_cast<const Dual *>(b.dtk), static_cast<Dual *>(b.out));
       else
         hipLaunchKernelGGL((k_adj_fwd_rhs<N, double>), dim3(nb), dim3(256), 0, s, a, b.stage,
                            static_cast<const double *>(b.Us), static_cast<const double *>(b.FE),
                            static_cast<double *>(b.R), static_cast<const double *>(b.U0),
                            static_cast<const double *>(b.U2), static_cast<const double *>(b.U3),
                            static_cast<const double *>(b.R3), static_cast<const double *>(b.dts),
-                           static_cast<double *>(b.out));
+                           static_cast<const double *>(b.dtk), static_cast<double *>(b.out));
       break;
     }
+    case ADJ_FWD_LTS:
+      if (b.dual)
+        hipLaunchKernelGGL((k_adj_fwd_lts<N, Dual>), dim3(adj_blocks(a.Kp)), dim3(256), 0, s, a, b.stage,
+                           static_cast<const Dual *>(b.SP), static_cast<const Dual *>(b.dtk_prev),
+                           static_cast<Dual *>(b.dtk), b.srck, b.wk);
+      else
+        hipLaunchKernelGGL((k_adj_fwd_lts<N, double>), dim3(adj_blocks(a.Kp)), dim3(256), 0, s, a, b.stage,
+                           static_cast<const double *>(b.SP), static_cast<const double *>(b.dtk_prev),
+                           static_cast<double *>(b.dtk), b.srck, b.wk);
+      break;
+    case ADJ_ADJ_SSP_LTS:
+      hipLaunchKernelGGL((k_adj_ssp_lts<N>), dim3(adj_blocks(a.Kp)), dim3(256), 0, s, a, b.stage, b.lam, b.lamU0,
+                         b.lamQ[1], b.lamQ[2], b.lamQ[3], b.lamQ[4], b.lamR, b.lamR3,
+                         static_cast<const double *>(b.Rs), static_cast<const double *>(b.R3),
+                         static_cast<const double *>(b.dtk), b.srck, b.wk, b.carry, b.lamSpeed);
+      break;
+    case ADJ_ADJ_LTS_WAVE:
+      hipLaunchKernelGGL((k_adj_lts_wave<N>), dim3(adj_blocks(nfp)), dim3(256), 0, s, a,
+                         static_cast<const double *>(b.Us), b.lamSpeed, b.lamOwn, b.lamFarPts);
+      break;
     case ADJ_ADJ_FACE:
 #define GOCFD_ADJ_FACE(FL)                                                                                 \
   hipLaunchKernelGGL((k_adj_face<N, FL>), dim3(adj_blocks(nfp)), dim3(256), 0, s, a,                        \

@@ -9,7 +9,7 @@ namespace gocfd {
 namespace capi {
 
 AdjointRunner::AdjointRunner(const Discretisation &d, const DeviceLayout &L, const dev::Phys &ph, bool io_tied,
-                             const AdjointGeometry &geo, hipStream_t stream)
+                             bool lts, const AdjointGeometry &geo, hipStream_t stream)
     : s_(stream) {
   if (L.nbfaces > 0)
     throw SetupError(GOCFD_ERR_ARG,
@@ -29,6 +29,11 @@ AdjointRunner::AdjointRunner(const Discretisation &d, const DeviceLayout &L, con
   a_.DivInt = divint_.p;
   a_.ph = ph;
   a_.io_tied = io_tied ? 1 : 0;
+  a_.lts = lts ? 1 : 0;
+  if (lts) {
+    wsrc_.upload(L.wsrc);
+    a_.wsrc = wsrc_.p;
+  }
   n_ = size_t(L.Kp) * 4 * NpInt;
   nfp_ = size_t(3) * L.Kp * NpEdge;
   for (int s = 0; s < 5; ++s) {
@@ -45,6 +50,13 @@ AdjointRunner::AdjointRunner(const Discretisation &d, const DeviceLayout &L, con
   time_.alloc(1);
   part_v_.alloc(4 * size_t(nparts_));
   part_i_.alloc(size_t(nparts_));
+  if (lts) {
+    scratch_small_.dtk.alloc(5 * size_t(L.Kp));
+    scratch_small_.srck.alloc(5 * size_t(L.Kp));
+    scratch_small_.wk.alloc(5 * size_t(L.Kp));
+    carry_.alloc(L.Kp);
+    lamSpeed_.alloc(nfp_);
+  }
 }
 
 double *AdjointRunner::state(size_t i) {
@@ -58,11 +70,29 @@ double *AdjointRunner::state(size_t i) {
 AdjointRunner::SmallRecord &AdjointRunner::small(size_t i) {
   while (smalls_.size() <= i) {
     smalls_.push_back(std::make_unique<SmallRecord>());
-    smalls_.back()->dts.alloc(5);
-    smalls_.back()->argmax.alloc(5);
-    smalls_.back()->Wv.alloc(5);
+    SmallRecord &sm = *smalls_.back();
+    sm.dts.alloc(5);
+    sm.argmax.alloc(5);
+    sm.Wv.alloc(5);
+    if (a_.lts) {
+      sm.dtk.alloc(5 * size_t(a_.Kp));
+      sm.srck.alloc(5 * size_t(a_.Kp));
+      sm.wk.alloc(5 * size_t(a_.Kp));
+    }
   }
   return *smalls_[i];
+}
+
+void AdjointRunner::fill_small(StageRecord &r, SmallRecord &sm) {
+  r.dts = sm.dts.p;
+  r.argmax = sm.argmax.p;
+  r.Wv = sm.Wv.p;
+  for (int s = 0; s < 5; ++s) {
+    const size_t o = size_t(s) * a_.Kp;
+    r.dtk[s] = sm.dtk.p ? static_cast<void *>(sm.dtk.p + o) : nullptr;
+    r.srck[s] = sm.srck.p ? sm.srck.p + o : nullptr;
+    r.wk[s] = sm.wk.p ? sm.wk.p + o : nullptr;
+  }
 }
 
 void AdjointRunner::launch(int op, dev::AdjBufs &b) {
@@ -76,6 +106,7 @@ AdjointRunner::StageRecord AdjointRunner::scratch_record() {
     r.U[s] = U_[s].p;
     r.R[s] = R_[s].p;
   }
+  fill_small(r, scratch_small_);
   r.dts = dts_.p;
   r.argmax = argmax_.p;
   r.Wv = Wv_.p;
@@ -108,7 +139,15 @@ void AdjointRunner::forward_step(bool dual, const void *Ustart, void *Uend, doub
     b.R3 = rec.R[3];
     b.out = s < 4 ? rec.U[s + 1] : Uend;
     launch(dev::ADJ_FWD_FACE, b);
-    launch(dev::ADJ_ARGMAX, b);
+    if (a_.lts) {
+      b.dtk = rec.dtk[s];
+      b.dtk_prev = s > 0 ? rec.dtk[s - 1] : nullptr;
+      b.srck = rec.srck[s];
+      b.wk = rec.wk[s];
+      launch(dev::ADJ_FWD_LTS, b);  // no time advance or FinalTime clamp, as in the solver
+    } else {
+      launch(dev::ADJ_ARGMAX, b);
+    }
     launch(dev::ADJ_FWD_RHS, b);
   }
 }
@@ -124,12 +163,16 @@ void AdjointRunner::adjoint_step(const double *Ustart, const StageRecord &rec, d
     b.lam = s == 4 ? b.lamU0 : b.lamQ[s + 1];
     b.Rs = rec.R[s];
     b.R3 = rec.R[3];
-    launch(dev::ADJ_ADJ_SSP, b);
+    b.dtk = rec.dtk[s];
+    b.srck = rec.srck[s];
+    b.wk = rec.wk[s];
+    launch(a_.lts ? dev::ADJ_ADJ_SSP_LTS : dev::ADJ_ADJ_SSP, b);
     b.Us = s == 0 ? static_cast<const void *>(Ustart) : static_cast<const void *>(rec.U[s]);
     b.target = s == 0 ? b.lamU0 : b.lamQ[s];
     launch(dev::ADJ_ADJ_FACE, b);
+    if (a_.lts) launch(dev::ADJ_ADJ_LTS_WAVE, b);
     launch(dev::ADJ_ADJ_ELEM, b);
-    launch(dev::ADJ_ADJ_WAVE, b);
+    if (!a_.lts) launch(dev::ADJ_ADJ_WAVE, b);
   }
 }
 
@@ -187,10 +230,7 @@ void AdjointRunner::vjp(const double *q0, double t0, int nsteps, int every, cons
     r.U[0] = nullptr;
     for (int s = 1; s < 5; ++s) r.U[s] = state(rec0 + 9 * i + size_t(s - 1));
     for (int s = 0; s < 5; ++s) r.R[s] = state(rec0 + 9 * i + 4 + size_t(s));
-    SmallRecord &sm = small(i);
-    r.dts = sm.dts.p;
-    r.argmax = sm.argmax.p;
-    r.Wv = sm.Wv.p;
+    fill_small(r, small(i));
   }
   // reverse sweep
   if (!lamU0_.p) {
@@ -209,6 +249,10 @@ void AdjointRunner::vjp(const double *q0, double t0, int nsteps, int every, cons
   HIPCHECK(hipMemcpyAsync(lamU0.p, lamN, bytes, hipMemcpyDeviceToDevice, s_));
   HIPCHECK(hipMemsetAsync(lamFarPts.p, 0, 4 * nfp_ * sizeof(double), s_));
   HIPCHECK(hipMemsetAsync(lamFarW.p, 0, 4 * sizeof(double), s_));
+  if (a_.lts) {
+    HIPCHECK(hipMemsetAsync(carry_.p, 0, carry_.n * sizeof(double), s_));
+    HIPCHECK(hipMemsetAsync(lamSpeed_.p, 0, lamSpeed_.n * sizeof(double), s_));
+  }
   dev::AdjBufs b{};
   b.dual = 0;
   b.part_v = part_v_.p;
@@ -222,6 +266,8 @@ void AdjointRunner::vjp(const double *q0, double t0, int nsteps, int every, cons
   b.lamFarPts = lamFarPts.p;
   b.lamFarW = lamFarW.p;
   b.lam_dt = lam_dt.p;
+  b.carry = carry_.p;
+  b.lamSpeed = lamSpeed_.p;
   for (int bk = nblocks - 1; bk >= 0; --bk) {
     const int j0 = bk * every_, cnt = std::min(every_, nsteps - j0);
     std::vector<const double *> starts(cnt);

@@ -27,7 +27,7 @@ struct AdjointGeometry {
 
 class AdjointRunner {
  public:
-  AdjointRunner(const Discretisation &d, const DeviceLayout &L, const dev::Phys &ph, bool io_tied,
+  AdjointRunner(const Discretisation &d, const DeviceLayout &L, const dev::Phys &ph, bool io_tied, bool lts,
                 const AdjointGeometry &geo, hipStream_t stream);
   // lam0 = (dq_n/dq_0)^T lamN and far = (dq_n/dqinf)^T lamN for nsteps steps
   // from q0 (device, element-major FP64) at time t0; every = 0 picks
@@ -48,6 +48,11 @@ class AdjointRunner {
     void *dts;
     int *argmax;
     double *Wv;
+    // local time stepping: per stage, the element dts, the points that set
+    // them and the maxima
+    void *dtk[5];
+    int *srck[5];
+    double *wk[5];
   };
   void launch(int op, dev::AdjBufs &b);
   void forward_step(bool dual, const void *Ustart, void *Uend, double *time, const StageRecord &rec);
@@ -59,15 +64,19 @@ class AdjointRunner {
   // small per-step record arrays
   double *state(size_t i);
   struct SmallRecord {
-    DevBuf<dev::ad_storage> dts;
-    DevBuf<int> argmax;
-    DevBuf<double> Wv;
+    DevBuf<dev::ad_storage> dts, dtk;  // dtk, srck, wk: [5][Kp] with local time stepping
+    DevBuf<int> argmax, srck;
+    DevBuf<double> Wv, wk;
   };
+  void fill_small(StageRecord &r, SmallRecord &sm);
   SmallRecord &small(size_t i);
   std::vector<std::unique_ptr<DevBuf<double>>> states_;
   std::vector<std::unique_ptr<SmallRecord>> smalls_;
   DevBuf<double> lamU0_, lamQ_[5], lamR_, lamR3_, lamOwn_, lamFarPts_, lamFarW_, lam_dt_, far_d_;
   DevBuf<dev::ad_storage> dualA_, dualB_;
+  SmallRecord scratch_small_;
+  DevBuf<int> wsrc_;
+  DevBuf<double> carry_, lamSpeed_;
   dev::AdjArgs a_{};
   hipStream_t s_;
   size_t n_ = 0, nfp_ = 0;  // element-array and face-point counts

@@ -131,16 +131,15 @@ struct gocfd_euler2d {
     return c;
   }
 
-  // The tangent-linear / adjoint driver (FP64, global time stepping, no
-  // limiter), on the solver's geometry and stream
+  // The tangent-linear / adjoint driver (FP64, global or local time
+  // stepping, no limiter), on the solver's geometry and stream
   capi::AdjointRunner &adjoint() {
     if (prec != 0) throw SetupError(GOCFD_ERR_ARG, "the tangent-linear and adjoint steps differentiate the FP64 step");
-    if (lts) throw SetupError(GOCFD_ERR_ARG, "the tangent-linear and adjoint steps use the global time step");
     if (shock) throw SetupError(GOCFD_ERR_ARG, "the tangent-linear and adjoint steps do not cover the PerssonC0 limiter");
     if (!adj) {
       capi::AdjointGeometry geo{Jdet.p, Jinv.p, IInII.p, Nx.p, Ny.p, oohK.p, Interp64.p, code.p};
       adj = std::make_unique<capi::AdjointRunner>(d, L, phys_of(fs, prm), prm.init_type != GOCFD_INIT_SHOCKTUBE,
-                                                  geo, stream);
+                                                  lts, geo, stream);
     }
     return *adj;
   }

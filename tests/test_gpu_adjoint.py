@@ -5,16 +5,19 @@
 * Dot-product test: <s, J v> from jvp equals <J^T s, v> from vjp to 1e-10
   relative, for random s and v over the state and the far-field state, on
   an airfoil (far field + wall), a doubly periodic mesh (periodic shared
-  faces, vortex initial state) and a shock tube (In/Out far states, which stay constant there),
-  every flux and order 1..4.
+  faces, vortex initial state) and a shock tube (In/Out far states, which
+  stay constant there), every flux, orders 0..4, global and local time
+  stepping.
 * Checkpoint schedules (every step, every 2, sqrt, one block; stage records
   kept per block or recomputed per step) give bit-identical gradients.
 * Lift gradients against central differences (h = 1e-6): of the GPU solver
   in Minf, angle of attack and two state directions, and of the CPU oracle
   (the reference's algorithm restated) in Minf and one state direction;
   within 1e-6 relative.
-* Refusals: FP32, LocalTimeStepping, PerssonC0, IVortex boundaries, a
-  window that reaches FinalTime.
+* LocalTimeStepping lift gradients against central differences of the
+  solver.
+* Refusals: FP32, PerssonC0, IVortex boundaries, a window that reaches
+  FinalTime.
 """
 import numpy as np
 import pytest
@@ -52,24 +55,29 @@ def _rel(a, b):
     return float(np.abs(a - b).max() / max(np.abs(b).max(), 1e-300))
 
 
-@pytest.mark.parametrize("N,flux", [(1, "Lax"), (2, "Roe"), (3, "Roe"), (4, "Lax"), (2, "Average"),
-                                    (3, "Roe-ER")])
-def test_tangent_linear_primal_matches_solver(meshes, N, flux):
-    e = _developed(meshes["foil"], _ip(N, flux))
+@pytest.mark.parametrize("N,flux,lts", [(1, "Lax", False), (2, "Roe", False), (3, "Roe", False), (4, "Lax", False),
+                                        (2, "Average", False), (3, "Roe-ER", False), (2, "Roe", True),
+                                        (3, "Lax", True)])
+def test_tangent_linear_primal_matches_solver(meshes, N, flux, lts):
+    e = _developed(meshes["foil"], _ip(N, flux, LocalTimeStepping=lts))
     qn, _ = e.jvp(5, np.zeros_like(e.Q))
     e.step(5)
     assert _rel(qn, e.Q) < 1e-12
 
 
-@pytest.mark.parametrize("mesh,N,flux,init", [
-    ("foil", 1, "Lax", "Freestream"), ("foil", 2, "Roe", "Freestream"), ("foil", 3, "Roe-ER", "Freestream"),
-    ("foil", 4, "Roe", "Freestream"), ("foil", 2, "Average", "Freestream"),
-    ("periodic", 2, "Roe", "IVortex"), ("periodic", 3, "Lax", "IVortex"),
-    ("tube", 2, "Roe", "shocktube"), ("tube", 1, "Lax", "shocktube")])
-def test_dot_product(meshes, mesh, N, flux, init):
-    ip = _ip(N, flux, init, M=0.0 if init == "shocktube" else 0.5, cfl=0.5 if init == "shocktube" else 1.0)
+@pytest.mark.parametrize("mesh,N,flux,init,lts", [
+    ("foil", 1, "Lax", "Freestream", False), ("foil", 2, "Roe", "Freestream", False),
+    ("foil", 3, "Roe-ER", "Freestream", False), ("foil", 4, "Roe", "Freestream", False),
+    ("foil", 2, "Average", "Freestream", False), ("foil", 0, "Roe", "Freestream", False),
+    ("periodic", 2, "Roe", "IVortex", False), ("periodic", 3, "Lax", "IVortex", False),
+    ("tube", 2, "Roe", "shocktube", False), ("tube", 1, "Lax", "shocktube", False),
+    ("foil", 2, "Roe", "Freestream", True), ("foil", 4, "Lax", "Freestream", True),
+    ("periodic", 3, "Roe", "IVortex", True), ("tube", 2, "Roe", "shocktube", True)])
+def test_dot_product(meshes, mesh, N, flux, init, lts):
+    ip = _ip(N, flux, init, M=0.0 if init == "shocktube" else 0.5, cfl=0.5 if init == "shocktube" else 1.0,
+             LocalTimeStepping=lts)
     e = _developed(meshes[mesh], ip)
-    rng = np.random.default_rng(N + 10 * len(mesh))
+    rng = np.random.default_rng(N + 10 * len(mesh) + 100 * lts)
     q = e.Q
     v = rng.standard_normal(q.shape) * np.abs(q)
     w = rng.standard_normal(4)
@@ -133,6 +141,34 @@ def test_lift_gradients_against_solver_differences(meshes):
         assert float(np.sum(g2["dJ_dQ0"] * v)) == pytest.approx(fd, rel=1e-6)
 
 
+def test_lift_gradients_with_local_time_stepping(meshes):
+    """LocalTimeStepping: each element's dt follows the fastest point of its
+    faces (or its previous stage's dt); gradients against central
+    differences of the solver in Minf (h = 1e-6) and a state direction
+    (h = 1e-7). The element maxima make the map piecewise smooth, and in a
+    random state direction some of the face-speed candidates sit within a
+    relative 1e-6 of each other: at h = 1e-6 the one-sided differences then
+    straddle a kink (forward and backward slopes differ by 2e-6 relative),
+    while below that gap the central difference converges to the adjoint
+    (1.4e-7 at h = 1e-7, 3.4e-8 at h = 1e-8)."""
+    mesh, n, h = meshes["foil"], 15, 1e-6
+    ip = _ip(2, LocalTimeStepping=True)
+    e = Euler(ip, mesh, precision="fp64")
+    g = adjoint.lift_gradients(e, n)
+    assert g["J"] == pytest.approx(_lift_after(mesh, ip, n), rel=1e-10)
+    fdM = (_lift_after(mesh, _ip(2, M=0.5 + h, LocalTimeStepping=True), n)
+           - _lift_after(mesh, _ip(2, M=0.5 - h, LocalTimeStepping=True), n)) / (2 * h)
+    assert g["dJ_dMinf"] == pytest.approx(fdM, rel=1e-6)
+    e2 = _developed(mesh, ip, 5)
+    q0 = e2.Q
+    e2.set_time(0.0)
+    g2 = adjoint.lift_gradients(e2, n)
+    v = np.random.default_rng(11).standard_normal(q0.shape) * np.abs(q0)
+    hs = 1e-7
+    fd = (_lift_after(mesh, ip, n, q0 + hs * v) - _lift_after(mesh, ip, n, q0 - hs * v)) / (2 * hs)
+    assert float(np.sum(g2["dJ_dQ0"] * v)) == pytest.approx(fd, rel=1e-6)
+
+
 def test_lift_gradients_against_oracle_differences(meshes):
     """Central differences of the CPU oracle (FP64, the reference's
     algorithm) in Minf and in a state direction."""
@@ -164,7 +200,6 @@ def test_refusals(meshes):
     z = None
     cases = [
         (dict(ip=_ip(2), precision="fp32"), "FP64"),
-        (dict(ip=_ip(2, LocalTimeStepping=True)), "global time step"),
         (dict(ip=_ip(2, Limiter="PerssonC0", Kappa=3.0)), "PerssonC0"),
     ]
     for kw, msg in cases:
