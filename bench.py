@@ -284,7 +284,9 @@ def run_airfoil(mesh, steps, warmup, h=1e-6, ndir=2, seed=3, developed=20):
     fd_seconds = time.perf_counter() - t1
     # tangent-linear dot product at full size: <dJ/dQn, J v> = <J^T dJ/dQn, v>
     v = rng.standard_normal(qd.shape) * np.abs(qd)
+    tj = time.perf_counter()
     qn, jv = ed.jvp(n, v)
+    jvp_seconds = time.perf_counter() - tj
     tlm = float(np.sum(adjoint.lift_state_gradient(ed, qn, a0) * jv))
     adj = float(np.sum(gd["dJ_dQ0"] * v))
     return dict(elapsed=elapsed, stage_ms=stage_ms, K=e.K, NpInt=e.NpInt, J=J, time=t_end, dt=dt, steps=n,
@@ -296,6 +298,7 @@ def run_airfoil(mesh, steps, warmup, h=1e-6, ndir=2, seed=3, developed=20):
                          "tangent_linear_dot_rel": abs(tlm - adj) / abs(tlm),
                          "vjp_seconds": vjp_seconds, "vjp_ms_per_step": vjp_seconds / n * 1e3,
                          "vjp_over_forward_step": (vjp_seconds / n) / (elapsed / steps),
+                         "jvp_ms_per_step": jvp_seconds / n * 1e3,
                          "gradient_seconds_with_primal": grad_seconds, "fd_seconds": fd_seconds,
                          "checkpoint_every": int(math.ceil(math.sqrt(n)))})
 
