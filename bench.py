@@ -512,8 +512,9 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32 (stage states q1..q4 and bulk flux/divergence arithmetic; FP64 geometry, dt "
-                 "reduction, SSP sums and accumulated state q0/R)",
+        "dtype": "f32",
+        "dtype_detail": "f32 stage states q1..q4 and bulk flux/divergence arithmetic; f64 geometry, dt "
+                        "reduction, SSP sums and accumulated state q0/R",
         "data": "synthetic: deterministic jittered-quad isentropic-vortex mesh",
         "config": {"workload": f"isentropic vortex, {K} triangles ({args.n}x{args.n} jittered quads), "
                                f"P=4, Roe, SSP54, CFL 1", "K": K, "NpInt": NpInt,
