@@ -145,7 +145,7 @@ def test_lift_gradients_with_local_time_stepping(meshes):
     """LocalTimeStepping: each element's dt follows the fastest point of its
     faces (or its previous stage's dt); gradients against central
     differences of the solver in Minf (h = 1e-6) and a state direction
-    (h = 1e-7). The element maxima make the map piecewise smooth, and in a
+    (h = 1e-7), and of the CPU oracle in Minf. The element maxima make the map piecewise smooth, and in a
     random state direction some of the face-speed candidates sit within a
     relative 1e-6 of each other: at h = 1e-6 the one-sided differences then
     straddle a kink (forward and backward slopes differ by 2e-6 relative),
@@ -167,6 +167,17 @@ def test_lift_gradients_with_local_time_stepping(meshes):
     hs = 1e-7
     fd = (_lift_after(mesh, ip, n, q0 + hs * v) - _lift_after(mesh, ip, n, q0 - hs * v)) / (2 * hs)
     assert float(np.sum(g2["dJ_dQ0"] * v)) == pytest.approx(fd, rel=1e-6)
+    # and against the CPU oracle's local time stepping (the reference's algorithm)
+
+    def J_oracle(M):
+        o = oracle.OracleSolver(mesh, 2, flux="roe", init="freestream", CFL=1.0, Minf=M, Alpha=2.0,
+                                LocalTimeStepping=True)
+        for _ in range(n):
+            o.step()
+        return adjoint.lift(e, o.state(), 2.0)
+
+    fdo = (J_oracle(0.5 + h) - J_oracle(0.5 - h)) / (2 * h)
+    assert g["dJ_dMinf"] == pytest.approx(fdo, rel=1e-6)
 
 
 def test_lift_gradients_against_oracle_differences(meshes):
