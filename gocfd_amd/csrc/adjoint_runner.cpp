@@ -203,18 +203,6 @@ void AdjointRunner::vjp(const double *q0, double t0, int nsteps, int every, cons
   HIPCHECK(hipMemsetAsync(flag_.p, 0, sizeof(int), s_));
   HIPCHECK(hipMemcpyAsync(time_.p, &t0, sizeof(double), hipMemcpyHostToDevice, s_));
   HIPCHECK(hipMemcpyAsync(ck[0], q0, bytes, hipMemcpyDeviceToDevice, s_));
-  // forward sweep: checkpoints at the block starts
-  const StageRecord scratch = scratch_record();
-  for (int bk = 0; bk < nblocks; ++bk) {
-    const int j0 = bk * every_, cnt = std::min(every_, nsteps - j0);
-    const double *src = ck[bk];
-    for (int i = 0; i < cnt; ++i) {
-      double *dst = i + 1 < cnt ? S[i + 1] : (bk + 1 < nblocks ? ck[bk + 1] : end);
-      forward_step(false, src, dst, time_.p, scratch);
-      src = dst;
-    }
-  }
-  check_flag();
   // The stage records of a whole block are kept when the states they still
   // need fit in a quarter of the free device memory; otherwise each step's
   // record is recomputed from its start just before its transposed stages.
@@ -224,6 +212,23 @@ void AdjointRunner::vjp(const double *q0, double t0, int nsteps, int every, cons
   stored_records_ = (want > have ? (want - have) * bytes : 0) <= free_b / 4;
   // GOCFD_ADJOINT_KEEP_RECORDS=0 forces the per-step recomputation
   if (const char *e = std::getenv("GOCFD_ADJOINT_KEEP_RECORDS")) stored_records_ = stored_records_ && e[0] != '0';
+  // With one block and its records kept, the record sweep is the forward
+  // sweep (with the FinalTime check); otherwise a forward sweep first stores
+  // the checkpoints at the block starts.
+  const bool one_sweep = nblocks == 1 && stored_records_;
+  const StageRecord scratch = scratch_record();
+  if (!one_sweep) {
+    for (int bk = 0; bk < nblocks; ++bk) {
+      const int j0 = bk * every_, cnt = std::min(every_, nsteps - j0);
+      const double *src = ck[bk];
+      for (int i = 0; i < cnt; ++i) {
+        double *dst = i + 1 < cnt ? S[i + 1] : (bk + 1 < nblocks ? ck[bk + 1] : end);
+        forward_step(false, src, dst, time_.p, scratch);
+        src = dst;
+      }
+    }
+    check_flag();
+  }
   std::vector<StageRecord> recs(stored_records_ ? size_t(every_) : 0);
   for (size_t i = 0; i < recs.size(); ++i) {
     StageRecord &r = recs[i];
@@ -274,10 +279,11 @@ void AdjointRunner::vjp(const double *q0, double t0, int nsteps, int every, cons
     starts[0] = ck[bk];
     for (int i = 0; i < cnt; ++i) {
       void *next = i + 1 < cnt ? static_cast<void *>(S[i + 1]) : static_cast<void *>(tmp_.p);
-      if (stored_records_) forward_step(false, starts[i], next, nullptr, recs[i]);
+      if (stored_records_) forward_step(false, starts[i], next, one_sweep ? time_.p : nullptr, recs[i]);
       else if (i + 1 < cnt) forward_step(false, starts[i], next, nullptr, scratch);
       if (i + 1 < cnt) starts[i + 1] = S[i + 1];
     }
+    if (one_sweep) check_flag();
     for (int i = cnt - 1; i >= 0; --i) {
       if (stored_records_) {
         adjoint_step(starts[i], recs[i], b);

@@ -170,7 +170,9 @@ int gocfd_euler2d_get_element_dt(gocfd_euler2d *h, double *DT);
 /* Vector-Jacobian product over nsteps steps: dQ0 = (dQn/dQ)^T dQn_bar and
  * dQinf = (dQn/dQinf)^T dQn_bar (dQinf may be NULL). The reverse sweep
  * recomputes the steps from a checkpoint every checkpoint_every steps (0:
- * ceil(sqrt(nsteps))) and each step's stages from its start. */
+ * ceil(sqrt(nsteps))) and each step's stages from its start; with
+ * checkpoint_every >= nsteps (one block) whose stage records fit in device
+ * memory, the one recording sweep is also the forward sweep. */
 int gocfd_euler2d_vjp(gocfd_euler2d *h, int32_t nsteps, int32_t checkpoint_every, const double *dQn_bar,
                       double *dQ0, double *dQinf);
 /* Jacobian-vector product over nsteps steps (tangent-linear step in dual

@@ -224,5 +224,6 @@ def test_refusals(meshes):
     with pytest.raises(GocfdError, match="IVortex"):
         e.vjp(2, np.zeros_like(e.Q))
     e = Euler(_ip(2, final=1e-4), meshes["foil"])
-    with pytest.raises(GocfdError, match="FinalTime"):
-        e.vjp(200, np.zeros_like(e.Q))
+    for every in (0, 200):  # checkpointed, and one block whose record sweep is the forward sweep
+        with pytest.raises(GocfdError, match="FinalTime"):
+            e.vjp(200, np.zeros_like(e.Q), every)
