@@ -54,6 +54,9 @@ struct AdjArgs {
   // (f Kp + kk) speed read from face wsrc[f Kp + kk] (the edge's owner side)
   int lts;
   const int *wsrc;
+  // unshared face points of real elements (boundary kinds), t = (f Kp + kk) NpEdge + pp
+  const int *bnd;
+  int nbnd;
 };
 
 // Buffers of one launch. Element arrays are element-major [Kp][4][NpInt];

@@ -483,8 +483,7 @@ __device__ __forceinline__ void lax_flux_vjp(double g, const double L[4], const 
 // arithmetic.
 // (Capping its registers for occupancy spills and measured slower.)
 template <int N, int FLUX>
-__global__ void __launch_bounds__(256) k_adj_face(AdjArgs a, const double *U, const double *lamR, double *lamOwn,
-                                                  double *lamFarPts) {
+__global__ void __launch_bounds__(256) k_adj_face(AdjArgs a, const double *U, const double *lamR, double *lamOwn) {
   using Sh = AdjShape<N>;
   constexpr int NpEdge = Sh::NpEdge;
   const size_t t = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
@@ -498,29 +497,28 @@ __global__ void __launch_bounds__(256) k_adj_face(AdjArgs a, const double *U, co
     return;
   }
   const int code = a.code[fi], kind = code & 15;
-  if (kind == FK_Shared && !(code & FC_FLUX_OWNER)) return;  // written by the owner
+  // unshared points: k_adj_face_bnd; a non-owner's point: written by its owner
+  if (kind != FK_Shared || !(code & FC_FLUX_OWNER)) return;
   double own[4];
   trace_at<N>(a, U, kk, f * NpEdge + pp, own);
-  if (kind == FK_Shared) {
-    const int nb = a.nbr_elem[fi], nf = a.nbr_face[fi], rp = NpEdge - 1 - pp;
-    const size_t fn = size_t(nf) * a.Kp + nb;
-    double oth[4], lfo[4], lfn[4], lF[4];
-    trace_at<N>(a, U, nb, nf * NpEdge + rp, oth);
-    edge_cotangent<N>(a, lamR, kk, f * NpEdge + pp, lfo);
-    edge_cotangent<N>(a, lamR, nb, nf * NpEdge + rp, lfn);
-    const double wo = a.IInII[fi], wn = a.IInII[fn];
-    for (int c = 0; c < 4; ++c) lF[c] = wo * lfo[c] - wn * lfn[c];
-    double *ln = lamOwn + 4 * (fn * NpEdge + rp);
-    if constexpr (FLUX == 1 || FLUX == 2) {
-      double bL[4] = {0., 0., 0., 0.}, bR[4] = {0., 0., 0., 0.};
-      if constexpr (FLUX == 2) roe_flux_vjp(a.ph.gamma, own, oth, a.Nx[fi], a.Ny[fi], lF, bL, bR);
-      else lax_flux_vjp(a.ph.gamma, own, oth, a.Nx[fi], a.Ny[fi], lF, bL, bR);
-      for (int c = 0; c < 4; ++c) {
-        lo[c] = bL[c];
-        ln[c] = bR[c];
-      }
-      return;
+  const int nb = a.nbr_elem[fi], nf = a.nbr_face[fi], rp = NpEdge - 1 - pp;
+  const size_t fn = size_t(nf) * a.Kp + nb;
+  double oth[4], lfo[4], lfn[4], lF[4];
+  trace_at<N>(a, U, nb, nf * NpEdge + rp, oth);
+  edge_cotangent<N>(a, lamR, kk, f * NpEdge + pp, lfo);
+  edge_cotangent<N>(a, lamR, nb, nf * NpEdge + rp, lfn);
+  const double wo = a.IInII[fi], wn = a.IInII[fn];
+  for (int c = 0; c < 4; ++c) lF[c] = wo * lfo[c] - wn * lfn[c];
+  double *ln = lamOwn + 4 * (fn * NpEdge + rp);
+  if constexpr (FLUX == 1 || FLUX == 2) {
+    double bL[4] = {0., 0., 0., 0.}, bR[4] = {0., 0., 0., 0.};
+    if constexpr (FLUX == 2) roe_flux_vjp(a.ph.gamma, own, oth, a.Nx[fi], a.Ny[fi], lF, bL, bR);
+    else lax_flux_vjp(a.ph.gamma, own, oth, a.Nx[fi], a.Ny[fi], lF, bL, bR);
+    for (int c = 0; c < 4; ++c) {
+      lo[c] = bL[c];
+      ln[c] = bR[c];
     }
+  } else {
     const Dual g(a.ph.gamma), nx(a.Nx[fi]), ny(a.Ny[fi]);
 #pragma unroll 1
     for (int j = 0; j < 8; ++j) {
@@ -536,29 +534,42 @@ __global__ void __launch_bounds__(256) k_adj_face(AdjArgs a, const double *U, co
       if (side == 0) lo[d] = sum;
       else ln[d] = sum;
     }
-    return;
   }
-  double lf[4], qfar[4];
+}
+
+// The unshared face points (boundary kinds) of real elements, from a list:
+// the boundary flux's transposed Jacobians with respect to the own trace
+// and the far state, in dual arithmetic, one thread per (point, input
+// direction): 0..3 the own trace, 4..7 the far state
+template <int N>
+__global__ void __launch_bounds__(256) k_adj_face_bnd(AdjArgs a, const double *U, const double *lamR,
+                                                      double *lamOwn, double *lamFarPts) {
+  using Sh = AdjShape<N>;
+  constexpr int NpEdge = Sh::NpEdge;
+  const int gid = blockIdx.x * blockDim.x + threadIdx.x;
+  if (gid >= 8 * a.nbnd) return;
+  const int ib = gid >> 3, j = gid & 7, side = j >> 2, d = j & 3;
+  const size_t t = size_t(a.bnd[ib]);
+  int f, kk, pp;
+  size_t fi;
+  face_point<N>(a, t, f, kk, pp, fi);
+  const int code = a.code[fi], kind = code & 15;
+  const bool far = kind == FK_Far && ((code >> 8) == 0 || a.io_tied);
+  if (side == 1 && !far) return;
+  double own[4], lf[4], qfar[4];
+  trace_at<N>(a, U, kk, f * NpEdge + pp, own);
   edge_cotangent<N>(a, lamR, kk, f * NpEdge + pp, lf);
   far_state(a, qfar);
-  const bool far = kind == FK_Far && ((code >> 8) == 0 || a.io_tied);
-  double *lq = lamFarPts + 4 * t;
-  // input directions: 0..3 the own trace, 4..7 the far state
-#pragma unroll 1
-  for (int j = 0; j < 8; ++j) {
-    const int side = j >> 2, d = j & 3;
-    if (side == 1 && !far) break;
-    Dual o[4], q[4], F[4];
-    for (int c = 0; c < 4; ++c) {
-      o[c] = Dual(own[c], side == 0 && c == d ? 1. : 0.);
-      q[c] = Dual(qfar[c], side == 1 && c == d ? 1. : 0.);
-    }
-    boundary_value(a, code, fi, o, q, F);
-    double sum = 0.;
-    for (int c = 0; c < 4; ++c) sum += lf[c] * F[c].d;
-    if (side == 0) lo[d] = sum;
-    else lq[d] += sum;
+  Dual o[4], q[4], F[4];
+  for (int c = 0; c < 4; ++c) {
+    o[c] = Dual(own[c], side == 0 && c == d ? 1. : 0.);
+    q[c] = Dual(qfar[c], side == 1 && c == d ? 1. : 0.);
   }
+  boundary_value(a, code, fi, o, q, F);
+  double sum = 0.;
+  for (int c = 0; c < 4; ++c) sum += lf[c] * F[c].d;
+  if (side == 0) lamOwn[4 * t + d] = sum;
+  else lamFarPts[4 * t + d] += sum;
 }
 
 // Per (element, solution point j): column j of the FluxEdgeInterp transpose
@@ -829,7 +840,7 @@ hipError_t adj_launch_n(int op, const AdjArgs &a, const AdjBufs &b, hipStream_t 
     case ADJ_ADJ_FACE:
 #define GOCFD_ADJ_FACE(FL)                                                                                 \
   hipLaunchKernelGGL((k_adj_face<N, FL>), dim3(adj_blocks(nfp)), dim3(256), 0, s, a,                        \
-                     static_cast<const double *>(b.Us), b.lamR, b.lamOwn, b.lamFarPts);
+                     static_cast<const double *>(b.Us), b.lamR, b.lamOwn);
       switch (a.ph.flux) {
         case 0: GOCFD_ADJ_FACE(0) break;
         case 1: GOCFD_ADJ_FACE(1) break;
@@ -837,6 +848,9 @@ hipError_t adj_launch_n(int op, const AdjArgs &a, const AdjBufs &b, hipStream_t 
         default: GOCFD_ADJ_FACE(3) break;
       }
 #undef GOCFD_ADJ_FACE
+      if (a.nbnd > 0)
+        hipLaunchKernelGGL((k_adj_face_bnd<N>), dim3(adj_blocks(8 * size_t(a.nbnd))), dim3(256), 0, s, a,
+                           static_cast<const double *>(b.Us), b.lamR, b.lamOwn, b.lamFarPts);
       break;
     case ADJ_ADJ_ELEM:
       hipLaunchKernelGGL((k_adj_elem<N>), dim3(adj_blocks(size_t(a.K) * Sh::NpInt)), dim3(256), 0, s, a,

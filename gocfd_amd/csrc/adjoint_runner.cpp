@@ -36,6 +36,18 @@ AdjointRunner::AdjointRunner(const Discretisation &d, const DeviceLayout &L, con
   }
   n_ = size_t(L.Kp) * 4 * NpInt;
   nfp_ = size_t(3) * L.Kp * NpEdge;
+  {
+    std::vector<int> bnd;
+    for (int f = 0; f < 3; ++f)
+      for (int kk = 0; kk < L.K; ++kk) {
+        const size_t fi = size_t(f) * L.Kp + kk;
+        if ((L.face_code[fi] & 15) == FK_Shared) continue;
+        for (int pp = 0; pp < NpEdge; ++pp) bnd.push_back(int(fi * NpEdge + pp));
+      }
+    bnd_.upload(bnd);
+    a_.bnd = bnd_.p;
+    a_.nbnd = int(bnd.size());
+  }
   for (int s = 0; s < 5; ++s) {
     U_[s].alloc(n_);
     R_[s].alloc(n_);

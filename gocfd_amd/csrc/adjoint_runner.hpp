@@ -75,7 +75,7 @@ class AdjointRunner {
   DevBuf<double> lamU0_, lamQ_[5], lamR_, lamR3_, lamOwn_, lamFarPts_, lamFarW_, lam_dt_, far_d_;
   DevBuf<dev::ad_storage> dualA_, dualB_;
   SmallRecord scratch_small_;
-  DevBuf<int> wsrc_;
+  DevBuf<int> wsrc_, bnd_;
   DevBuf<double> carry_, lamSpeed_;
   dev::AdjArgs a_{};
   hipStream_t s_;
