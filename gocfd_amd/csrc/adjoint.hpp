@@ -1,5 +1,6 @@
 // Tangent-linear and discrete-adjoint kernels of the FP64 SSP54 step
-// (global or local time stepping, no limiter): interface between the host driver
+// (global or local time stepping; PerssonC0 while its sensor is quiet):
+// interface between the host driver
 // (adjoint_runner.cpp) and the per-order kernel translation units.
 //
 // The differentiated map is one SSP54 step of the reference (rkAdvance,
@@ -57,6 +58,16 @@ struct AdjArgs {
   // unshared face points of real elements (boundary kinds), t = (f Kp + kk) NpEdge + pp
   const int *bnd;
   int nbnd;
+  // PerssonC0 while its sensor is quiet (sigma = 0 everywhere): the modal
+  // filter F = V diag(1, mf_1, mf_2, ...) Vinv [NpInt][NpInt] of the RHS at
+  // every stage and of the stage state at stage 2 (limitAndFilterSolution
+  // dissipation.go:606-622, euler.go:603-608); the sensor's matrices
+  // P, M, D [NpInt][NpInt] each and its threshold S0 - Kappa (sigma > 0 from
+  // there on, UpdateShockFinderSigma dissipation.go:491-518)
+  int shock;
+  const double *filt;
+  const double *sens;
+  double sens_left;
 };
 
 // Buffers of one launch. Element arrays are element-major [Kp][4][NpInt];
@@ -114,6 +125,9 @@ enum AdjOp : int {
   ADJ_FWD_LTS,       // dtk, srck, wk <- local dt of each element from SP and dtk_prev
   ADJ_ADJ_SSP_LTS,   // the SSP transpose with element dts; lamSpeed, carry <- the dt terms
   ADJ_ADJ_LTS_WAVE,  // lamOwn, lamFarPts += the face-point speed cotangents (then cleared)
+  ADJ_SENSOR,        // flag |= 4 where the shock sensor of Us reaches its ramp
+  ADJ_FILTER,        // out <- F out per element and variable (in place; dual or double)
+  ADJ_FILTER_T,      // lam <- F^T lam per element and variable (in place)
   ADJ_PACK,          // out (dual) <- (pack_v, pack_d)
   ADJ_UNPACK,        // (unpack_v, unpack_d) <- Us (dual)
 };

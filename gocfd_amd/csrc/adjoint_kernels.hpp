@@ -224,9 +224,8 @@ __global__ void __launch_bounds__(256) k_adj_fwd_rhs(AdjArgs a, int stage, const
     }
   }
   __syncthreads();
-  if (!live) return;
   T acc[4] = {T(0.), T(0.), T(0.), T(0.)};
-  if (real) {
+  if (live && real) {
     const double *D = a.DivInt + i * NpFlux;
     for (int j = 0; j < NpInt; ++j) {
 #pragma unroll
@@ -242,6 +241,25 @@ __global__ void __launch_bounds__(256) k_adj_fwd_rhs(AdjArgs a, int stage, const
 #pragma unroll
     for (int c = 0; c < 4; ++c) acc[c] = acc[c] * m;
   }
+  if (a.shock) {
+    // the modal filter of the RHS: row i of F times the element's rows
+    __syncthreads();  // every thread has read sF
+    if (live) {
+#pragma unroll
+      for (int c = 0; c < 4; ++c) sF[el][c][i] = acc[c];
+    }
+    __syncthreads();
+    if (live && real) {
+      const double *Fi = a.filt + i * NpInt;
+#pragma unroll
+      for (int c = 0; c < 4; ++c) {
+        T t = T(0.);
+        for (int k = 0; k < NpInt; ++k) t += Fi[k] * sF[el][c][k];
+        acc[c] = t;
+      }
+    }
+  }
+  if (!live) return;
   const T dt = a.lts ? dtk[kk] : dts[stage];  // the element's dt with local time stepping
 #pragma unroll
   for (int c = 0; c < 4; ++c) {
@@ -296,6 +314,50 @@ __global__ void __launch_bounds__(256) k_adj_fwd_lts(AdjArgs a, int stage, const
   dtk[kk] = a.ph.cfl / w;
   srck[kk] = src;
   wk[kk] = ad::val(w);
+}
+
+// PerssonC0 sensor of each element's density (UpdateSeMoment
+// DG2D/dfr_shock_capturing.go:142-166): Se = log10(sum DQ.(P r) / sum r.(M r))
+// with DQ = D r; flags the stage (flag |= 4) where Se reaches S0 - Kappa,
+// the start of the sigma ramp, beyond which the quiet-sensor derivative
+// does not hold.
+template <int N, class T>
+__global__ void __launch_bounds__(256) k_adj_sensor(AdjArgs a, const T *U, int *flag) {
+  constexpr int NpInt = AdjShape<N>::NpInt;
+  const int kk = blockIdx.x * blockDim.x + threadIdx.x;
+  if (kk >= a.K) return;
+  const T *rho = U + size_t(kk) * AdjShape<N>::NV;
+  const double *Pm = a.sens, *Mm = Pm + NpInt * NpInt, *Dm = Mm + NpInt * NpInt;
+  double num = 0., den = 0.;
+  for (int i = 0; i < NpInt; ++i) {
+    double X = 0., Y = 0., DQ = 0.;
+    for (int j = 0; j < NpInt; ++j) {
+      const double r = ad::val(rho[j]);
+      X += Pm[i * NpInt + j] * r;
+      Y += Mm[i * NpInt + j] * r;
+      DQ += Dm[i * NpInt + j] * r;
+    }
+    num += DQ * X;
+    den += ad::val(rho[i]) * Y;
+  }
+  const double se = log10(num / den);
+  if (se >= a.sens_left) atomicOr(flag, 4);
+}
+
+// X <- F X (or F^T X) for each element and variable, in place
+template <int N, class T>
+__global__ void __launch_bounds__(256) k_adj_filter(AdjArgs a, T *X, int transpose) {
+  constexpr int NpInt = AdjShape<N>::NpInt;
+  const size_t gid = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (gid >= size_t(a.K) * 4) return;
+  T *x = X + gid * NpInt;  // element gid / 4, variable gid % 4
+  T u[NpInt];
+  for (int i = 0; i < NpInt; ++i) u[i] = x[i];
+  for (int i = 0; i < NpInt; ++i) {
+    T t = T(0.);
+    for (int k = 0; k < NpInt; ++k) t += (transpose ? a.filt[k * NpInt + i] : a.filt[i * NpInt + k]) * u[k];
+    x[i] = t;
+  }
 }
 
 // ---- adjoint ----
@@ -836,6 +898,25 @@ hipError_t adj_launch_n(int op, const AdjArgs &a, const AdjBufs &b, hipStream_t 
     case ADJ_ADJ_LTS_WAVE:
       hipLaunchKernelGGL((k_adj_lts_wave<N>), dim3(adj_blocks(nfp)), dim3(256), 0, s, a,
                          static_cast<const double *>(b.Us), b.lamSpeed, b.lamOwn, b.lamFarPts);
+      break;
+    case ADJ_SENSOR:
+      if (b.dual)
+        hipLaunchKernelGGL((k_adj_sensor<N, Dual>), dim3(adj_blocks(a.K)), dim3(256), 0, s, a,
+                           static_cast<const Dual *>(b.Us), b.flag);
+      else
+        hipLaunchKernelGGL((k_adj_sensor<N, double>), dim3(adj_blocks(a.K)), dim3(256), 0, s, a,
+                           static_cast<const double *>(b.Us), b.flag);
+      break;
+    case ADJ_FILTER:
+      if (b.dual)
+        hipLaunchKernelGGL((k_adj_filter<N, Dual>), dim3(adj_blocks(size_t(a.K) * 4)), dim3(256), 0, s, a,
+                           static_cast<Dual *>(b.out), 0);
+      else
+        hipLaunchKernelGGL((k_adj_filter<N, double>), dim3(adj_blocks(size_t(a.K) * 4)), dim3(256), 0, s, a,
+                           static_cast<double *>(b.out), 0);
+      break;
+    case ADJ_FILTER_T:
+      hipLaunchKernelGGL((k_adj_filter<N, double>), dim3(adj_blocks(size_t(a.K) * 4)), dim3(256), 0, s, a, b.lam, 1);
       break;
     case ADJ_ADJ_FACE:
 #define GOCFD_ADJ_FACE(FL)                                                                                 \

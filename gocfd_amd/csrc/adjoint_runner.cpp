@@ -9,7 +9,7 @@ namespace gocfd {
 namespace capi {
 
 AdjointRunner::AdjointRunner(const Discretisation &d, const DeviceLayout &L, const dev::Phys &ph, bool io_tied,
-                             bool lts, const AdjointGeometry &geo, hipStream_t stream)
+                             bool lts, bool shock, double kappa, const AdjointGeometry &geo, hipStream_t stream)
     : s_(stream) {
   if (L.nbfaces > 0)
     throw SetupError(GOCFD_ERR_ARG,
@@ -30,6 +30,26 @@ AdjointRunner::AdjointRunner(const Discretisation &d, const DeviceLayout &L, con
   a_.ph = ph;
   a_.io_tied = io_tied ? 1 : 0;
   a_.lts = lts ? 1 : 0;
+  a_.shock = shock ? 1 : 0;
+  if (shock) {
+    // the shock operator block: P, Mass, D, V, Vinv [Np^2] each, then mf [Np]
+    const std::vector<double> ops = build_shock_operators(d.ops);
+    const size_t np = size_t(NpInt), n2 = np * np;
+    const double *V = ops.data() + 3 * n2, *Vi = V + n2, *mf = Vi + n2;
+    std::vector<double> F(n2, 0.0);
+    for (size_t i = 0; i < np; ++i)
+      for (size_t k = 0; k < np; ++k) {
+        double acc = 0.;
+        for (size_t m = 0; m < np; ++m) acc += V[i * np + m] * (m == 0 ? 1.0 : mf[m]) * Vi[m * np + k];
+        F[i * np + k] = acc;
+      }
+    filt_.upload(F);
+    sens_.upload(std::vector<double>(ops.begin(), ops.begin() + 3 * n2));
+    a_.filt = filt_.p;
+    a_.sens = sens_.p;
+    const double np1 = double(d.N + 1);
+    a_.sens_left = 4. / (np1 * np1 * np1 * np1) - kappa;
+  }
   if (lts) {
     wsrc_.upload(L.wsrc);
     a_.wsrc = wsrc_.p;
@@ -142,6 +162,15 @@ void AdjointRunner::forward_step(bool dual, const void *Ustart, void *Uend, doub
     const void *Us = s == 0 ? Ustart : static_cast<const void *>(rec.U[s]);
     b.stage = s;
     b.Us = Us;
+    if (a_.shock) {
+      // the sensor on the stage input (checked in the sweeps that check the
+      // time), then the stage-2 state filter in place
+      if (time) launch(dev::ADJ_SENSOR, b);
+      if (s == 2) {
+        b.out = rec.U[2];
+        launch(dev::ADJ_FILTER, b);
+      }
+    }
     b.FE = FE_.p;
     b.R = rec.R[s];
     b.SP = SP_.p;
@@ -179,12 +208,22 @@ void AdjointRunner::adjoint_step(const double *Ustart, const StageRecord &rec, d
     b.srck = rec.srck[s];
     b.wk = rec.wk[s];
     launch(a_.lts ? dev::ADJ_ADJ_SSP_LTS : dev::ADJ_ADJ_SSP, b);
+    if (a_.shock) {  // the RHS filter, transposed
+      dev::AdjBufs bf = b;
+      bf.lam = b.lamR;
+      launch(dev::ADJ_FILTER_T, bf);
+    }
     b.Us = s == 0 ? static_cast<const void *>(Ustart) : static_cast<const void *>(rec.U[s]);
     b.target = s == 0 ? b.lamU0 : b.lamQ[s];
     launch(dev::ADJ_ADJ_FACE, b);
     if (a_.lts) launch(dev::ADJ_ADJ_LTS_WAVE, b);
     launch(dev::ADJ_ADJ_ELEM, b);
     if (!a_.lts) launch(dev::ADJ_ADJ_WAVE, b);
+    if (a_.shock && s == 2) {  // the stage-2 state filter, transposed
+      dev::AdjBufs bf = b;
+      bf.lam = b.lamQ[2];
+      launch(dev::ADJ_FILTER_T, bf);
+    }
   }
 }
 
@@ -197,6 +236,10 @@ void AdjointRunner::check_flag() {
                      "the steps reach FinalTime (a stage dt would be clamped); the derivative of the clamp is not "
                      "formed");
   if (f & 2) throw SetupError(GOCFD_ERR_ARG, "the mesh has no face that bounds the time step");
+  if (f & 4)
+    throw SetupError(GOCFD_ERR_ARG,
+                     "the PerssonC0 shock sensor is active in the window (an element's Se reaches S0 - Kappa); "
+                     "the derivative through the sensor and the artificial dissipation is not formed");
 }
 
 void AdjointRunner::vjp(const double *q0, double t0, int nsteps, int every, const double *lamN, double *lam0,

@@ -27,8 +27,10 @@ struct AdjointGeometry {
 
 class AdjointRunner {
  public:
+  // shock: PerssonC0 with this Kappa, differentiated while its sensor is
+  // quiet (sigma = 0 everywhere; a window where it is not is refused)
   AdjointRunner(const Discretisation &d, const DeviceLayout &L, const dev::Phys &ph, bool io_tied, bool lts,
-                const AdjointGeometry &geo, hipStream_t stream);
+                bool shock, double kappa, const AdjointGeometry &geo, hipStream_t stream);
   // lam0 = (dq_n/dq_0)^T lamN and far = (dq_n/dqinf)^T lamN for nsteps steps
   // from q0 (device, element-major FP64) at time t0; every = 0 picks
   // ceil(sqrt(nsteps)) steps per checkpoint
@@ -76,6 +78,7 @@ class AdjointRunner {
   DevBuf<dev::ad_storage> dualA_, dualB_;
   SmallRecord scratch_small_;
   DevBuf<int> wsrc_, bnd_;
+  DevBuf<double> filt_, sens_;
   DevBuf<double> carry_, lamSpeed_;
   dev::AdjArgs a_{};
   hipStream_t s_;

@@ -132,14 +132,17 @@ struct gocfd_euler2d {
   }
 
   // The tangent-linear / adjoint driver (FP64, global or local time
-  // stepping, no limiter), on the solver's geometry and stream
+  // stepping; PerssonC0 with the global time step while its sensor is
+  // quiet), on the solver's geometry and stream
   capi::AdjointRunner &adjoint() {
     if (prec != 0) throw SetupError(GOCFD_ERR_ARG, "the tangent-linear and adjoint steps differentiate the FP64 step");
-    if (shock) throw SetupError(GOCFD_ERR_ARG, "the tangent-linear and adjoint steps do not cover the PerssonC0 limiter");
+    if (shock && lts)
+      throw SetupError(GOCFD_ERR_ARG,
+                       "the tangent-linear and adjoint steps cover the PerssonC0 limiter with the global time step only");
     if (!adj) {
       capi::AdjointGeometry geo{Jdet.p, Jinv.p, IInII.p, Nx.p, Ny.p, oohK.p, Interp64.p, code.p};
       adj = std::make_unique<capi::AdjointRunner>(d, L, phys_of(fs, prm), prm.init_type != GOCFD_INIT_SHOCKTUBE,
-                                                  lts, geo, stream);
+                                                  lts, shock, kappa, geo, stream);
     }
     return *adj;
   }

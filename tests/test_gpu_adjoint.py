@@ -16,8 +16,10 @@
   within 1e-6 relative.
 * LocalTimeStepping lift gradients against central differences of the
   solver.
-* Refusals: FP32, PerssonC0, IVortex boundaries, a window that reaches
-  FinalTime.
+* PerssonC0 with a quiet sensor (its modal filters differentiated):
+  primal, dot product and lift gradients against the solver.
+* Refusals: FP32, PerssonC0 with local time stepping or with an active
+  sensor, IVortex boundaries, a window that reaches FinalTime.
 """
 import numpy as np
 import pytest
@@ -180,6 +182,40 @@ def test_lift_gradients_with_local_time_stepping(meshes):
     assert g["dJ_dMinf"] == pytest.approx(fdo, rel=1e-6)
 
 
+def test_perssonc0_with_a_quiet_sensor(meshes):
+    """PerssonC0 (Kappa 3) on the airfoil: the sensor stays below S0 - Kappa
+    (max Se about -7.6), so sigma and the dissipation are zero and the
+    limiter acts through its modal filters of the RHS (every stage) and of
+    the stage-2 state. The dual forward's primal equals the solver's steps,
+    the dot-product test holds, and the lift gradients match central
+    differences of the solver (h = 1e-6)."""
+    mesh, n, h = meshes["foil"], 12, 1e-6
+
+    def ipk(M=0.5):
+        return _ip(2, M=M, Limiter="PerssonC0", Kappa=3.0)
+
+    e = _developed(mesh, ipk(), 3)
+    q = e.Q
+    qn, _ = e.jvp(5, np.zeros_like(q))
+    ref = Euler(ipk(), mesh, precision="fp64")
+    ref.step(3)
+    ref.step(5)
+    assert _rel(qn, ref.Q) < 1e-12
+    rng = np.random.default_rng(21)
+    v, w, sbar = rng.standard_normal(q.shape) * np.abs(q), rng.standard_normal(4), rng.standard_normal(q.shape)
+    _, jv = e.jvp(4, v, w)
+    d0, dfar = e.vjp(4, sbar)
+    lhs, rhs = float(np.sum(sbar * jv)), float(np.sum(d0 * v) + dfar @ w)
+    assert abs(lhs - rhs) <= 1e-10 * abs(lhs)
+    g = adjoint.lift_gradients(Euler(ipk(), mesh, precision="fp64"), n)
+    fdM = (_lift_after(mesh, ipk(0.5 + h), n) - _lift_after(mesh, ipk(0.5 - h), n)) / (2 * h)
+    assert g["dJ_dMinf"] == pytest.approx(fdM, rel=1e-6)
+    e.set_time(0.0)
+    g2 = adjoint.lift_gradients(e, n)
+    fd = (_lift_after(mesh, ipk(), n, q + h * v) - _lift_after(mesh, ipk(), n, q - h * v)) / (2 * h)
+    assert float(np.sum(g2["dJ_dQ0"] * v)) == pytest.approx(fd, rel=1e-6)
+
+
 def test_lift_gradients_against_oracle_differences(meshes):
     """Central differences of the CPU oracle (FP64, the reference's
     algorithm) in Minf and in a state direction."""
@@ -211,7 +247,7 @@ def test_refusals(meshes):
     z = None
     cases = [
         (dict(ip=_ip(2), precision="fp32"), "FP64"),
-        (dict(ip=_ip(2, Limiter="PerssonC0", Kappa=3.0)), "PerssonC0"),
+        (dict(ip=_ip(2, Limiter="PerssonC0", Kappa=3.0, LocalTimeStepping=True)), "global time step only"),
     ]
     for kw, msg in cases:
         e = Euler(kw["ip"], meshes["foil"], precision=kw.get("precision", "fp64"))
@@ -220,6 +256,10 @@ def test_refusals(meshes):
             e.vjp(2, z)
         with pytest.raises(GocfdError, match=msg):
             e.jvp(2, z)
+    # PerssonC0 with Kappa 10: the sensor flags elements in the window
+    e = Euler(_ip(2, Limiter="PerssonC0", Kappa=10.0), meshes["foil"])
+    with pytest.raises(GocfdError, match="sensor is active"):
+        e.vjp(25, np.zeros_like(e.Q))
     e = Euler(_ip(2, init="IVortex"), meshes["foil"])  # its wall edges take the vortex state
     with pytest.raises(GocfdError, match="IVortex"):
         e.vjp(2, np.zeros_like(e.Q))
