@@ -476,6 +476,37 @@ __device__ __forceinline__ void roe_flux_vjp(double g, const double L[4], const 
   bR[3] += b_pR * GM1 + bR3;
 }
 
+// Reverse-mode derivative of flux_xy (FluxCalcBase, fluxes.go:76-87), by
+// hand: adds (dFx/dq)^T bFx + (dFy/dq)^T bFy to bq, with u = rhoU/rho,
+// v = rhoV/rho, p = (g-1)(E - (rhoU^2 + rhoV^2)/(2 rho)):
+// Fx = (rhoU, rhoU u + p, rhoU v, u (E + p)), Fy = (rhoV, rhoV u, rhoV v + p, v (E + p)).
+__device__ __forceinline__ void flux_xy_vjp(double g, const double q[4], const double bFx[4], const double bFy[4],
+                                            double bq[4]) {
+  const double GM1 = g - 1.;
+  const double r = q[0], u = q[1] / r, v = q[2] / r, E = q[3];
+  const double p = GM1 * (E - 0.5 * r * (u * u + v * v)), H = E + p;
+  // cotangents of u, v, p, and the direct rhoU, rhoV, E terms
+  double b_u = bFx[1] * q[1] + bFx[3] * H + bFy[1] * q[2];
+  double b_v = bFx[2] * q[1] + bFy[2] * q[2] + bFy[3] * H;
+  const double b_p = bFx[1] + bFy[2] + bFx[3] * u + bFy[3] * v;
+  double b_ru = bFx[0] + bFx[1] * u + bFx[2] * v;
+  double b_rv = bFy[0] + bFy[1] * u + bFy[2] * v;
+  double b_E = bFx[3] * u + bFy[3] * v;
+  // p
+  b_E += GM1 * b_p;
+  b_ru += -GM1 * u * b_p;
+  b_rv += -GM1 * v * b_p;
+  double b_r = GM1 * 0.5 * (u * u + v * v) * b_p;
+  // u = rhoU / rho, v = rhoV / rho
+  b_ru += b_u / r;
+  b_rv += b_v / r;
+  b_r += -(b_u * u + b_v * v) / r;
+  bq[0] += b_r;
+  bq[1] += b_ru;
+  bq[2] += b_rv;
+  bq[3] += b_E;
+}
+
 // Reverse-mode derivative of lax_flux (LaxFlux, fluxes.go:161-190), by
 // hand: adds (dF*/dL)^T bF to bL and (dF*/dR)^T bF to bR. The maximum of
 // the two sides' |V| + c takes the left side's branch on a tie, as the
@@ -572,10 +603,22 @@ __global__ void __launch_bounds__(256) k_adj_face(AdjArgs a, const double *U, co
   const double wo = a.IInII[fi], wn = a.IInII[fn];
   for (int c = 0; c < 4; ++c) lF[c] = wo * lfo[c] - wn * lfn[c];
   double *ln = lamOwn + 4 * (fn * NpEdge + rp);
-  if constexpr (FLUX == 1 || FLUX == 2) {
+  if constexpr (FLUX <= 2) {
     double bL[4] = {0., 0., 0., 0.}, bR[4] = {0., 0., 0., 0.};
-    if constexpr (FLUX == 2) roe_flux_vjp(a.ph.gamma, own, oth, a.Nx[fi], a.Ny[fi], lF, bL, bR);
-    else lax_flux_vjp(a.ph.gamma, own, oth, a.Nx[fi], a.Ny[fi], lF, bL, bR);
+    if constexpr (FLUX == 2) {
+      roe_flux_vjp(a.ph.gamma, own, oth, a.Nx[fi], a.Ny[fi], lF, bL, bR);
+    } else if constexpr (FLUX == 1) {
+      lax_flux_vjp(a.ph.gamma, own, oth, a.Nx[fi], a.Ny[fi], lF, bL, bR);
+    } else {
+      // AvgFlux: 0.5 (n . F(L) + n . F(R)) (fluxes.go:135-159)
+      double bx[4], by[4];
+      for (int c = 0; c < 4; ++c) {
+        bx[c] = 0.5 * a.Nx[fi] * lF[c];
+        by[c] = 0.5 * a.Ny[fi] * lF[c];
+      }
+      flux_xy_vjp(a.ph.gamma, own, bx, by, bL);
+      flux_xy_vjp(a.ph.gamma, oth, bx, by, bR);
+    }
     for (int c = 0; c < 4; ++c) {
       lo[c] = bL[c];
       ln[c] = bR[c];
