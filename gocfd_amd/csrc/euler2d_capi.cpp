@@ -132,13 +132,21 @@ struct gocfd_euler2d {
   }
 
   // The tangent-linear / adjoint driver (FP64, global or local time
-  // stepping; PerssonC0 with the global time step while its sensor is
-  // quiet), on the solver's geometry and stream
+  // stepping; PerssonC0 while its sensor is quiet), on the solver's
+  // geometry and stream
   capi::AdjointRunner &adjoint() {
     if (prec != 0) throw SetupError(GOCFD_ERR_ARG, "the tangent-linear and adjoint steps differentiate the FP64 step");
-    if (shock && lts)
-      throw SetupError(GOCFD_ERR_ARG,
-                       "the tangent-linear and adjoint steps cover the PerssonC0 limiter with the global time step only");
+    if (shock && lts) {
+      // the element viscous dt (DTv, never reset in the reference) must be
+      // unset: it stays zero while the sensor has been quiet since the start
+      std::vector<double> dtv(DTv.n);
+      HIPCHECK(hipMemcpy(dtv.data(), DTv.p, dtv.size() * sizeof(double), hipMemcpyDeviceToHost));
+      for (double x : dtv)
+        if (x != 0.)
+          throw SetupError(GOCFD_ERR_ARG,
+                           "LocalTimeStepping with PerssonC0: the element viscous time step is already set (the "
+                           "sensor was active before the window); its derivative is not formed");
+    }
     if (!adj) {
       capi::AdjointGeometry geo{Jdet.p, Jinv.p, IInII.p, Nx.p, Ny.p, oohK.p, Interp64.p, code.p};
       adj = std::make_unique<capi::AdjointRunner>(d, L, phys_of(fs, prm), prm.init_type != GOCFD_INIT_SHOCKTUBE,
@@ -589,8 +597,8 @@ int gocfd_euler2d_vjp(gocfd_euler2d *h, int32_t nsteps, int32_t checkpoint_every
   NEED_DEVICE(h);
   if (nsteps < 1 || checkpoint_every < 0) return fail(GOCFD_ERR_ARG, "nsteps must be positive, checkpoint_every >= 0");
   HIPCHECK(hipSetDevice(h->device));
-  capi::AdjointRunner &r = h->adjoint();
   HIPCHECK(hipStreamSynchronize(h->stream));
+  capi::AdjointRunner &r = h->adjoint();
   const double t0 = h->current_time();
   const size_t tot = size_t(h->L.Kp) * h->NV;
   DevBuf<double> lamN, lam0;
@@ -612,8 +620,8 @@ int gocfd_euler2d_jvp(gocfd_euler2d *h, int32_t nsteps, const double *dQ0, const
   NEED_DEVICE(h);
   if (nsteps < 1) return fail(GOCFD_ERR_ARG, "nsteps must be positive");
   HIPCHECK(hipSetDevice(h->device));
-  capi::AdjointRunner &r = h->adjoint();
   HIPCHECK(hipStreamSynchronize(h->stream));
+  capi::AdjointRunner &r = h->adjoint();
   const double t0 = h->current_time();
   const size_t tot = size_t(h->L.Kp) * h->NV;
   DevBuf<double> dq0, qn, dqn;

@@ -162,9 +162,10 @@ int gocfd_euler2d_get_element_dt(gocfd_euler2d *h, double *DT);
  *      far-field state FSFar.Qinf [4] (rho, rhoU, rhoV, E; FSFar's pressure
  *      and sound speed follow it); the In/Out states follow it too except in
  *      the shock tube. Needs an FP64 solver without IVortex boundary and
- *      steps that stay short of FinalTime; with the PerssonC0 limiter, the
- *      global time step and a sensor that stays below S0 - Kappa in every
- *      element (its modal filters are then the limiter's only action)
+ *      steps that stay short of FinalTime; with the PerssonC0 limiter, a
+ *      sensor that stays below S0 - Kappa in every element (its modal
+ *      filters are then the limiter's only action) and, with
+ *      LocalTimeStepping, an element viscous dt not yet set
  *      (GOCFD_ERR_ARG otherwise). The dt of a stage (of an element, with LocalTimeStepping)
  *      follows the face point of its maximum wave speed (ties: the first in
  *      the device order; with LocalTimeStepping the previous stage's
