@@ -152,11 +152,13 @@ __device__ __forceinline__ void roeer_flux(C g, const C L[4], const C R[4], C nx
   const C U1x = n1x * u, U1y = n1y * v, U2x = n2x * u, U2y = n2y * v;
   const C Urot = fsqrt(alp1 * alp1 * (U1x * U1x + U1y * U1y)) + fsqrt(alp2 * alp2 * (U2x * U2x + U2y * U2y));
   const C sigma = fmax(Uabs, fmin(Uef, Urot));
-  const C UabsP = Uabs - C(0.25) * fmax(C(0), UR - UL) * (copysign(C(1), U + cc) - copysign(C(1), U - cc));
+  // The reference's sign = math.Copysign: sign(x, 1) has the magnitude of x
+  // and a positive sign, i.e. |x| (fluxes.go:426,491,494)
+  const C UabsP = Uabs - C(0.25) * fmax(C(0), UR - UL) * (fabs(U + cc) - fabs(U - cc));
   const C dU = UR - UL, dP = pR - pL, dRho = rhoR - rhoL, dRU = rhoR * uR - rhoL * uL;
   const C dRV = rhoR * vR - rhoL * vL, dE = ER - EL;
   const C dPu = rho * dU * fmax(C(0), cc - UabsP);
-  const C swt = copysign(C(1), U) * fmin(UabsP, cc);
+  const C swt = Uabs * fmin(UabsP, cc);
   const C dPp = swt * dP * ooC, dUu = swt * dU * ooC;
   F[0] -= C(0.5) * (sigma * dRho + (dPu + dPp) * C(0) + dUu * rho);
   F[1] -= C(0.5) * (sigma * dRU + (dPu + dPp) * nx + dUu * rho * u);

@@ -84,6 +84,9 @@ def lib():
         L.or_freestream.argtypes = [C.c_double, C.c_double, C.c_double, _dp, _dp, _dp]
         L.or_numerical_flux.argtypes = [C.c_int, C.c_double, _dp, _dp, _dp, _dp]
         L.or_riemann_bc.argtypes = [C.c_double, C.c_double, C.c_double, C.c_double, _dp, _dp, _dp, _dp]
+        L.or_set_nan_scan.argtypes = [C.c_void_p, C.c_int]
+        L.or_sigma_ramp.argtypes = [C.c_double, C.c_double, C.c_int]
+        L.or_sigma_ramp.restype = C.c_double
         _lib = L
     return _lib
 
@@ -146,6 +149,11 @@ class OracleSolver:
         A = np.abs(self.operators()["DivInt"])
         J = np.abs(self.geometry()["Jdet"])
         return np.einsum("ij,njk->nik", A, np.abs(F)) / J[None, None, :]
+
+    def set_nan_scan(self, mode: int):
+        """NaN scan before each stage: 0 none, 1 stage input, q0 and DT (the
+        default), 2 the reference's eleven arrays (euler.go:471-484)"""
+        lib().or_set_nan_scan(self._h, int(mode))
 
     def step(self):
         rc = lib().or_step(self._h)
@@ -289,3 +297,8 @@ def riemann_bc(gamma, minf, pinf, cinf, Qint, Qinf, n):
     n = np.ascontiguousarray(n, dtype=np.float64); Q = np.empty(4)
     lib().or_riemann_bc(gamma, minf, pinf, cinf, _p(Qint), _p(Qinf), _p(n), _p(Q))
     return Q
+
+
+def sigma_ramp(se, kappa, N):
+    """The stepping path's sensor ramp sigma(Se) (dissipation.go:491-518)."""
+    return lib().or_sigma_ramp(float(se), float(kappa), int(N))

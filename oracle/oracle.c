@@ -757,6 +757,7 @@ struct or_solver {
   double *EdgeFlux, *EdgeQ, *EdgeAgg, *EdgeAggVisc, *EdgeVisc;
   double time, gdt;
   int steps;
+  int nan_scan;   /* 0 none, 1 stage input + q0 + DT, 2 the reference's eleven arrays */
   /* shock capturing */
   int shock;
   double *Se, *Sigma, *Eps, *SigmaV, *EpsV, *EpsF, *SigmaF, *DissX, *DissY, *Clip, *Dmat, *Pmat;
@@ -932,12 +933,13 @@ static void roeer_flux(double g, const double QL[4], const double QR[4], const d
   double U1x = n1x * u, U1y = n1y * v, U2x = n2x * u, U2y = n2y * v;
   double Urot = sqrt(alp1 * alp1 * (U1x * U1x + U1y * U1y)) + sqrt(alp2 * alp2 * (U2x * U2x + U2y * U2y));
   double sigma = fmax(Uabs, fmin(Uef, Urot));
-  double sgnp = copysign(1, U + C), sgnm = copysign(1, U - C);
+  /* sign = math.Copysign (fluxes.go:426): sign(x, 1) is |x|, not the sign of x */
+  double sgnp = fabs(U + C), sgnm = fabs(U - C);
   double UabsPrime = Uabs - 0.25 * fmax(0, UR - UL) * (sgnp - sgnm);
   double dU = UR - UL, dP = pR - pL, dRho = rhoR - rhoL, dRhoU = rhoR * uR - rhoL * uL;
   double dRhoV = rhoR * vR - rhoL * vL, dE = ER - EL;
   double dPu = rho * dU * fmax(0, C - UabsPrime);
-  double swt = copysign(1, U) * fmin(UabsPrime, C);
+  double swt = fabs(U) * fmin(UabsPrime, C);
   double dPp = swt * dP * ooC, dUu = swt * dU * ooC;
   F[0] -= 0.5 * (sigma * dRho + (dPu + dPp) * 0 + dUu * rho);
   F[1] -= 0.5 * (sigma * dRhoU + (dPu + dPp) * nx + dUu * rho * u);
@@ -1170,6 +1172,7 @@ or_solver *or_create(const char *meshfile, const or_params *p, char *err, int er
   or_mesh *m = or_mesh_read(meshfile, err, errlen);
   if (!m) return NULL;
   or_solver *s = calloc(1, sizeof(or_solver));
+  s->nan_scan = 1;
   s->p = *p;
   if (p->flux < 0 || p->flux > 3) { seterr(err, errlen, "unable to use flux type %d", p->flux); goto fail; }
   if (build_ops(p->N, &s->o, err, errlen)) goto fail;
@@ -1255,6 +1258,9 @@ int or_NpFlux(const or_solver *s) { return s->NpFlux; }
 int or_num_edges(const or_solver *s) { return s->ne; }
 void or_get_state(const or_solver *s, double *Q) { memcpy(Q, s->Q, sizeof(double) * 4 * s->NpInt * (size_t)s->K); }
 void or_set_state(or_solver *s, const double *Q) { memcpy(s->Q, Q, sizeof(double) * 4 * s->NpInt * (size_t)s->K); }
+/* NaN scan before each stage: 0 none, 1 stage input, q0 and DT (the
+ * default), 2 the reference's per-stage list (euler.go:471-484) */
+void or_set_nan_scan(or_solver *s, int mode) { s->nan_scan = mode; }
 double or_time(const or_solver *s) { return s->time; }
 double or_dt(const or_solver *s) { return s->gdt; }
 int or_step_count(const or_solver *s) { return s->steps; }
@@ -1594,8 +1600,17 @@ int or_step(or_solver *s) {
           }
       }
     }
-    /* rkAdvance: the reference checks every array for NaN first (euler.go:471-492) */
-    if (has_nan(QQQ, nq) || has_nan(s->Q, nq) || has_nan(s->DT, K)) return 1;
+    /* rkAdvance: the reference checks every array for NaN first (checkFunc,
+     * euler.go:471-492: QQQ, F_RT_DOF, RHSQ, Jinv, Jdet, Q, Q1..Q4, DT) */
+    if (s->nan_scan == 2) {
+      size_t nf = 4 * (size_t)s->NpFlux * K;
+      if (has_nan(QQQ, nq) || has_nan(s->FRT, nf) || has_nan(s->RHSQ, nq) || has_nan(s->Jinv, 4 * (size_t)K) ||
+          has_nan(s->Jdet, K) || has_nan(s->Q, nq) || has_nan(s->Q1, nq) || has_nan(s->Q2, nq) ||
+          has_nan(s->Q3, nq) || has_nan(s->Q4, nq) || has_nan(s->DT, K))
+        return 1;
+    } else if (s->nan_scan == 1) {
+      if (has_nan(QQQ, nq) || has_nan(s->Q, nq) || has_nan(s->DT, K)) return 1;
+    }
     rt_flux_internal(s, QQQ);
     rt_flux_edges(s);
     rhs_internal(s, s->RHSQ);
@@ -1696,6 +1711,7 @@ static void setup_shock(or_solver *s) {
 /* UpdateSeMoment (dfr_shock_capturing.go:142-166) */
 static void sensor_se(or_solver *s, const double *rho, double *Se) {
   int K = s->K, Np = s->NpInt;
+#pragma omp parallel for schedule(static)
   for (int k = 0; k < K; k++) {
     double num = 0, den = 0;
     for (int i = 0; i < Np; i++) {
@@ -1757,9 +1773,12 @@ void or_vertex_max(const or_solver *s, const double *elem, double *vert) { verte
 /* limitAndFilterSolution (dissipation.go:606-622) applied to nvars variables */
 static void limit_filter(or_solver *s, double *U, int nvars) {
   int K = s->K, Np = s->NpInt;
+#pragma omp parallel
+  {
   double *m = malloc(sizeof(double) * Np);
   for (int n = 0; n < nvars; n++) {
     double *u = U + (size_t)n * Np * K;
+#pragma omp for schedule(static)
     for (int k = 0; k < K; k++) {
       double alpha = sin(0.5 * M_PI * s->Sigma[k]);
       for (int i = 0; i < Np; i++) {
@@ -1776,7 +1795,21 @@ static void limit_filter(or_solver *s, double *U, int nvars) {
     }
   }
   free(m);
+  }
 }
+
+/* The sensor ramp of UpdateShockFinderSigma (dissipation.go:491-518):
+ * 0 below S0 - kappa, 1 above S0 + kappa, 1/2 (1 + sin(pi (Se - S0) / (2 kappa)))
+ * between, with the stepping path's S0 = 4/(N+1)^4 (dissipation.go:500) */
+static double sigma_ramp(double se, double kappa, int N) {
+  double S0 = 4. / pow((double)(N + 1), 4.);
+  double left = S0 - kappa, right = S0 + kappa, ook = 0.5 / kappa, sg = 0.;
+  if (se < left) sg = 0.;
+  else if (se >= left && se <= right) sg = 0.5 * (1. + sin(M_PI * ook * (se - S0)));
+  else if (se > right) sg = 1.;
+  return sg;
+}
+double or_sigma_ramp(double se, double kappa, int N) { return sigma_ramp(se, kappa, N); }
 
 /* Stage phases 1-3 of StepWorker (euler.go:574-616): sensor, sigma, eps,
  * vertex max-merge, vertex->element sigma mean, C0 interpolation of eps and
@@ -1784,27 +1817,26 @@ static void limit_filter(or_solver *s, double *U, int nvars) {
 static void shock_stage_pre(or_solver *s, double *QQQ, int rkStep) {
   int K = s->K, NpFlux = s->NpFlux, N = s->p.N;
   sensor_se(s, QQQ, s->Se);
-  /* UpdateShockFinderSigma (dissipation.go:491-518): S0 = 4/(N+1)^4 */
-  double kappa = s->kappa_sd, S0 = 4. / pow((double)(N + 1), 4.);
-  double left = S0 - kappa, right = S0 + kappa, ook = 0.5 / kappa;
+  /* UpdateShockFinderSigma (dissipation.go:491-518) */
+  double kappa = s->kappa_sd;
   double Np12 = (double)((N + 1) * (N + 1));
+#pragma omp parallel for schedule(static)
   for (int k = 0; k < K; k++) {
-    double se = s->Se[k], sg = 0;
-    if (se < left) sg = 0.;
-    else if (se >= left && se <= right) sg = 0.5 * (1. + sin(M_PI * ook * (se - S0)));
-    else if (se > right) sg = 1.;
+    double sg = sigma_ramp(s->Se[k], kappa, N);
     s->Sigma[k] = sg;
     /* CalculateElementViscosity (:397-412) */
     s->Eps[k] = s->eps0 * (s->EdgeLenMax[k] / Np12) * sg;
   }
   vertex_max(s, s->Sigma, s->SigmaV);
   vertex_max(s, s->Eps, s->EpsV);
+#pragma omp parallel for schedule(static)
   for (int k = 0; k < K; k++) {
     double acc = 0;
     for (int a = 0; a < 3; a++) acc = acc + s->SigmaV[s->EToV[3 * k + a]];
     s->Sigma[k] = acc / 3.;
   }
   /* InterpolateEpsilonSigma (:219-242) */
+#pragma omp parallel for schedule(static)
   for (int k = 0; k < K; k++) {
     int *v = &s->EToV[3 * k];
     for (int i = 0; i < NpFlux; i++) {
@@ -1827,7 +1859,10 @@ static void shock_stage_pre(or_solver *s, double *QQQ, int rkStep) {
  * owner's stored trace (read reversed by the non-owner), then Div. */
 static void rt_gradient(or_solver *s, const double *QQQ, int n, const double *eps, double *DX, double *DY) {
   int K = s->K, NpInt = s->NpInt, NpEdge = s->NpEdge, NpFlux = s->NpFlux;
+#pragma omp parallel
+  {
   double *DOFX = malloc(sizeof(double) * NpFlux), *DOFY = malloc(sizeof(double) * NpFlux);
+#pragma omp for schedule(static)
   for (int k = 0; k < K; k++) {
     double *Ji = &s->Jinv[4 * k], ooJd = 1. / s->Jdet[k];
     for (int i = 0; i < NpInt; i++) {
@@ -1857,14 +1892,15 @@ static void rt_gradient(or_solver *s, const double *QQQ, int n, const double *ep
     }
   }
   free(DOFX); free(DOFY);
+  }
 }
 
 static void shock_gradient_and_viscous_edges(or_solver *s, const double *QQQ) {
   int K = s->K, NpInt = s->NpInt, NpEdge = s->NpEdge, NpFlux = s->NpFlux, N = s->p.N;
-#pragma omp parallel for schedule(static)
   for (int n = 0; n < 4; n++)
     rt_gradient(s, QQQ, n, s->EpsF, s->DissX + (size_t)n * NpFlux * K, s->DissY + (size_t)n * NpFlux * K);
   double Omega = 1. * (double)(N * N);
+#pragma omp parallel for schedule(static)
   for (int ei = 0; ei < s->ne; ei++) {
     edge_t *E = &s->E[ei];
     int kL = E->tri[0], enL = E->enr[0], shiftL = NpEdge * enL;
@@ -1904,9 +1940,12 @@ static void shock_gradient_and_viscous_edges(or_solver *s, const double *QQQ) {
 /* AddDissipation (dissipation.go:274-346) */
 static void shock_add_dissipation(or_solver *s, double *RHS) {
   int K = s->K, NpInt = s->NpInt, NpEdge = s->NpEdge, NpFlux = s->NpFlux;
+#pragma omp parallel
+  {
   double *DOF = malloc(sizeof(double) * NpFlux);
   for (int n = 0; n < 4; n++) {
     const double *DX = s->DissX + (size_t)n * NpFlux * K, *DY = s->DissY + (size_t)n * NpFlux * K;
+#pragma omp for schedule(static)
     for (int k = 0; k < K; k++) {
       double Jd = s->Jdet[k], *Ji = &s->Jinv[4 * k];
       for (int i = 0; i < NpInt; i++) {
@@ -1933,6 +1972,7 @@ static void shock_add_dissipation(or_solver *s, double *RHS) {
     }
   }
   free(DOF);
+  }
 }
 
 void or_get_shock(const or_solver *s, double *Se, double *Sigma, double *Eps) {

@@ -68,6 +68,9 @@ int or_rhs(or_solver *s, const double *Q, double t, double *RHS);
 void or_get_flux_dofs(const or_solver *s, double *F);
 /* One SSP54 step (euler.go:408-653); returns 0, or 1 on NaN ("NAN found") */
 int or_step(or_solver *s);
+/* NaN scan before each stage: 0 none, 1 the stage input, q0 and DT (the
+ * default), 2 the reference's per-stage list of eleven arrays (euler.go:471-484) */
+void or_set_nan_scan(or_solver *s, int mode);
 double or_time(const or_solver *s);
 double or_dt(const or_solver *s);
 int or_step_count(const or_solver *s);
@@ -92,6 +95,9 @@ void or_get_connectivity(const or_solver *s, int *EtoE, int *FaceEdge);
 void or_get_shock(const or_solver *s, double *Se, double *Sigma, double *Eps);
 /* Persson indicator of the DG2D shock finder on a density field (dfr_shock_capturing.go:189-209) */
 void or_shock_indicator(const or_solver *s, const double *rho, double kappa, double *sigma);
+/* The stepping path's sensor ramp sigma(Se) for Kappa and order N
+ * (UpdateShockFinderSigma, dissipation.go:491-518) */
+double or_sigma_ramp(double se, double kappa, int N);
 /* Vertex max-merge of an element scalar (euler.go:1048-1065) */
 void or_vertex_max(const or_solver *s, const double *elem, double *vert);
 /* RT-element gradient with eps = 1 (GetSolutionGradient, euler.go:864-918) */
