@@ -23,6 +23,7 @@ EXPORTS = [
     "gocfd_euler2d_get_edges", "gocfd_euler2d_get_connectivity", "gocfd_euler2d_get_operators",
     "gocfd_euler2d_get_geometry", "gocfd_euler2d_get_coords", "gocfd_euler2d_kernel_time",
     "gocfd_euler2d_tiling", "gocfd_euler2d_get_shock", "gocfd_euler2d_get_element_dt",
+    "gocfd_euler2d_interp_matrix",
     "gocfd_euler2d_vjp", "gocfd_euler2d_jvp",
     "gocfd_ensemble_create", "gocfd_ensemble_destroy", "gocfd_ensemble_case_dims", "gocfd_ensemble_step",
     "gocfd_ensemble_get_state", "gocfd_ensemble_time", "gocfd_ensemble_kernel_time",
@@ -97,6 +98,7 @@ def load() -> C.CDLL:
     L.gocfd_euler2d_get_shock.argtypes = [_vp, _dp, _dp, _dp]
     L.gocfd_euler2d_get_element_dt.argtypes = [_vp, _dp]
     L.gocfd_euler2d_vjp.argtypes = [_vp, C.c_int32, C.c_int32, _dp, _dp, _dp]
+    L.gocfd_euler2d_interp_matrix.argtypes = [_vp, C.c_int32, _dp, _dp, _dp]
     L.gocfd_euler2d_jvp.argtypes = [_vp, C.c_int32, _dp, _dp, _dp, _dp]
     L.gocfd_ensemble_create.argtypes = [C.c_int32, C.POINTER(_vp), C.POINTER(Params), C.POINTER(_vp)]
     L.gocfd_ensemble_destroy.argtypes = [_vp]

@@ -303,6 +303,14 @@ Operators build_operators(int N) {
   return o;
 }
 
+std::vector<double> interp_matrix(const Operators &o, int npts, const double *r, const double *s) {
+  const std::vector<double> R(r, r + npts), S(s, s + npts);
+  Mat Vp = vandermonde(o.N, R, S, 0, npts);
+  Mat Vi(o.NpInt, o.NpInt);
+  Vi.a = o.Vinv;
+  return matmul(Vp, Vi).a;
+}
+
 // ===========================================================================
 // Mesh ingest
 // ===========================================================================

@@ -51,6 +51,10 @@ struct Operators {
   std::vector<double> V, Vinv, Mass; // solution element [NpInt x NpInt]
 };
 Operators build_operators(int N);
+// The solution basis interpolated to arbitrary reference points (r, s):
+// [npts x NpInt] = V(r, s) Vinv (JacobiBasis2D.GetInterpMatrix,
+// DG2D/basis_polynomials.go:49-53)
+std::vector<double> interp_matrix(const Operators &o, int npts, const double *r, const double *s);
 
 struct Edge {
   uint64_t key = 0;

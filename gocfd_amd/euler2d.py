@@ -146,6 +146,7 @@ class Euler:
         self.CFL, self.FinalTime, self.MaxIterations = ip.CFL, ip.FinalTime, ip.MaxIterations
         self.precision = precision
         mesh = Mesh.read(meshFile) if isinstance(meshFile, str) else meshFile
+        self._mesh_xy = mesh.arrays()  # VX, VY, EToV for the output writers
         h = C.c_void_p()
         _lib.check(L.gocfd_euler2d_create(mesh._h, C.byref(p), C.byref(h)))
         self._h = h
@@ -265,9 +266,11 @@ class Euler:
         """euler.go:796-801"""
         return Time >= self.FinalTime or steps >= self.MaxIterations
 
-    def Solve(self) -> float:
+    def Solve(self, output_dir: Optional[str] = ".") -> float:
         """euler.go:138-220: step until FinalTime or MaxIterations; returns the
-        step-loop wall time in seconds (the scope the reference times)."""
+        step-loop wall time in seconds (the scope the reference times). Then
+        OutputFinal (euler.go:222-318) writes the case's files into
+        output_dir (None: no files)."""
         self.PrintInitialization()
         steps, elapsed, finished = 0, 0.0, False
         while not finished:
@@ -283,7 +286,15 @@ class Euler:
             if finished or steps == 1 or steps % 100 == 0:
                 self.PrintUpdate(Time, dt, steps)
         self.PrintFinal(elapsed, steps)
+        if output_dir is not None:
+            self.OutputFinal(output_dir)
         return elapsed
+
+    def OutputFinal(self, output_dir: str = ".") -> list:
+        """euler.go:222-318: shocktube.dat and shocktube_analytic.dat for the
+        shock tube, plotfile.dat for freestream cases with PlotFields."""
+        from . import output
+        return output.output_final(self, output_dir)
 
     def PrintInitialization(self) -> None:
         print(f"Using mesh from file: [{self.MeshFile}]")
@@ -303,6 +314,20 @@ class Euler:
         print(f"\nRate of execution = {rate:8.5f} us/(element*iteration) over {steps} iterations")
 
     # ---- setup introspection --------------------------------------------
+    def mesh_arrays(self):
+        """VX [Nv], VY [Nv], EToV [K][3] of the mesh the solver was built on"""
+        VX, VY, EToV = self._mesh_xy
+        return VX, VY, np.asarray(EToV).reshape(-1, 3)
+
+    def interp_matrix(self, r, s) -> np.ndarray:
+        """The solution basis at reference points (r, s): [npts][NpInt]
+        (GetInterpMatrix, DG2D/basis_polynomials.go:49-53)"""
+        r = np.ascontiguousarray(r, dtype=np.float64)
+        s = np.ascontiguousarray(s, dtype=np.float64)
+        out = np.empty((r.size, self.NpInt))
+        _lib.check(_lib.load().gocfd_euler2d_interp_matrix(self._h, r.size, _dp(r), _dp(s), _dp(out)))
+        return out
+
     def edges(self) -> np.ndarray:
         e = np.empty((self.nedges, 10), dtype=np.int64)
         _lib.check(_lib.load().gocfd_euler2d_get_edges(self._h, e.ctypes.data_as(C.POINTER(C.c_int64))))
