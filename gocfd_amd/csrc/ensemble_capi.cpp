@@ -17,6 +17,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cstdlib>
 #include <cstring>
 #include <map>
 #include <tuple>
@@ -34,6 +35,40 @@ using namespace gocfd::capi;
 namespace {
 
 constexpr int kGraphPeriod = 6;  // steps after which every per-stage argument repeats
+
+// rocprofv3 exports these to the program it runs (kernel tracing, counter
+// collection). A captured graph replayed under its kernel-trace interception
+// ended in a segmentation fault inside the HIP runtime's graph launch; the
+// kernels are the same when launched one by one, so with the profiler
+// attached an ensemble launches eagerly (graph_steps 0) unless told otherwise.
+bool profiler_attached() {
+  for (const char *v : {"ROCPROF_KERNEL_TRACE", "ROCPROF_COUNTER_COLLECTION"}) {
+    const char *e = std::getenv(v);
+    if (e && *e && std::strcmp(e, "0") != 0) return true;
+  }
+  return false;
+}
+
+// Ends a stream capture that an exception interrupts: the stream leaves
+// capture mode, the partial graph is destroyed and the stage counters that
+// the capture advanced are restored, so that the handle stays usable.
+struct CaptureGuard {
+  hipStream_t stream;
+  long long *gstage;
+  int *step_index;
+  long long g0;
+  int s0;
+  bool active = true;
+  ~CaptureGuard() {
+    if (!active) return;
+    hipGraph_t partial = nullptr;
+    (void)hipStreamEndCapture(stream, &partial);
+    if (partial) (void)hipGraphDestroy(partial);
+    (void)hipGetLastError();
+    *gstage = g0;
+    *step_index = s0;
+  }
+};
 
 struct EnsCase {
   gocfd_params prm{};
@@ -195,32 +230,13 @@ struct gocfd_ensemble {
   // graph_steps; from there whole blocks replay the captured graph (captured
   // on first use; the groups fork from the ensemble's stream and join it
   // again inside the graph when `parallel`), and the remainder runs eagerly.
+  // graph_steps 0: every step runs eagerly.
   void enqueue_steps(int nsteps) {
     HIPCHECK(hipEventRecord(ev0, stream));
     int left = nsteps;
     while (left > 0) {
-      if (step_index % graph_steps == 1 && left >= graph_steps) {
-        if (!gexec) {
-          HIPCHECK(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
-          const long long g0 = gstage;
-          const int s0 = step_index;
-          const bool br = parallel && groups.size() > 1;
-          if (br) {
-            HIPCHECK(hipEventRecord(fork, stream));
-            for (auto &g : groups) HIPCHECK(hipStreamWaitEvent(g->stream, fork, 0));
-          }
-          for (int j = 0; j < graph_steps; ++j) enqueue_step(br);
-          if (br) {
-            for (auto &g : groups) {
-              HIPCHECK(hipEventRecord(g->done, g->stream));
-              HIPCHECK(hipStreamWaitEvent(stream, g->done, 0));
-            }
-          }
-          HIPCHECK(hipStreamEndCapture(stream, &graph));
-          HIPCHECK(hipGraphInstantiate(&gexec, graph, nullptr, nullptr, 0));
-          gstage = g0;
-          step_index = s0;
-        }
+      if (graph_steps > 0 && step_index % graph_steps == 1 && left >= graph_steps) {
+        if (!gexec) capture_block();
         HIPCHECK(hipGraphLaunch(gexec, stream));
         gstage += 5LL * graph_steps;
         step_index += graph_steps;
@@ -234,6 +250,45 @@ struct gocfd_ensemble {
     last_steps = nsteps;
   }
 
+  // Captures graph_steps steps into `graph` and instantiates `gexec`; the
+  // stage counters are left where they were.
+  void capture_block() {
+    const long long g0 = gstage;
+    const int s0 = step_index;
+    hipGraph_t g = nullptr;
+    {
+      HIPCHECK(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
+      CaptureGuard guard{stream, &gstage, &step_index, g0, s0};
+      const bool br = parallel && groups.size() > 1;
+      if (br) {
+        HIPCHECK(hipEventRecord(fork, stream));
+        for (auto &gr : groups) HIPCHECK(hipStreamWaitEvent(gr->stream, fork, 0));
+      }
+      for (int j = 0; j < graph_steps; ++j) enqueue_step(br);
+      if (br) {
+        for (auto &gr : groups) {
+          HIPCHECK(hipEventRecord(gr->done, gr->stream));
+          HIPCHECK(hipStreamWaitEvent(stream, gr->done, 0));
+        }
+      }
+      guard.active = false;
+      const hipError_t e = hipStreamEndCapture(stream, &g);
+      gstage = g0;
+      step_index = s0;
+      if (e != hipSuccess) {
+        if (g) (void)hipGraphDestroy(g);
+        HIPCHECK(e);
+      }
+    }
+    const hipError_t e = hipGraphInstantiate(&gexec, g, nullptr, nullptr, 0);
+    if (e != hipSuccess) {
+      (void)hipGraphDestroy(g);
+      gexec = nullptr;
+      HIPCHECK(e);
+    }
+    graph = g;
+  }
+
   // Back to the initial state and time 0 (the stage states and R are
   // written before they are read in every step).
   void reset_state() {
@@ -245,8 +300,8 @@ struct gocfd_ensemble {
   }
 
   int set_launch(int par, int gsteps, int xcd) {
-    if (gsteps < kGraphPeriod || gsteps % kGraphPeriod != 0 || gsteps > 600)
-      return fail(GOCFD_ERR_ARG, "graph_steps must be a positive multiple of 6 (at most 600)");
+    if (gsteps < 0 || gsteps % kGraphPeriod != 0 || gsteps > 600)
+      return fail(GOCFD_ERR_ARG, "graph_steps must be 0 (eager launches) or a multiple of 6 (at most 600)");
     HIPCHECK(hipStreamSynchronize(stream));
     drop_graph();
     parallel = par ? 1 : 0;
@@ -262,20 +317,25 @@ struct gocfd_ensemble {
       int par, gs, xcd;
     };
     std::vector<Cfg> cand;
-    for (int par = 0; par <= (groups.size() > 1 ? 1 : 0); ++par)
-      for (int gs : {kGraphPeriod, 4 * kGraphPeriod})
-        for (int xcd : {1, 0}) cand.push_back({par, gs, xcd});
+    if (profiler_attached()) {  // eager launches only (see profiler_attached)
+      for (int xcd : {1, 0}) cand.push_back({0, 0, xcd});
+    } else {
+      for (int par = 0; par <= (groups.size() > 1 ? 1 : 0); ++par)
+        for (int gs : {kGraphPeriod, 4 * kGraphPeriod})
+          for (int xcd : {1, 0}) cand.push_back({par, gs, xcd});
+    }
     double best = 1e300;
     Cfg pick = cand[0];
     for (const Cfg &c : cand) {
       int rc = set_launch(c.par, c.gs, c.xcd);
       if (rc != GOCFD_OK) return rc;
+      const int bsteps = c.gs > 0 ? c.gs : kGraphPeriod;
       reset_state();
-      enqueue_steps(1 + c.gs);  // eager first step, then the block that captures the graph
+      enqueue_steps(1 + bsteps);  // eager first step, then the block that captures the graph
       if ((rc = finish()) != GOCFD_OK) return rc;
-      enqueue_steps(blocks * c.gs);
+      enqueue_steps(blocks * bsteps);
       if ((rc = finish()) != GOCFD_OK) return rc;
-      const double ms = last_ms / (blocks * c.gs);
+      const double ms = last_ms / (blocks * bsteps);
       if (ms < best) {
         best = ms;
         pick = c;
@@ -458,6 +518,7 @@ int gocfd_ensemble_create(int32_t ncases, const gocfd_mesh *const *meshes, const
   h->prec = params[0].precision;
   h->flux = params[0].flux_type;
   h->ssize = h->prec == 0 ? sizeof(double) : sizeof(float);
+  if (profiler_attached()) h->graph_steps = 0;  // eager launches under rocprofv3
   for (int i = 0; i < ncases; ++i) {
     const gocfd_params &p = params[i];
     if (!meshes[i]) return fail(GOCFD_ERR_ARG, "null mesh");
