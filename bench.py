@@ -303,17 +303,35 @@ def run_airfoil(mesh, steps, warmup, h=1e-6, ndir=2, seed=3, developed=20):
                          "checkpoint_every": int(math.ceil(math.sqrt(n)))})
 
 
-def cpu_baseline(mesh, target_seconds=15.0):
+def host_cpu():
+    """Model name of the host CPU (/proc/cpuinfo, what lscpu prints) and its
+    logical CPU count."""
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for ln in f:
+                if ln.startswith("model name"):
+                    model = ln.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return model, os.cpu_count()
+
+
+def cpu_baseline(mesh, target_seconds=15.0, nan_scan=2, max_steps=50):
     """The CPU oracle (plain C, FP64, OpenMP over the host cores) on the same mesh and order:
-    whole SSP54 steps until about `target_seconds` of CPU work."""
+    whole SSP54 steps until about `target_seconds` of CPU work. nan_scan 2 scans
+    the reference's eleven arrays for NaN before every stage (euler.go:471-484,
+    on in the reference), 0 none."""
     import oracle
     s = oracle.OracleSolver(mesh, 4, flux="roe", init="ivortex", CFL=1.0, FinalTime=1e9)
+    s.set_nan_scan(nan_scan)
     n, t0 = 0, time.perf_counter()
     while True:
         s.step()
         n += 1
         el = time.perf_counter() - t0
-        if el >= target_seconds or n >= 50:
+        if el >= target_seconds or n >= max_steps:
             break
     return dict(value=s.K * s.NpInt * 5 * n / el, steps=n, seconds=el, K=s.K, threads=oracle.threads(),
                 rho=s.state()[0])
@@ -436,11 +454,19 @@ def main():
     ach32 = b32 / (r32["stage_ms"] * 1e-3) / 1e9
     ach64 = b64 / (r64["stage_ms"] * 1e-3) / 1e9
 
-    traffic = None
-    tfile = os.path.join(ROOT, "profiles", "stage_fp32_p4_k80000_traffic.json")
-    if os.path.exists(tfile):
+    # HBM bytes per launch of the stage kernel at this workload, from the
+    # committed rocprofv3 FETCH_SIZE/WRITE_SIZE summary (calibrated by a copy
+    # of known size); None when the summary is absent
+    traffic = traffic64 = None
+    traffic_src = "profiles/stage_fp32_p4_k80000_traffic.json"
+    tfile = os.path.join(ROOT, traffic_src)
+    if os.path.exists(tfile) and args.n == 200:
         with open(tfile) as f:
-            traffic = json.load(f).get("bytes_per_launch")
+            tj = json.load(f)
+        traffic, traffic64 = tj.get("bytes_per_launch"), tj.get("fp64_bytes_per_launch")
+        traffic_src += f" (from {tj.get('source', '?')})"
+    else:
+        traffic_src = None
 
     shock = None
     if wedge is not None:
@@ -495,11 +521,17 @@ def main():
     cpu = None
     vs_oracle = None
     if not args.no_cpu:
-        c = cpu_baseline(mesh, args.cpu_seconds)
+        model, ncpu = host_cpu()
+        c = cpu_baseline(mesh, args.cpu_seconds, nan_scan=2)
         vs_oracle = gpu_vs_oracle(mesh, c["steps"], c["rho"])
+        c0 = cpu_baseline(mesh, args.cpu_seconds, nan_scan=0, max_steps=c["steps"])
         cpu = {"value": c["value"], "unit": "DOF-stage/s", "cores": c["threads"], "kind": "port",
                "sample": f"CPU oracle (plain C, FP64, OpenMP {c['threads']} threads), same {K}-triangle P=4 Roe "
-                         f"mesh, {c['steps']} SSP54 steps in {c['seconds']:.1f} s"}
+                         f"mesh, {c['steps']} SSP54 steps in {c['seconds']:.1f} s, with the reference's "
+                         f"per-stage NaN scan of eleven arrays (euler.go:471-484)",
+               "host_cpu": model, "host_logical_cpus": ncpu,
+               "no_nan_scan": {"value": c0["value"], "steps": c0["steps"], "seconds": c0["seconds"],
+                               "cores": c0["threads"]}}
 
     line = {
         "metric": METRIC,
@@ -520,14 +552,17 @@ def main():
                                f"P=4, Roe, SSP54, CFL 1", "K": K, "NpInt": NpInt,
                    "dof": K * NpInt, "precision": "fp32-bulk", "tile": r32["tiling"][0]},
         "fp64": {"value": v64, "ms_per_step": r64["elapsed"] / args.steps * 1e3,
-                 "stage_ms": r64["stage_ms"], "achieved_GBs": ach64,
-                 "frac_hbm_peak": ach64 / HBM_PEAK_GBS},
+                 "stage_ms": r64["stage_ms"],
+                 "roofline": {"bound": "hbm", "achieved": ach64, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                              "frac": ach64 / HBM_PEAK_GBS, "traffic": traffic64,
+                              "kernel": "k_stage<4,double,double>", "avg_launch_ms": r64["stage_ms"],
+                              "alg_bytes_per_launch": b64}},
         "fp32_over_fp64": v32 / v64,
         "accuracy": {"rho_l2_fp32_vs_fp64": l2_32_64, "rho_linf_fp32_vs_fp64": linf_32_64,
                      "rho_l2_fp64_vs_analytic": l2_64_an, "rho_l2_fp32_vs_analytic": l2_32_an,
                      "time": r64["time"], "vs_oracle": vs_oracle},
         "roofline": {"bound": "hbm", "achieved": ach32, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": ach32 / HBM_PEAK_GBS, "traffic": traffic,
+                     "frac": ach32 / HBM_PEAK_GBS, "traffic": traffic, "traffic_source": traffic_src,
                      "kernel": "k_stage<4,float,float>", "avg_launch_ms": r32["stage_ms"],
                      "alg_bytes_per_launch": b32},
         "cpu_baseline": cpu,

@@ -88,4 +88,6 @@ def test_wedge_full_size_shock_parity(tmp_path):
     assert _l2(g.Q[0], o.state()[0]) < 1e-10
     _, sg_g, _ = g.shock()
     _, sg_o, _ = o.shock()
+    print(f"wedge: {(sg_o > 0).sum()} of {o.K} elements flagged after 10 steps", flush=True)
+    assert (sg_o > 0).sum() > 0  # the comparison below is not vacuous
     assert np.array_equal(sg_g > 0, sg_o > 0)

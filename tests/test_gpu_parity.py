@@ -85,12 +85,12 @@ def test_initial_state_and_topology(vortex_meshes):
     assert _rel(gpu.Q, ora.state()) < 1e-15
 
 
-@pytest.mark.parametrize("flux,N", [("Lax", 2), ("Roe", 4), ("Roe", 1)])
+@pytest.mark.parametrize("flux,N", [("Lax", 2), ("Roe", 4), ("Roe", 3), ("Roe-ER", 3), ("Roe", 1)])
 def test_step_parity_vortex(vortex_meshes, flux, N):
     """The CPU-runnable vortex workload: 20x20 jittered quads (800 triangles),
     CFL 1, 100 SSP54 steps; density within 1e-10 relative L2 of the oracle."""
     gpu, ora = _pair(vortex_meshes[20], N, flux, "ivortex")
-    nsteps = 100 if N <= 2 else 30
+    nsteps = 100
     gpu.step(nsteps)
     for _ in range(nsteps):
         ora.step()
