@@ -348,15 +348,6 @@ int gocfd_euler2d_create(const gocfd_mesh *mesh, const gocfd_params *p, gocfd_eu
     h->sh_ops.upload(sops);
     h->sh_opsF.upload(std::vector<float>(sops.begin(), sops.end()));
     h->evert.upload(L.evert); h->hK.upload(L.hK);
-int gocfd_euler2d_interp_matrix(gocfd_euler2d *h, int32_t npts, const double *r, const double *s, double *out) {
-  GUARD_BEGIN
-  if (!h || npts < 0 || (npts > 0 && (!r || !s || !out))) return fail(GOCFD_ERR_ARG, "bad argument");
-  const std::vector<double> M = interp_matrix(h->d.ops, npts, r, s);
-  std::memcpy(out, M.data(), M.size() * sizeof(double));
-  return GOCFD_OK;
-  GUARD_END
-}
-
     h->nbr_elem.upload(L.nbr_elem); h->nbr_face.upload(L.nbr_face);
     h->Nxo.upload(L.Nxo); h->Nyo.upload(L.Nyo); h->vooL.upload(L.vooL);
     h->sigV.alloc(size_t(2) * std::max(h->d.Nv, 1));
@@ -564,6 +555,15 @@ int gocfd_euler2d_get_coords(gocfd_euler2d *h, double *SolX, double *SolY, doubl
   };
   cp(SolX, d.SolX); cp(SolY, d.SolY); cp(FluxX, d.FluxX); cp(FluxY, d.FluxY);
   return GOCFD_OK;
+}
+
+int gocfd_euler2d_interp_matrix(gocfd_euler2d *h, int32_t npts, const double *r, const double *s, double *out) {
+  GUARD_BEGIN
+  if (!h || npts < 0 || (npts > 0 && (!r || !s || !out))) return fail(GOCFD_ERR_ARG, "bad argument");
+  const std::vector<double> M = interp_matrix(h->d.ops, npts, r, s);
+  std::memcpy(out, M.data(), M.size() * sizeof(double));
+  return GOCFD_OK;
+  GUARD_END
 }
 
 int gocfd_euler2d_get_shock(gocfd_euler2d *h, double *Se, double *Sigma, double *Eps) {

@@ -142,6 +142,14 @@ int gocfd_euler2d_get_geometry(gocfd_euler2d *h, double *Jdet, double *Jinv, dou
 int gocfd_euler2d_get_coords(gocfd_euler2d *h, double *SolX, double *SolY, double *FluxX,
                              double *FluxY);
 
+/* The solution basis interpolated to npts reference points (r, s):
+ * out[npts][NpInt] = V(r, s) Vinv (JacobiBasis2D.GetInterpMatrix,
+ * DG2D/basis_polynomials.go:49-53); what the reference's output writers
+ * (SODShockTube.calculateInterpolation, sod_shock_tube/shock_tube.go:60-79;
+ * GraphInterp, DG2D/dfr_startup.go:62-63) evaluate the state with. */
+int gocfd_euler2d_interp_matrix(gocfd_euler2d *h, int32_t npts, const double *r, const double *s,
+                                double *out);
+
 /* ---- Shock capturing (Limiter: PerssonC0) and local time stepping ---- */
 /* Per element, of the last stage evaluated: the sensor Se (UpdateSeMoment,
  * DG2D/dfr_shock_capturing.go:142-166), sigma after the vertex averaging
@@ -158,14 +166,6 @@ int gocfd_euler2d_get_element_dt(gocfd_euler2d *h, double *DT);
  *      map is its SSP54 step, rkAdvance euler.go:460-566, with the
  *      per-stage global dt of calculateGlobalDT euler.go:945-971). Both start
  *      from the solver's current state and time and leave the solver
-/* The solution basis interpolated to npts reference points (r, s):
- * out[npts][NpInt] = V(r, s) Vinv (JacobiBasis2D.GetInterpMatrix,
- * DG2D/basis_polynomials.go:49-53); what the reference's output writers
- * (SODShockTube.calculateInterpolation, sod_shock_tube/shock_tube.go:60-79;
- * GraphInterp, DG2D/dfr_startup.go:62-63) evaluate the state with. */
-int gocfd_euler2d_interp_matrix(gocfd_euler2d *h, int32_t npts, const double *r, const double *s,
-                                double *out);
-
  *      unchanged. Inputs of the map: the state Q [4][NpInt][K] and the
  *      far-field state FSFar.Qinf [4] (rho, rhoU, rhoV, E; FSFar's pressure
  *      and sound speed follow it); the In/Out states follow it too except in
