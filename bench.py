@@ -217,14 +217,15 @@ def run_ensemble(ncases, steps, warmup):
                 seq_rate=seq_rate, errs=errs, tuned=tuned, tune_seconds=t_tune)
 
 
-def run_airfoil(mesh, steps, warmup, h=1e-6, ndir=2, seed=3, developed=20):
-    """The adjoint workload: 4-digit airfoil in an O-domain, P=3, M=0.5,
-    AoA 2 deg, Roe, FP64, n = warmup + steps SSP54 steps from the freestream.
-    Times the forward step; computes the lift functional's gradients by one
-    discrete-adjoint sweep (Euler.vjp, checkpointed) with respect to Mach,
-    angle of attack (degrees) and the initial state, and checks them against
-    central finite differences (step h) of the solver: in Mach and angle from
-    the freestream start, and in `ndir` seeded state directions from a state
+def run_airfoil(mesh, steps, warmup, h=1e-6, ndir=16, seed=3, developed=20, window=200, every=20):
+    """The adjoint workload: 4-digit airfoil in an O-domain (Delaunay mesh of
+    50k wall-clustered points), P=3, M=0.5, AoA 2 deg, Roe, FP64. Times the
+    forward step (warmup + steps); computes the lift functional's gradients
+    over a `window`-step run by one discrete-adjoint sweep (Euler.vjp,
+    checkpoint every `every` steps) with respect to Mach, angle of attack
+    (degrees) and the initial state, and checks them against central finite
+    differences (step h) of the solver: in Mach and angle from the freestream
+    start, and in `ndir` seeded random state directions from a state
     `developed` steps into the run (at the uniform start every wave-speed
     candidate of the dt maximum ties, so the state derivative is one-sided
     there). Also the tangent-linear run's dot product against the adjoint."""
@@ -233,7 +234,7 @@ def run_airfoil(mesh, steps, warmup, h=1e-6, ndir=2, seed=3, developed=20):
     def make(M, a):
         ip = InputParameters2D(Title="airfoil", CFL=1.0, FluxType="Roe", InitType="Freestream",
                                PolynomialOrder=3, Minf=M, Alpha=a, FinalTime=1e9,
-                               MaxIterations=warmup + steps + developed)
+                               MaxIterations=warmup + steps + developed + window)
         return Euler(ip, mesh, precision="fp64")
 
     M0, a0 = 0.5, 2.0
@@ -245,40 +246,45 @@ def run_airfoil(mesh, steps, warmup, h=1e-6, ndir=2, seed=3, developed=20):
     elapsed = time.perf_counter() - t0
     stage_ms, _ = e.kernel_time()
     t_end, dt, _ = e.time_info()
-    n = warmup + steps
-    J = adjoint.lift(e, e.Q, a0)
+    n = window
 
-    def J_of(M, a, q0=None):
+    def J_of(M, a):
         g = make(M, a)
-        if q0 is not None:
-            g.Q = q0
-            g.set_time(0.0)
         g.step(n)
         return adjoint.lift(g, g.Q, a)
 
     # adjoint from the freestream start: Mach and angle
     e0 = make(M0, a0)
     ta = time.perf_counter()
-    ga = adjoint.lift_gradients(e0, n)
+    ga = adjoint.lift_gradients(e0, n, every)
     grad_seconds = time.perf_counter() - ta
-    seed_n = adjoint.lift_state_gradient(e0, e.Q, a0)
+    seed_n = adjoint.lift_state_gradient(e0, e0.Q, a0)
     tv = time.perf_counter()
-    e0.vjp(n, seed_n)
+    e0.vjp(n, seed_n, every)
     vjp_seconds = time.perf_counter() - tv
     t1 = time.perf_counter()
     fdM = (J_of(M0 + h, a0) - J_of(M0 - h, a0)) / (2 * h)
     fda = (J_of(M0, a0 + h) - J_of(M0, a0 - h)) / (2 * h)
-    # state directions from a developed state
+    # state directions from a developed state; one solver re-run from each
+    # perturbed state (set_state / set_time restart the step sequence)
     ed = make(M0, a0)
     ed.step(developed)
     qd = ed.Q
     ed.set_time(0.0)
-    gd = adjoint.lift_gradients(ed, n)
+    gd = adjoint.lift_gradients(ed, n, every)
     rng = np.random.default_rng(seed)
+    fd_run = make(M0, a0)
+
+    def J_from(q0):
+        fd_run.Q = q0
+        fd_run.set_time(0.0)
+        fd_run.step(n)
+        return adjoint.lift(fd_run, fd_run.Q, a0)
+
     dirs = []
     for _ in range(ndir):
         v = rng.standard_normal(qd.shape) * np.abs(qd)
-        fd = (J_of(M0, a0, qd + h * v) - J_of(M0, a0, qd - h * v)) / (2 * h)
+        fd = (J_from(qd + h * v) - J_from(qd - h * v)) / (2 * h)
         ad = float(np.sum(gd["dJ_dQ0"] * v))
         dirs.append({"adjoint": ad, "fd": fd, "rel_err": abs(ad - fd) / abs(fd)})
     fd_seconds = time.perf_counter() - t1
@@ -289,18 +295,21 @@ def run_airfoil(mesh, steps, warmup, h=1e-6, ndir=2, seed=3, developed=20):
     jvp_seconds = time.perf_counter() - tj
     tlm = float(np.sum(adjoint.lift_state_gradient(ed, qn, a0) * jv))
     adj = float(np.sum(gd["dJ_dQ0"] * v))
-    return dict(elapsed=elapsed, stage_ms=stage_ms, K=e.K, NpInt=e.NpInt, J=J, time=t_end, dt=dt, steps=n,
-                adjoint={"dJ_dMinf": ga["dJ_dMinf"], "fd_dJ_dMinf": fdM,
+    errs = [d["rel_err"] for d in dirs]
+    return dict(elapsed=elapsed, stage_ms=stage_ms, K=e.K, NpInt=e.NpInt, J=ga["J"], time=t_end, dt=dt, steps=n,
+                adjoint={"window_steps": n, "checkpoint_every": every, "h": h,
+                         "dJ_dMinf": ga["dJ_dMinf"], "fd_dJ_dMinf": fdM,
                          "rel_err_Minf": abs(ga["dJ_dMinf"] - fdM) / abs(fdM),
                          "dJ_dalpha_deg": ga["dJ_dalpha_deg"], "fd_dJ_dalpha_deg": fda,
                          "rel_err_alpha": abs(ga["dJ_dalpha_deg"] - fda) / abs(fda),
+                         "state_dirs": len(dirs), "state_dir_rel_err_max": max(errs),
+                         "state_dir_rel_err_median": float(np.median(errs)),
                          "state_dirs_from_developed_state": dirs,
                          "tangent_linear_dot_rel": abs(tlm - adj) / abs(tlm),
                          "vjp_seconds": vjp_seconds, "vjp_ms_per_step": vjp_seconds / n * 1e3,
                          "vjp_over_forward_step": (vjp_seconds / n) / (elapsed / steps),
                          "jvp_ms_per_step": jvp_seconds / n * 1e3,
-                         "gradient_seconds_with_primal": grad_seconds, "fd_seconds": fd_seconds,
-                         "checkpoint_every": int(math.ceil(math.sqrt(n)))})
+                         "gradient_seconds_with_primal": grad_seconds, "fd_seconds": fd_seconds})
 
 
 def host_cpu():
@@ -416,7 +425,7 @@ def main():
         return
     airfoil_mesh = None
     if args.only == "airfoil" or (args.only is None and not args.no_airfoil):
-        airfoil_mesh = meshgen.airfoil_mesh(os.path.join(tmpdir, "airfoil.su2"), seed=1)
+        airfoil_mesh = meshgen.airfoil_delaunay_mesh(os.path.join(tmpdir, "airfoil.su2"), seed=1)
     if args.only == "airfoil":
         r = run_airfoil(airfoil_mesh, args.airfoil_steps, args.warmup)
         r["value"] = r["K"] * r["NpInt"] * 5 * args.airfoil_steps / r["elapsed"]
@@ -511,8 +520,9 @@ def main():
     if airfoil_mesh is not None:
         ra = run_airfoil(airfoil_mesh, args.airfoil_steps, args.warmup)
         airfoil = {"workload": f"symmetric 4-digit airfoil (12 %, chord 1) in an O-domain of radius 20, {ra['K']} "
-                               f"triangles (400 x 125 wall-clustered O-mesh, seeded jitter), P=3, M=0.5, AoA 2 deg, "
-                               f"Roe, FP64, {args.airfoil_steps} steps",
+                               f"triangles (Delaunay of 50,400 seeded wall-clustered points), P=3, M=0.5, AoA 2 deg, "
+                               f"Roe, FP64, {args.airfoil_steps} timed steps; gradients over 200 steps, "
+                               f"checkpoints every 20, 16 random state directions",
                    "value": ra["K"] * ra["NpInt"] * 5 * args.airfoil_steps / ra["elapsed"], "unit": "DOF-stage/s",
                    "ms_per_step": ra["elapsed"] / args.airfoil_steps * 1e3, "stage_ms": ra["stage_ms"],
                    "time": ra["time"], "dt": ra["dt"], "lift_functional": ra["J"],

@@ -136,7 +136,7 @@ def naca4_thickness(x, t=0.12):
 
 
 def airfoil_mesh(path: str, n_around: int = 400, n_radial: int = 125, radius: float = 20.0, seed: int = 1,
-                 jitter: float = 0.1, stretch: float = 8.0) -> str:
+                 jitter: float = 0.1, stretch: float = 8.0, points_only: bool = False):
     """Symmetric 4-digit airfoil (chord 1, 12 % thick, closed trailing edge)
     in an O-domain of the given radius about (0.5, 0): n_around surface
     points (cosine-clustered toward the leading edge) by n_radial
@@ -147,7 +147,8 @@ def airfoil_mesh(path: str, n_around: int = 400, n_radial: int = 125, radius: fl
     +-jitter of the local spacing (SplitMix64 stream from `seed`); each quad is
     split into two counter-clockwise triangles: 2 n_around n_radial
     triangles (100,000 at the defaults, 50,000 points). Tags: "wall" on the
-    airfoil, "far" on the circle."""
+    airfoil, "far" on the circle. points_only: return the points (VX, VY)
+    ordered layer by layer (n_around per layer, the wall first) instead."""
     assert n_around % 2 == 0
     m = n_around // 2
     # clustered toward the leading edge only: a cusp-side clustering would
@@ -182,6 +183,8 @@ def airfoil_mesh(path: str, n_around: int = 400, n_radial: int = 125, radius: fl
     X[1:-1] += (2 * r[0] - 1) * jitter * h
     Y[1:-1] += (2 * r[1] - 1) * jitter * h
     VX, VY = X.ravel(), Y.ravel()
+    if points_only:
+        return VX, VY
 
     def v(j, i):
         return j * n_around + (i % n_around)
@@ -259,4 +262,45 @@ def wedge_mesh(path: str, n_points: int = 125_000, seed: int = 1, x_ramp: float 
         (far if s_i in (2, 4) else walls).extend(edges)
     tags = {"wall": walls, "far": far}
     write_su2(path, P[:, 0], P[:, 1], tri.astype(np.int32), tags)
+    return path
+
+
+def airfoil_delaunay_mesh(path: str, n_around: int = 400, n_radial: int = 125, radius: float = 20.0,
+                          seed: int = 1, jitter: float = 0.1, stretch: float = 8.0) -> str:
+    """The airfoil workload as a Delaunay mesh: the Delaunay triangulation
+    (scipy) of the seeded, wall-clustered point set of airfoil_mesh
+    (n_around x (n_radial + 1) points: 50,400 at the defaults), without the
+    triangles inside the airfoil. The points form a jittered lattice, as for
+    the wedge channel: independent uniform points would leave sliver
+    triangles whose CFL limit stalls the time step. Every airfoil surface
+    segment and every circle segment must be an edge of the triangulation
+    (checked). Tags: "wall" on the airfoil, "far" on the circle."""
+    from scipy.spatial import Delaunay
+    VX, VY = airfoil_mesh(path, n_around, n_radial, radius, seed, jitter, stretch, points_only=True)
+    P = np.stack([VX, VY], axis=1)
+    tri = Delaunay(P).simplices.astype(np.int64)
+    # drop triangles inside the airfoil (all three vertices on the surface
+    # and the centroid within the section)
+    on_wall = np.arange(len(VX)) < n_around
+    c = P[tri].mean(axis=1)
+    xc = np.clip(c[:, 0], 0.0, 1.0)
+    inside = on_wall[tri].all(axis=1) & (c[:, 0] > 0) & (c[:, 0] < 1) & (np.abs(c[:, 1]) < naca4_thickness(xc))
+    tri = tri[~inside]
+    a, b, cc = P[tri[:, 0]], P[tri[:, 1]], P[tri[:, 2]]
+    cross = (b[:, 0] - a[:, 0]) * (cc[:, 1] - a[:, 1]) - (b[:, 1] - a[:, 1]) * (cc[:, 0] - a[:, 0])
+    keep = np.abs(cross) > 0
+    tri, cross = tri[keep], cross[keep]
+    tri[cross < 0] = tri[cross < 0][:, [0, 2, 1]]
+    wall = [(i, (i + 1) % n_around) for i in range(n_around)]
+    o = n_radial * n_around
+    far = [(o + i, o + (i + 1) % n_around) for i in range(n_around)]
+    edges = set()
+    for t in tri:
+        for k in range(3):
+            u, w = int(t[k]), int(t[(k + 1) % 3])
+            edges.add((min(u, w), max(u, w)))
+    missing = [e for e in wall + far if (min(e), max(e)) not in edges]
+    if missing:
+        raise RuntimeError(f"{len(missing)} boundary segments are not Delaunay edges")
+    write_su2(path, VX, VY, tri.astype(np.int32), {"wall": wall, "far": far})
     return path

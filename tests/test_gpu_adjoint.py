@@ -38,6 +38,8 @@ def meshes(tmp_path_factory):
     d = tmp_path_factory.mktemp("adj")
     return dict(
         foil=meshgen.airfoil_mesh(str(d / "foil.su2"), n_around=60, n_radial=12, radius=8.0, seed=2),
+        foil_delaunay=meshgen.airfoil_delaunay_mesh(str(d / "foil_del.su2"), n_around=60, n_radial=12, radius=8.0,
+                                                    seed=2),
         periodic=meshgen.periodic_mesh(str(d / "per.su2"), 6, seed=4),
         tube=meshgen.shocktube_mesh(str(d / "st.su2"), 30, 2, height=0.05),
     )
@@ -246,15 +248,17 @@ def test_perssonc0_with_local_time_stepping(meshes):
     assert g["dJ_dMinf"] == pytest.approx(fdM, rel=1e-6)
 
 
-def test_lift_gradients_against_oracle_differences(meshes):
+@pytest.mark.parametrize("mesh_name,N,flux", [("foil", 2, "Roe"), ("foil", 3, "Roe"), ("foil_delaunay", 3, "Roe")])
+def test_lift_gradients_against_oracle_differences(meshes, mesh_name, N, flux):
     """Central differences of the CPU oracle (FP64, the reference's
-    algorithm) in Minf and in a state direction."""
-    mesh, n, h, N = meshes["foil"], 10, 1e-6, 2
-    ip = _ip(N)
+    algorithm) in Minf and in a state direction; P=3 with Roe is the
+    airfoil workload's order and flux, also on a Delaunay foil mesh."""
+    mesh, n, h = meshes[mesh_name], 10, 1e-6
+    ip = _ip(N, flux)
     e = Euler(ip, mesh, precision="fp64")
 
     def J_oracle(M, q0=None):
-        o = oracle.OracleSolver(mesh, N, flux="roe", init="freestream", CFL=1.0, Minf=M, Alpha=2.0)
+        o = oracle.OracleSolver(mesh, N, flux=flux.lower(), init="freestream", CFL=1.0, Minf=M, Alpha=2.0)
         if q0 is not None:
             o.set_state(q0)
         for _ in range(n):
@@ -296,3 +300,36 @@ def test_refusals(meshes):
     for every in (0, 200):  # checkpointed, and one block whose record sweep is the forward sweep
         with pytest.raises(GocfdError, match="FinalTime"):
             e.vjp(200, np.zeros_like(e.Q), every)
+
+
+def test_lts_shock_viscous_dt_refusal(meshes):
+    """With LocalTimeStepping and PerssonC0 the element viscous dt, once set
+    by an active sensor, is never reset by the reference; vjp and jvp refuse
+    such a state ('viscous time step'), and set_state / set_time, which
+    restart the step sequence and clear it, lift that refusal."""
+    ip = _ip(2, Limiter="PerssonC0", Kappa=10.0, LocalTimeStepping=True)
+    e = Euler(ip, meshes["foil"], precision="fp64")
+    for _ in range(20):
+        e.step(5)
+        _, sigma, _ = e.shock()
+        if (sigma > 0).any():
+            break
+    assert (sigma > 0).any(), "the sensor never fired"
+    z = np.zeros_like(e.Q)
+    with pytest.raises(GocfdError, match="viscous time step"):
+        e.vjp(1, z)
+    with pytest.raises(GocfdError, match="viscous time step"):
+        e.jvp(1, z)
+    t, _, _ = e.time_info()
+    e.set_time(t)
+    # the viscous dt is clear again: whatever else the window meets, it is
+    # not refused for that reason
+    try:
+        e.vjp(1, z)
+    except GocfdError as err:
+        assert "viscous time step" not in str(err)
+    e.Q = e.Q
+    try:
+        e.jvp(1, z)
+    except GocfdError as err:
+        assert "viscous time step" not in str(err)

@@ -1,7 +1,10 @@
 """GPU parity: the HIP path through the C ABI against the CPU oracle.
 
-Bars (FP64): RHS within 1e-12 of the magnitude of the summed divergence
-terms (and within 1e-11 plain inf-norm relative); density after 100 SSP54
+Bars (FP64): RHS within 1e-14 of the magnitude of the summed divergence
+terms and within 1e-12 plain inf-norm relative (3e-12 at N=1, where the RT
+edge points sit 0.056 apart and the divergence cancels three digits;
+measured plain-relative maxima per order: N=0 5e-15, N=1 1.2e-12, N=2 4e-14,
+N=3 7.5e-14, N=4 2.1e-13, term-scaled 5.4e-16); density after 100 SSP54
 steps within 1e-10 relative L2; mesh topology bit-exact. FP32-bulk: density
 L2 within 1e-5 of the FP64 oracle.
 """
@@ -37,6 +40,11 @@ def _rel(a, b):
     return np.abs(a - b).max() / max(np.abs(b).max(), 1e-300)
 
 
+def _plain_bar(N):
+    """1e-12 plain relative, except at N=1 (see the module docstring)"""
+    return 3e-12 if N == 1 else 1e-12
+
+
 def _rhs_err(r_gpu, ora):
     """RHS error of the GPU against the oracle's last RHS evaluation, relative
     to the magnitude of the terms the divergence sums (sum |DivInt||F|/|J|):
@@ -57,8 +65,8 @@ def test_rhs_parity_vortex(vortex_meshes, N, flux):
         r_gpu = gpu.rhs(q, t)
         r_ora = ora.rhs(q, t)
         ora.rhs_last = r_ora
-        assert _rhs_err(r_gpu, ora) < 1e-12, (N, flux, t, _rhs_err(r_gpu, ora))
-        assert _rel(r_gpu, r_ora) < 1e-11, (N, flux, t, _rel(r_gpu, r_ora))
+        assert _rhs_err(r_gpu, ora) < 1e-14, (N, flux, t, _rhs_err(r_gpu, ora))
+        assert _rel(r_gpu, r_ora) < _plain_bar(N), (N, flux, t, _rel(r_gpu, r_ora))
 
 
 @pytest.mark.parametrize("mesh,N,init,Minf,Alpha", [
@@ -75,8 +83,8 @@ def test_rhs_parity_reference_meshes(mesh, N, init, Minf, Alpha, flux):
     q = _perturbed(ora.state(), 3)
     r_gpu, r_ora = gpu.rhs(q, 0.25), ora.rhs(q, 0.25)
     ora.rhs_last = r_ora
-    assert _rhs_err(r_gpu, ora) < 1e-12
-    assert _rel(r_gpu, r_ora) < 1e-11
+    assert _rhs_err(r_gpu, ora) < 1e-14
+    assert _rel(r_gpu, r_ora) < _plain_bar(N)
 
 
 def test_initial_state_and_topology(vortex_meshes):
