@@ -25,6 +25,7 @@ EXPORTS = [
     "gocfd_euler2d_tiling", "gocfd_euler2d_get_shock", "gocfd_euler2d_get_element_dt",
     "gocfd_euler2d_interp_matrix",
     "gocfd_euler2d_vjp", "gocfd_euler2d_jvp",
+    "gocfd_euler2d_rhs_noalloc",
     "gocfd_ensemble_create", "gocfd_ensemble_destroy", "gocfd_ensemble_case_dims", "gocfd_ensemble_step",
     "gocfd_ensemble_get_state", "gocfd_ensemble_time", "gocfd_ensemble_kernel_time",
     "gocfd_ensemble_set_launch", "gocfd_ensemble_get_launch", "gocfd_ensemble_tune",
@@ -83,6 +84,7 @@ def load() -> C.CDLL:
     L.gocfd_euler2d_get_state.argtypes = [_vp, _dp]
     L.gocfd_euler2d_get_residual.argtypes = [_vp, _dp]
     L.gocfd_euler2d_rhs.argtypes = [_vp, _dp, C.c_double, _dp]
+    L.gocfd_euler2d_rhs_noalloc.argtypes = [_vp, _vp, C.c_double, _vp, _vp]
     L.gocfd_euler2d_step.argtypes = [_vp, C.c_int32]
     L.gocfd_euler2d_step_async.argtypes = [_vp, C.c_int32]
     L.gocfd_euler2d_sync.argtypes = [_vp]

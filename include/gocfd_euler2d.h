@@ -105,6 +105,22 @@ int gocfd_euler2d_get_state(gocfd_euler2d *h, double *Q);
  *      SetRTFluxInternal/SetRTFluxOnEdges). Q and RHS in the reference layout. ---- */
 int gocfd_euler2d_rhs(gocfd_euler2d *h, const double *Q, double t, double *RHS);
 
+/* ---- RHS of a device-resident state with no device allocation. No
+ *      counterpart in the reference: for callers that cannot allocate device
+ *      memory (code under HIP graph capture, tight serving loops, harnesses
+ *      that forbid hipMalloc). Same result as gocfd_euler2d_rhs, bit for
+ *      bit. Q and RHS are DEVICE pointers on the handle's device, in the
+ *      reference layout [4][NpInt][K]. stream: a hipStream_t to enqueue on
+ *      (the call then returns without waiting), or NULL for the handle's own
+ *      stream (the call returns when the RHS is written). The element-major
+ *      staging lives in a fixed 32 MiB device array per device: meshes with
+ *      2 * Kp * 4 * NpInt * 8 bytes above that (Kp = K rounded up to the
+ *      64-element tile; about K > 34,900 at N=4) get GOCFD_ERR_ARG. Every
+ *      handle on a device shares that array: calls that overlap on one
+ *      device race, and the caller orders the call after step work of the
+ *      same handle. ---- */
+int gocfd_euler2d_rhs_noalloc(gocfd_euler2d *h, const double *Q, double t, double *RHS, void *stream);
+
 /* ---- nsteps SSP54 steps: (*RungeKutta5SSP).Step (euler.go:408-418) with
  *      the Solve loop's Time += GlobalDT, StepCount++ (euler.go:175-182).
  *      Stops early, like CheckIfFinished (euler.go:796-801), once Time >=
