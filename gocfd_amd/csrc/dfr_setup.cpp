@@ -872,6 +872,11 @@ DeviceLayout build_device_layout(const Discretisation &d, int E) {
       std::copy(fl[t].begin(), fl[t].end(), L.flux_list.begin() + size_t(t) * L.Fmax);
     }
   }
+  // Non-shared faces per tile (the boundary loop of the stage kernel)
+  L.bnd_cnt.assign(L.ntiles, 0);
+  for (int kk = 0; kk < Kp; ++kk)
+    for (int f = 0; f < 3; ++f)
+      if ((L.face_code[size_t(f) * Kp + kk] & 15) != FK_Shared) ++L.bnd_cnt[kk / E];
   // Faces whose owner-side trace enters the CFL wave-speed maximum, per tile
   std::vector<std::vector<int>> waves(L.ntiles);
   for (int kk = 0; kk < K; ++kk)

@@ -114,6 +114,10 @@ struct DeviceLayout {
   int Fmax = 0;
   std::vector<int> flux_list;
   std::vector<int> flux_cnt;                            // [ntiles]
+  // Faces of the tile's elements (padding included) that are not shared:
+  // boundary, inert and padding faces, whose flux the boundary loop sets.
+  // Tiles with none skip that loop. [ntiles]
+  std::vector<int> bnd_cnt;
   std::vector<double> bpoint_xy;                        // IVortex boundary points [nb][NpEdge][2]
   int nbfaces = 0;
   // shock capturing and local time stepping, per face [3][Kp] / element [Kp]

@@ -86,7 +86,7 @@ struct EnsGroup {
   int Nv = 0;                       // vertices of the group's cases
   std::vector<int> cases;
   DevBuf<double> Jdet, Jinv, IInII, Nx, Ny, oohK, bxy, Interp64;
-  DevBuf<int> code, nbr_slot, halo_elem, halo_face, wave_list, wave_cnt, flux_list, flux_cnt, tile_case;
+  DevBuf<int> code, nbr_slot, halo_elem, halo_face, wave_list, wave_cnt, flux_list, flux_cnt, bnd_cnt, tile_case;
   void *q[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
   void *R = nullptr;
   std::vector<double> q0_init;         // initial element-major state (tuning restores it)
@@ -96,7 +96,8 @@ struct EnsGroup {
   // shock capturing
   DevBuf<double> sh_ops, hK, Se, sigma, eps, epsF, diss, Nxo, Nyo, vooL;
   DevBuf<float> sh_opsF;
-  DevBuf<int> evert, nbr_elem, nbr_face, active;
+  DevBuf<int> evert, nbr_elem, nbr_face, active, act_list;
+  DevBuf<unsigned> act_cnt;
   DevBuf<unsigned long long> sigV, epsV;
   hipStream_t stream = nullptr;        // the group's branch of a parallel graph
   hipEvent_t done = nullptr;
@@ -160,7 +161,7 @@ struct gocfd_ensemble {
     a.g.oohK = g.oohK.p; a.g.code = g.code.p; a.g.nbr_slot = g.nbr_slot.p;
     a.g.halo_elem = g.halo_elem.p; a.g.halo_face = g.halo_face.p; a.g.bxy = g.bxy.p;
     a.g.wave_list = g.wave_list.p; a.g.wave_cnt = g.wave_cnt.p;
-    a.g.flux_list = g.flux_list.p; a.g.flux_cnt = g.flux_cnt.p;
+    a.g.flux_list = g.flux_list.p; a.g.flux_cnt = g.flux_cnt.p; a.g.bnd_cnt = g.bnd_cnt.p;
     a.Interp64 = g.Interp64.p;
     const EnsCase &c0 = *cases[g.cases[0]];
     a.ph = phys_of(c0.fs, c0.prm);
@@ -171,6 +172,8 @@ struct gocfd_ensemble {
       a.sh.sigV = g.sigV.p; a.sh.epsV = g.epsV.p; a.sh.Nv = g.Nv;
       a.sh.Se = g.Se.p; a.sh.sigma = g.sigma.p; a.sh.eps = g.eps.p; a.sh.epsF = g.epsF.p; a.sh.diss = g.diss.p;
       a.sh.active = g.active.p;
+      a.sh.act_list = g.act_list.p;
+      a.sh.act_cnt = g.act_cnt.p;
       a.sh.nbr_elem = g.nbr_elem.p; a.sh.nbr_face = g.nbr_face.p;
       a.sh.Nxo = g.Nxo.p; a.sh.Nyo = g.Nyo.p; a.sh.vooL = g.vooL.p;
       a.sh.kappa = g.kappa; a.sh.eps0 = 5. / 1.5;
@@ -217,7 +220,7 @@ struct gocfd_ensemble {
         if (g->shock) {
           a.sh.vbuf = int(gstage & 1);
           a.sh.filter_state = s == 2 ? 1 : 0;
-          for (int pass = 0; pass < 3; ++pass) HIPCHECK(dev::launch_shock(g->N, prec, pass, a, st));
+          for (int pass = 0; pass < dev::kShockPasses; ++pass) HIPCHECK(dev::launch_shock(g->N, prec, pass, a, st));
         }
         HIPCHECK(dev::launch_stage(g->N, prec, g->shock ? 1 : 0, a, st));
       }
@@ -387,7 +390,7 @@ void build_group(gocfd_ensemble &h, EnsGroup &g) {
   std::vector<int> code(3 * Kp), nbr_slot(3 * Kp), tile_case(g.ntiles);
   std::vector<int> halo_elem(size_t(g.ntiles) * g.Hmax, -1), halo_face(size_t(g.ntiles) * g.Hmax, 0);
   std::vector<int> wave_list(size_t(g.ntiles) * g.Wmax, 0), wave_cnt(g.ntiles, 0);
-  std::vector<int> flux_list(size_t(g.ntiles) * g.Fmax, 0), flux_cnt(g.ntiles, 0);
+  std::vector<int> flux_list(size_t(g.ntiles) * g.Fmax, 0), flux_cnt(g.ntiles, 0), bnd_cnt(g.ntiles, 0);
   std::vector<double> q0(Kp * g.NV);
   for (int ci : g.cases) {
     const EnsCase &c = *h.cases[ci];
@@ -420,6 +423,7 @@ void build_group(gocfd_ensemble &h, EnsGroup &g) {
       wave_cnt[t0 + t] = L.wave_cnt[t];
       for (int i = 0; i < L.wave_cnt[t]; ++i) wave_list[size_t(t0 + t) * g.Wmax + i] = L.wave_list[size_t(t) * L.Wmax + i];
       flux_cnt[t0 + t] = L.flux_cnt[t];
+      bnd_cnt[t0 + t] = L.bnd_cnt[t];
       for (int i = 0; i < L.flux_cnt[t]; ++i) flux_list[size_t(t0 + t) * g.Fmax + i] = L.flux_list[size_t(t) * L.Fmax + i];
     }
     fill_element_major(L, NpInt, c.Q0.data(), q0.data() + o * g.NV);
@@ -428,7 +432,8 @@ void build_group(gocfd_ensemble &h, EnsGroup &g) {
   g.oohK.upload(oohK); g.bxy.upload(bxy); g.code.upload(code); g.nbr_slot.upload(nbr_slot);
   g.halo_elem.upload(halo_elem); g.halo_face.upload(halo_face);
   g.wave_list.upload(wave_list); g.wave_cnt.upload(wave_cnt);
-  g.flux_list.upload(flux_list); g.flux_cnt.upload(flux_cnt); g.tile_case.upload(tile_case);
+  g.flux_list.upload(flux_list); g.flux_cnt.upload(flux_cnt); g.bnd_cnt.upload(bnd_cnt);
+  g.tile_case.upload(tile_case);
   const EnsCase &c0 = *h.cases[g.cases[0]];
   g.Interp64.upload(c0.d.ops.Interp);
   std::vector<double> divT(c0.d.ops.DivInt.size());
@@ -492,6 +497,8 @@ void build_group(gocfd_ensemble &h, EnsGroup &g) {
       g.epsF.alloc(Kp * NR);
       g.diss.alloc(Kp * 4 * 2 * NR);  // FP64 or FP32 values (the larger size)
       g.active.alloc(Kp);
+      g.act_list.alloc(Kp);
+      g.act_cnt.alloc(1);
     }
   }
   HIPCHECK(hipStreamCreateWithFlags(&g.stream, hipStreamNonBlocking));

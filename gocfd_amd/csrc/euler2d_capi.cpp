@@ -114,7 +114,7 @@ int gocfd_euler2d_create(const gocfd_mesh *mesh, const gocfd_params *p, gocfd_eu
   h->code.upload(L.face_code); h->nbr_slot.upload(L.nbr_slot);
   h->halo_elem.upload(L.halo_elem); h->halo_face.upload(L.halo_face);
   h->wave_list.upload(L.wave_list); h->wave_cnt.upload(L.wave_cnt);
-  h->flux_list.upload(L.flux_list); h->flux_cnt.upload(L.flux_cnt);
+  h->flux_list.upload(L.flux_list); h->flux_cnt.upload(L.flux_cnt); h->bnd_cnt.upload(L.bnd_cnt);
   h->perm.upload(L.perm);
   noalloc_setup(h->device);
   {
@@ -146,6 +146,8 @@ int gocfd_euler2d_create(const gocfd_mesh *mesh, const gocfd_params *p, gocfd_eu
     h->epsF.alloc(size_t(L.Kp) * NR);
     h->diss.alloc(size_t(L.Kp) * 4 * 2 * NR);  // FP64 or FP32 values (the larger size)
     h->active.alloc(L.Kp);
+    h->act_list.alloc(L.Kp);
+    h->act_cnt.alloc(1);
   }
   std::vector<double> divT(h->d.ops.DivInt.size());  // [NpFlux][NpInt] for contiguous per-DOF rows
   for (int i = 0; i < h->d.ops.NpInt; ++i)

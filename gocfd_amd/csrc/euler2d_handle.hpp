@@ -38,7 +38,7 @@ struct gocfd_euler2d {
   void *scratch_in = nullptr;
   DevBuf<double> rhs_out;
   DevBuf<double> Jdet, Jinv, IInII, Nx, Ny, oohK, bxy, Interp64;
-  DevBuf<int> code, nbr_slot, halo_elem, halo_face, wave_list, wave_cnt, flux_list, flux_cnt;
+  DevBuf<int> code, nbr_slot, halo_elem, halo_face, wave_list, wave_cnt, flux_list, flux_cnt, bnd_cnt;
   DevBuf<int> perm;  // [Kp] device element -> reference element (-1: padding)
   // local time stepping
   bool lts = false;
@@ -49,7 +49,8 @@ struct gocfd_euler2d {
   double kappa = 0, eps0 = 0;
   DevBuf<double> sh_ops, hK, Se, sigma, eps, epsF, diss, Nxo, Nyo, vooL;
   DevBuf<float> sh_opsF;
-  DevBuf<int> evert, nbr_elem, nbr_face, active;
+  DevBuf<int> evert, nbr_elem, nbr_face, active, act_list;
+  DevBuf<unsigned> act_cnt;
   DevBuf<unsigned long long> sigV, epsV;
   // a single solver is a batch of one case
   DevBuf<dev::CaseInfo> cases;
@@ -84,7 +85,7 @@ struct gocfd_euler2d {
     a.g.oohK = oohK.p; a.g.code = code.p; a.g.nbr_slot = nbr_slot.p;
     a.g.halo_elem = halo_elem.p; a.g.halo_face = halo_face.p; a.g.bxy = bxy.p;
     a.g.wave_list = wave_list.p; a.g.wave_cnt = wave_cnt.p;
-    a.g.flux_list = flux_list.p; a.g.flux_cnt = flux_cnt.p;
+    a.g.flux_list = flux_list.p; a.g.flux_cnt = flux_cnt.p; a.g.bnd_cnt = bnd_cnt.p;
     a.Interp64 = Interp64.p;
     a.lts.on = lts ? 1 : 0;
     a.lts.fw = fw.p; a.lts.fwv = fwv.p; a.lts.wsrc = wsrc.p; a.lts.DTk = DTk.p; a.lts.DTv = DTv.p;
@@ -93,6 +94,8 @@ struct gocfd_euler2d {
       a.sh.sigV = sigV.p; a.sh.epsV = epsV.p; a.sh.Nv = d.Nv;
       a.sh.Se = Se.p; a.sh.sigma = sigma.p; a.sh.eps = eps.p; a.sh.epsF = epsF.p; a.sh.diss = diss.p;
       a.sh.active = active.p;
+      a.sh.act_list = act_list.p;
+      a.sh.act_cnt = act_cnt.p;
       a.sh.nbr_elem = nbr_elem.p; a.sh.nbr_face = nbr_face.p;
       a.sh.Nxo = Nxo.p; a.sh.Nyo = Nyo.p; a.sh.vooL = vooL.p;
       a.sh.kappa = kappa; a.sh.eps0 = eps0;
@@ -191,7 +194,7 @@ struct gocfd_euler2d {
     if (shock) {
       a.sh.vbuf = int(gstage & 1);
       a.sh.filter_state = a.stage == 2 ? 1 : 0;
-      for (int pass = 0; pass < 3; ++pass) HIPCHECK(dev::launch_shock(N, prec, pass, a, s));
+      for (int pass = 0; pass < dev::kShockPasses; ++pass) HIPCHECK(dev::launch_shock(N, prec, pass, a, s));
     }
     HIPCHECK(dev::launch_stage(N, prec, shock ? 1 : 0, a, s));
   }

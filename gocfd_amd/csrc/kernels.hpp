@@ -35,6 +35,7 @@ struct Geom {
   const int *halo_elem, *halo_face;                    // [ntiles][Hmax]
   const int *wave_list, *wave_cnt;                     // [ntiles][Wmax], [ntiles]
   const int *flux_list, *flux_cnt;                     // [ntiles][Fmax], [ntiles]
+  const int *bnd_cnt;                                  // [ntiles] non-shared faces
   const double *bxy;                                   // IVortex boundary points
 };
 
@@ -91,6 +92,8 @@ struct ShockArgs {
   int *active;                       // [Kp] 1 if a vertex of the element carries eps > 0;
                                      // else eps = 0 at all its RT points and diss is
                                      // neither written nor read (it is zero)
+  int *act_list;                     // [Kp] the active elements of this stage (any order)
+  unsigned *act_cnt;                 // their number (cleared by the sensor pass)
   const int *nbr_elem, *nbr_face;    // [3][Kp]
   const double *Nxo, *Nyo;           // [3][Kp] own outward normals
   const double *vooL;                // [3][Kp] 1 / owner edge length (shared faces)
@@ -127,7 +130,9 @@ struct StageArgs {
 hipError_t launch_stage(int N, int prec, int shock, const StageArgs &a, hipStream_t s);
 hipError_t launch_prologue(int N, int prec, const StageArgs &a, hipStream_t s);
 // Shock-capturing passes of one stage: 0 sensor + vertex max, 1 element
-// fields (+ stage-state filter), 2 gradient, viscosity and wave speeds.
+// fields (+ stage-state filter, active-element list), 2 wave speeds,
+// 3 eps * gradient of the active elements.
+constexpr int kShockPasses = 4;
 hipError_t launch_shock(int N, int prec, int pass, const StageArgs &a, hipStream_t s);
 hipError_t launch_ctl_init(Ctl *ctl, double time, hipStream_t s, int ncases = 1);
 hipError_t launch_fill_u64(unsigned long long *p, size_t n, unsigned long long v, hipStream_t s);

@@ -313,6 +313,7 @@ k_stage(StageArgs a) {
     const int it = tid + j * kThreads;
     fpk[j] = it < nfl ? flist[it / NpEdge] : 0;
   }
+  const int nbnd = a.g.bnd_cnt[tile];  // non-shared faces: the boundary loop runs only if any
   int hk0 = -1, hf0 = 0;
   if (tid < a.Hmax * 4) {
     const int h = tid % a.Hmax;
@@ -342,6 +343,10 @@ k_stage(StageArgs a) {
   const bool first = tile == ci.tile0;  // does the case's control bookkeeping
   Ctl *ctl = a.ctl + cs;
   const double time = rhs_only ? a.t_rhs : ctl->time[par];
+  // this stage's wave-speed maxima, written by the previous launch: read now
+  // so that phase E does not wait for them
+  const unsigned long long wmax_bits = rhs_only ? 0ull : ctl->wmax[a.ring];
+  const unsigned long long wmaxv_bits = (rhs_only || !SHOCK) ? 0ull : ctl->wmaxv[a.ring];
   if (step_skipped(a, ctl, ph.final_time)) {
     if (a.stage == 4 && first && threadIdx.x == 0) ctl->time[par ^ 1] = time;
     return;
@@ -490,7 +495,7 @@ k_stage(StageArgs a) {
     }
   }
 #pragma unroll 1
-  for (int it = tid; it < E * NF; it += kThreads) {
+  for (int it = tid; it < (nbnd > 0 ? E * NF : 0); it += kThreads) {
     const int e = it % E, p = it / E, f = p / NpEdge, pp = p - f * NpEdge;
     const size_t fi = size_t(f) * Kp + k0 + e;
     const int code = a.g.code[fi];
@@ -661,7 +666,8 @@ k_stage(StageArgs a) {
 
   // the owned-face list of phase G, fetched under phase E's loads
   constexpr int ITG = (3 * E * NpEdge + kThreads - 1) / kThreads;
-  int wpk[ITG];
+  int wpk[ITG], wcode[ITG];
+  double woohk[ITG];
   int nw = 0;
   if constexpr (!SHOCK) {
     nw = a.g.wave_cnt[tile] * NpEdge;
@@ -671,6 +677,14 @@ k_stage(StageArgs a) {
       const int it = tid + j * kThreads;
       wpk[j] = it < nw ? wlist[it / NpEdge] : 0;
     }
+    // their face codes and 1/hK, under phase E's loads
+#pragma unroll
+    for (int j = 0; j < ITG; ++j) {
+      const int it = tid + j * kThreads;
+      const int e = wpk[j] & 255, f = wpk[j] >> 8;
+      wcode[j] = it < nw ? a.g.code[size_t(f) * Kp + k0 + e] : 0;
+      woohk[j] = it < nw ? a.g.oohK[k0 + e] : 0.0;
+    }
   }
 
   // ---- E: SSP54 stage combination (euler.go:511-565), FP64 arithmetic ----
@@ -678,9 +692,9 @@ k_stage(StageArgs a) {
   const bool lts = a.lts.on;
   double dt = 0.0;  // GlobalDT; stays 0 with local time stepping (never computed)
   if (!lts) {
-    dt = ph.cfl / from_bits(ctl->wmax[a.ring]);
+    dt = ph.cfl / from_bits(wmax_bits);
     if constexpr (SHOCK) {
-      const double dtv = a.sh.cdiff / from_bits(ctl->wmaxv[a.ring]);
+      const double dtv = a.sh.cdiff / from_bits(wmaxv_bits);
       dt = fmin(dt, dtv);
     }
     if (time + dt > ph.final_time) dt = ph.final_time - time;
@@ -842,7 +856,7 @@ k_stage(StageArgs a) {
         const int e = packed & 255, f = packed >> 8, p = f * NpEdge + pp;
         const int kk = k0 + e;
         const size_t fi = size_t(f) * Kp + kk;
-        const int code = a.g.code[fi];
+        const int code = wcode[j];
         const int kind = code & 15;
         C own[4];
 #pragma unroll
@@ -850,7 +864,7 @@ k_stage(StageArgs a) {
         double q[4] = {double(own[0]), double(own[1]), double(own[2]), double(own[3])};
         if (kind == FK_Far || kind == FK_IVortex)
           boundary_state(kind, ph, a.g.bxy, code >> 8, NpEdge, pp, tnext, own, a.g.Nx[fi], a.g.Ny[fi], q);
-        const double wsp = point_speed(ph.gamma, q, a.g.oohK[kk]);
+        const double wsp = point_speed(ph.gamma, q, woohk[j]);
         if (!(wsp == wsp)) bad = 1;
         wl = fmax(wl, wsp);
         if (lts) atomicMax(&sWf[e * 3 + f], bits(wsp));
@@ -957,6 +971,8 @@ __global__ void __launch_bounds__(256) k_shock_sensor(StageArgs a) {
   using Sh = Shape<N>;
   constexpr int NpInt = Sh::NpInt, NV = Sh::NV;
   const int kk = blockIdx.x * blockDim.x + threadIdx.x;
+  // the element-fields pass of this stage builds the active-element list
+  if (kk == 0) *a.sh.act_cnt = 0u;
   if (kk >= a.Kp) return;
   // the element's case (a batch holds several, each with its own control)
   const int cs = a.tile_case[kk / kTile];
@@ -1040,6 +1056,17 @@ __global__ void __launch_bounds__(256) k_shock_prep(StageArgs a) {
   const double e0 = from_bits(ev[v0]), e1 = from_bits(ev[v1]), e2 = from_bits(ev[v2]);
   const bool act = e0 > 0. || e1 > 0. || e2 > 0.;
   a.sh.active[kk] = act ? 1 : 0;
+  {
+    // append to the active-element list: one atomic per wave
+    const unsigned long long m = __ballot(act);
+    if (m) {
+      const int lane = threadIdx.x & 63, leader = __ffsll(m) - 1;
+      unsigned base = 0;
+      if (lane == leader) base = atomicAdd(a.sh.act_cnt, unsigned(__popcll(m)));
+      base = __shfl(base, leader, 64);
+      if (act) a.sh.act_list[base + unsigned(__popcll(m & ((1ull << lane) - 1ull)))] = kk;
+    }
+  }
   // an inactive element's eps is zero at every RT point; readers check the
   // flag instead of reading epsF
   if (act) {
@@ -1073,100 +1100,64 @@ __global__ void __launch_bounds__(256) k_shock_prep(StageArgs a) {
   }
 }
 
-// Pass 2: eps * grad Q by the RT element (CalculateEpsilonGradient
-// dissipation.go:244-272 via GetSolutionGradientUsingRTElement
-// euler.go:864-918) at the interior and edge RT points, from interior DOFs
-// Jinv*Q and edge DOFs n*IInII/Jdet times the owner's trace (read reversed
-// by the non-owner); then the stage's owner-side wave speeds and viscous
-// speeds eps/hK^2 (StoreEdgeAggregates edges.go:246-289). Workgroup = tile of
-// 64 elements x 4 variables.
+// Pass 2: the stage's owner-side wave speeds (|U|+c)/hK and viscous speeds
+// eps/hK^2 (StoreEdgeAggregates edges.go:246-289) from the own traces, one
+// tile of 64 elements x 4 variables per workgroup; the tile's state is
+// staged through LDS with coalesced loads.
 template <int N, class SIN, class D>
 __global__ void __launch_bounds__(kThreads) k_shock_grad(StageArgs a) {
   using Sh = Shape<N>;
   constexpr int NpInt = Sh::NpInt, NpEdge = Sh::NpEdge, NpFlux = Sh::NpFlux, NF = Sh::NF, NV = Sh::NV,
-                NR = Sh::NR, E = kTile;
-  __shared__ D sTr[4][NF][E];
+                E = kTile, P = kTile + 1;
+  // the tile state [NV][P], then (aliased) the traces [4][NF][E]
+  constexpr int LW = NV * P > 4 * NF * E ? NV * P : 4 * NF * E;
+  __shared__ D sbuf[LW];
   __shared__ double red[2][kThreads / 64];
   const int tile = blockIdx.x, tid = threadIdx.x;
   // the tile's case (a batch holds several, each with its own control)
   const int cs = a.tile_case[tile];
   const CaseInfo &ci = a.cases[cs];
   Ctl *ctl = a.ctl + cs;
-  if (step_skipped(a, ctl, ci.ph.final_time)) return;
-  const int te = tid % E, tn = tid / E, k0 = tile * E, kk = k0 + te, Kp = a.Kp;
+  if (a.stage < 0 || step_skipped(a, ctl, ci.ph.final_time)) return;
+  const int te = tid % E, tn = tid / E, k0 = tile * E, Kp = a.Kp;
   const SIN *Q = static_cast<const SIN *>(a.qs);
   const D *Itp = ops_base<D>() + NpFlux * NpInt;
-  const D *Div = shock_ops<D>(a.sh) + Sh::oDiv;
-  D u[NpInt];
+  {
+    constexpr int LPT = (E * NV + kThreads - 1) / kThreads;
+    const SIN *qt = Q + size_t(k0) * NV;
+    SIN v[LPT];
 #pragma unroll
-  for (int i = 0; i < NpInt; ++i) u[i] = D(Q[size_t(kk) * NV + tn * NpInt + i]);
-  // eps is zero at every RT point of an inactive element: its eps*grad is
-  // zero, the stage kernel does not read it, and only the own traces (for
-  // the wave speeds) are formed
-  const bool act = a.sh.active[kk] != 0;
-  // edge values: the owner's trace at this face's points, in this face's order
-  D ue[NF];
-#pragma unroll
-  for (int f = 0; f < 3; ++f) {
-    const size_t fi = size_t(f) * Kp + kk;
-#pragma unroll
-    for (int i = 0; i < NpEdge; ++i) {
-      D t = 0;
-#pragma unroll
-      for (int j = 0; j < NpInt; ++j) t += Itp[(f * NpEdge + i) * NpInt + j] * u[j];
-      sTr[tn][f * NpEdge + i][te] = t;
-      ue[f * NpEdge + i] = t;
+    for (int j = 0; j < LPT; ++j) {
+      const int idx = tid + j * kThreads;
+      v[j] = idx < E * NV ? qt[idx] : SIN(0);
     }
-    if (!act) continue;
-    const int code = a.g.code[fi];
-    // unshared faces (boundary, padding) are always their own edge's owner
-    const bool owner = (code & FC_FLUX_OWNER) || (code & 15) != FK_Shared;
-    if (!owner) {
-      const int nb = a.sh.nbr_elem[fi], nf = a.sh.nbr_face[fi];
-      const SIN *qn = Q + size_t(nb) * NV + tn * NpInt;
 #pragma unroll
-      for (int i = 0; i < NpEdge; ++i) {
-        const int p = nf * NpEdge + (NpEdge - 1 - i);
-        D t = 0;
-#pragma unroll
-        for (int j = 0; j < NpInt; ++j) t += Itp[p * NpInt + j] * D(qn[j]);
-        ue[f * NpEdge + i] = t;
+    for (int j = 0; j < LPT; ++j) {
+      const int idx = tid + j * kThreads;
+      if (idx < E * NV) {
+        const int e = idx / NV, r = idx - e * NV;
+        sbuf[r * P + e] = D(v[j]);
       }
     }
   }
-  if (act) {
-    const double Jd = a.g.Jdet[kk], ooJd = 1. / Jd;
-    const D J0 = D(a.g.Jinv[kk]), J1 = D(a.g.Jinv[size_t(Kp) + kk]);
-    const D J2 = D(a.g.Jinv[2 * size_t(Kp) + kk]), J3 = D(a.g.Jinv[3 * size_t(Kp) + kk]);
-    const double *epsF = a.sh.epsF + kk;                                    // [NR][Kp]
-    D *dout = static_cast<D *>(a.sh.diss) + size_t(tn * 2 * NR) * Kp + kk;  // [4][2][NR][Kp]
-#pragma unroll 1
-    for (int dir = 0; dir < 2; ++dir) {
-      const D jr = dir == 0 ? J0 : J1, js = dir == 0 ? J2 : J3;
-      D dof[NpFlux];
+  __syncthreads();
+  D t[NF];
+  {
+    D u[NpInt];
 #pragma unroll
-      for (int i = 0; i < NpInt; ++i) {
-        dof[i] = jr * u[i];
-        dof[NpInt + i] = js * u[i];
-      }
+    for (int i = 0; i < NpInt; ++i) u[i] = sbuf[(tn * NpInt + i) * P + te];
 #pragma unroll
-      for (int f = 0; f < 3; ++f) {
-        const size_t fi = size_t(f) * Kp + kk;
-        const double nrm = dir == 0 ? a.sh.Nxo[fi] : a.sh.Nyo[fi];
-        const D w = D(ooJd * nrm * a.g.IInII[fi]);
+    for (int p = 0; p < NF; ++p) {
+      D acc = 0;
 #pragma unroll
-        for (int i = 0; i < NpEdge; ++i) dof[2 * NpInt + f * NpEdge + i] = w * ue[f * NpEdge + i];
-      }
-#pragma unroll 3
-      for (int r = 0; r < NR; ++r) {
-        D acc = 0;
-#pragma unroll
-        for (int j = 0; j < NpFlux; ++j) acc += Div[r * NpFlux + j] * dof[j];
-        dout[size_t(dir * NR + r) * Kp] = acc * D(epsF[size_t(r) * Kp]);
-      }
+      for (int j = 0; j < NpInt; ++j) acc += Itp[p * NpInt + j] * u[j];
+      t[p] = acc;
     }
   }
-  if (a.stage < 0) return;
+  __syncthreads();  // sbuf now holds the traces
+  D *sTr = sbuf;    // [4][NF][E]
+#pragma unroll
+  for (int p = 0; p < NF; ++p) sTr[(tn * NF + p) * E + te] = t[p];
   __syncthreads();
   // wave speeds of the owned faces: thread (element, face)
   double wl = 0.0, wv = 0.0;
@@ -1183,15 +1174,16 @@ __global__ void __launch_bounds__(kThreads) k_shock_grad(StageArgs a) {
       const double *ef = a.sh.epsF + size_t(NpInt + f * NpEdge) * Kp + k;
       double fwl = 0.0, fwv = 0.0;
       for (int pp = 0; pp < NpEdge; ++pp) {
-        double q[4] = {double(sTr[0][f * NpEdge + pp][e]), double(sTr[1][f * NpEdge + pp][e]),
-                       double(sTr[2][f * NpEdge + pp][e]), double(sTr[3][f * NpEdge + pp][e])};
+        const int p = f * NpEdge + pp;
+        double q[4] = {double(sTr[(0 * NF + p) * E + e]), double(sTr[(1 * NF + p) * E + e]),
+                       double(sTr[(2 * NF + p) * E + e]), double(sTr[(3 * NF + p) * E + e])};
         if (kind == FK_Far || kind == FK_IVortex) {
           const double own[4] = {q[0], q[1], q[2], q[3]};
           boundary_state(kind, ci.ph, a.g.bxy, code >> 8, NpEdge, pp, time, own, a.g.Nx[fi], a.g.Ny[fi], q);
         }
-        const double s = point_speed(ci.ph.gamma, q, oohK);
-        if (!(s == s)) bad = 1;
-        fwl = fmax(fwl, s);
+        const double sp = point_speed(ci.ph.gamma, q, oohK);
+        if (!(sp == sp)) bad = 1;
+        fwl = fmax(fwl, sp);
         if (actk) fwv = fmax(fwv, oohK * oohK * ef[size_t(pp) * Kp]);
       }
       if (a.lts.on) {
@@ -1218,6 +1210,90 @@ __global__ void __launch_bounds__(kThreads) k_shock_grad(StageArgs a) {
     atomic_max_bits(&ctl->wmax[a.ring], m);
     atomic_max_bits(&ctl->wmaxv[a.ring], mv);
     if (bad) atomicOr(&ctl->nan, 1);
+  }
+}
+
+// Pass 3: eps * grad Q by the RT element (CalculateEpsilonGradient
+// dissipation.go:244-272 via GetSolutionGradientUsingRTElement
+// euler.go:864-918) at the interior and edge RT points of the active
+// elements (elsewhere eps is zero at every RT point and the stage kernel does
+// not read it), from interior DOFs Jinv*Q and edge DOFs n*IInII/Jdet times
+// the owner's trace (read reversed by the non-owner). Thread = (list entry,
+// variable, direction); a grid-stride loop over the list that the
+// element-fields pass built.
+constexpr int kGradActBlocks = 1024;
+template <int N, class SIN, class D>
+__global__ void __launch_bounds__(256) k_shock_grad_act(StageArgs a) {
+  using Sh = Shape<N>;
+  constexpr int NpInt = Sh::NpInt, NpEdge = Sh::NpEdge, NpFlux = Sh::NpFlux, NF = Sh::NF, NV = Sh::NV,
+                NR = Sh::NR;
+  const size_t n = size_t(*a.sh.act_cnt) * 8;
+  const int Kp = a.Kp;
+  const SIN *Q = static_cast<const SIN *>(a.qs);
+  const D *Itp = ops_base<D>() + NpFlux * NpInt;
+  const D *Div = shock_ops<D>(a.sh) + Sh::oDiv;
+  for (size_t it = size_t(blockIdx.x) * blockDim.x + threadIdx.x; it < n; it += size_t(gridDim.x) * blockDim.x) {
+    const int kk = a.sh.act_list[it >> 3], tn = int(it >> 1) & 3, dir = int(it) & 1;
+    D u[NpInt];
+#pragma unroll
+    for (int i = 0; i < NpInt; ++i) u[i] = D(Q[size_t(kk) * NV + tn * NpInt + i]);
+    // edge values: the owner's trace at this face's points, in this face's order
+    D ue[NF];
+#pragma unroll
+    for (int f = 0; f < 3; ++f) {
+      const size_t fi = size_t(f) * Kp + kk;
+      const int code = a.g.code[fi];
+      // unshared faces (boundary, padding) are always their own edge's owner
+      const bool owner = (code & FC_FLUX_OWNER) || (code & 15) != FK_Shared;
+      if (owner) {
+#pragma unroll
+        for (int i = 0; i < NpEdge; ++i) {
+          D t = 0;
+#pragma unroll
+          for (int j = 0; j < NpInt; ++j) t += Itp[(f * NpEdge + i) * NpInt + j] * u[j];
+          ue[f * NpEdge + i] = t;
+        }
+      } else {
+        const int nb = a.sh.nbr_elem[fi], nf = a.sh.nbr_face[fi];
+        const SIN *qn = Q + size_t(nb) * NV + tn * NpInt;
+        D un[NpInt];
+#pragma unroll
+        for (int j = 0; j < NpInt; ++j) un[j] = D(qn[j]);
+#pragma unroll
+        for (int i = 0; i < NpEdge; ++i) {
+          const int p = nf * NpEdge + (NpEdge - 1 - i);
+          D t = 0;
+#pragma unroll
+          for (int j = 0; j < NpInt; ++j) t += Itp[p * NpInt + j] * un[j];
+          ue[f * NpEdge + i] = t;
+        }
+      }
+    }
+    const double Jd = a.g.Jdet[kk], ooJd = 1. / Jd;
+    const D jr = D(a.g.Jinv[size_t(dir) * Kp + kk]), js = D(a.g.Jinv[size_t(2 + dir) * Kp + kk]);
+    D dof[NpFlux];
+#pragma unroll
+    for (int i = 0; i < NpInt; ++i) {
+      dof[i] = jr * u[i];
+      dof[NpInt + i] = js * u[i];
+    }
+#pragma unroll
+    for (int f = 0; f < 3; ++f) {
+      const size_t fi = size_t(f) * Kp + kk;
+      const double nrm = dir == 0 ? a.sh.Nxo[fi] : a.sh.Nyo[fi];
+      const D w = D(ooJd * nrm * a.g.IInII[fi]);
+#pragma unroll
+      for (int i = 0; i < NpEdge; ++i) dof[2 * NpInt + f * NpEdge + i] = w * ue[f * NpEdge + i];
+    }
+    const double *epsF = a.sh.epsF + kk;                                   // [NR][Kp]
+    D *dout = static_cast<D *>(a.sh.diss) + size_t(tn * 2 * NR) * Kp + kk;  // [4][2][NR][Kp]
+#pragma unroll 3
+    for (int r = 0; r < NR; ++r) {
+      D acc = 0;
+#pragma unroll
+      for (int j = 0; j < NpFlux; ++j) acc += Div[r * NpFlux + j] * dof[j];
+      dout[size_t(dir * NR + r) * Kp] = acc * D(epsF[size_t(r) * Kp]);
+    }
   }
 }
 
@@ -1298,6 +1374,10 @@ hipError_t launch_shock_t(int pass, const StageArgs &a, hipStream_t s) {
       break;
     case 2:
       hipLaunchKernelGGL((k_shock_grad<N, SIN, D>), dim3(a.ntiles), dim3(kThreads), 0, s, a);
+      break;
+    case 3:
+      hipLaunchKernelGGL((k_shock_grad_act<N, SIN, D>),
+                         dim3(std::max(1, std::min(kGradActBlocks, (a.Kp * 8 + 255) / 256))), dim3(256), 0, s, a);
       break;
     default:
       return hipErrorInvalidValue;
