@@ -148,6 +148,36 @@ def run_gpu_shock(mesh, steps, warmup, precision):
                 rho=e.Q[0], flags=sigma > 0, active=float(np.mean(act)))
 
 
+def wedge_flags_vs_oracle(mesh):
+    """The FP64 wedge from the freestream for 2000 steps: the sensor flags
+    after 10, 500, 1000 and 2000 steps against the oracle's, bit for bit
+    (tests/golden/wedge_flags_2000.npz; the oracle takes about 90 minutes)."""
+    from gocfd_amd import Euler, InputParameters2D
+    gold = np.load(os.path.join(ROOT, "tests", "golden", "wedge_flags_2000.npz"))
+    ip = InputParameters2D(Title="wedge", CFL=1.0, FluxType="Roe", InitType="Freestream", PolynomialOrder=2,
+                           Minf=2.0, FinalTime=1e9, Limiter="PerssonC0", Kappa=SHOCK_KAPPA)
+    e = Euler(ip, mesh, precision="fp64")
+    K = int(gold["K"])
+    out = {"checkpoints": [], "flagged": [], "flags_identical": [], "time_rel": [], "dt_rel": []}
+    if e.K != K:
+        return {"error": f"mesh has {e.K} triangles, the oracle fixture {K}"}
+    done = 0
+    for n in (int(x) for x in gold["checkpoints"]):
+        e.step(n - done)
+        done = n
+        t, dt, _ = e.time_info()
+        _, sg, _ = e.shock()
+        want = np.unpackbits(gold[f"flags_{n}"])[:K].astype(bool)
+        out["checkpoints"].append(n)
+        out["flagged"].append(int((sg > 0).sum()))
+        out["flags_identical"].append(bool(np.array_equal(sg > 0, want)))
+        out["time_rel"].append(abs(t - float(gold[f"time_{n}"])) / t)
+        out["dt_rel"].append(abs(dt - float(gold[f"dt_{n}"])) / dt)
+    rho = e.Q[0]
+    out["rho_sum_rel_2000"] = abs(rho.sum() - float(gold["rho_sum_2000"])) / abs(float(gold["rho_sum_2000"]))
+    return out
+
+
 def ensemble_cases(ncases: int, seed: int = 7):
     """The batched workload: ncases independent cases, K log-uniform in
     [2k, 40k] triangles (jittered quads, n = sqrt(K/2) per side), order drawn
@@ -508,6 +538,8 @@ def main():
                  "rho_l2_fp32_vs_fp64": float(np.linalg.norm(d) / np.linalg.norm(r64s["rho"])),
                  "rho_linf_fp32_vs_fp64": float(np.abs(d).max() / np.abs(r64s["rho"]).max()),
                  "flags_agree_fp32_vs_fp64": float(np.mean(r32s["flags"] == r64s["flags"]))}
+        if SHOCK_KAPPA == 10.0 and args.shock_points == 125_000:
+            shock["fp64_flags_vs_oracle"] = wedge_flags_vs_oracle(wedge)
 
     ensemble = None
     if not args.no_ensemble:

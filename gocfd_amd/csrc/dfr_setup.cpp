@@ -861,7 +861,7 @@ DeviceLayout build_device_layout(const Discretisation &d, int E) {
         const bool in_tile = !(uint32_t(L.nbr_slot[fi]) >> 31);
         const bool owner = code & FC_FLUX_OWNER;
         if (in_tile && !owner) continue;  // the in-tile owner fills this slot
-        fl[kk / E].push_back((kk % E) | (f << 8) | ((in_tile && owner) ? 1 << 10 : 0));
+        fl[kk / E].push_back((kk % E) | (f << 8) | ((in_tile && owner) ? 1 << 10 : 0) | (in_tile ? 0 : 1 << 11));
       }
     L.Fmax = 1;
     for (auto &v : fl) L.Fmax = std::max<int>(L.Fmax, int(v.size()));
@@ -870,6 +870,28 @@ DeviceLayout build_device_layout(const Discretisation &d, int E) {
     for (int t = 0; t < L.ntiles; ++t) {
       L.flux_cnt[t] = int(fl[t].size());
       std::copy(fl[t].begin(), fl[t].end(), L.flux_list.begin() + size_t(t) * L.Fmax);
+    }
+  }
+  // Cross-tile faces per tile, with the neighbour face each one exports to
+  {
+    std::vector<std::vector<std::pair<int, int>>> xp(L.ntiles);
+    for (int kk = 0; kk < K; ++kk)
+      for (int f = 0; f < 3; ++f) {
+        const size_t fi = size_t(f) * Kp + kk;
+        if ((L.face_code[fi] & 15) != FK_Shared || !(uint32_t(L.nbr_slot[fi]) >> 31)) continue;
+        xp[kk / E].push_back({(kk % E) | (f << 8), L.nbr_face[fi] * Kp + L.nbr_elem[fi]});
+      }
+    L.Xmax = 1;
+    for (auto &v : xp) L.Xmax = std::max<int>(L.Xmax, int(v.size()));
+    L.xp_list.assign(size_t(L.ntiles) * L.Xmax, 0);
+    L.xp_dst.assign(size_t(L.ntiles) * L.Xmax, 0);
+    L.xp_cnt.assign(L.ntiles, 0);
+    for (int t = 0; t < L.ntiles; ++t) {
+      L.xp_cnt[t] = int(xp[t].size());
+      for (size_t i = 0; i < xp[t].size(); ++i) {
+        L.xp_list[size_t(t) * L.Xmax + i] = xp[t][i].first;
+        L.xp_dst[size_t(t) * L.Xmax + i] = xp[t][i].second;
+      }
     }
   }
   // Non-shared faces per tile (the boundary loop of the stage kernel)

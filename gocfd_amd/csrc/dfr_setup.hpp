@@ -111,6 +111,7 @@ struct DeviceLayout {
   // Shared faces whose Riemann flux this tile evaluates: every face with an
   // out-of-tile neighbour, and the owner side of in-tile pairs (which also
   // fills the partner's slot). [ntiles x Fmax]: e | face << 8 | partner << 10
+  // | (neighbour out of tile) << 11
   int Fmax = 0;
   std::vector<int> flux_list;
   std::vector<int> flux_cnt;                            // [ntiles]
@@ -118,6 +119,11 @@ struct DeviceLayout {
   // boundary, inert and padding faces, whose flux the boundary loop sets.
   // Tiles with none skip that loop. [ntiles]
   std::vector<int> bnd_cnt;
+  // Faces whose neighbour lies in another tile, per tile (the export list of
+  // the halo traces): [ntiles x Xmax] e | face << 8, the neighbour's face
+  // index nbr_face * Kp + nbr_elem, and [ntiles] counts
+  int Xmax = 0;
+  std::vector<int> xp_list, xp_dst, xp_cnt;
   std::vector<double> bpoint_xy;                        // IVortex boundary points [nb][NpEdge][2]
   int nbfaces = 0;
   // shock capturing and local time stepping, per face [3][Kp] / element [Kp]

@@ -80,13 +80,16 @@ struct EnsCase {
 };
 
 struct EnsGroup {
-  int N = 0, Kp = 0, ntiles = 0, Hmax = 1, Wmax = 1, Fmax = 1, NV = 0;
+  int N = 0, Kp = 0, ntiles = 0, Hmax = 1, Wmax = 1, Fmax = 1, Xmax = 1, NV = 0;
   bool shock = false, lts = false;  // PerssonC0 limiter, LocalTimeStepping (the same for the group)
   double kappa = 0;
   int Nv = 0;                       // vertices of the group's cases
   std::vector<int> cases;
   DevBuf<double> Jdet, Jinv, IInII, Nx, Ny, oohK, bxy, Interp64;
   DevBuf<int> code, nbr_slot, halo_elem, halo_face, wave_list, wave_cnt, flux_list, flux_cnt, bnd_cnt, tile_case;
+  // halo traces (without shock capturing): export lists and the two buffers
+  DevBuf<int> xp_list, xp_dst, xp_cnt;
+  DevBuf<double> ht;
   void *q[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
   void *R = nullptr;
   std::vector<double> q0_init;         // initial element-major state (tuning restores it)
@@ -152,6 +155,7 @@ struct gocfd_ensemble {
   dev::StageArgs args(const EnsGroup &g) const {
     dev::StageArgs a{};
     a.K = g.Kp; a.Kp = g.Kp; a.ntiles = g.ntiles; a.Hmax = g.Hmax; a.Wmax = g.Wmax; a.Fmax = g.Fmax;
+    a.Xmax = g.Xmax;
     a.xcd_tiles = xcd_tiles;
     a.q0 = g.q[0]; a.q1 = g.q[1]; a.q2 = g.q[2]; a.q3 = g.q[3]; a.q4 = g.q[4]; a.R = g.R;
     a.ctl = ctl;
@@ -162,6 +166,8 @@ struct gocfd_ensemble {
     a.g.halo_elem = g.halo_elem.p; a.g.halo_face = g.halo_face.p; a.g.bxy = g.bxy.p;
     a.g.wave_list = g.wave_list.p; a.g.wave_cnt = g.wave_cnt.p;
     a.g.flux_list = g.flux_list.p; a.g.flux_cnt = g.flux_cnt.p; a.g.bnd_cnt = g.bnd_cnt.p;
+    a.g.xp_list = g.xp_list.p; a.g.xp_dst = g.xp_dst.p; a.g.xp_cnt = g.xp_cnt.p;
+    a.ht = g.ht.p;
     a.Interp64 = g.Interp64.p;
     const EnsCase &c0 = *cases[g.cases[0]];
     a.ph = phys_of(c0.fs, c0.prm);
@@ -200,8 +206,10 @@ struct gocfd_ensemble {
       a.qs = g->q[0];
       a.ring = 0;
       a.par2 = 0;
+      a.hpar = 0;
       a.step_index = 0;
       HIPCHECK(dev::launch_prologue(g->N, prec, a, stream));
+      HIPCHECK(dev::launch_halo_export(g->N, prec, a, stream));
     }
   }
 
@@ -215,6 +223,7 @@ struct gocfd_ensemble {
         a.step_index = step_index;
         a.ring = int(gstage % 3);
         a.par2 = int(gstage & 1);
+        a.hpar = a.par2;
         a.qs = g->q[s];
         hipStream_t st = branches ? g->stream : stream;
         if (g->shock) {
@@ -380,6 +389,7 @@ void build_group(gocfd_ensemble &h, EnsGroup &g) {
     g.Hmax = std::max(g.Hmax, c.L.Hmax);
     g.Wmax = std::max(g.Wmax, c.L.Wmax);
     g.Fmax = std::max(g.Fmax, c.L.Fmax);
+    g.Xmax = std::max(g.Xmax, c.L.Xmax);
   }
   g.Kp = off;
   g.ntiles = off / E;
@@ -391,6 +401,7 @@ void build_group(gocfd_ensemble &h, EnsGroup &g) {
   std::vector<int> halo_elem(size_t(g.ntiles) * g.Hmax, -1), halo_face(size_t(g.ntiles) * g.Hmax, 0);
   std::vector<int> wave_list(size_t(g.ntiles) * g.Wmax, 0), wave_cnt(g.ntiles, 0);
   std::vector<int> flux_list(size_t(g.ntiles) * g.Fmax, 0), flux_cnt(g.ntiles, 0), bnd_cnt(g.ntiles, 0);
+  std::vector<int> xp_list(size_t(g.ntiles) * g.Xmax, 0), xp_dst(size_t(g.ntiles) * g.Xmax, 0), xp_cnt(g.ntiles, 0);
   std::vector<double> q0(Kp * g.NV);
   for (int ci : g.cases) {
     const EnsCase &c = *h.cases[ci];
@@ -425,6 +436,13 @@ void build_group(gocfd_ensemble &h, EnsGroup &g) {
       flux_cnt[t0 + t] = L.flux_cnt[t];
       bnd_cnt[t0 + t] = L.bnd_cnt[t];
       for (int i = 0; i < L.flux_cnt[t]; ++i) flux_list[size_t(t0 + t) * g.Fmax + i] = L.flux_list[size_t(t) * L.Fmax + i];
+      xp_cnt[t0 + t] = L.xp_cnt[t];
+      for (int i = 0; i < L.xp_cnt[t]; ++i) {
+        // the neighbour face f * kp + kk of the case -> f * Kp + off + kk of the group
+        const int d = L.xp_dst[size_t(t) * L.Xmax + i], f = d / int(kp), kk = d - f * int(kp);
+        xp_list[size_t(t0 + t) * g.Xmax + i] = L.xp_list[size_t(t) * L.Xmax + i];
+        xp_dst[size_t(t0 + t) * g.Xmax + i] = f * int(Kp) + int(o) + kk;
+      }
     }
     fill_element_major(L, NpInt, c.Q0.data(), q0.data() + o * g.NV);
   }
@@ -434,6 +452,10 @@ void build_group(gocfd_ensemble &h, EnsGroup &g) {
   g.wave_list.upload(wave_list); g.wave_cnt.upload(wave_cnt);
   g.flux_list.upload(flux_list); g.flux_cnt.upload(flux_cnt); g.bnd_cnt.upload(bnd_cnt);
   g.tile_case.upload(tile_case);
+  if (!g.shock) {
+    g.xp_list.upload(xp_list); g.xp_dst.upload(xp_dst); g.xp_cnt.upload(xp_cnt);
+    g.ht.alloc(size_t(2) * 3 * Kp * NpEdge * 4);  // FP64 size, enough for either arithmetic
+  }
   const EnsCase &c0 = *h.cases[g.cases[0]];
   g.Interp64.upload(c0.d.ops.Interp);
   std::vector<double> divT(c0.d.ops.DivInt.size());

@@ -116,6 +116,10 @@ int gocfd_euler2d_create(const gocfd_mesh *mesh, const gocfd_params *p, gocfd_eu
   h->wave_list.upload(L.wave_list); h->wave_cnt.upload(L.wave_cnt);
   h->flux_list.upload(L.flux_list); h->flux_cnt.upload(L.flux_cnt); h->bnd_cnt.upload(L.bnd_cnt);
   h->perm.upload(L.perm);
+  if (!h->shock) {
+    h->xp_list.upload(L.xp_list); h->xp_dst.upload(L.xp_dst); h->xp_cnt.upload(L.xp_cnt);
+    h->ht.alloc(size_t(2) * 3 * L.Kp * (h->d.N + 2) * 4);  // FP64 size, enough for either arithmetic
+  }
   noalloc_setup(h->device);
   {
     dev::CaseInfo ci{};
@@ -221,14 +225,7 @@ int gocfd_euler2d_rhs(gocfd_euler2d *h, const double *Q, double t, double *RHS) 
   if (!h->scratch_in) HIPCHECK(hipMalloc(&h->scratch_in, tot * sizeof(double)));
   if (h->rhs_out.n < tot) h->rhs_out.alloc(tot);
   h->to_device(Q, h->scratch_in);
-  dev::StageArgs a = h->args();
-  a.stage = -1;
-  a.t_rhs = t;
-  a.qs = h->scratch_in;
-  a.rhs_out = h->rhs_out.p;
-  a.ring = int(h->gstage % 3);
-  a.par2 = int(h->gstage & 1);
-  a.step_index = h->step_index;
+  dev::StageArgs a = h->rhs_args(h->scratch_in, t, h->rhs_out.p, h->stream);
   h->launch_one_stage(a);
   if (h->shock) {  // the vertex buffer this evaluation used must be clear for the next stage
     HIPCHECK(dev::launch_fill_u64(h->sigV.p, h->sigV.n, 0ull, h->stream));

@@ -40,6 +40,9 @@ struct gocfd_euler2d {
   DevBuf<double> Jdet, Jinv, IInII, Nx, Ny, oohK, bxy, Interp64;
   DevBuf<int> code, nbr_slot, halo_elem, halo_face, wave_list, wave_cnt, flux_list, flux_cnt, bnd_cnt;
   DevBuf<int> perm;  // [Kp] device element -> reference element (-1: padding)
+  // halo traces (without shock capturing): export lists and the two buffers
+  DevBuf<int> xp_list, xp_dst, xp_cnt;
+  DevBuf<double> ht;
   // local time stepping
   bool lts = false;
   DevBuf<double> fw, fwv, DTk, DTv;
@@ -78,6 +81,7 @@ struct gocfd_euler2d {
   dev::StageArgs args() const {
     dev::StageArgs a{};
     a.K = L.K; a.Kp = L.Kp; a.ntiles = L.ntiles; a.Hmax = L.Hmax; a.Wmax = L.Wmax; a.Fmax = L.Fmax;
+    a.Xmax = L.Xmax;
     a.xcd_tiles = 1;
     a.q0 = q[0]; a.q1 = q[1]; a.q2 = q[2]; a.q3 = q[3]; a.q4 = q[4]; a.R = R;
     a.ctl = ctl;
@@ -86,6 +90,8 @@ struct gocfd_euler2d {
     a.g.halo_elem = halo_elem.p; a.g.halo_face = halo_face.p; a.g.bxy = bxy.p;
     a.g.wave_list = wave_list.p; a.g.wave_cnt = wave_cnt.p;
     a.g.flux_list = flux_list.p; a.g.flux_cnt = flux_cnt.p; a.g.bnd_cnt = bnd_cnt.p;
+    a.g.xp_list = xp_list.p; a.g.xp_dst = xp_dst.p; a.g.xp_cnt = xp_cnt.p;
+    a.ht = ht.p;
     a.Interp64 = Interp64.p;
     a.lts.on = lts ? 1 : 0;
     a.lts.fw = fw.p; a.lts.fwv = fwv.p; a.lts.wsrc = wsrc.p; a.lts.DTk = DTk.p; a.lts.DTv = DTv.p;
@@ -181,9 +187,28 @@ struct gocfd_euler2d {
       a.qs = q[0];
       a.ring = 0;
       a.par2 = 0;
+      a.hpar = 0;
       a.step_index = 0;
       HIPCHECK(dev::launch_prologue(prm.polynomial_order, prec, a, stream));
+      HIPCHECK(dev::launch_halo_export(prm.polynomial_order, prec, a, stream));
     }
+  }
+
+  // Arguments of an RHS evaluation of the element-major FP64 state qs into
+  // out, with its halo traces in the buffer the next stage does not read
+  // (the one the last stage read); on stream s
+  dev::StageArgs rhs_args(const void *qs, double t, double *out, hipStream_t s) {
+    dev::StageArgs a = args();
+    a.stage = -1;
+    a.t_rhs = t;
+    a.qs = const_cast<void *>(qs);
+    a.rhs_out = out;
+    a.ring = int(gstage % 3);
+    a.par2 = int(gstage & 1);
+    a.hpar = a.par2 ^ 1;
+    a.step_index = step_index;
+    if (!shock) HIPCHECK(dev::launch_halo_export(prm.polynomial_order, prec, a, s));
+    return a;
   }
 
   // One stage: the shock-capturing passes (if any), then the stage kernel,
@@ -208,6 +233,7 @@ struct gocfd_euler2d {
         a.step_index = step_index;
         a.ring = int(gstage % 3);
         a.par2 = int(gstage & 1);
+        a.hpar = a.par2;
         a.qs = q[s];
         launch_one_stage(a);
         ++gstage;

@@ -37,6 +37,11 @@ struct Geom {
   const int *flux_list, *flux_cnt;                     // [ntiles][Fmax], [ntiles]
   const int *bnd_cnt;                                  // [ntiles] non-shared faces
   const double *bxy;                                   // IVortex boundary points
+  // Faces whose neighbour lies in another tile, per tile: after a stage the
+  // tile writes its traces of the new state there into the neighbour's face
+  // slots of the halo-trace buffer. [ntiles][Xmax]: e | face << 8, and the
+  // neighbour's face index (f * Kp + kk); [ntiles] counts
+  const int *xp_list, *xp_dst, *xp_cnt;
 };
 
 // A far-field state of the Riemann-invariant boundary (FreeStream,
@@ -102,12 +107,17 @@ struct ShockArgs {
 };
 
 struct StageArgs {
-  int K, Kp, ntiles, Hmax, Wmax, Fmax;
+  int K, Kp, ntiles, Hmax, Wmax, Fmax, Xmax;
   int xcd_tiles;       // 1: each XCD takes a contiguous tile range; 0: tile = workgroup id
   int stage;           // 0..4, or -1 for the RHS-only evaluation
   int step_index;      // steps since the state was set (selects time parity)
   int ring;            // global stage counter mod 3
   int par2;            // global stage counter mod 2
+  int hpar;            // halo-trace buffer holding this stage's input traces
+  // Halo-trace buffers [2][3 Kp][NpEdge][4] in the stage arithmetic type
+  // (without shock capturing): the traces of each cross-tile face's
+  // neighbour, at the face's own points, written by the neighbour's tile
+  void *ht;
   double t_rhs;        // time for the RHS-only evaluation
   void *qs;            // stage input state (element-major)
   void *q0, *q1, *q2, *q3, *q4, *R;
@@ -129,6 +139,9 @@ struct StageArgs {
 // shock != 0 selects the PerssonC0 stage kernel.
 hipError_t launch_stage(int N, int prec, int shock, const StageArgs &a, hipStream_t s);
 hipError_t launch_prologue(int N, int prec, const StageArgs &a, hipStream_t s);
+// Halo traces of the FP64 state a.qs into buffer a.hpar (a state set outside
+// the step sequence; stage launches write them for their new state)
+hipError_t launch_halo_export(int N, int prec, const StageArgs &a, hipStream_t s);
 // Shock-capturing passes of one stage: 0 sensor + vertex max, 1 element
 // fields (+ stage-state filter, active-element list), 2 wave speeds,
 // 3 eps * gradient of the active elements.
@@ -145,6 +158,7 @@ size_t stage_lds_bytes(int N, int prec, int Hmax);
 // Per-order entry points (one translation unit per order)
 template <int N> hipError_t launch_stage_n(int prec, int shock, const StageArgs &a, hipStream_t s);
 template <int N> hipError_t launch_prologue_n(int prec, const StageArgs &a, hipStream_t s);
+template <int N> hipError_t launch_halo_export_n(int prec, const StageArgs &a, hipStream_t s);
 template <int N> hipError_t launch_shock_n(int prec, int pass, const StageArgs &a, hipStream_t s);
 template <int N> hipError_t upload_operators_n(const double *divT, const double *interp);
 

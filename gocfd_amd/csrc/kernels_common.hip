@@ -30,6 +30,12 @@ hipError_t launch_prologue(int N, int prec, const StageArgs &a, hipStream_t s) {
 #undef CALL
 }
 
+hipError_t launch_halo_export(int N, int prec, const StageArgs &a, hipStream_t s) {
+#define CALL(n) launch_halo_export_n<n>(prec, a, s)
+  GOCFD_DISPATCH_ORDER(N, CALL)
+#undef CALL
+}
+
 hipError_t launch_shock(int N, int prec, int pass, const StageArgs &a, hipStream_t s) {
 #define CALL(n) launch_shock_n<n>(prec, pass, a, s)
   GOCFD_DISPATCH_ORDER(N, CALL)
@@ -42,8 +48,9 @@ hipError_t upload_operators(int N, const double *divT, const double *interp) {
 #undef CALL
 }
 
-// Dynamic LDS of the stage kernel: the tile state [NV][kTile+1] and the
-// trace/flux buffer [4][3 NpEdge kTile + NpEdge Hmax].
+// Dynamic LDS of the stage kernel, at most: the tile state [NV][kTile+1]
+// and the trace/flux buffer [4][3 NpEdge kTile + NpEdge Hmax] (the halo
+// part only with shock capturing).
 size_t stage_lds_bytes(int N, int prec, int Hmax) {
   if (N < 0 || N > 4) return 0;
   const size_t cs = prec == 0 ? sizeof(double) : sizeof(float);

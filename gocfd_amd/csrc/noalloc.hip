@@ -101,14 +101,7 @@ extern "C" int gocfd_euler2d_rhs_noalloc(gocfd_euler2d *h, const double *Q, doub
   hipLaunchKernelGGL(k_gather_state, dim3(grid_for(vals)), dim3(256), 0, s, Q, h->perm.p, h->L.K, h->L.Kp, NpInt,
                      in);
   HIPCHECK(hipGetLastError());
-  dev::StageArgs a = h->args();
-  a.stage = -1;
-  a.t_rhs = t;
-  a.qs = in;
-  a.rhs_out = out;
-  a.ring = int(h->gstage % 3);
-  a.par2 = int(h->gstage & 1);
-  a.step_index = h->step_index;
+  dev::StageArgs a = h->rhs_args(in, t, out, s);
   h->launch_one_stage(a, s);
   if (h->shock) {  // the vertex buffers this evaluation used must be clear for the next stage
     HIPCHECK(dev::launch_fill_u64(h->sigV.p, h->sigV.n, 0ull, s));

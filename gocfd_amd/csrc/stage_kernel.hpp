@@ -76,6 +76,13 @@ struct UseMfma {
 };
 typedef float floatx4 __attribute__((ext_vector_type(4)));
 typedef double doublex4 __attribute__((ext_vector_type(4)));
+// the four variables of one trace point, one vector access
+template <class C>
+struct Vec4;
+template <>
+struct Vec4<float> { using type = floatx4; };
+template <>
+struct Vec4<double> { using type = doublex4; };
 
 // 16x16x4 MFMA tiles in FP32 and FP64. Operands: lane l holds A[l%16][l/16]
 // and B[l/16][l%16]; result element v of lane l is D[row(l/16, v)][l%16],
@@ -284,13 +291,20 @@ k_stage(StageArgs a) {
   constexpr int E = kTile, P = kTile + 1;
   static_assert(kThreads == 4 * kTile, "one thread per (element, variable)");
   extern __shared__ __align__(16) unsigned char smem_raw[];
+  // Without shock capturing the traces of cross-tile neighbours come from
+  // the halo-trace buffer, which the neighbours' tiles filled after the
+  // previous stage; with it (the stage state may be filtered after the
+  // previous stage) each tile interpolates them from the neighbours' state
+  // into a halo part of the trace buffer.
+  constexpr bool HX = !SHOCK;
+  using V4 = typename Vec4<C>::type;
   C *sQ = reinterpret_cast<C *>(smem_raw);  // [NV][P]
-  const int TS = NF * E + NpEdge * a.Hmax;
+  const int TS = NF * E + (HX ? 0 : NpEdge * a.Hmax);
   C *sT = sQ + NV * P;                       // [4][TS]: traces, then fluxes
   C *sR = sT;                                // [NV][P]: RHS (aliases sT)
   // local time stepping only: element dt and per-face speed maxima, after
   // the tile buffers (the launch adds their bytes only when LTS is on)
-  double *sDT = reinterpret_cast<double *>(smem_raw + lds_tile_bytes<N>(sizeof(C), a.Hmax));
+  double *sDT = reinterpret_cast<double *>(smem_raw + lds_tile_bytes<N>(sizeof(C), HX ? 0 : a.Hmax));
   unsigned long long *sWf = reinterpret_cast<unsigned long long *>(sDT + kTile);
   __shared__ double red[kThreads / 64];
 
@@ -315,7 +329,7 @@ k_stage(StageArgs a) {
   }
   const int nbnd = a.g.bnd_cnt[tile];  // non-shared faces: the boundary loop runs only if any
   int hk0 = -1, hf0 = 0;
-  if (tid < a.Hmax * 4) {
+  if (!HX && tid < a.Hmax * 4) {
     const int h = tid % a.Hmax;
     hk0 = a.g.halo_elem[size_t(tile) * a.Hmax + h];
     hf0 = a.g.halo_face[size_t(tile) * a.Hmax + h];
@@ -332,6 +346,21 @@ k_stage(StageArgs a) {
     for (int j = 0; j < LPT; ++j) {
       const int idx = tid + j * kThreads;
       v[j] = (idx < E * NV) ? qt[idx] : S(0);
+    }
+  }
+  // the neighbour traces of the cross-tile faces, under the state loads
+  V4 hx[HX ? ITB : 1];
+  if constexpr (HX) {
+    const V4 *hin = static_cast<const V4 *>(a.ht) + size_t(a.hpar) * 3 * size_t(Kp) * NpEdge;
+#pragma unroll
+    for (int j = 0; j < ITB; ++j) {
+      const int it = tid + j * kThreads;
+      hx[j] = V4{C(0), C(0), C(0), C(0)};
+      if (it < nfl && (fpk[j] & (1 << 11))) {
+        const int w = it / NpEdge, pp = it - w * NpEdge;
+        const int e = fpk[j] & 255, f = (fpk[j] >> 8) & 3;
+        hx[j] = hin[(size_t(f) * Kp + k0 + e) * NpEdge + pp];
+      }
     }
   }
 
@@ -380,7 +409,7 @@ k_stage(StageArgs a) {
       sT[tn * TS + p * E + te] = t;
     }
   }
-  for (int it = tid; it < a.Hmax * 4; it += kThreads) {
+  for (int it = tid; it < (HX ? 0 : a.Hmax * 4); it += kThreads) {
     const int h = it % a.Hmax, n = it / a.Hmax;
     const int kh = it == tid ? hk0 : a.g.halo_elem[size_t(tile) * a.Hmax + h];
     if (kh < 0) continue;
@@ -424,11 +453,13 @@ k_stage(StageArgs a) {
       const int base = int(slot & 0x7fffffffu);
       const int stride = (slot >> 31) ? 1 : E;
       const int rp = NpEdge - 1 - pp;
-      const int idx = base + rp * stride;
+      const bool xt = HX && (packed & (1 << 11));  // neighbour traces in hx[j]
+      const int idx = xt ? p * E + e : base + rp * stride;
 #pragma unroll
       for (int c = 0; c < 4; ++c) {
         own[c] = sT[c * TS + p * E + e];
         oth[c] = sT[c * TS + idx];
+        if (xt) oth[c] = hx[j][c];
       }
       const C nx = C(a.g.Nx[fi]), ny = C(a.g.Ny[fi]);
       const bool owner = code & FC_FLUX_OWNER;
@@ -846,7 +877,21 @@ k_stage(StageArgs a) {
     }
     __syncthreads();
 
-    // ---- G: owner-side wave speed of the next stage, max over the tile ----
+    // ---- G: owner-side wave speed of the next stage, max over the tile,
+    // and the traces of the cross-tile faces to the neighbours' slots of the
+    // next halo-trace buffer (at the neighbour's point order); the export
+    // list loads go out first ----
+    const int nxp = a.g.xp_cnt[tile] * NpEdge;
+    int xpk[ITG], xpd[ITG];
+    {
+      const int *xl = a.g.xp_list + size_t(tile) * a.Xmax, *xd = a.g.xp_dst + size_t(tile) * a.Xmax;
+#pragma unroll
+      for (int j = 0; j < ITG; ++j) {
+        const int it = tid + j * kThreads;
+        xpk[j] = it < nxp ? xl[it / NpEdge] : 0;
+        xpd[j] = it < nxp ? xd[it / NpEdge] : 0;
+      }
+    }
 #pragma unroll
     for (int j = 0; j < ITG; ++j) {
       const int it = tid + j * kThreads;
@@ -868,6 +913,17 @@ k_stage(StageArgs a) {
         if (!(wsp == wsp)) bad = 1;
         wl = fmax(wl, wsp);
         if (lts) atomicMax(&sWf[e * 3 + f], bits(wsp));
+      }
+    }
+    V4 *hout = static_cast<V4 *>(a.ht) + size_t(a.hpar ^ 1) * 3 * size_t(Kp) * NpEdge;
+#pragma unroll
+    for (int j = 0; j < ITG; ++j) {
+      const int it = tid + j * kThreads;
+      if (it < nxp) {
+        const int w = it / NpEdge, pp = it - w * NpEdge;
+        const int e = xpk[j] & 255, f = (xpk[j] >> 8) & 3, p = f * NpEdge + pp;
+        hout[size_t(xpd[j]) * NpEdge + (NpEdge - 1 - pp)] =
+            V4{sT[p * E + e], sT[TS + p * E + e], sT[2 * TS + p * E + e], sT[3 * TS + p * E + e]};
       }
     }
   }
@@ -947,6 +1003,39 @@ __global__ void __launch_bounds__(256) k_prologue(StageArgs a) {
   if (a.lts.on) a.lts.fw[size_t(a.par2) * 3 * a.Kp + fi] = wl;
   atomic_max_bits(&ctl->wmax[a.ring], wl);
   if (bad) atomicOr(&ctl->nan, 1);
+}
+
+// Halo traces of a state set outside the step sequence (a restart, an RHS
+// evaluation), into halo-trace buffer a.hpar: thread (tile, export item,
+// variable) interpolates one variable of one element to the NpEdge points of
+// one cross-tile face, in the stage kernel's arithmetic type C, and stores
+// them at the neighbour's point order. The stage kernel does the same for
+// its new state after every stage.
+template <int N, class SIN, class C>
+__global__ void __launch_bounds__(256) k_halo_export(StageArgs a) {
+  using Sh = Shape<N>;
+  constexpr int NpInt = Sh::NpInt, NpEdge = Sh::NpEdge, NV = Sh::NV;
+  const int it = blockIdx.x * blockDim.x + threadIdx.x;
+  const int n = it & 3, item = it >> 2;
+  const int tile = item / a.Xmax, w = item - tile * a.Xmax;
+  if (tile >= a.ntiles || w >= a.g.xp_cnt[tile]) return;
+  const int packed = a.g.xp_list[size_t(tile) * a.Xmax + w];
+  const size_t dst = size_t(a.g.xp_dst[size_t(tile) * a.Xmax + w]);
+  const int e = packed & 255, f = (packed >> 8) & 3;
+  const SIN *qn = static_cast<const SIN *>(a.qs) + size_t(tile * kTile + e) * NV + n * NpInt;
+  C q[NpInt];
+#pragma unroll
+  for (int i = 0; i < NpInt; ++i) q[i] = C(qn[i]);
+  const C *Itp = ops_base<C>() + Sh::NpFlux * NpInt;
+  C *hout = static_cast<C *>(a.ht) + size_t(a.hpar) * 3 * size_t(a.Kp) * NpEdge * 4;
+#pragma unroll
+  for (int pp = 0; pp < NpEdge; ++pp) {
+    const C *row = Itp + (f * NpEdge + pp) * NpInt;
+    C t = C(0);
+#pragma unroll
+    for (int i = 0; i < NpInt; ++i) t += row[i] * q[i];
+    hout[(dst * NpEdge + (NpEdge - 1 - pp)) * 4 + n] = t;
+  }
 }
 
 // ---- Shock capturing ----
@@ -1306,7 +1395,7 @@ size_t lds_bytes_n(size_t csize, int Hmax, bool lts) {
 
 template <int N, class S, class SIN, class C, int FLUX, bool LAST, bool SHOCK>
 hipError_t launch_stage_t(const StageArgs &a, hipStream_t s) {
-  const size_t lds = lds_bytes_n<N>(sizeof(C), a.Hmax, a.lts.on != 0);
+  const size_t lds = lds_bytes_n<N>(sizeof(C), SHOCK ? a.Hmax : 0, a.lts.on != 0);
   static size_t attr_bytes = 0;  // dynamic LDS the kernel is currently allowed
   if (lds > 64 * 1024 && lds > attr_bytes) {
     hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void *>(&k_stage<N, S, SIN, C, FLUX, LAST, SHOCK>),
@@ -1360,6 +1449,16 @@ hipError_t launch_prologue_n(int prec, const StageArgs &a, hipStream_t s) {
   const int blocks = int((items + 255) / 256);
   (void)prec;  // the prologue reads q0, which is FP64 in every precision
   hipLaunchKernelGGL((k_prologue<N, double>), dim3(blocks), dim3(256), 0, s, a);
+  return hipGetLastError();
+}
+
+template <int N>
+hipError_t launch_halo_export_n(int prec, const StageArgs &a, hipStream_t s) {
+  if (!a.ht) return hipErrorInvalidValue;
+  const long items = long(a.ntiles) * a.Xmax * 4;
+  const int blocks = int((items + 255) / 256);
+  if (prec == 0) hipLaunchKernelGGL((k_halo_export<N, double, double>), dim3(blocks), dim3(256), 0, s, a);
+  else hipLaunchKernelGGL((k_halo_export<N, double, BulkC<N>>), dim3(blocks), dim3(256), 0, s, a);
   return hipGetLastError();
 }
 
@@ -1456,6 +1555,7 @@ hipError_t upload_operators_n(const double *divT, const double *interp) {
   namespace dev {                                                                                        \
   template hipError_t launch_stage_n<NN>(int, int, const StageArgs &, hipStream_t);                      \
   template hipError_t launch_prologue_n<NN>(int, const StageArgs &, hipStream_t);                        \
+  template hipError_t launch_halo_export_n<NN>(int, const StageArgs &, hipStream_t);                     \
   template hipError_t launch_shock_n<NN>(int, int, const StageArgs &, hipStream_t);                      \
   template hipError_t upload_operators_n<NN>(const double *, const double *);                           \
   }                                                                                                      \

@@ -10,7 +10,9 @@ with OpenMP over the host cores).
   1e-5 of FP64.
 * The 125k-point wedge channel (about 250k triangles), P=2, Roe, PerssonC0
   (Kappa 10): 10 FP64 steps from the freestream match the oracle to 1e-10,
-  with identical shock-cell flags.
+  with identical shock-cell flags; after 10, 500, 1000 and 2000 steps the
+  flags are the oracle's bit for bit (a committed fixture: the oracle needs
+  about 90 minutes for 2000 steps).
 """
 import math
 import os
@@ -91,3 +93,33 @@ def test_wedge_full_size_shock_parity(tmp_path):
     print(f"wedge: {(sg_o > 0).sum()} of {o.K} elements flagged after 10 steps", flush=True)
     assert (sg_o > 0).sum() > 0  # the comparison below is not vacuous
     assert np.array_equal(sg_g > 0, sg_o > 0)
+
+
+def test_wedge_flags_2000_steps_match_oracle(tmp_path):
+    """The sensor flags (sigma > 0 per element) of the FP64 wedge after 10,
+    500, 1000 and 2000 steps are the oracle's, bit for bit
+    (tests/golden/wedge_flags_2000.npz, made by make_wedge_flags.py beside
+    it); time and dt agree to 1e-12 relative, and the density's sum and max
+    at 2000 steps to 1e-12."""
+    gold = np.load(os.path.join(os.path.dirname(__file__), "golden", "wedge_flags_2000.npz"))
+    mesh = meshgen.wedge_mesh(str(tmp_path / "wedge.su2"), 125_000, seed=1)
+    ip = InputParameters2D(CFL=1.0, FluxType="Roe", InitType="Freestream", PolynomialOrder=2, Minf=2.0,
+                           FinalTime=1e9, Limiter="PerssonC0", Kappa=10.0)
+    g = Euler(ip, mesh, precision="fp64")
+    K = int(gold["K"])
+    assert g.K == K
+    done = 0
+    for n in gold["checkpoints"]:
+        g.step(int(n) - done)
+        done = int(n)
+        t, dt, steps = g.time_info()
+        _, sg, _ = g.shock()
+        want = np.unpackbits(gold[f"flags_{n}"])[:K].astype(bool)
+        print(f"wedge step {n}: {int((sg > 0).sum())} flagged (oracle {int(want.sum())})", flush=True)
+        assert want.sum() > 0
+        assert np.array_equal(sg > 0, want), f"step {n}: {int(np.sum((sg > 0) != want))} flags differ"
+        assert abs(t - float(gold[f"time_{n}"])) <= 1e-12 * t
+        assert abs(dt - float(gold[f"dt_{n}"])) <= 1e-12 * dt
+    rho = g.Q[0]
+    assert abs(rho.sum() - float(gold["rho_sum_2000"])) <= 1e-12 * abs(float(gold["rho_sum_2000"]))
+    assert abs(rho.max() - float(gold["rho_max_2000"])) <= 1e-12 * float(gold["rho_max_2000"])
