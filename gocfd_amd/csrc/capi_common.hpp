@@ -58,6 +58,18 @@ struct DevBuf {
   ~DevBuf() { free(); }
 };
 
+// A host array of doubles on the device in the stage kernel's arithmetic
+// type: FP32 for the FP32-bulk precision from N=2 on, else FP64
+struct StageTypeBuf {
+  DevBuf<double> d;
+  DevBuf<float> f;
+  void upload(const std::vector<double> &v, int prec, int N) {
+    if (prec != 0 && N >= 2) f.upload(std::vector<float>(v.begin(), v.end()));
+    else d.upload(v);
+  }
+  const void *ptr() const { return f.p ? static_cast<const void *>(f.p) : static_cast<const void *>(d.p); }
+};
+
 // Freestream (model_problems/Euler2D/fluids.go:245-273)
 struct FreeStream {
   double gamma = 1.4, minf = 0, alpha = 0, q[4] = {1, 0, 0, 0}, pinf = 0, cinf = 0;

@@ -2,6 +2,7 @@
 #include "dfr_setup.hpp"
 
 #include <algorithm>
+#include <array>
 #include <cctype>
 #include <cmath>
 #include <cstdio>
@@ -853,6 +854,7 @@ DeviceLayout build_device_layout(const Discretisation &d, int E) {
   // Shared faces with Riemann work, per tile
   {
     std::vector<std::vector<int>> fl(L.ntiles);
+    std::vector<std::vector<std::array<double, 4>>> fg(L.ntiles);
     for (int kk = 0; kk < K; ++kk)
       for (int f = 0; f < 3; ++f) {
         const size_t fi = size_t(f) * Kp + kk;
@@ -861,15 +863,27 @@ DeviceLayout build_device_layout(const Discretisation &d, int E) {
         const bool in_tile = !(uint32_t(L.nbr_slot[fi]) >> 31);
         const bool owner = code & FC_FLUX_OWNER;
         if (in_tile && !owner) continue;  // the in-tile owner fills this slot
-        fl[kk / E].push_back((kk % E) | (f << 8) | ((in_tile && owner) ? 1 << 10 : 0) | (in_tile ? 0 : 1 << 11));
+        const int base = int(uint32_t(L.nbr_slot[fi]) & 0x7fffffffu);
+        if (base >= (1 << 18)) throw SetupError(GOCFD_ERR_MESH, "tile halo too large for the flux-list encoding");
+        fl[kk / E].push_back((kk % E) | (f << 8) | ((in_tile && owner) ? 1 << 10 : 0) | (in_tile ? 0 : 1 << 11) |
+                             (owner ? 1 << 12 : 0) | (base << 13));
+        double w2 = 0.0;
+        if (in_tile) {  // the partner slot pf * NpEdge * E + e2
+          const int pf = base / (NpEdge * E), e2 = base - pf * NpEdge * E;
+          w2 = -L.IInII[size_t(pf) * Kp + (kk / E) * E + e2];
+        }
+        fg[kk / E].push_back({L.Nx[fi], L.Ny[fi], owner ? L.IInII[fi] : -L.IInII[fi], w2});
       }
     L.Fmax = 1;
     for (auto &v : fl) L.Fmax = std::max<int>(L.Fmax, int(v.size()));
     L.flux_list.assign(size_t(L.ntiles) * L.Fmax, 0);
+    L.flux_geo.assign(size_t(L.ntiles) * L.Fmax * 4, 0.0);
     L.flux_cnt.assign(L.ntiles, 0);
     for (int t = 0; t < L.ntiles; ++t) {
       L.flux_cnt[t] = int(fl[t].size());
       std::copy(fl[t].begin(), fl[t].end(), L.flux_list.begin() + size_t(t) * L.Fmax);
+      for (size_t i = 0; i < fg[t].size(); ++i)
+        for (int c = 0; c < 4; ++c) L.flux_geo[(size_t(t) * L.Fmax + i) * 4 + c] = fg[t][i][c];
     }
   }
   // Cross-tile faces per tile, with the neighbour face each one exports to

@@ -111,9 +111,13 @@ struct DeviceLayout {
   // Shared faces whose Riemann flux this tile evaluates: every face with an
   // out-of-tile neighbour, and the owner side of in-tile pairs (which also
   // fills the partner's slot). [ntiles x Fmax]: e | face << 8 | partner << 10
-  // | (neighbour out of tile) << 11
+  // | (neighbour out of tile) << 11 | (flux owner) << 12 | nbr_slot base << 13
   int Fmax = 0;
   std::vector<int> flux_list;
+  // per item [ntiles x Fmax][4]: the owner normal (nx, ny), this side's
+  // IInII signed by the side (+ owner, - other side) and the in-tile
+  // partner's IInII negated (0 without partner)
+  std::vector<double> flux_geo;
   std::vector<int> flux_cnt;                            // [ntiles]
   // Faces of the tile's elements (padding included) that are not shared:
   // boundary, inert and padding faces, whose flux the boundary loop sets.

@@ -90,6 +90,7 @@ struct EnsGroup {
   // halo traces (without shock capturing): export lists and the two buffers
   DevBuf<int> xp_list, xp_dst, xp_cnt;
   DevBuf<double> ht;
+  StageTypeBuf flux_geo;
   void *q[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
   void *R = nullptr;
   std::vector<double> q0_init;         // initial element-major state (tuning restores it)
@@ -166,6 +167,7 @@ struct gocfd_ensemble {
     a.g.halo_elem = g.halo_elem.p; a.g.halo_face = g.halo_face.p; a.g.bxy = g.bxy.p;
     a.g.wave_list = g.wave_list.p; a.g.wave_cnt = g.wave_cnt.p;
     a.g.flux_list = g.flux_list.p; a.g.flux_cnt = g.flux_cnt.p; a.g.bnd_cnt = g.bnd_cnt.p;
+    a.g.flux_geo = g.flux_geo.ptr();
     a.g.xp_list = g.xp_list.p; a.g.xp_dst = g.xp_dst.p; a.g.xp_cnt = g.xp_cnt.p;
     a.ht = g.ht.p;
     a.Interp64 = g.Interp64.p;
@@ -402,6 +404,7 @@ void build_group(gocfd_ensemble &h, EnsGroup &g) {
   std::vector<int> wave_list(size_t(g.ntiles) * g.Wmax, 0), wave_cnt(g.ntiles, 0);
   std::vector<int> flux_list(size_t(g.ntiles) * g.Fmax, 0), flux_cnt(g.ntiles, 0), bnd_cnt(g.ntiles, 0);
   std::vector<int> xp_list(size_t(g.ntiles) * g.Xmax, 0), xp_dst(size_t(g.ntiles) * g.Xmax, 0), xp_cnt(g.ntiles, 0);
+  std::vector<double> flux_geo(size_t(g.ntiles) * g.Fmax * 4, 0.0);
   std::vector<double> q0(Kp * g.NV);
   for (int ci : g.cases) {
     const EnsCase &c = *h.cases[ci];
@@ -435,7 +438,11 @@ void build_group(gocfd_ensemble &h, EnsGroup &g) {
       for (int i = 0; i < L.wave_cnt[t]; ++i) wave_list[size_t(t0 + t) * g.Wmax + i] = L.wave_list[size_t(t) * L.Wmax + i];
       flux_cnt[t0 + t] = L.flux_cnt[t];
       bnd_cnt[t0 + t] = L.bnd_cnt[t];
-      for (int i = 0; i < L.flux_cnt[t]; ++i) flux_list[size_t(t0 + t) * g.Fmax + i] = L.flux_list[size_t(t) * L.Fmax + i];
+      for (int i = 0; i < L.flux_cnt[t]; ++i) {
+        flux_list[size_t(t0 + t) * g.Fmax + i] = L.flux_list[size_t(t) * L.Fmax + i];
+        for (int c = 0; c < 4; ++c)
+          flux_geo[(size_t(t0 + t) * g.Fmax + i) * 4 + c] = L.flux_geo[(size_t(t) * L.Fmax + i) * 4 + c];
+      }
       xp_cnt[t0 + t] = L.xp_cnt[t];
       for (int i = 0; i < L.xp_cnt[t]; ++i) {
         // the neighbour face f * kp + kk of the case -> f * Kp + off + kk of the group
@@ -451,6 +458,7 @@ void build_group(gocfd_ensemble &h, EnsGroup &g) {
   g.halo_elem.upload(halo_elem); g.halo_face.upload(halo_face);
   g.wave_list.upload(wave_list); g.wave_cnt.upload(wave_cnt);
   g.flux_list.upload(flux_list); g.flux_cnt.upload(flux_cnt); g.bnd_cnt.upload(bnd_cnt);
+  g.flux_geo.upload(flux_geo, h.prec, N);
   g.tile_case.upload(tile_case);
   if (!g.shock) {
     g.xp_list.upload(xp_list); g.xp_dst.upload(xp_dst); g.xp_cnt.upload(xp_cnt);

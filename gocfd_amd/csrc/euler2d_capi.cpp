@@ -115,6 +115,7 @@ int gocfd_euler2d_create(const gocfd_mesh *mesh, const gocfd_params *p, gocfd_eu
   h->halo_elem.upload(L.halo_elem); h->halo_face.upload(L.halo_face);
   h->wave_list.upload(L.wave_list); h->wave_cnt.upload(L.wave_cnt);
   h->flux_list.upload(L.flux_list); h->flux_cnt.upload(L.flux_cnt); h->bnd_cnt.upload(L.bnd_cnt);
+  h->flux_geo.upload(L.flux_geo, h->prec, p->polynomial_order);
   h->perm.upload(L.perm);
   if (!h->shock) {
     h->xp_list.upload(L.xp_list); h->xp_dst.upload(L.xp_dst); h->xp_cnt.upload(L.xp_cnt);

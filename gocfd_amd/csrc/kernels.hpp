@@ -35,6 +35,9 @@ struct Geom {
   const int *halo_elem, *halo_face;                    // [ntiles][Hmax]
   const int *wave_list, *wave_cnt;                     // [ntiles][Wmax], [ntiles]
   const int *flux_list, *flux_cnt;                     // [ntiles][Fmax], [ntiles]
+  // per flux-list item, in the stage arithmetic type: owner normal (nx, ny),
+  // this side's signed IInII, the in-tile partner's IInII negated
+  const void *flux_geo;                                // [ntiles][Fmax][4]
   const int *bnd_cnt;                                  // [ntiles] non-shared faces
   const double *bxy;                                   // IVortex boundary points
   // Faces whose neighbour lies in another tile, per tile: after a stage the

@@ -248,12 +248,17 @@ __device__ __forceinline__ double point_speed(double g, const double q[4], doubl
 }
 
 // Dynamic LDS of the stage kernel: the tile state [NV][kTile+1] and the
-// trace/flux buffer [4][3 NpEdge kTile + NpEdge Hmax], rounded to 16 bytes;
-// local time stepping adds the element dt and face-speed buffers.
+// trace/flux buffer [4][3 NpEdge kTile + NpEdge Hmax] (rounded to 32 bytes),
+// the Riemann items' geometry [Fmax][4] and the elements' metric terms
+// [6][kTile]; local time stepping adds the element dt and face-speed buffers.
 template <int N>
-__host__ __device__ constexpr size_t lds_tile_bytes(size_t csize, int Hmax) {
+__host__ __device__ constexpr size_t lds_trace_bytes(size_t csize, int Hmax) {
   return (csize * (size_t(Shape<N>::NV) * (kTile + 1) +
-                   4 * (size_t(Shape<N>::NF) * kTile + size_t(Shape<N>::NpEdge) * Hmax)) + 15) / 16 * 16;
+                   4 * (size_t(Shape<N>::NF) * kTile + size_t(Shape<N>::NpEdge) * Hmax)) + 31) / 32 * 32;
+}
+template <int N>
+__host__ __device__ constexpr size_t lds_tile_bytes(size_t csize, int Hmax, int Fmax) {
+  return lds_trace_bytes<N>(csize, Hmax) + (csize * (4 * size_t(Fmax) + 6 * kTile) + 15) / 16 * 16;
 }
 constexpr size_t lds_lts_bytes = sizeof(double) * kTile + sizeof(unsigned long long) * 3 * kTile;
 
@@ -302,9 +307,12 @@ k_stage(StageArgs a) {
   const int TS = NF * E + (HX ? 0 : NpEdge * a.Hmax);
   C *sT = sQ + NV * P;                       // [4][TS]: traces, then fluxes
   C *sR = sT;                                // [NV][P]: RHS (aliases sT)
+  // the Riemann items' geometry and the metric terms (Jdet, Jinv[4], -1/Jdet)
+  V4 *sFG = reinterpret_cast<V4 *>(smem_raw + lds_trace_bytes<N>(sizeof(C), HX ? 0 : a.Hmax));  // [Fmax]
+  C *sJ = reinterpret_cast<C *>(sFG + a.Fmax);                                                 // [6][E]
   // local time stepping only: element dt and per-face speed maxima, after
   // the tile buffers (the launch adds their bytes only when LTS is on)
-  double *sDT = reinterpret_cast<double *>(smem_raw + lds_tile_bytes<N>(sizeof(C), HX ? 0 : a.Hmax));
+  double *sDT = reinterpret_cast<double *>(smem_raw + lds_tile_bytes<N>(sizeof(C), HX ? 0 : a.Hmax, a.Fmax));
   unsigned long long *sWf = reinterpret_cast<unsigned long long *>(sDT + kTile);
   __shared__ double red[kThreads / 64];
 
@@ -326,6 +334,16 @@ k_stage(StageArgs a) {
   for (int j = 0; j < ITB; ++j) {
     const int it = tid + j * kThreads;
     fpk[j] = it < nfl ? flist[it / NpEdge] : 0;
+  }
+  // the items' geometry and the elements' metric terms, to LDS below
+  V4 fgv = V4{C(0), C(0), C(0), C(0)};
+  if (tid < a.g.flux_cnt[tile]) fgv = static_cast<const V4 *>(a.g.flux_geo)[size_t(tile) * a.Fmax + tid];
+  // (wave 0: Jdet and Jinv[3]; wave c = 1..3: Jinv[c-1] of element te)
+  double jm0, jm1 = 0.;
+  {
+    const int c = tid / E, e = tid - c * E;
+    jm0 = c == 0 ? a.g.Jdet[k0 + e] : a.g.Jinv[size_t(c - 1) * Kp + k0 + e];
+    if (c == 0) jm1 = a.g.Jinv[3 * size_t(Kp) + k0 + e];
   }
   const int nbnd = a.g.bnd_cnt[tile];  // non-shared faces: the boundary loop runs only if any
   int hk0 = -1, hf0 = 0;
@@ -393,6 +411,15 @@ k_stage(StageArgs a) {
       }
     }
   }
+  if (tid < a.Fmax) sFG[tid] = fgv;
+  {
+    const int c = tid / E, e = tid - c * E;
+    sJ[c * E + e] = C(jm0);
+    if (c == 0) {
+      sJ[4 * E + e] = C(jm1);
+      sJ[5 * E + e] = -C(1. / jm0);
+    }
+  }
   __syncthreads();
   const int te = tid % E, tn = tid / E;  // (element, variable) of this thread
   if constexpr (UseMfma<N, C, SHOCK>::value) {
@@ -447,11 +474,10 @@ k_stage(StageArgs a) {
       const int packed = fpk[j];
       const int e = packed & 255, f = (packed >> 8) & 3, p = f * NpEdge + pp;
       const size_t fi = size_t(f) * Kp + k0 + e;
-      const int code = a.g.code[fi];
+      const V4 gq = sFG[w];
       C own[4], oth[4], F[4];
-      const unsigned slot = unsigned(a.g.nbr_slot[fi]);
-      const int base = int(slot & 0x7fffffffu);
-      const int stride = (slot >> 31) ? 1 : E;
+      const int base = packed >> 13;                    // the neighbour's slot
+      const int stride = (packed & (1 << 11)) ? 1 : E;  // halo: point-contiguous
       const int rp = NpEdge - 1 - pp;
       const bool xt = HX && (packed & (1 << 11));  // neighbour traces in hx[j]
       const int idx = xt ? p * E + e : base + rp * stride;
@@ -461,8 +487,8 @@ k_stage(StageArgs a) {
         oth[c] = sT[c * TS + idx];
         if (xt) oth[c] = hx[j][c];
       }
-      const C nx = C(a.g.Nx[fi]), ny = C(a.g.Ny[fi]);
-      const bool owner = code & FC_FLUX_OWNER;
+      const C nx = gq[0], ny = gq[1];
+      const bool owner = packed & (1 << 12);
       if (owner) numflux(FLUX, g, own, oth, nx, ny, F);
       else numflux(FLUX, g, oth, own, nx, ny, F);
       if constexpr (SHOCK) {
@@ -505,23 +531,17 @@ k_stage(StageArgs a) {
       const int w = it / NpEdge, pp = it - w * NpEdge;
       const int packed = fpk[j];
       const int e = packed & 255, f = (packed >> 8) & 3, p = f * NpEdge + pp;
-      const size_t fi = size_t(f) * Kp + k0 + e;
-      const C wn = C(a.g.IInII[fi]);
-      if (a.g.code[fi] & FC_FLUX_OWNER) {
+      const V4 gq = sFG[w];
+      const C wn = gq[2];  // IInII, negated on the non-owner side
 #pragma unroll
-        for (int c = 0; c < 4; ++c) sT[c * TS + p * E + e] = fl[j][c] * wn;
-      } else {
-#pragma unroll
-        for (int c = 0; c < 4; ++c) sT[c * TS + p * E + e] = -fl[j][c] * wn;
-      }
+      for (int c = 0; c < 4; ++c) sT[c * TS + p * E + e] = fl[j][c] * wn;
       if (packed & (1 << 10)) {
-        // the in-tile partner: slot pf*NpEdge*E + e2, at the reversed point
-        const int base = a.g.nbr_slot[fi];
-        const int pf = base / (NpEdge * E), e2 = base - pf * NpEdge * E;
-        const C w2 = C(a.g.IInII[size_t(pf) * Kp + k0 + e2]);
-        const int idx = base + (NpEdge - 1 - pp) * E;
+        // the in-tile partner's slot (pf*NpEdge*E + e2) at the reversed
+        // point, with the partner's IInII negated
+        const int idx = (packed >> 13) + (NpEdge - 1 - pp) * E;
+        const C w2 = gq[3];
 #pragma unroll
-        for (int c = 0; c < 4; ++c) sT[c * TS + idx] = -fl[j][c] * w2;
+        for (int c = 0; c < 4; ++c) sT[c * TS + idx] = fl[j][c] * w2;
       }
     }
   }
@@ -588,11 +608,8 @@ k_stage(StageArgs a) {
     typename M::vec out[4];
 #pragma unroll
     for (int b = 0; b < 4; ++b) {
-      const int e = mfma_elem(b, col), kk = k0 + e;
-      const double Jd = a.g.Jdet[kk];
-      const C Jdc = C(Jd);
-      const C J0 = C(a.g.Jinv[kk]), J1 = C(a.g.Jinv[size_t(Kp) + kk]);
-      const C J2 = C(a.g.Jinv[2 * size_t(Kp) + kk]), J3 = C(a.g.Jinv[3 * size_t(Kp) + kk]);
+      const int e = mfma_elem(b, col);
+      const C Jdc = sJ[e], J0 = sJ[E + e], J1 = sJ[2 * E + e], J2 = sJ[3 * E + e], J3 = sJ[4 * E + e];
       typename M::vec acc = M::zero();
 #pragma unroll
       for (int g2 = 0; g2 < Sh::G; ++g2) {
@@ -611,7 +628,7 @@ k_stage(StageArgs a) {
         const int p = min(4 * s2 + krow, NF - 1);
         acc = M::mma(af[2 * Sh::G + s2], sT[tn * TS + p * E + e], acc);
       }
-      out[b] = acc * (-C(1. / Jd));
+      out[b] = acc * sJ[5 * E + e];  // -1/Jdet
     }
     __syncthreads();  // every wave has read its fluxes; sR may now overwrite sT
 #pragma unroll
@@ -627,10 +644,7 @@ k_stage(StageArgs a) {
     const int kk = k0 + te;
     bool act = false;  // shock capturing: eps*grad Q is nonzero somewhere in the element
     if constexpr (SHOCK) act = a.sh.active[kk] != 0;
-    const double Jd = a.g.Jdet[kk];
-    const C Jdc = C(Jd);
-    const C J0 = C(a.g.Jinv[kk]), J1 = C(a.g.Jinv[size_t(Kp) + kk]);
-    const C J2 = C(a.g.Jinv[2 * size_t(Kp) + kk]), J3 = C(a.g.Jinv[3 * size_t(Kp) + kk]);
+    const C Jdc = sJ[te], J0 = sJ[E + te], J1 = sJ[2 * E + te], J2 = sJ[3 * E + te], J3 = sJ[4 * E + te];
     C acc[NpInt];
 #pragma unroll
     for (int i = 0; i < NpInt; ++i) acc[i] = C(0);
@@ -657,7 +671,7 @@ k_stage(StageArgs a) {
 #pragma unroll
       for (int i = 0; i < NpInt; ++i) acc[i] += DvT[(2 * NpInt + p) * NpInt + i] * Fe;
     }
-    const C moojd = -C(1. / Jd);
+    const C moojd = sJ[5 * E + te];
 #pragma unroll
     for (int i = 0; i < NpInt; ++i) acc[i] *= moojd;
     if constexpr (SHOCK) {
@@ -695,28 +709,7 @@ k_stage(StageArgs a) {
     return;
   }
 
-  // the owned-face list of phase G, fetched under phase E's loads
   constexpr int ITG = (3 * E * NpEdge + kThreads - 1) / kThreads;
-  int wpk[ITG], wcode[ITG];
-  double woohk[ITG];
-  int nw = 0;
-  if constexpr (!SHOCK) {
-    nw = a.g.wave_cnt[tile] * NpEdge;
-    const int *wlist = a.g.wave_list + size_t(tile) * a.Wmax;
-#pragma unroll
-    for (int j = 0; j < ITG; ++j) {
-      const int it = tid + j * kThreads;
-      wpk[j] = it < nw ? wlist[it / NpEdge] : 0;
-    }
-    // their face codes and 1/hK, under phase E's loads
-#pragma unroll
-    for (int j = 0; j < ITG; ++j) {
-      const int it = tid + j * kThreads;
-      const int e = wpk[j] & 255, f = wpk[j] >> 8;
-      wcode[j] = it < nw ? a.g.code[size_t(f) * Kp + k0 + e] : 0;
-      woohk[j] = it < nw ? a.g.oohK[k0 + e] : 0.0;
-    }
-  }
 
   // ---- E: SSP54 stage combination (euler.go:511-565), FP64 arithmetic ----
   const int stage = a.stage;
@@ -881,6 +874,26 @@ k_stage(StageArgs a) {
     // and the traces of the cross-tile faces to the neighbours' slots of the
     // next halo-trace buffer (at the neighbour's point order); the export
     // list loads go out first ----
+    // the owned-face list with its face codes and 1/hK, and the export
+    // list: all loads go out first
+    const int nw = a.g.wave_cnt[tile] * NpEdge;
+    int wpk[ITG], wcode[ITG];
+    double woohk[ITG];
+    {
+      const int *wlist = a.g.wave_list + size_t(tile) * a.Wmax;
+#pragma unroll
+      for (int j = 0; j < ITG; ++j) {
+        const int it = tid + j * kThreads;
+        wpk[j] = it < nw ? wlist[it / NpEdge] : 0;
+      }
+#pragma unroll
+      for (int j = 0; j < ITG; ++j) {
+        const int it = tid + j * kThreads;
+        const int e = wpk[j] & 255, f = wpk[j] >> 8;
+        wcode[j] = it < nw ? a.g.code[size_t(f) * Kp + k0 + e] : 0;
+        woohk[j] = it < nw ? a.g.oohK[k0 + e] : 0.0;
+      }
+    }
     const int nxp = a.g.xp_cnt[tile] * NpEdge;
     int xpk[ITG], xpd[ITG];
     {
@@ -1389,13 +1402,14 @@ __global__ void __launch_bounds__(256) k_shock_grad_act(StageArgs a) {
 // ---- launchers ----
 
 template <int N>
-size_t lds_bytes_n(size_t csize, int Hmax, bool lts) {
-  return lds_tile_bytes<N>(csize, Hmax) + (lts ? lds_lts_bytes : 0);
+size_t lds_bytes_n(size_t csize, int Hmax, int Fmax, bool lts) {
+  return lds_tile_bytes<N>(csize, Hmax, Fmax) + (lts ? lds_lts_bytes : 0);
 }
 
 template <int N, class S, class SIN, class C, int FLUX, bool LAST, bool SHOCK>
 hipError_t launch_stage_t(const StageArgs &a, hipStream_t s) {
-  const size_t lds = lds_bytes_n<N>(sizeof(C), SHOCK ? a.Hmax : 0, a.lts.on != 0);
+  if (a.Fmax > kThreads) return hipErrorInvalidValue;  // one thread stages each item's geometry
+  const size_t lds = lds_bytes_n<N>(sizeof(C), SHOCK ? a.Hmax : 0, a.Fmax, a.lts.on != 0);
   static size_t attr_bytes = 0;  // dynamic LDS the kernel is currently allowed
   if (lds > 64 * 1024 && lds > attr_bytes) {
     hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void *>(&k_stage<N, S, SIN, C, FLUX, LAST, SHOCK>),
