@@ -57,6 +57,7 @@ size_t stage_lds_bytes(int N, int prec, int Hmax) {
   const size_t NpInt = size_t(N + 1) * (N + 2) / 2, NpEdge = size_t(N) + 2;
   return (cs * (4 * NpInt * (kTile + 1) + 4 * (3 * NpEdge * kTile + NpEdge * size_t(Hmax))) + 31) / 32 * 32 +
          cs * (4 * 3 * kTile + 6 * kTile) +                                  // item geometry (Fmax <= 3 kTile), metric
+         sizeof(double) * kTile + 2 * sizeof(int) * 6 * kTile +              // phase G lists (Wmax, Xmax <= 3 kTile)
          sizeof(double) * kTile + sizeof(unsigned long long) * 3 * kTile;  // with the LTS buffers
 }
 

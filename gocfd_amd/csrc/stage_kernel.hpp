@@ -250,15 +250,19 @@ __device__ __forceinline__ double point_speed(double g, const double q[4], doubl
 // Dynamic LDS of the stage kernel: the tile state [NV][kTile+1] and the
 // trace/flux buffer [4][3 NpEdge kTile + NpEdge Hmax] (rounded to 32 bytes),
 // the Riemann items' geometry [Fmax][4] and the elements' metric terms
-// [6][kTile]; local time stepping adds the element dt and face-speed buffers.
+// [6][kTile]; without shock capturing, phase G's lists: the elements' 1/hK
+// [kTile] (FP64), the owned faces with their codes [2][Wmax] and the export
+// items with their destinations [2][Xmax] (ints); local time stepping adds
+// the element dt and face-speed buffers.
 template <int N>
 __host__ __device__ constexpr size_t lds_trace_bytes(size_t csize, int Hmax) {
   return (csize * (size_t(Shape<N>::NV) * (kTile + 1) +
                    4 * (size_t(Shape<N>::NF) * kTile + size_t(Shape<N>::NpEdge) * Hmax)) + 31) / 32 * 32;
 }
 template <int N>
-__host__ __device__ constexpr size_t lds_tile_bytes(size_t csize, int Hmax, int Fmax) {
-  return lds_trace_bytes<N>(csize, Hmax) + (csize * (4 * size_t(Fmax) + 6 * kTile) + 15) / 16 * 16;
+__host__ __device__ constexpr size_t lds_tile_bytes(size_t csize, int Hmax, int Fmax, int WXmax) {
+  return lds_trace_bytes<N>(csize, Hmax) + csize * (4 * size_t(Fmax) + 6 * kTile) +
+         (WXmax > 0 ? (sizeof(double) * kTile + 2 * sizeof(int) * size_t(WXmax) + 15) / 16 * 16 : 0);
 }
 constexpr size_t lds_lts_bytes = sizeof(double) * kTile + sizeof(unsigned long long) * 3 * kTile;
 
@@ -310,9 +314,15 @@ k_stage(StageArgs a) {
   // the Riemann items' geometry and the metric terms (Jdet, Jinv[4], -1/Jdet)
   V4 *sFG = reinterpret_cast<V4 *>(smem_raw + lds_trace_bytes<N>(sizeof(C), HX ? 0 : a.Hmax));  // [Fmax]
   C *sJ = reinterpret_cast<C *>(sFG + a.Fmax);                                                 // [6][E]
+  // phase G's lists (without shock capturing)
+  double *sOK = reinterpret_cast<double *>(sJ + 6 * E);  // [E] 1/hK
+  int *sWP = reinterpret_cast<int *>(sOK + E);           // [Wmax] owned faces e | f << 8
+  int *sWC = sWP + a.Wmax;                               // [Wmax] their codes
+  int *sXP = sWC + a.Wmax, *sXD = sXP + a.Xmax;          // [Xmax] export items, destinations
   // local time stepping only: element dt and per-face speed maxima, after
   // the tile buffers (the launch adds their bytes only when LTS is on)
-  double *sDT = reinterpret_cast<double *>(smem_raw + lds_tile_bytes<N>(sizeof(C), HX ? 0 : a.Hmax, a.Fmax));
+  double *sDT = reinterpret_cast<double *>(
+      smem_raw + lds_tile_bytes<N>(sizeof(C), HX ? 0 : a.Hmax, a.Fmax, HX ? a.Wmax + a.Xmax : 0));
   unsigned long long *sWf = reinterpret_cast<unsigned long long *>(sDT + kTile);
   __shared__ double red[kThreads / 64];
 
@@ -344,6 +354,21 @@ k_stage(StageArgs a) {
     const int c = tid / E, e = tid - c * E;
     jm0 = c == 0 ? a.g.Jdet[k0 + e] : a.g.Jinv[size_t(c - 1) * Kp + k0 + e];
     if (c == 0) jm1 = a.g.Jinv[3 * size_t(Kp) + k0 + e];
+  }
+  // phase G's lists (owned faces, their codes, 1/hK; export items)
+  const int nwc = HX ? a.g.wave_cnt[tile] : 0, nxc = HX ? a.g.xp_cnt[tile] : 0;
+  int gw = 0, gc = 0, gx = 0, gd = 0;
+  double gok = 0.;
+  if constexpr (HX) {
+    if (tid < nwc) {
+      gw = a.g.wave_list[size_t(tile) * a.Wmax + tid];
+      gc = a.g.code[size_t(gw >> 8) * Kp + k0 + (gw & 255)];
+    }
+    if (tid < nxc) {
+      gx = a.g.xp_list[size_t(tile) * a.Xmax + tid];
+      gd = a.g.xp_dst[size_t(tile) * a.Xmax + tid];
+    }
+    if (tid < E) gok = a.g.oohK[k0 + tid];
   }
   const int nbnd = a.g.bnd_cnt[tile];  // non-shared faces: the boundary loop runs only if any
   int hk0 = -1, hf0 = 0;
@@ -394,10 +419,6 @@ k_stage(StageArgs a) {
   // so that phase E does not wait for them
   const unsigned long long wmax_bits = rhs_only ? 0ull : ctl->wmax[a.ring];
   const unsigned long long wmaxv_bits = (rhs_only || !SHOCK) ? 0ull : ctl->wmaxv[a.ring];
-  if (step_skipped(a, ctl, ph.final_time)) {
-    if (a.stage == 4 && first && threadIdx.x == 0) ctl->time[par ^ 1] = time;
-    return;
-  }
   const C *DvT = ops_base<C>();                         // DivInt^T [NpFlux][NpInt]
   const C *Itp = DvT + Sh::NpFlux * NpInt;              // FluxEdgeInterp [NF][NpInt]
   const C g = C(ph.gamma);
@@ -412,6 +433,17 @@ k_stage(StageArgs a) {
     }
   }
   if (tid < a.Fmax) sFG[tid] = fgv;
+  if constexpr (HX) {
+    if (tid < a.Wmax) {
+      sWP[tid] = gw;
+      sWC[tid] = gc;
+    }
+    if (tid < a.Xmax) {
+      sXP[tid] = gx;
+      sXD[tid] = gd;
+    }
+    if (tid < E) sOK[tid] = gok;
+  }
   {
     const int c = tid / E, e = tid - c * E;
     sJ[c * E + e] = C(jm0);
@@ -421,6 +453,12 @@ k_stage(StageArgs a) {
     }
   }
   __syncthreads();
+  // the control-block chain (tile -> case -> time) has had the tile loads and
+  // the LDS stores to arrive under
+  if (step_skipped(a, ctl, ph.final_time)) {
+    if (a.stage == 4 && first && threadIdx.x == 0) ctl->time[par ^ 1] = time;
+    return;
+  }
   const int te = tid % E, tn = tid / E;  // (element, variable) of this thread
   if constexpr (UseMfma<N, C, SHOCK>::value) {
     traces_mfma<N, C>(sQ, sT, tn, te, TS);
@@ -872,49 +910,19 @@ k_stage(StageArgs a) {
 
     // ---- G: owner-side wave speed of the next stage, max over the tile,
     // and the traces of the cross-tile faces to the neighbours' slots of the
-    // next halo-trace buffer (at the neighbour's point order); the export
-    // list loads go out first ----
-    // the owned-face list with its face codes and 1/hK, and the export
-    // list: all loads go out first
-    const int nw = a.g.wave_cnt[tile] * NpEdge;
-    int wpk[ITG], wcode[ITG];
-    double woohk[ITG];
-    {
-      const int *wlist = a.g.wave_list + size_t(tile) * a.Wmax;
-#pragma unroll
-      for (int j = 0; j < ITG; ++j) {
-        const int it = tid + j * kThreads;
-        wpk[j] = it < nw ? wlist[it / NpEdge] : 0;
-      }
-#pragma unroll
-      for (int j = 0; j < ITG; ++j) {
-        const int it = tid + j * kThreads;
-        const int e = wpk[j] & 255, f = wpk[j] >> 8;
-        wcode[j] = it < nw ? a.g.code[size_t(f) * Kp + k0 + e] : 0;
-        woohk[j] = it < nw ? a.g.oohK[k0 + e] : 0.0;
-      }
-    }
-    const int nxp = a.g.xp_cnt[tile] * NpEdge;
-    int xpk[ITG], xpd[ITG];
-    {
-      const int *xl = a.g.xp_list + size_t(tile) * a.Xmax, *xd = a.g.xp_dst + size_t(tile) * a.Xmax;
-#pragma unroll
-      for (int j = 0; j < ITG; ++j) {
-        const int it = tid + j * kThreads;
-        xpk[j] = it < nxp ? xl[it / NpEdge] : 0;
-        xpd[j] = it < nxp ? xd[it / NpEdge] : 0;
-      }
-    }
+    // next halo-trace buffer (at the neighbour's point order); both lists
+    // are in LDS ----
+    const int nw = nwc * NpEdge, nxp = nxc * NpEdge;
 #pragma unroll
     for (int j = 0; j < ITG; ++j) {
       const int it = tid + j * kThreads;
       if (it < nw) {
         const int w = it / NpEdge, pp = it - w * NpEdge;
-        const int packed = wpk[j];
+        const int packed = sWP[w];
         const int e = packed & 255, f = packed >> 8, p = f * NpEdge + pp;
         const int kk = k0 + e;
         const size_t fi = size_t(f) * Kp + kk;
-        const int code = wcode[j];
+        const int code = sWC[w];
         const int kind = code & 15;
         C own[4];
 #pragma unroll
@@ -922,7 +930,7 @@ k_stage(StageArgs a) {
         double q[4] = {double(own[0]), double(own[1]), double(own[2]), double(own[3])};
         if (kind == FK_Far || kind == FK_IVortex)
           boundary_state(kind, ph, a.g.bxy, code >> 8, NpEdge, pp, tnext, own, a.g.Nx[fi], a.g.Ny[fi], q);
-        const double wsp = point_speed(ph.gamma, q, woohk[j]);
+        const double wsp = point_speed(ph.gamma, q, sOK[e]);
         if (!(wsp == wsp)) bad = 1;
         wl = fmax(wl, wsp);
         if (lts) atomicMax(&sWf[e * 3 + f], bits(wsp));
@@ -934,8 +942,9 @@ k_stage(StageArgs a) {
       const int it = tid + j * kThreads;
       if (it < nxp) {
         const int w = it / NpEdge, pp = it - w * NpEdge;
-        const int e = xpk[j] & 255, f = (xpk[j] >> 8) & 3, p = f * NpEdge + pp;
-        hout[size_t(xpd[j]) * NpEdge + (NpEdge - 1 - pp)] =
+        const int xk = sXP[w];
+        const int e = xk & 255, f = (xk >> 8) & 3, p = f * NpEdge + pp;
+        hout[size_t(sXD[w]) * NpEdge + (NpEdge - 1 - pp)] =
             V4{sT[p * E + e], sT[TS + p * E + e], sT[2 * TS + p * E + e], sT[3 * TS + p * E + e]};
       }
     }
@@ -1402,14 +1411,16 @@ __global__ void __launch_bounds__(256) k_shock_grad_act(StageArgs a) {
 // ---- launchers ----
 
 template <int N>
-size_t lds_bytes_n(size_t csize, int Hmax, int Fmax, bool lts) {
-  return lds_tile_bytes<N>(csize, Hmax, Fmax) + (lts ? lds_lts_bytes : 0);
+size_t lds_bytes_n(size_t csize, int Hmax, int Fmax, int WXmax, bool lts) {
+  return lds_tile_bytes<N>(csize, Hmax, Fmax, WXmax) + (lts ? lds_lts_bytes : 0);
 }
 
 template <int N, class S, class SIN, class C, int FLUX, bool LAST, bool SHOCK>
 hipError_t launch_stage_t(const StageArgs &a, hipStream_t s) {
-  if (a.Fmax > kThreads) return hipErrorInvalidValue;  // one thread stages each item's geometry
-  const size_t lds = lds_bytes_n<N>(sizeof(C), SHOCK ? a.Hmax : 0, a.Fmax, a.lts.on != 0);
+  // one thread stages each list item into LDS
+  if (a.Fmax > kThreads || (!SHOCK && (a.Wmax > kThreads || a.Xmax > kThreads))) return hipErrorInvalidValue;
+  const size_t lds = lds_bytes_n<N>(sizeof(C), SHOCK ? a.Hmax : 0, a.Fmax, SHOCK ? 0 : a.Wmax + a.Xmax,
+                                    a.lts.on != 0);
   static size_t attr_bytes = 0;  // dynamic LDS the kernel is currently allowed
   if (lds > 64 * 1024 && lds > attr_bytes) {
     hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void *>(&k_stage<N, S, SIN, C, FLUX, LAST, SHOCK>),
