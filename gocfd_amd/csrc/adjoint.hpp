@@ -1,5 +1,6 @@
 // Tangent-linear and discrete-adjoint kernels of the FP64 SSP54 step
-// (global or local time stepping; PerssonC0 while its sensor is quiet):
+// (global or local time stepping; PerssonC0 with global time stepping, or
+// with local time stepping while its sensor is quiet):
 // interface between the host driver
 // (adjoint_runner.cpp) and the per-order kernel translation units.
 //
@@ -58,7 +59,8 @@ struct AdjArgs {
   // unshared face points of real elements (boundary kinds), t = (f Kp + kk) NpEdge + pp
   const int *bnd;
   int nbnd;
-  // PerssonC0 while its sensor is quiet (sigma = 0 everywhere): the modal
+  // PerssonC0 with local time stepping, while its sensor is quiet (sigma =
+  // 0 everywhere): the modal
   // filter F = V diag(1, mf_1, mf_2, ...) Vinv [NpInt][NpInt] of the RHS at
   // every stage and of the stage state at stage 2 (limitAndFilterSolution
   // dissipation.go:606-622, euler.go:603-608); the sensor's matrices
@@ -68,6 +70,24 @@ struct AdjArgs {
   const double *filt;
   const double *sens;
   double sens_left;
+  // PerssonC0 with an active sensor (global time stepping): the sensor
+  // ramp sigma(Se) (UpdateShockFinderSigma dissipation.go:491-518), the
+  // element viscosity eps = eps0 hK sigma (CalculateElementViscosity
+  // :397-412), the vertex max-merges (euler.go:1048-1065), the vertex-mean
+  // sigma's modal filters (dissipation.go:606-622), eps at the RT points
+  // (InterpolateEpsilonSigma :219-242), the RT gradient times eps
+  // (CalculateEpsilonGradient :244-272, euler.go:864-918), the viscous edge
+  // flux (StoreEdgeViscousFlux edges.go:151-244), AddDissipation
+  // (dissipation.go:274-346) and the viscous dt, all differentiated
+  int act;
+  const double *shops;          // the shock operator block (P, M, D, V, Vinv, mf, DivR, Bary)
+  const double *hK;             // [Kp] EdgeLenMax / (N+1)^2
+  const double *Nxo, *Nyo;      // [3][Kp] own outward normals
+  const double *vooL;           // [3][Kp] 1 / owner edge length
+  const int *evert;             // [3][Kp] vertex ids
+  const int *vptr, *vinc;       // vertex -> (kk << 2 | local vertex) of its real elements, CSR
+  int Nv;
+  double kappa, S0, eps0, cdiff, omega;
 };
 
 // Buffers of one launch. Element arrays are element-major [Kp][4][NpInt];
@@ -108,6 +128,24 @@ struct AdjBufs {
   double *carry;        // cotangent of the previous stage's element dt [Kp]
   double *lamSpeed;     // cotangents of the face-point speeds [3 Kp NpEdge]
   double *target;       // cotangent of this stage's input (accumulated)
+  // active PerssonC0: forward fields of this stage (T; the records double
+  // in the double sweeps) and their cotangents
+  void *sig, *eps;      // [Kp] element sigma and eps
+  void *sigv, *epsv;    // [Nv] vertex maxima
+  int *args, *arge;     // [Nv] the elements that attain them
+  void *sigb;           // [Kp] vertex-mean sigma
+  void *epsf;           // [NR][Kp] eps at the used RT points
+  void *dis;            // [4][2][NR][Kp] eps grad Q there
+  void *SPV;            // [3 Kp NpEdge] viscous speeds of owner-side face points, -1 elsewhere
+  double *Rpre;         // this stage's RHS before its modal filter (double sweeps)
+  const double *Xpre;   // the pre-filter values a transposed filter differentiates at
+  int *argmaxv, *vbr;   // [5] the viscous arg-max point; 1 where the viscous dt binds
+  double *Wvv;          // [5] the maximum viscous speed
+  double *part_v2;      // viscous arg-max partials
+  int *part_i2;
+  int accumulate;       // transposed filter: add to lamSigb (else set)
+  const double *Usens;  // the sensor's input (the stage input before the stage-2 filter)
+  double *lamSigb, *lamEpsf, *lamDis, *lamEQv, *lamEQg, *lamSigv, *lamEpsv;
   const double *pack_v, *pack_d;  // value and tangent arrays (pack/unpack)
   double *unpack_v, *unpack_d;
   size_t n;             // element-array length (Kp * 4 * NpInt) or reduction length
@@ -128,6 +166,14 @@ enum AdjOp : int {
   ADJ_SENSOR,        // flag |= 4 where the shock sensor of Us reaches its ramp
   ADJ_FILTER,        // out <- F out per element and variable (in place; dual or double)
   ADJ_FILTER_T,      // lam <- F^T lam per element and variable (in place)
+  ADJ_FWD_SHOCK,     // sig, eps, sigv, epsv, args, arge, sigb, epsf, SPV <- the sensor path of Us
+  ADJ_FILTER_SIG,    // out <- F(sigb) out per element and variable (in place; dual or double)
+  ADJ_FWD_VISC,      // dis <- eps grad Q of Us; FE -= the viscous edge values
+  ADJ_ADJ_FILTER_SIG_T,  // lam <- F(sigb)^T lam; lamSigb (+)= its sigma term at Xpre
+  ADJ_ADJ_VISC,      // lamDis, lamEpsf, lamEQv, lamEQg, target <- viscous flux and gradient transposes;
+                     // lamOwn += the owner-trace terms
+  ADJ_ADJ_VDT,       // lamEpsf += the viscous dt term where it binds
+  ADJ_ADJ_SENSOR,    // target += the vertex-merge, ramp and sensor transposes at Usens
   ADJ_PACK,          // out (dual) <- (pack_v, pack_d)
   ADJ_UNPACK,        // (unpack_v, unpack_d) <- Us (dual)
 };

@@ -46,14 +46,8 @@ __global__ void __launch_bounds__(256) k_adj_argmax_part(const T *SP, size_t n, 
   }
 }
 
-// Final arg-max over the partials; dt = CFL / W (in T), the FinalTime check
-// of the solver's clamp, and the time advance after the last stage
-template <class T>
-__global__ void __launch_bounds__(256) k_adj_argmax_final(const T *SP, const double *pv, const int *pi, int nparts,
-                                                          int stage, double cfl, double final_time, T *dts,
-                                                          int *argmax, double *Wv, double *time, int *flag) {
-  __shared__ double sv[256];
-  __shared__ int si[256];
+// The arg-max over the partials in one block (ties to the smallest index)
+__device__ int argmax_of_parts(const double *pv, const int *pi, int nparts, double *sv, int *si) {
   double bv = -2.;
   int bi = -1;
   for (int i = threadIdx.x; i < nparts; i += blockDim.x) {
@@ -73,9 +67,32 @@ __global__ void __launch_bounds__(256) k_adj_argmax_final(const T *SP, const dou
     }
     __syncthreads();
   }
+  const int r = si[0];
+  __syncthreads();
+  return r;
+}
+
+// Final arg-max over the partials; dt = CFL / W (in T), with an active
+// shock sensor the smaller of that and Cdiff / Wvisc (the maximum viscous
+// speed; calculateGlobalDT euler.go:945-971), the FinalTime check of the
+// solver's clamp, and the time advance after the last stage
+template <class T>
+__global__ void __launch_bounds__(256) k_adj_argmax_final(const T *SP, const double *pv, const int *pi, int nparts,
+                                                          int stage, double cfl, double final_time, T *dts,
+                                                          int *argmax, double *Wv, double *time, int *flag,
+                                                          const T *SPV, const double *pv2, const int *pi2,
+                                                          double cdiff, int *argmaxv, double *Wvv, int *vbr) {
+  __shared__ double sv[256];
+  __shared__ int si[256];
+  const int idx = argmax_of_parts(pv, pi, nparts, sv, si);
+  const int idv = SPV ? argmax_of_parts(pv2, pi2, nparts, sv, si) : -1;
   if (threadIdx.x == 0) {
-    const int idx = si[0];
     argmax[stage] = idx;
+    if (SPV) {
+      argmaxv[stage] = idv;
+      Wvv[stage] = idv >= 0 ? val_of(SPV[idv]) : 0.;
+      vbr[stage] = 0;
+    }
     if (idx < 0) {
       *flag |= 2;  // no wave face
       dts[stage] = T(0.);
@@ -83,7 +100,14 @@ __global__ void __launch_bounds__(256) k_adj_argmax_final(const T *SP, const dou
       return;
     }
     const T W = SP[idx];
-    const T dt = cfl / W;
+    T dt = cfl / W;
+    if (idv >= 0 && val_of(SPV[idv]) > 0.) {
+      const T dtv = cdiff / SPV[idv];
+      if (val_of(dtv) < val_of(dt)) {
+        dt = dtv;
+        vbr[stage] = 1;
+      }
+    }
     dts[stage] = dt;
     Wv[stage] = val_of(W);
     if (time) {
@@ -220,15 +244,25 @@ hipError_t adj_launch_common(int op, const AdjArgs &a, const AdjBufs &b, hipStre
       if (b.dual) {
         hipLaunchKernelGGL((k_adj_argmax_part<Dual>), dim3(b.nparts), dim3(256), 0, s,
                            static_cast<const Dual *>(b.SP), nfp, b.part_v, b.part_i);
+        if (a.act)
+          hipLaunchKernelGGL((k_adj_argmax_part<Dual>), dim3(b.nparts), dim3(256), 0, s,
+                             static_cast<const Dual *>(b.SPV), nfp, b.part_v2, b.part_i2);
         hipLaunchKernelGGL((k_adj_argmax_final<Dual>), dim3(1), dim3(256), 0, s, static_cast<const Dual *>(b.SP),
                            b.part_v, b.part_i, b.nparts, b.stage, a.ph.cfl, a.ph.final_time,
-                           static_cast<Dual *>(b.dts), b.argmax, b.Wv, b.time, b.flag);
+                           static_cast<Dual *>(b.dts), b.argmax, b.Wv, b.time, b.flag,
+                           a.act ? static_cast<const Dual *>(b.SPV) : nullptr, b.part_v2, b.part_i2, a.cdiff,
+                           b.argmaxv, b.Wvv, b.vbr);
       } else {
         hipLaunchKernelGGL((k_adj_argmax_part<double>), dim3(b.nparts), dim3(256), 0, s,
                            static_cast<const double *>(b.SP), nfp, b.part_v, b.part_i);
+        if (a.act)
+          hipLaunchKernelGGL((k_adj_argmax_part<double>), dim3(b.nparts), dim3(256), 0, s,
+                             static_cast<const double *>(b.SPV), nfp, b.part_v2, b.part_i2);
         hipLaunchKernelGGL((k_adj_argmax_final<double>), dim3(1), dim3(256), 0, s,
                            static_cast<const double *>(b.SP), b.part_v, b.part_i, b.nparts, b.stage, a.ph.cfl,
-                           a.ph.final_time, static_cast<double *>(b.dts), b.argmax, b.Wv, b.time, b.flag);
+                           a.ph.final_time, static_cast<double *>(b.dts), b.argmax, b.Wv, b.time, b.flag,
+                           a.act ? static_cast<const double *>(b.SPV) : nullptr, b.part_v2, b.part_i2, a.cdiff,
+                           b.argmaxv, b.Wvv, b.vbr);
       }
       break;
     }

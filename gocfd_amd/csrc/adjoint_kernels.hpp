@@ -192,10 +192,25 @@ struct RhsElems {
   static constexpr int value = 256 / AdjShape<N>::NpInt;
 };
 
+template <int N>
+struct AdjShock {
+  using Sh = AdjShape<N>;
+  static constexpr int NpInt = Sh::NpInt, NpEdge = Sh::NpEdge, NpFlux = Sh::NpFlux;
+  static constexpr int NR = NpInt + 3 * NpEdge;  // the used RT points: interior, then the edges
+  static constexpr int n2 = NpInt * NpInt;
+  static constexpr int oP = 0, oM = n2, oD = 2 * n2, oV = 3 * n2, oVi = 4 * n2, omf = 5 * n2, oDR = 5 * n2 + NpInt,
+                       oBa = oDR + NR * NpFlux;
+  // eps grad Q of variable n, direction dir, RT row r of element kk in [4][2][NR][Kp]
+  __device__ static size_t dis_at(int Kp, int n, int dir, int r, int kk) {
+    return (size_t((n * 2 + dir) * NR + r)) * Kp + kk;
+  }
+};
+
 template <int N, class T>
 __global__ void __launch_bounds__(256) k_adj_fwd_rhs(AdjArgs a, int stage, const T *U, const T *FE, T *R,
                                                      const T *U0, const T *U2, const T *U3, const T *R3,
-                                                     const T *dts, const T *dtk, T *out) {
+                                                     const T *dts, const T *dtk, T *out, const T *dis,
+                                                     const T *sigb, double *Rpre) {
   using Sh = AdjShape<N>;
   constexpr int NpInt = Sh::NpInt, NpEdge = Sh::NpEdge, NpFlux = Sh::NpFlux, EB = RhsElems<N>::value;
   // Fr of the 4 variables, then Fs, per element: [EB][8][NpInt] (raw
@@ -217,6 +232,13 @@ __global__ void __launch_bounds__(256) k_adj_fwd_rhs(AdjArgs a, int stage, const
     const T q[4] = {u[i], u[NpInt + i], u[2 * NpInt + i], u[3 * NpInt + i]};
     T Fx[4], Fy[4];
     flux_xy(T(a.ph.gamma), q, Fx, Fy);
+    if (a.act) {  // the physical flux minus eps grad Q (AddDissipation)
+#pragma unroll
+      for (int c = 0; c < 4; ++c) {
+        Fx[c] -= dis[AdjShock<N>::dis_at(a.Kp, c, 0, i, kk)];
+        Fy[c] -= dis[AdjShock<N>::dis_at(a.Kp, c, 1, i, kk)];
+      }
+    }
 #pragma unroll
     for (int c = 0; c < 4; ++c) {
       sF[el][c][i] = Jd * (J0 * Fx[c] + J1 * Fy[c]);
@@ -247,9 +269,42 @@ __global__ void __launch_bounds__(256) k_adj_fwd_rhs(AdjArgs a, int stage, const
     if (live) {
 #pragma unroll
       for (int c = 0; c < 4; ++c) sF[el][c][i] = acc[c];
+      if (Rpre) {
+#pragma unroll
+        for (int c = 0; c < 4; ++c) Rpre[size_t(kk) * Sh::NV + c * NpInt + i] = ad::val(acc[c]);
+      }
     }
     __syncthreads();
-    if (live && real) {
+    if (a.act) {
+      // F(sigma) = V diag(1, mf (1 - sin(pi sigma / 2))) Vinv, sigma the
+      // vertex mean: modal coefficients, scaled, back to the points
+      using Sk = AdjShock<N>;
+      const double *V = a.shops + Sk::oV, *Vi = a.shops + Sk::oVi, *mf = a.shops + Sk::omf;
+      T mm[4];
+      if (live && real) {
+        const T fac = 1. - sin((0.5 * M_PI) * sigb[kk]);
+#pragma unroll
+        for (int c = 0; c < 4; ++c) {
+          T t = T(0.);
+          for (int j = 0; j < NpInt; ++j) t += Vi[i * NpInt + j] * sF[el][c][j];
+          mm[c] = i == 0 ? t : t * (mf[i] * fac);
+        }
+      }
+      __syncthreads();
+      if (live && real) {
+#pragma unroll
+        for (int c = 0; c < 4; ++c) sF[el][c][i] = mm[c];
+      }
+      __syncthreads();
+      if (live && real) {
+#pragma unroll
+        for (int c = 0; c < 4; ++c) {
+          T t = T(0.);
+          for (int m = 0; m < NpInt; ++m) t += V[i * NpInt + m] * sF[el][c][m];
+          acc[c] = t;
+        }
+      }
+    } else if (live && real) {
       const double *Fi = a.filt + i * NpInt;
 #pragma unroll
       for (int c = 0; c < 4; ++c) {
@@ -843,9 +898,10 @@ __global__ void __launch_bounds__(256) k_adj_lts_wave(AdjArgs a, const double *U
 // state for a far-field face). One thread.
 template <int N>
 __global__ void k_adj_wave(AdjArgs a, int stage, const double *U, const int *argmax, const double *Wv,
-                           const double *lam_dt, double *target, double *lamFarW) {
+                           const double *lam_dt, double *target, double *lamFarW, const int *vbr) {
   using Sh = AdjShape<N>;
   if (blockIdx.x != 0 || threadIdx.x != 0) return;
+  if (vbr && vbr[stage]) return;  // the viscous dt bound this stage
   const int idx = argmax[stage];
   if (idx < 0) return;
   int f, kk, pp;
@@ -880,6 +936,582 @@ __global__ void k_adj_wave(AdjArgs a, int stage, const double *U, const int *arg
     for (int i = 0; i < Sh::NpInt; ++i) tg[c * Sh::NpInt + i] += row[i] * lo[c];
 }
 
+// ---- PerssonC0 with an active sensor ----
+//
+// Forward (T = double or ad::Dual), per stage: the sensor fields of the
+// stage input (k_adj_fwd_sens, k_adj_fwd_vmax, k_adj_fwd_efield), the
+// stage-2 state filter with the vertex-mean sigma (k_adj_filter_sig), the
+// RT gradient times eps (k_adj_fwd_grad) and the viscous edge values
+// (k_adj_fwd_visc). Adjoint: their transposes (k_adj_filter_sig_t,
+// k_adj_visc_t, k_adj_grad_t, k_adj_eq_gather, k_adj_vdt, k_adj_vert_t,
+// k_adj_sens_t), deterministic (each cotangent is written by one thread or
+// gathered in a fixed order).
+
+// sigma(Se) of UpdateShockFinderSigma (dissipation.go:491-518) with the
+// stepping path's S0 = 4/(N+1)^4, and its slope
+template <class T>
+__device__ __forceinline__ T sigma_ramp(const AdjArgs &a, T se) {
+  const double left = a.S0 - a.kappa, right = a.S0 + a.kappa, ook = 0.5 / a.kappa;
+  if (se < left) return T(0.);
+  if (se <= right) return 0.5 * (1. + sin((M_PI * ook) * (se - a.S0)));
+  if (se > right) return T(1.);
+  return T(0.);
+}
+__device__ __forceinline__ double sigma_slope(const AdjArgs &a, double se) {
+  const double left = a.S0 - a.kappa, right = a.S0 + a.kappa, ook = 0.5 / a.kappa;
+  if (!(se >= left && se <= right)) return 0.;
+  return 0.5 * (M_PI * ook) * cos((M_PI * ook) * (se - a.S0));
+}
+
+// Per element: Se of the stage input's density (UpdateSeMoment
+// DG2D/dfr_shock_capturing.go:142-166), sigma and eps = eps0 hK sigma
+template <int N, class T>
+__global__ void __launch_bounds__(256) k_adj_fwd_sens(AdjArgs a, const T *U, T *sig, T *eps) {
+  using Sk = AdjShock<N>;
+  constexpr int NpInt = Sk::NpInt;
+  const int kk = blockIdx.x * blockDim.x + threadIdx.x;
+  if (kk >= a.Kp) return;
+  if (kk >= a.K) {
+    sig[kk] = T(0.);
+    eps[kk] = T(0.);
+    return;
+  }
+  const T *rho = U + size_t(kk) * AdjShape<N>::NV;
+  const double *Pm = a.shops + Sk::oP, *Mm = a.shops + Sk::oM, *Dm = a.shops + Sk::oD;
+  T num = T(0.), den = T(0.);
+  for (int i = 0; i < NpInt; ++i) {
+    T X = T(0.), Y = T(0.), DQ = T(0.);
+    for (int j = 0; j < NpInt; ++j) {
+      const T r = rho[j];
+      X += Pm[i * NpInt + j] * r;
+      Y += Mm[i * NpInt + j] * r;
+      DQ += Dm[i * NpInt + j] * r;
+    }
+    num += DQ * X;
+    den += rho[i] * Y;
+  }
+  const T sg = sigma_ramp(a, log10(num / den));
+  sig[kk] = sg;
+  eps[kk] = (a.eps0 * a.hK[kk]) * sg;
+}
+
+// Vertex maxima of sigma and eps over the real elements at the vertex
+// (MergeElementScalarToVertices euler.go:1048-1065), with the element that
+// attains each (the first in incidence order among equals)
+template <int N, class T>
+__global__ void __launch_bounds__(256) k_adj_fwd_vmax(AdjArgs a, const T *sig, const T *eps, T *sigv, T *epsv,
+                                                      int *args, int *arge) {
+  const int v = blockIdx.x * blockDim.x + threadIdx.x;
+  if (v >= a.Nv) return;
+  T bs = T(0.), be = T(0.);
+  int is = -1, ie = -1;
+  for (int q = a.vptr[v]; q < a.vptr[v + 1]; ++q) {
+    const int kk = a.vinc[q] >> 2;
+    const T s = sig[kk], e = eps[kk];
+    if (is < 0 || s > bs) {
+      bs = s;
+      is = kk;
+    }
+    if (ie < 0 || e > be) {
+      be = e;
+      ie = kk;
+    }
+  }
+  sigv[v] = bs;
+  epsv[v] = be;
+  args[v] = is;
+  arge[v] = ie;
+}
+
+// Per element: sigma as the mean of its vertices' maxima, eps at the used
+// RT points by barycentric interpolation of its vertices' maxima
+// (InterpolateEpsilonSigma dissipation.go:219-242), and eps / hK^2 at its
+// owner-side face points, the viscous speed of StoreEdgeAggregates
+// (edges.go:246-289); -1 at the other face points
+template <int N, class T>
+__global__ void __launch_bounds__(256) k_adj_fwd_efield(AdjArgs a, const T *sigv, const T *epsv, T *sigb, T *epsf,
+                                                        T *SPV) {
+  using Sk = AdjShock<N>;
+  constexpr int NR = Sk::NR, NpInt = Sk::NpInt, NpEdge = Sk::NpEdge;
+  const int kk = blockIdx.x * blockDim.x + threadIdx.x;
+  if (kk >= a.Kp) return;
+  const size_t Kp = size_t(a.Kp);
+  if (kk >= a.K) {
+    sigb[kk] = T(0.);
+    for (int r = 0; r < NR; ++r) epsf[r * Kp + kk] = T(0.);
+    for (int f = 0; f < 3; ++f)
+      for (int pp = 0; pp < NpEdge; ++pp) SPV[(f * Kp + kk) * NpEdge + pp] = T(-1.);
+    return;
+  }
+  const int v0 = a.evert[kk], v1 = a.evert[Kp + kk], v2 = a.evert[2 * Kp + kk];
+  sigb[kk] = ((sigv[v0] + sigv[v1]) + sigv[v2]) / 3.;
+  const T e0 = epsv[v0], e1 = epsv[v1], e2 = epsv[v2];
+  const double *B = a.shops + Sk::oBa;
+  for (int r = 0; r < NR; ++r) epsf[r * Kp + kk] = B[3 * r] * e0 + B[3 * r + 1] * e1 + B[3 * r + 2] * e2;
+  const double oohK = a.oohK[kk];
+  for (int f = 0; f < 3; ++f) {
+    const bool owner = a.code[f * Kp + kk] & FC_WAVE;
+    for (int pp = 0; pp < NpEdge; ++pp)
+      SPV[(f * Kp + kk) * NpEdge + pp] = owner ? (oohK * oohK) * epsf[(NpInt + f * NpEdge + pp) * Kp + kk] : T(-1.);
+  }
+}
+
+// X <- F(sigma) X per element and variable, in place (limitAndFilterSolution
+// dissipation.go:606-622 with the vertex-mean sigma)
+template <int N, class T>
+__global__ void __launch_bounds__(256) k_adj_filter_sig(AdjArgs a, T *X, const T *sigb) {
+  using Sk = AdjShock<N>;
+  constexpr int NpInt = Sk::NpInt;
+  const size_t gid = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (gid >= size_t(a.K) * 4) return;
+  const int kk = int(gid >> 2);
+  T *x = X + gid * NpInt;
+  const double *V = a.shops + Sk::oV, *Vi = a.shops + Sk::oVi, *mf = a.shops + Sk::omf;
+  const T fac = 1. - sin((0.5 * M_PI) * sigb[kk]);
+  T u[NpInt], m[NpInt];
+  for (int i = 0; i < NpInt; ++i) u[i] = x[i];
+  for (int i = 0; i < NpInt; ++i) {
+    T t = T(0.);
+    for (int j = 0; j < NpInt; ++j) t += Vi[i * NpInt + j] * u[j];
+    m[i] = i == 0 ? t : t * (mf[i] * fac);
+  }
+  for (int i = 0; i < NpInt; ++i) {
+    T t = T(0.);
+    for (int j = 0; j < NpInt; ++j) t += V[i * NpInt + j] * m[j];
+    x[i] = t;
+  }
+}
+
+// Variable n of the edge-owner's stored trace that face point (f, i) of
+// element kk reads: its own trace where it owns the edge (or the face is
+// unshared), else the owner's at the reversed point (edges.go:225-237,
+// euler.go:889-903)
+template <int N, class T>
+__device__ __forceinline__ T owner_trace(const AdjArgs &a, const T *U, int kk, int f, int i, int n) {
+  using Sh = AdjShape<N>;
+  const size_t fi = size_t(f) * a.Kp + kk;
+  const int code = a.code[fi];
+  int ke = kk, p = f * Sh::NpEdge + i;
+  if ((code & 15) == FK_Shared && !(code & FC_FLUX_OWNER)) {
+    ke = a.nbr_elem[fi];
+    p = a.nbr_face[fi] * Sh::NpEdge + (Sh::NpEdge - 1 - i);
+  }
+  const T *u = U + size_t(ke) * Sh::NV + n * Sh::NpInt;
+  const double *row = a.Interp + p * Sh::NpInt;
+  T t = T(0.);
+  for (int j = 0; j < Sh::NpInt; ++j) t += row[j] * u[j];
+  return t;
+}
+
+// The RT-element gradient DOFs of variable n of element kk
+// (GetSolutionGradientUsingRTElement euler.go:864-918): interior Jinv U,
+// edges n IInII / Jdet times the owner's stored trace
+template <int N, class T>
+__device__ __forceinline__ void grad_dofs(const AdjArgs &a, const T *U, int kk, int n, T *DX, T *DY) {
+  using Sk = AdjShock<N>;
+  constexpr int NpInt = Sk::NpInt, NpEdge = Sk::NpEdge;
+  const size_t Kp = size_t(a.Kp);
+  const double ooJd = 1. / a.Jdet[kk];
+  const double J0 = a.Jinv[kk], J1 = a.Jinv[Kp + kk], J2 = a.Jinv[2 * Kp + kk], J3 = a.Jinv[3 * Kp + kk];
+  const T *u = U + size_t(kk) * AdjShape<N>::NV + n * NpInt;
+  for (int i = 0; i < NpInt; ++i) {
+    const T un = u[i];
+    DX[i] = J0 * un;
+    DY[i] = J1 * un;
+    DX[i + NpInt] = J2 * un;
+    DY[i + NpInt] = J3 * un;
+  }
+  for (int f = 0; f < 3; ++f) {
+    const size_t fi = f * Kp + kk;
+    const double nx = a.Nxo[fi], ny = a.Nyo[fi], w = a.IInII[fi];
+    for (int i = 0; i < NpEdge; ++i) {
+      const T q = owner_trace<N>(a, U, kk, f, i, n);
+      DX[2 * NpInt + f * NpEdge + i] = ooJd * nx * w * q;
+      DY[2 * NpInt + f * NpEdge + i] = ooJd * ny * w * q;
+    }
+  }
+}
+
+__device__ __forceinline__ bool any_nonzero(double x) { return x != 0.; }
+__device__ __forceinline__ bool any_nonzero(const Dual &x) { return x.v != 0. || x.d != 0.; }
+
+// eps grad Q at the used RT points, one thread per (element, variable)
+// (CalculateEpsilonGradient dissipation.go:244-272); zero where eps is
+// zero at every point of the element
+template <int N, class T>
+__global__ void __launch_bounds__(256) k_adj_fwd_grad(AdjArgs a, const T *U, const T *epsf, T *dis) {
+  using Sk = AdjShock<N>;
+  constexpr int NR = Sk::NR, NpFlux = Sk::NpFlux;
+  const size_t gid = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (gid >= size_t(a.Kp) * 4) return;
+  const int kk = int(gid >> 2), n = int(gid & 3);
+  const size_t Kp = size_t(a.Kp);
+  bool act = false;
+  if (kk < a.K)
+    for (int r = 0; r < NR; ++r) act = act || any_nonzero(epsf[r * Kp + kk]);
+  if (!act) {
+    for (int r = 0; r < NR; ++r) {
+      dis[Sk::dis_at(a.Kp, n, 0, r, kk)] = T(0.);
+      dis[Sk::dis_at(a.Kp, n, 1, r, kk)] = T(0.);
+    }
+    return;
+  }
+  T DX[NpFlux], DY[NpFlux];
+  grad_dofs<N>(a, U, kk, n, DX, DY);
+  const double *DR = a.shops + Sk::oDR;
+  for (int r = 0; r < NR; ++r) {
+    T ax = T(0.), ay = T(0.);
+    for (int j = 0; j < NpFlux; ++j) {
+      ax += DR[r * NpFlux + j] * DX[j];
+      ay += DR[r * NpFlux + j] * DY[j];
+    }
+    const T e = epsf[r * Kp + kk];
+    dis[Sk::dis_at(a.Kp, n, 0, r, kk)] = ax * e;
+    dis[Sk::dis_at(a.Kp, n, 1, r, kk)] = ay * e;
+  }
+}
+
+// The viscous edge flux (StoreEdgeViscousFlux edges.go:151-244) taken off
+// the edge values: at a shared edge's owner point i, the mean of both
+// sides' eps grad Q along the owner normal minus the penalty
+// N^2 mean(eps) / L (Q_i - Q_rev i) on the owner's stored trace, written to
+// both sides like the numerical flux; at an unshared point the own side's
+// eps grad Q along its normal
+template <int N, class T>
+__global__ void __launch_bounds__(256) k_adj_fwd_visc(AdjArgs a, const T *U, const T *epsf, const T *dis, T *FE) {
+  using Sk = AdjShock<N>;
+  constexpr int NpInt = Sk::NpInt, NpEdge = Sk::NpEdge;
+  const size_t t = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (t >= size_t(3) * a.Kp * NpEdge) return;
+  int f, kk, pp;
+  size_t fi;
+  face_point<N>(a, t, f, kk, pp, fi);
+  if (kk >= a.K) return;
+  const size_t Kp = size_t(a.Kp);
+  const int code = a.code[fi];
+  const double nx = a.Nx[fi], ny = a.Ny[fi];
+  const int rowL = NpInt + f * NpEdge + pp;
+  if ((code & 15) == FK_Shared) {
+    if (!(code & FC_FLUX_OWNER)) return;  // written by the owner
+    const int nb = a.nbr_elem[fi], nf = a.nbr_face[fi], rp = NpEdge - 1 - pp;
+    const int rowR = NpInt + nf * NpEdge + rp;
+    const size_t fn = size_t(nf) * Kp + nb, t2 = fn * NpEdge + rp;
+    const T lam = 0.5 * (epsf[rowL * Kp + kk] + epsf[rowR * Kp + nb]);
+    const T coef = (a.omega * lam) * a.vooL[fi];
+    const double wo = a.IInII[fi], wn = a.IInII[fn];
+    for (int n = 0; n < 4; ++n) {
+      const T vFL = nx * dis[Sk::dis_at(a.Kp, n, 0, rowL, kk)] + ny * dis[Sk::dis_at(a.Kp, n, 1, rowL, kk)];
+      const T vFR = nx * dis[Sk::dis_at(a.Kp, n, 0, rowR, nb)] + ny * dis[Sk::dis_at(a.Kp, n, 1, rowR, nb)];
+      T VF = 0.5 * (vFL + vFR);
+      VF -= coef * (owner_trace<N>(a, U, kk, f, pp, n) - owner_trace<N>(a, U, kk, f, rp, n));
+      FE[4 * t + n] -= wo * VF;
+      FE[4 * t2 + n] += wn * VF;
+    }
+    return;
+  }
+  const double w = a.IInII[fi];
+  for (int n = 0; n < 4; ++n) {
+    const T VF = nx * dis[Sk::dis_at(a.Kp, n, 0, rowL, kk)] + ny * dis[Sk::dis_at(a.Kp, n, 1, rowL, kk)];
+    FE[4 * t + n] -= w * VF;
+  }
+}
+
+// ---- adjoint of the active sensor path ----
+
+// lam <- F(sigma)^T lam for the element's four variables (in place) and
+// lamSigb[kk] (+)= sum_n lam_n . dF/dsigma X_n at the pre-filter values X
+template <int N>
+__global__ void __launch_bounds__(256) k_adj_filter_sig_t(AdjArgs a, double *lam, const double *X, const double *sigb,
+                                                          double *lamSigb, int accumulate) {
+  using Sk = AdjShock<N>;
+  constexpr int NpInt = Sk::NpInt;
+  const int kk = blockIdx.x * blockDim.x + threadIdx.x;
+  if (kk >= a.K) return;
+  const double *V = a.shops + Sk::oV, *Vi = a.shops + Sk::oVi, *mf = a.shops + Sk::omf;
+  const double sb = sigb[kk];
+  const double fac = 1. - sin((0.5 * M_PI) * sb), dfac = -(0.5 * M_PI) * cos((0.5 * M_PI) * sb);
+  double dsum = 0.;
+  for (int n = 0; n < 4; ++n) {
+    double *l = lam + size_t(kk) * AdjShape<N>::NV + n * NpInt;
+    const double *x = X + size_t(kk) * AdjShape<N>::NV + n * NpInt;
+    double w[NpInt];
+    for (int m = 0; m < NpInt; ++m) {
+      double t = 0., mx = 0.;
+      for (int i = 0; i < NpInt; ++i) t += V[i * NpInt + m] * l[i];
+      for (int j = 0; j < NpInt; ++j) mx += Vi[m * NpInt + j] * x[j];
+      if (m > 0) {
+        dsum += t * mf[m] * dfac * mx;
+        t *= mf[m] * fac;
+      }
+      w[m] = t;
+    }
+    for (int j = 0; j < NpInt; ++j) {
+      double t = 0.;
+      for (int m = 0; m < NpInt; ++m) t += Vi[m * NpInt + j] * w[m];
+      l[j] = t;
+    }
+  }
+  lamSigb[kk] = accumulate ? lamSigb[kk] + dsum : dsum;
+}
+
+// Cotangents of the viscous edge flux, one thread per face point; the
+// owner point of a shared edge writes both sides' rows: eps grad Q at the
+// edge rows (lamDis), eps at the edge rows from the penalty (lamEpsf), and
+// its own stored trace's cotangent from the penalty jumps of its point and
+// of the reversed one (lamEQv; zero at every other point)
+template <int N>
+__global__ void __launch_bounds__(256) k_adj_visc_t(AdjArgs a, const double *U, const double *lamR, const double *epsf,
+                                                    double *lamDis, double *lamEpsf, double *lamEQv) {
+  using Sk = AdjShock<N>;
+  constexpr int NpInt = Sk::NpInt, NpEdge = Sk::NpEdge;
+  const size_t t = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (t >= size_t(3) * a.Kp * NpEdge) return;
+  int f, kk, pp;
+  size_t fi;
+  face_point<N>(a, t, f, kk, pp, fi);
+  for (int c = 0; c < 4; ++c) lamEQv[4 * t + c] = 0.;
+  if (kk >= a.K) return;
+  const size_t Kp = size_t(a.Kp);
+  const int code = a.code[fi];
+  const double nx = a.Nx[fi], ny = a.Ny[fi];
+  const int rowL = NpInt + f * NpEdge + pp;
+  if ((code & 15) == FK_Shared) {
+    if (!(code & FC_FLUX_OWNER)) return;
+    const int nb = a.nbr_elem[fi], nf = a.nbr_face[fi], rp = NpEdge - 1 - pp;
+    const int rowR = NpInt + nf * NpEdge + rp;
+    const size_t fn = size_t(nf) * Kp + nb;
+    const double wo = a.IInII[fi], wn = a.IInII[fn];
+    // the edge values' cotangents at this point and at the reversed one
+    double lfo[4], lfn[4], lfor[4], lfnr[4], lV[4], lVr[4];
+    edge_cotangent<N>(a, lamR, kk, f * NpEdge + pp, lfo);
+    edge_cotangent<N>(a, lamR, nb, nf * NpEdge + rp, lfn);
+    edge_cotangent<N>(a, lamR, kk, f * NpEdge + rp, lfor);
+    edge_cotangent<N>(a, lamR, nb, nf * NpEdge + pp, lfnr);
+    for (int c = 0; c < 4; ++c) {
+      lV[c] = -(wo * lfo[c] - wn * lfn[c]);
+      lVr[c] = -(wo * lfor[c] - wn * lfnr[c]);
+    }
+    const double pc = a.omega * 0.5 * a.vooL[fi];
+    const double coef = pc * (epsf[rowL * Kp + kk] + epsf[rowR * Kp + nb]);
+    const double coefr = pc * (epsf[(NpInt + f * NpEdge + rp) * Kp + kk] + epsf[(NpInt + nf * NpEdge + pp) * Kp + nb]);
+    double jump = 0.;
+    for (int c = 0; c < 4; ++c) {
+      const double qi = owner_trace<N>(a, U, kk, f, pp, c), qr = owner_trace<N>(a, U, kk, f, rp, c);
+      jump += (qi - qr) * lV[c];
+      lamDis[Sk::dis_at(a.Kp, c, 0, rowL, kk)] = 0.5 * nx * lV[c];
+      lamDis[Sk::dis_at(a.Kp, c, 1, rowL, kk)] = 0.5 * ny * lV[c];
+      lamDis[Sk::dis_at(a.Kp, c, 0, rowR, nb)] = 0.5 * nx * lV[c];
+      lamDis[Sk::dis_at(a.Kp, c, 1, rowR, nb)] = 0.5 * ny * lV[c];
+      lamEQv[4 * t + c] = -coef * lV[c] + coefr * lVr[c];
+    }
+    lamEpsf[rowL * Kp + kk] = -pc * jump;
+    lamEpsf[rowR * Kp + nb] = -pc * jump;
+    return;
+  }
+  const double w = a.IInII[fi];
+  double lf[4];
+  edge_cotangent<N>(a, lamR, kk, f * NpEdge + pp, lf);
+  for (int c = 0; c < 4; ++c) {
+    const double lv = -w * lf[c];
+    lamDis[Sk::dis_at(a.Kp, c, 0, rowL, kk)] = nx * lv;
+    lamDis[Sk::dis_at(a.Kp, c, 1, rowL, kk)] = ny * lv;
+  }
+  lamEpsf[rowL * Kp + kk] = 0.;
+}
+
+// Cotangents through eps grad Q, one thread per (element, variable), four
+// consecutive lanes per element: the interior rows' cotangent from the
+// interior flux (the flux is F - eps grad Q), the edge rows' from
+// k_adj_visc_t; then eps's (summed over the variables: interior rows set,
+// edge rows added), the interior state's (added to target) and the
+// owner-stored traces' at the element's face points (lamEQg). Elements with
+// eps zero everywhere take none (their eps cotangent is set to zero).
+template <int N>
+__global__ void __launch_bounds__(256) k_adj_grad_t(AdjArgs a, const double *U, const double *lamR, const double *epsf,
+                                                    const double *lamDis, double *lamEpsf, double *lamEQg,
+                                                    double *target) {
+  using Sk = AdjShock<N>;
+  constexpr int NR = Sk::NR, NpInt = Sk::NpInt, NpEdge = Sk::NpEdge, NpFlux = Sk::NpFlux;
+  const size_t gid = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (gid >= size_t(a.Kp) * 4) return;  // Kp * 4 is a multiple of 64: whole element groups
+  const int kk = int(gid >> 2), n = int(gid & 3);
+  const size_t Kp = size_t(a.Kp);
+  bool act = false;
+  if (kk < a.K)
+    for (int r = 0; r < NR; ++r) act = act || epsf[r * Kp + kk] != 0.;
+  if (!act) {
+    for (int f = 0; f < 3; ++f)
+      for (int i = 0; i < NpEdge; ++i) lamEQg[4 * ((f * Kp + kk) * NpEdge + i) + n] = 0.;
+    if (n == 0)
+      for (int r = 0; r < NR; ++r) lamEpsf[r * Kp + kk] = 0.;
+    return;
+  }
+  const double Jd = a.Jdet[kk], m = -(1. / Jd), ooJd = 1. / Jd;
+  const double J0 = a.Jinv[kk], J1 = a.Jinv[Kp + kk], J2 = a.Jinv[2 * Kp + kk], J3 = a.Jinv[3 * Kp + kk];
+  const double *lr = lamR + size_t(kk) * AdjShape<N>::NV + n * NpInt;
+  const double *D = a.DivInt, *DR = a.shops + Sk::oDR;
+  double DX[NpFlux], DY[NpFlux];
+  grad_dofs<N>(a, U, kk, n, DX, DY);
+  double gX[NpFlux], gY[NpFlux];  // cotangents of the gradient DOFs
+  for (int j = 0; j < NpFlux; ++j) {
+    gX[j] = 0.;
+    gY[j] = 0.;
+  }
+  for (int r = 0; r < NR; ++r) {
+    double lDX, lDY;
+    if (r < NpInt) {
+      // the interior flux's cotangent at point r (k_adj_elem), negated
+      double lr_ = 0., ls_ = 0.;
+      for (int i = 0; i < NpInt; ++i) {
+        lr_ += D[i * NpFlux + r] * lr[i];
+        ls_ += D[i * NpFlux + NpInt + r] * lr[i];
+      }
+      lr_ *= m;
+      ls_ *= m;
+      lDX = -(Jd * (J0 * lr_ + J2 * ls_));
+      lDY = -(Jd * (J1 * lr_ + J3 * ls_));
+    } else {
+      lDX = lamDis[Sk::dis_at(a.Kp, n, 0, r, kk)];
+      lDY = lamDis[Sk::dis_at(a.Kp, n, 1, r, kk)];
+    }
+    double ax = 0., ay = 0.;
+    for (int j = 0; j < NpFlux; ++j) {
+      ax += DR[r * NpFlux + j] * DX[j];
+      ay += DR[r * NpFlux + j] * DY[j];
+    }
+    double le = lDX * ax + lDY * ay;  // eps's cotangent from this variable
+    le += __shfl_xor(le, 1, 64);
+    le += __shfl_xor(le, 2, 64);
+    if (n == 0) lamEpsf[r * Kp + kk] = (r < NpInt ? 0. : lamEpsf[r * Kp + kk]) + le;
+    const double e = epsf[r * Kp + kk], bx = e * lDX, by = e * lDY;
+    for (int j = 0; j < NpFlux; ++j) {
+      gX[j] += DR[r * NpFlux + j] * bx;
+      gY[j] += DR[r * NpFlux + j] * by;
+    }
+  }
+  double *tg = target + size_t(kk) * AdjShape<N>::NV + n * NpInt;
+  for (int i = 0; i < NpInt; ++i) tg[i] += J0 * gX[i] + J2 * gX[i + NpInt] + J1 * gY[i] + J3 * gY[i + NpInt];
+  for (int f = 0; f < 3; ++f) {
+    const size_t fi = f * Kp + kk;
+    const double nx = a.Nxo[fi], ny = a.Nyo[fi], w = a.IInII[fi];
+    for (int i = 0; i < NpEdge; ++i) {
+      const int j = 2 * NpInt + f * NpEdge + i;
+      lamEQg[4 * ((f * Kp + kk) * NpEdge + i) + n] = ooJd * w * (nx * gX[j] + ny * gY[j]);
+    }
+  }
+}
+
+// The owner-stored traces' cotangents into the face-point trace cotangents
+// of the edge owner: its viscous-flux term, its own gradient term and, for a
+// shared edge, the partner's gradient term at the reversed point
+template <int N>
+__global__ void __launch_bounds__(256) k_adj_eq_gather(AdjArgs a, const double *lamEQv, const double *lamEQg,
+                                                       double *lamOwn) {
+  constexpr int NpEdge = AdjShape<N>::NpEdge;
+  const size_t t = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (t >= size_t(3) * a.Kp * NpEdge) return;
+  int f, kk, pp;
+  size_t fi;
+  face_point<N>(a, t, f, kk, pp, fi);
+  if (kk >= a.K) return;
+  const int code = a.code[fi];
+  const bool shared = (code & 15) == FK_Shared;
+  if (shared && !(code & FC_FLUX_OWNER)) return;
+  size_t t2 = 0;
+  if (shared) t2 = (size_t(a.nbr_face[fi]) * a.Kp + a.nbr_elem[fi]) * NpEdge + (NpEdge - 1 - pp);
+  for (int c = 0; c < 4; ++c) {
+    double s = lamEQv[4 * t + c] + lamEQg[4 * t + c];
+    if (shared) s += lamEQg[4 * t2 + c];
+    lamOwn[4 * t + c] += s;
+  }
+}
+
+// The viscous dt term where it set the stage's dt (dt = Cdiff / Wv at the
+// arg-max owner face point): into eps at that point's RT row. One thread.
+template <int N>
+__global__ void k_adj_vdt(AdjArgs a, int stage, const int *vbr, const int *argmaxv, const double *Wvv,
+                          const double *lam_dt, double *lamEpsf) {
+  using Sh = AdjShape<N>;
+  if (blockIdx.x != 0 || threadIdx.x != 0) return;
+  if (!vbr[stage]) return;
+  int f, kk, pp;
+  size_t fi;
+  face_point<N>(a, size_t(argmaxv[stage]), f, kk, pp, fi);
+  const double W = Wvv[stage];
+  const double lW = -(a.cdiff / (W * W)) * lam_dt[stage];
+  const double oohK = a.oohK[kk];
+  lamEpsf[size_t(Sh::NpInt + f * Sh::NpEdge + pp) * a.Kp + kk] += (oohK * oohK) * lW;
+}
+
+// Per vertex: the cotangents of its eps and sigma maxima, summed over its
+// elements in incidence order (the transposes of the barycentric
+// interpolation and of the vertex mean)
+template <int N>
+__global__ void __launch_bounds__(256) k_adj_vert_t(AdjArgs a, const double *lamEpsf, const double *lamSigb,
+                                                    double *lamEpsv, double *lamSigv) {
+  using Sk = AdjShock<N>;
+  const int v = blockIdx.x * blockDim.x + threadIdx.x;
+  if (v >= a.Nv) return;
+  const double *B = a.shops + Sk::oBa;
+  double le = 0., ls = 0.;
+  for (int q = a.vptr[v]; q < a.vptr[v + 1]; ++q) {
+    const int kk = a.vinc[q] >> 2, lv = a.vinc[q] & 3;
+    for (int r = 0; r < Sk::NR; ++r) le += B[3 * r + lv] * lamEpsf[size_t(r) * a.Kp + kk];
+    ls += lamSigb[kk] / 3.;
+  }
+  lamEpsv[v] = le;
+  lamSigv[v] = ls;
+}
+
+// Per element: the cotangents of the vertex maxima it attains, through
+// eps = eps0 hK sigma and the ramp to Se and the density of the sensor's
+// input (the stage input before the stage-2 filter), added to target
+template <int N>
+__global__ void __launch_bounds__(256) k_adj_sens_t(AdjArgs a, const double *Us, const int *args, const int *arge,
+                                                    const double *lamEpsv, const double *lamSigv, double *target) {
+  using Sk = AdjShock<N>;
+  constexpr int NpInt = Sk::NpInt;
+  const int kk = blockIdx.x * blockDim.x + threadIdx.x;
+  if (kk >= a.K) return;
+  const size_t Kp = size_t(a.Kp);
+  double le = 0., ls = 0.;
+  for (int c = 0; c < 3; ++c) {
+    const int v = a.evert[c * Kp + kk];
+    if (arge[v] == kk) le += lamEpsv[v];
+    if (args[v] == kk) ls += lamSigv[v];
+  }
+  ls += (a.eps0 * a.hK[kk]) * le;
+  if (ls == 0.) return;
+  const double *rho = Us + size_t(kk) * AdjShape<N>::NV;
+  const double *Pm = a.shops + Sk::oP, *Mm = a.shops + Sk::oM, *Dm = a.shops + Sk::oD;
+  double X[NpInt], Y[NpInt], DQ[NpInt], num = 0., den = 0.;
+  for (int i = 0; i < NpInt; ++i) {
+    double x = 0., y = 0., dq = 0.;
+    for (int j = 0; j < NpInt; ++j) {
+      x += Pm[i * NpInt + j] * rho[j];
+      y += Mm[i * NpInt + j] * rho[j];
+      dq += Dm[i * NpInt + j] * rho[j];
+    }
+    X[i] = x;
+    Y[i] = y;
+    DQ[i] = dq;
+    num += dq * x;
+    den += rho[i] * y;
+  }
+  const double slope = sigma_slope(a, log10(num / den));
+  if (slope == 0.) return;
+  const double lse = slope * ls / 2.302585092994045684;
+  double *tg = target + size_t(kk) * AdjShape<N>::NV;  // variable 0: the density
+  for (int j = 0; j < NpInt; ++j) {
+    double gn = 0., gd = Y[j];
+    for (int i = 0; i < NpInt; ++i) {
+      gn += Dm[i * NpInt + j] * X[i] + DQ[i] * Pm[i * NpInt + j];
+      gd += rho[i] * Mm[i * NpInt + j];
+    }
+    tg[j] += lse * (gn / num - gd / den);
+  }
+}
+
 inline unsigned adj_blocks(size_t n) { return unsigned((n + 255) / 256); }
 
 template <int N>
@@ -912,14 +1544,16 @@ hipError_t adj_launch_n(int op, const AdjArgs &a, const AdjBufs &b, hipStream_t 
                            static_cast<Dual *>(b.R), static_cast<const Dual *>(b.U0),
                            static_cast<const Dual *>(b.U2), static_cast<const Dual *>(b.U3),
                            static_cast<const Dual *>(b.R3), static_cast<const Dual *>(b.dts),
-                           static_cast<const Dual *>(b.dtk), static_cast<Dual *>(b.out));
+                           static_cast<const Dual *>(b.dtk), static_cast<Dual *>(b.out),
+                           static_cast<const Dual *>(b.dis), static_cast<const Dual *>(b.sigb), nullptr);
       else
         hipLaunchKernelGGL((k_adj_fwd_rhs<N, double>), dim3(nb), dim3(256), 0, s, a, b.stage,
                            static_cast<const double *>(b.Us), static_cast<const double *>(b.FE),
                            static_cast<double *>(b.R), static_cast<const double *>(b.U0),
                            static_cast<const double *>(b.U2), static_cast<const double *>(b.U3),
                            static_cast<const double *>(b.R3), static_cast<const double *>(b.dts),
-                           static_cast<const double *>(b.dtk), static_cast<double *>(b.out));
+                           static_cast<const double *>(b.dtk), static_cast<double *>(b.out),
+                           static_cast<const double *>(b.dis), static_cast<const double *>(b.sigb), b.Rpre);
       break;
     }
     case ADJ_FWD_LTS:
@@ -980,9 +1614,72 @@ hipError_t adj_launch_n(int op, const AdjArgs &a, const AdjBufs &b, hipStream_t 
       hipLaunchKernelGGL((k_adj_elem<N>), dim3(adj_blocks(size_t(a.K) * Sh::NpInt)), dim3(256), 0, s, a,
                          static_cast<const double *>(b.Us), b.lamR, b.lamOwn, b.target);
       break;
+    case ADJ_FWD_SHOCK:
+#define GOCFD_FWD_SHOCK(TT)                                                                                       \
+  hipLaunchKernelGGL((k_adj_fwd_sens<N, TT>), dim3(adj_blocks(a.Kp)), dim3(256), 0, s, a,                          \
+                     static_cast<const TT *>(b.Us), static_cast<TT *>(b.sig), static_cast<TT *>(b.eps));            \
+  hipLaunchKernelGGL((k_adj_fwd_vmax<N, TT>), dim3(adj_blocks(a.Nv)), dim3(256), 0, s, a,                          \
+                     static_cast<const TT *>(b.sig), static_cast<const TT *>(b.eps), static_cast<TT *>(b.sigv),     \
+                     static_cast<TT *>(b.epsv), b.args, b.arge);                                                    \
+  hipLaunchKernelGGL((k_adj_fwd_efield<N, TT>), dim3(adj_blocks(a.Kp)), dim3(256), 0, s, a,                        \
+                     static_cast<const TT *>(b.sigv), static_cast<const TT *>(b.epsv), static_cast<TT *>(b.sigb),   \
+                     static_cast<TT *>(b.epsf), static_cast<TT *>(b.SPV));
+      if (b.dual) {
+        GOCFD_FWD_SHOCK(Dual)
+      } else {
+        GOCFD_FWD_SHOCK(double)
+      }
+#undef GOCFD_FWD_SHOCK
+      break;
+    case ADJ_FILTER_SIG:
+      if (b.dual)
+        hipLaunchKernelGGL((k_adj_filter_sig<N, Dual>), dim3(adj_blocks(size_t(a.K) * 4)), dim3(256), 0, s, a,
+                           static_cast<Dual *>(b.out), static_cast<const Dual *>(b.sigb));
+      else
+        hipLaunchKernelGGL((k_adj_filter_sig<N, double>), dim3(adj_blocks(size_t(a.K) * 4)), dim3(256), 0, s, a,
+                           static_cast<double *>(b.out), static_cast<const double *>(b.sigb));
+      break;
+    case ADJ_FWD_VISC:
+#define GOCFD_FWD_VISC(TT)                                                                                        \
+  hipLaunchKernelGGL((k_adj_fwd_grad<N, TT>), dim3(adj_blocks(size_t(a.Kp) * 4)), dim3(256), 0, s, a,              \
+                     static_cast<const TT *>(b.Us), static_cast<const TT *>(b.epsf), static_cast<TT *>(b.dis));     \
+  hipLaunchKernelGGL((k_adj_fwd_visc<N, TT>), dim3(adj_blocks(nfp)), dim3(256), 0, s, a,                           \
+                     static_cast<const TT *>(b.Us), static_cast<const TT *>(b.epsf), static_cast<const TT *>(b.dis),\
+                     static_cast<TT *>(b.FE));
+      if (b.dual) {
+        GOCFD_FWD_VISC(Dual)
+      } else {
+        GOCFD_FWD_VISC(double)
+      }
+#undef GOCFD_FWD_VISC
+      break;
+    case ADJ_ADJ_FILTER_SIG_T:
+      hipLaunchKernelGGL((k_adj_filter_sig_t<N>), dim3(adj_blocks(a.K)), dim3(256), 0, s, a, b.lam, b.Xpre,
+                         static_cast<const double *>(b.sigb), b.lamSigb, b.accumulate);
+      break;
+    case ADJ_ADJ_VISC:
+      hipLaunchKernelGGL((k_adj_visc_t<N>), dim3(adj_blocks(nfp)), dim3(256), 0, s, a,
+                         static_cast<const double *>(b.Us), b.lamR, static_cast<const double *>(b.epsf), b.lamDis,
+                         b.lamEpsf, b.lamEQv);
+      hipLaunchKernelGGL((k_adj_grad_t<N>), dim3(adj_blocks(size_t(a.Kp) * 4)), dim3(256), 0, s, a,
+                         static_cast<const double *>(b.Us), b.lamR, static_cast<const double *>(b.epsf), b.lamDis,
+                         b.lamEpsf, b.lamEQg, b.target);
+      hipLaunchKernelGGL((k_adj_eq_gather<N>), dim3(adj_blocks(nfp)), dim3(256), 0, s, a, b.lamEQv, b.lamEQg,
+                         b.lamOwn);
+      break;
+    case ADJ_ADJ_VDT:
+      hipLaunchKernelGGL((k_adj_vdt<N>), dim3(1), dim3(64), 0, s, a, b.stage, b.vbr, b.argmaxv, b.Wvv, b.lam_dt,
+                         b.lamEpsf);
+      break;
+    case ADJ_ADJ_SENSOR:
+      hipLaunchKernelGGL((k_adj_vert_t<N>), dim3(adj_blocks(a.Nv)), dim3(256), 0, s, a, b.lamEpsf, b.lamSigb,
+                         b.lamEpsv, b.lamSigv);
+      hipLaunchKernelGGL((k_adj_sens_t<N>), dim3(adj_blocks(a.K)), dim3(256), 0, s, a, b.Usens, b.args, b.arge,
+                         b.lamEpsv, b.lamSigv, b.target);
+      break;
     case ADJ_ADJ_WAVE:
       hipLaunchKernelGGL((k_adj_wave<N>), dim3(1), dim3(64), 0, s, a, b.stage, static_cast<const double *>(b.Us),
-                         b.argmax, b.Wv, b.lam_dt, b.target, b.lamFarW);
+                         b.argmax, b.Wv, b.lam_dt, b.target, b.lamFarW, a.act ? b.vbr : nullptr);
       break;
     default:
       return hipErrorInvalidValue;

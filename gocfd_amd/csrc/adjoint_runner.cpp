@@ -49,6 +49,42 @@ AdjointRunner::AdjointRunner(const Discretisation &d, const DeviceLayout &L, con
     a_.sens = sens_.p;
     const double np1 = double(d.N + 1);
     a_.sens_left = 4. / (np1 * np1 * np1 * np1) - kappa;
+    if (!lts) {
+      // the whole sensor path, an active sensor included
+      a_.act = 1;
+      shops_.upload(ops);
+      hK_.upload(L.hK);
+      Nxo_.upload(L.Nxo);
+      Nyo_.upload(L.Nyo);
+      vooL_.upload(L.vooL);
+      evert_.upload(L.evert);
+      // vertex -> (element << 2 | local vertex) of the real elements, in
+      // element order
+      std::vector<int> vptr(size_t(d.Nv) + 1, 0), vinc(3 * size_t(L.K));
+      for (int kk = 0; kk < L.K; ++kk)
+        for (int c = 0; c < 3; ++c) ++vptr[size_t(L.evert[size_t(c) * L.Kp + kk]) + 1];
+      for (int v = 0; v < d.Nv; ++v) vptr[v + 1] += vptr[v];
+      std::vector<int> fill(vptr.begin(), vptr.end() - 1);
+      for (int kk = 0; kk < L.K; ++kk)
+        for (int c = 0; c < 3; ++c) vinc[size_t(fill[L.evert[size_t(c) * L.Kp + kk]]++)] = (kk << 2) | c;
+      vptr_.upload(vptr);
+      vinc_.upload(vinc);
+      a_.shops = shops_.p;
+      a_.hK = hK_.p;
+      a_.Nxo = Nxo_.p;
+      a_.Nyo = Nyo_.p;
+      a_.vooL = vooL_.p;
+      a_.evert = evert_.p;
+      a_.vptr = vptr_.p;
+      a_.vinc = vinc_.p;
+      a_.Nv = d.Nv;
+      a_.kappa = kappa;
+      a_.S0 = 4. / (np1 * np1 * np1 * np1);
+      a_.eps0 = 5. / 1.5;
+      a_.cdiff = 1. / (np1 * np1);
+      a_.omega = double(d.N * d.N);
+      NR_ = NpInt + 3 * NpEdge;
+    }
   }
   if (lts) {
     wsrc_.upload(L.wsrc);
@@ -82,6 +118,33 @@ AdjointRunner::AdjointRunner(const Discretisation &d, const DeviceLayout &L, con
   time_.alloc(1);
   part_v_.alloc(4 * size_t(nparts_));
   part_i_.alloc(size_t(nparts_));
+  if (a_.act) {
+    const size_t Kp = size_t(L.Kp), Nv = size_t(d.Nv), nr = size_t(NR_) * Kp;
+    sig_.alloc(Kp);
+    eps_.alloc(Kp);
+    sigv_.alloc(Nv);
+    epsv_.alloc(Nv);
+    dis_.alloc(8 * nr);
+    SPV_.alloc(3 * Kp * NpEdge);
+    sigbR_.alloc(5 * Kp);
+    epsfR_.alloc(5 * nr);
+    argsR_.alloc(5 * Nv);
+    argeR_.alloc(5 * Nv);
+    argmaxv_.alloc(5);
+    vbr_.alloc(5);
+    Wvv_.alloc(5);
+    RpreR_.alloc(5 * n_);
+    U2pre_.alloc(n_);
+    part_v2_.alloc(size_t(nparts_));
+    part_i2_.alloc(size_t(nparts_));
+    lamSigb_.alloc(Kp);
+    lamEpsf_.alloc(nr);
+    lamDis_.alloc(8 * nr);
+    lamEQv_.alloc(4 * nfp_);
+    lamEQg_.alloc(4 * nfp_);
+    lamSigv_.alloc(Nv);
+    lamEpsv_.alloc(Nv);
+  }
   if (lts) {
     scratch_small_.dtk.alloc(5 * size_t(L.Kp));
     scratch_small_.srck.alloc(5 * size_t(L.Kp));
@@ -142,6 +205,20 @@ AdjointRunner::StageRecord AdjointRunner::scratch_record() {
   r.dts = dts_.p;
   r.argmax = argmax_.p;
   r.Wv = Wv_.p;
+  if (a_.act) {
+    const size_t Kp = size_t(a_.Kp), Nv = size_t(a_.Nv), nr = size_t(NR_) * Kp;
+    for (int s = 0; s < 5; ++s) {
+      r.sigb[s] = sigbR_.p + s * Kp;
+      r.epsf[s] = epsfR_.p + s * nr;
+      r.args[s] = argsR_.p + s * Nv;
+      r.arge[s] = argeR_.p + s * Nv;
+      r.Rpre[s] = RpreR_.p + s * n_;
+    }
+    r.U2pre = U2pre_.p;
+    r.argmaxv = argmaxv_.p;
+    r.vbr = vbr_.p;
+    r.Wvv = Wvv_.p;
+  }
   return r;
 }
 
@@ -162,7 +239,26 @@ void AdjointRunner::forward_step(bool dual, const void *Ustart, void *Uend, doub
     const void *Us = s == 0 ? Ustart : static_cast<const void *>(rec.U[s]);
     b.stage = s;
     b.Us = Us;
-    if (a_.shock) {
+    if (a_.act) {
+      // the sensor fields of the stage input, then the stage-2 state filter
+      // in place (its input kept for the adjoint)
+      b.sig = sig_.p;
+      b.eps = eps_.p;
+      b.sigv = sigv_.p;
+      b.epsv = epsv_.p;
+      b.args = rec.args[s];
+      b.arge = rec.arge[s];
+      b.sigb = rec.sigb[s];
+      b.epsf = rec.epsf[s];
+      b.SPV = SPV_.p;
+      b.dis = dis_.p;
+      launch(dev::ADJ_FWD_SHOCK, b);
+      if (s == 2) {
+        if (!dual) HIPCHECK(hipMemcpyAsync(rec.U2pre, rec.U[2], n_ * sizeof(double), hipMemcpyDeviceToDevice, s_));
+        b.out = rec.U[2];
+        launch(dev::ADJ_FILTER_SIG, b);
+      }
+    } else if (a_.shock) {
       // the sensor on the stage input (checked in the sweeps that check the
       // time), then the stage-2 state filter in place
       if (time) launch(dev::ADJ_SENSOR, b);
@@ -180,6 +276,13 @@ void AdjointRunner::forward_step(bool dual, const void *Ustart, void *Uend, doub
     b.R3 = rec.R[3];
     b.out = s < 4 ? rec.U[s + 1] : Uend;
     launch(dev::ADJ_FWD_FACE, b);
+    if (a_.act) launch(dev::ADJ_FWD_VISC, b);
+    b.argmaxv = rec.argmaxv;
+    b.vbr = rec.vbr;
+    b.Wvv = rec.Wvv;
+    b.part_v2 = part_v2_.p;
+    b.part_i2 = part_i2_.p;
+    b.Rpre = (a_.act && !dual) ? rec.Rpre[s] : nullptr;
     if (a_.lts) {
       b.dtk = rec.dtk[s];
       b.dtk_prev = s > 0 ? rec.dtk[s - 1] : nullptr;
@@ -208,7 +311,14 @@ void AdjointRunner::adjoint_step(const double *Ustart, const StageRecord &rec, d
     b.srck = rec.srck[s];
     b.wk = rec.wk[s];
     launch(a_.lts ? dev::ADJ_ADJ_SSP_LTS : dev::ADJ_ADJ_SSP, b);
-    if (a_.shock) {  // the RHS filter, transposed
+    if (a_.act) {  // the RHS filter, transposed; its sigma term sets lamSigb
+      dev::AdjBufs bf = b;
+      bf.lam = b.lamR;
+      bf.Xpre = rec.Rpre[s];
+      bf.sigb = rec.sigb[s];
+      bf.accumulate = 0;
+      launch(dev::ADJ_ADJ_FILTER_SIG_T, bf);
+    } else if (a_.shock) {  // the RHS filter, transposed
       dev::AdjBufs bf = b;
       bf.lam = b.lamR;
       launch(dev::ADJ_FILTER_T, bf);
@@ -216,13 +326,38 @@ void AdjointRunner::adjoint_step(const double *Ustart, const StageRecord &rec, d
     b.Us = s == 0 ? static_cast<const void *>(Ustart) : static_cast<const void *>(rec.U[s]);
     b.target = s == 0 ? b.lamU0 : b.lamQ[s];
     launch(dev::ADJ_ADJ_FACE, b);
+    if (a_.act) {  // the viscous edge flux and eps grad Q, transposed
+      b.epsf = rec.epsf[s];
+      launch(dev::ADJ_ADJ_VISC, b);
+    }
     if (a_.lts) launch(dev::ADJ_ADJ_LTS_WAVE, b);
     launch(dev::ADJ_ADJ_ELEM, b);
-    if (!a_.lts) launch(dev::ADJ_ADJ_WAVE, b);
+    if (!a_.lts) {
+      if (a_.act) {  // the viscous dt where it bound, into eps
+        b.vbr = rec.vbr;
+        b.argmaxv = rec.argmaxv;
+        b.Wvv = rec.Wvv;
+        launch(dev::ADJ_ADJ_VDT, b);
+      }
+      launch(dev::ADJ_ADJ_WAVE, b);
+    }
     if (a_.shock && s == 2) {  // the stage-2 state filter, transposed
       dev::AdjBufs bf = b;
       bf.lam = b.lamQ[2];
-      launch(dev::ADJ_FILTER_T, bf);
+      if (a_.act) {
+        bf.Xpre = rec.U2pre;
+        bf.sigb = rec.sigb[2];
+        bf.accumulate = 1;
+        launch(dev::ADJ_ADJ_FILTER_SIG_T, bf);
+      } else {
+        launch(dev::ADJ_FILTER_T, bf);
+      }
+    }
+    if (a_.act) {  // the vertex merges, the ramp and the sensor, at the sensor's input
+      b.Usens = s == 2 ? rec.U2pre : static_cast<const double *>(b.Us);
+      b.args = rec.args[s];
+      b.arge = rec.arge[s];
+      launch(dev::ADJ_ADJ_SENSOR, b);
     }
   }
 }
@@ -267,6 +402,8 @@ void AdjointRunner::vjp(const double *q0, double t0, int nsteps, int every, cons
   stored_records_ = (want > have ? (want - have) * bytes : 0) <= free_b / 4;
   // GOCFD_ADJOINT_KEEP_RECORDS=0 forces the per-step recomputation
   if (const char *e = std::getenv("GOCFD_ADJOINT_KEEP_RECORDS")) stored_records_ = stored_records_ && e[0] != '0';
+  // the active sensor path keeps one step's record (its fields per stage)
+  if (a_.act) stored_records_ = false;
   // With one block and its records kept, the record sweep is the forward
   // sweep (with the FinalTime check); otherwise a forward sweep first stores
   // the checkpoints at the block starts.
@@ -328,6 +465,13 @@ void AdjointRunner::vjp(const double *q0, double t0, int nsteps, int every, cons
   b.lam_dt = lam_dt.p;
   b.carry = carry_.p;
   b.lamSpeed = lamSpeed_.p;
+  b.lamSigb = lamSigb_.p;
+  b.lamEpsf = lamEpsf_.p;
+  b.lamDis = lamDis_.p;
+  b.lamEQv = lamEQv_.p;
+  b.lamEQg = lamEQg_.p;
+  b.lamSigv = lamSigv_.p;
+  b.lamEpsv = lamEpsv_.p;
   for (int bk = nblocks - 1; bk >= 0; --bk) {
     const int j0 = bk * every_, cnt = std::min(every_, nsteps - j0);
     std::vector<const double *> starts(cnt);

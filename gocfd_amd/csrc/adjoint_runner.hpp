@@ -27,8 +27,10 @@ struct AdjointGeometry {
 
 class AdjointRunner {
  public:
-  // shock: PerssonC0 with this Kappa, differentiated while its sensor is
-  // quiet (sigma = 0 everywhere; a window where it is not is refused)
+  // shock: PerssonC0 with this Kappa. With global time stepping the whole
+  // sensor path is differentiated (an active sensor included); with local
+  // time stepping only while the sensor is quiet (sigma = 0 everywhere; a
+  // window where it is not is refused)
   AdjointRunner(const Discretisation &d, const DeviceLayout &L, const dev::Phys &ph, bool io_tied, bool lts,
                 bool shock, double kappa, const AdjointGeometry &geo, hipStream_t stream);
   // lam0 = (dq_n/dq_0)^T lamN and far = (dq_n/dqinf)^T lamN for nsteps steps
@@ -55,6 +57,15 @@ class AdjointRunner {
     void *dtk[5];
     int *srck[5];
     double *wk[5];
+    // active PerssonC0, per stage: vertex-mean sigma, eps at the used RT
+    // points, the vertex maxima's elements, the RHS before its filter; the
+    // stage-2 state before its filter; the viscous dt's point and branch
+    void *sigb[5], *epsf[5];
+    int *args[5], *arge[5];
+    double *Rpre[5];
+    double *U2pre;
+    int *argmaxv, *vbr;
+    double *Wvv;
   };
   void launch(int op, dev::AdjBufs &b);
   void forward_step(bool dual, const void *Ustart, void *Uend, double *time, const StageRecord &rec);
@@ -94,6 +105,14 @@ class AdjointRunner {
   DevBuf<int> argmax_, flag_, part_i_;
   DevBuf<double> Wv_, time_, part_v_;
   int nparts_ = 1024;
+  // active PerssonC0 (global time stepping)
+  DevBuf<double> shops_, hK_, Nxo_, Nyo_, vooL_;
+  DevBuf<int> evert_, vptr_, vinc_;
+  DevBuf<dev::ad_storage> sig_, eps_, sigv_, epsv_, dis_, SPV_, sigbR_, epsfR_;
+  DevBuf<int> argsR_, argeR_, argmaxv_, vbr_, part_i2_;
+  DevBuf<double> RpreR_, U2pre_, Wvv_, part_v2_;
+  DevBuf<double> lamSigb_, lamEpsf_, lamDis_, lamEQv_, lamEQg_, lamSigv_, lamEpsv_;
+  int NR_ = 0;
 };
 
 }  // namespace capi

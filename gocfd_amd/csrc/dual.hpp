@@ -74,6 +74,9 @@ __device__ __forceinline__ Dual exp(Dual a) {
   const double e = ::exp(a.v);
   return Dual(e, e * a.d);
 }
+__device__ __forceinline__ Dual log10(Dual a) { return Dual(::log10(a.v), a.d / (a.v * 2.302585092994045684)); }
+__device__ __forceinline__ Dual sin(Dual a) { return Dual(::sin(a.v), ::cos(a.v) * a.d); }
+__device__ __forceinline__ Dual cos(Dual a) { return Dual(::cos(a.v), -::sin(a.v) * a.d); }
 // x^p for a constant exponent: p x^(p-1) dx, written p (x^p / x) dx
 __device__ __forceinline__ Dual pow(Dual a, double p) {
   const double r = ::pow(a.v, p);

@@ -147,8 +147,9 @@ struct gocfd_euler2d {
   }
 
   // The tangent-linear / adjoint driver (FP64, global or local time
-  // stepping; PerssonC0 while its sensor is quiet), on the solver's
-  // geometry and stream
+  // stepping; PerssonC0 with global time stepping, or with local time
+  // stepping while its sensor is quiet), on the solver's geometry and
+  // stream
   capi::AdjointRunner &adjoint() {
     if (prec != 0) throw SetupError(GOCFD_ERR_ARG, "the tangent-linear and adjoint steps differentiate the FP64 step");
     if (shock && lts) {

@@ -186,14 +186,19 @@ int gocfd_euler2d_get_element_dt(gocfd_euler2d *h, double *DT);
  *      far-field state FSFar.Qinf [4] (rho, rhoU, rhoV, E; FSFar's pressure
  *      and sound speed follow it); the In/Out states follow it too except in
  *      the shock tube. Needs an FP64 solver without IVortex boundary and
- *      steps that stay short of FinalTime; with the PerssonC0 limiter, a
- *      sensor that stays below S0 - Kappa in every element (its modal
- *      filters are then the limiter's only action) and, with
- *      LocalTimeStepping, an element viscous dt not yet set
- *      (GOCFD_ERR_ARG otherwise). The dt of a stage (of an element, with LocalTimeStepping)
- *      follows the face point of its maximum wave speed (ties: the first in
- *      the device order; with LocalTimeStepping the previous stage's
- *      element dt when that is larger, as in CalculateLocalDT). ---- */
+ *      steps that stay short of FinalTime; with the PerssonC0 limiter and
+ *      global time stepping the whole shock-capturing path is
+ *      differentiated (sensor, sigma ramp, element viscosity, vertex
+ *      max-merges, sigma-dependent modal filters, eps grad Q, the viscous
+ *      edge flux and the viscous dt; a vertex maximum follows the first
+ *      element that attains it); with LocalTimeStepping a sensor that
+ *      stays below S0 - Kappa in every element and an element viscous dt
+ *      not yet set (GOCFD_ERR_ARG otherwise). The dt of a stage (of an
+ *      element, with LocalTimeStepping) follows the face point of its
+ *      maximum wave speed, or of its maximum viscous speed where that dt is
+ *      smaller (ties: the first in the device order; with
+ *      LocalTimeStepping the previous stage's element dt when that is
+ *      larger, as in CalculateLocalDT). ---- */
 /* Vector-Jacobian product over nsteps steps: dQ0 = (dQn/dQ)^T dQn_bar and
  * dQinf = (dQn/dQinf)^T dQn_bar (dQinf may be NULL). The reverse sweep
  * recomputes the steps from a checkpoint every checkpoint_every steps (0:

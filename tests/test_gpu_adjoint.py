@@ -20,8 +20,9 @@
   primal, dot product and lift gradients against the solver.
 * PerssonC0 with local time stepping and a quiet sensor: primal, dot
   product and the Mach gradient.
-* Refusals: FP32, PerssonC0 with an active sensor, IVortex boundaries, a
-  window that reaches FinalTime.
+* Refusals: FP32, PerssonC0 with an active sensor and local time stepping,
+  IVortex boundaries, a window that reaches FinalTime. (An active sensor
+  with global time stepping: tests/test_gpu_adjoint_shock.py.)
 """
 import numpy as np
 import pytest
@@ -289,8 +290,10 @@ def test_refusals(meshes):
             e.vjp(2, z)
         with pytest.raises(GocfdError, match=msg):
             e.jvp(2, z)
-    # PerssonC0 with Kappa 10: the sensor flags elements in the window
-    e = Euler(_ip(2, Limiter="PerssonC0", Kappa=10.0), meshes["foil"])
+    # PerssonC0 (Kappa 10) with local time stepping: the sensor flags
+    # elements in the window (with global time stepping that case is
+    # differentiated, tests/test_gpu_adjoint_shock.py)
+    e = Euler(_ip(2, Limiter="PerssonC0", Kappa=10.0, LocalTimeStepping=True), meshes["foil"])
     with pytest.raises(GocfdError, match="sensor is active"):
         e.vjp(25, np.zeros_like(e.Q))
     e = Euler(_ip(2, init="IVortex"), meshes["foil"])  # its wall edges take the vortex state
