@@ -19,6 +19,7 @@ LIB = os.path.join(HERE, "libgocfd_euler2d.so")
 
 SOURCES = ["dfr_setup.cpp", "capi_common.cpp", "euler2d_capi.cpp", "ensemble_capi.cpp", "adjoint_runner.cpp",
            "kernels_common.hip", "adjoint_common.hip", "noalloc.hip"] + [f"stage_n{n}.hip" for n in range(5)] + [
+           f"stage_multi_f{f}.hip" for f in range(4)] + [
            f"adjoint_n{n}.hip" for n in range(5)]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")

@@ -6,13 +6,16 @@
 // time-stepping mode form a group whose device layout is the concatenation
 // of the cases' layouts (every case starts on a tile boundary, so a tile
 // belongs to one case); each stage is one launch per group (after the three
-// shock-capturing passes in a PerssonC0 group). Every
+// shock-capturing passes in a PerssonC0 group), or one launch for all the
+// groups without shock capturing (a multi-order kernel: each workgroup runs
+// its tile's order) plus one per PerssonC0 group. Every
 // case keeps its own control block (time, dt, wave-speed maxima) and
 // physics constants, selected per tile. A run of 6 steps -- the period of
 // the per-stage ring and time-parity arguments -- or a multiple of it is
 // captured once as a HIP graph and replayed. The launch configuration (the
-// groups serialized on one stream or as parallel graph branches, the steps
-// per graph, the XCD tile order) can be picked by timing the candidates
+// groups serialized on one stream, as parallel graph branches or in one
+// multi-order launch, the steps per graph, the XCD tile order) can be picked
+// by timing the candidates
 // (gocfd_ensemble_tune); it does not change the results.
 #include <hip/hip_runtime.h>
 
@@ -131,7 +134,11 @@ struct gocfd_ensemble {
   double last_ms = 0;
   int last_steps = 0;
   // launch configuration
-  int parallel = 0;              // groups as parallel graph branches
+  int parallel = 0;              // 0 groups in sequence, 1 parallel graph branches, 2 one multi-order launch
+  // the groups without shock capturing as one multi-order launch per stage
+  DevBuf<dev::StageArgs> margs;  // their arguments (device)
+  std::vector<int> mgroups;      // their indices
+  bool multi_ok = false;
   int graph_steps = kGraphPeriod;
   int xcd_tiles = 1;
 
@@ -218,8 +225,24 @@ struct gocfd_ensemble {
   // One SSP54 step of every group; with `branches` each group's stages go
   // to the group's own stream (inside a graph capture: parallel branches).
   void enqueue_step(bool branches) {
+    const bool multi = parallel == 2 && multi_ok;
     for (int s = 0; s < 5; ++s) {
+      if (multi) {
+        dev::MultiArgs m{};
+        m.ngroups = int(mgroups.size());
+        m.xcd_tiles = xcd_tiles;
+        m.l = dev::MultiLaunch{s, step_index, int(gstage % 3), int(gstage & 1), int(gstage & 1)};
+        int t0 = 0;
+        for (size_t i = 0; i < mgroups.size(); ++i) {
+          const EnsGroup &g = *groups[size_t(mgroups[i])];
+          m.g[i] = dev::MultiGroup{margs.p + i, t0, g.N, g.Fmax, g.Wmax + g.Xmax, g.lts ? 1 : 0};
+          t0 += g.ntiles;
+        }
+        m.ntiles = t0;
+        HIPCHECK(dev::launch_stage_multi(prec, flux, m, stream));
+      }
       for (auto &g : groups) {
+        if (multi && !g->shock) continue;
         dev::StageArgs a = args(*g);
         a.stage = s;
         a.step_index = step_index;
@@ -273,7 +296,7 @@ struct gocfd_ensemble {
     {
       HIPCHECK(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
       CaptureGuard guard{stream, &gstage, &step_index, g0, s0};
-      const bool br = parallel && groups.size() > 1;
+      const bool br = parallel == 1 && groups.size() > 1;
       if (br) {
         HIPCHECK(hipEventRecord(fork, stream));
         for (auto &gr : groups) HIPCHECK(hipStreamWaitEvent(gr->stream, fork, 0));
@@ -316,9 +339,14 @@ struct gocfd_ensemble {
   int set_launch(int par, int gsteps, int xcd) {
     if (gsteps < 0 || gsteps % kGraphPeriod != 0 || gsteps > 600)
       return fail(GOCFD_ERR_ARG, "graph_steps must be 0 (eager launches) or a multiple of 6 (at most 600)");
+    if (par < 0 || par > 2) return fail(GOCFD_ERR_ARG, "parallel must be 0, 1 or 2");
+    if (par == 2 && !multi_ok)
+      return fail(GOCFD_ERR_ARG, "no multi-order launch for this ensemble (FP32, no group without shock capturing, "
+                                 "more than " + std::to_string(dev::kMultiMax) +
+                                     " such groups, or a work list longer than a workgroup)");
     HIPCHECK(hipStreamSynchronize(stream));
     drop_graph();
-    parallel = par ? 1 : 0;
+    parallel = par;
     graph_steps = gsteps;
     xcd_tiles = xcd ? 1 : 0;
     return GOCFD_OK;
@@ -332,11 +360,15 @@ struct gocfd_ensemble {
     };
     std::vector<Cfg> cand;
     if (profiler_attached()) {  // eager launches only (see profiler_attached)
-      for (int xcd : {1, 0}) cand.push_back({0, 0, xcd});
+      for (int par : {0, 2})
+        if (par == 0 || multi_ok)
+          for (int xcd : {1, 0}) cand.push_back({par, 0, xcd});
     } else {
-      for (int par = 0; par <= (groups.size() > 1 ? 1 : 0); ++par)
+      for (int par = 0; par <= 2; ++par) {
+        if ((par == 1 && groups.size() < 2) || (par == 2 && !multi_ok)) continue;
         for (int gs : {kGraphPeriod, 4 * kGraphPeriod})
           for (int xcd : {1, 0}) cand.push_back({par, gs, xcd});
+      }
     }
     double best = 1e300;
     Cfg pick = cand[0];
@@ -471,6 +503,7 @@ void build_group(gocfd_ensemble &h, EnsGroup &g) {
     for (int j = 0; j < c0.d.ops.NpFlux; ++j)
       divT[size_t(j) * NpInt + i] = c0.d.ops.DivInt[size_t(i) * c0.d.ops.NpFlux + j];
   HIPCHECK(dev::upload_operators(N, divT.data(), c0.d.ops.Interp.data()));
+  if (!g.shock) HIPCHECK(dev::upload_operators_multi(h.flux, N, divT.data(), c0.d.ops.Interp.data()));
   const size_t nval = Kp * g.NV;
   HIPCHECK(hipMalloc(&g.q[0], nval * sizeof(double)));
   HIPCHECK(hipMemcpy(g.q[0], q0.data(), nval * sizeof(double), hipMemcpyHostToDevice));
@@ -607,6 +640,22 @@ int gocfd_ensemble_create(int32_t ncases, const gocfd_mesh *const *meshes, const
   }
   h->cinfo.upload(ci);
   HIPCHECK(hipMalloc(&h->ctl, sizeof(dev::Ctl) * ncases));
+  {
+    // the groups without shock capturing, for the multi-order launch
+    std::vector<dev::StageArgs> ma;
+    bool ok = true;
+    for (size_t i = 0; i < h->groups.size(); ++i) {
+      const EnsGroup &g = *h->groups[i];
+      if (g.shock) continue;
+      ok = ok && g.Fmax <= dev::kThreads && g.Wmax <= dev::kThreads && g.Xmax <= dev::kThreads;
+      h->mgroups.push_back(int(i));
+      ma.push_back(h->args(g));
+    }
+    // FP64 only: the FP32 multi-order kernel faulted the GPU in its first
+    // run and stays off until that is explained
+    h->multi_ok = ok && h->prec == 0 && !ma.empty() && ma.size() <= size_t(dev::kMultiMax);
+    if (h->multi_ok) h->margs.upload(ma);
+  }
   h->restart();
   HIPCHECK(hipStreamSynchronize(h->stream));
   *out = h.release();

@@ -158,12 +158,38 @@ hipError_t launch_fill_f64(double *p, size_t n, double v, hipStream_t s);
 hipError_t upload_operators(int N, const double *divT, const double *interp);
 size_t stage_lds_bytes(int N, int prec, int Hmax);
 
+// All non-shock order groups of an ensemble in one stage launch: tiles are
+// numbered group after group; each workgroup finds its group and runs that
+// group's order. The group arguments live in device memory; the launch
+// values (stage, step, rings) are the same for every group.
+constexpr int kMultiMax = 10;
+struct MultiGroup {
+  const StageArgs *a;   // the group's arguments (device memory)
+  int tile0;            // its first tile in the launch's numbering
+  int N;
+  int Fmax, WXmax, lts; // its LDS layout (host side)
+};
+struct MultiLaunch {
+  int stage, step_index, ring, par2, hpar;
+};
+struct MultiArgs {
+  int ngroups, ntiles, xcd_tiles;
+  MultiLaunch l;
+  MultiGroup g[kMultiMax];
+};
+hipError_t launch_stage_multi(int prec, int flux, const MultiArgs &m, hipStream_t s);
+// The order-N operators in the tables of the multi-order kernels of a flux
+hipError_t upload_operators_multi(int flux, int N, const double *divT, const double *interp);
+
 // Per-order entry points (one translation unit per order)
 template <int N> hipError_t launch_stage_n(int prec, int shock, const StageArgs &a, hipStream_t s);
 template <int N> hipError_t launch_prologue_n(int prec, const StageArgs &a, hipStream_t s);
 template <int N> hipError_t launch_halo_export_n(int prec, const StageArgs &a, hipStream_t s);
 template <int N> hipError_t launch_shock_n(int prec, int pass, const StageArgs &a, hipStream_t s);
 template <int N> hipError_t upload_operators_n(const double *divT, const double *interp);
+// Per-flux entry points of the multi-order kernels (one translation unit per flux)
+template <int FLUX> hipError_t launch_stage_multi_f(int prec, const MultiArgs &m, hipStream_t s);
+template <int FLUX> hipError_t upload_operators_multi_f(int N, const double *divT, const double *interp);
 
 }  // namespace dev
 }  // namespace gocfd

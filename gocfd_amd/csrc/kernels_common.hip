@@ -42,6 +42,26 @@ hipError_t launch_shock(int N, int prec, int pass, const StageArgs &a, hipStream
 #undef CALL
 }
 
+hipError_t launch_stage_multi(int prec, int flux, const MultiArgs &m, hipStream_t s) {
+  switch (flux) {
+    case 0: return launch_stage_multi_f<0>(prec, m, s);
+    case 1: return launch_stage_multi_f<1>(prec, m, s);
+    case 2: return launch_stage_multi_f<2>(prec, m, s);
+    case 3: return launch_stage_multi_f<3>(prec, m, s);
+  }
+  return hipErrorInvalidValue;
+}
+
+hipError_t upload_operators_multi(int flux, int N, const double *divT, const double *interp) {
+  switch (flux) {
+    case 0: return upload_operators_multi_f<0>(N, divT, interp);
+    case 1: return upload_operators_multi_f<1>(N, divT, interp);
+    case 2: return upload_operators_multi_f<2>(N, divT, interp);
+    case 3: return upload_operators_multi_f<3>(N, divT, interp);
+  }
+  return hipErrorInvalidValue;
+}
+
 hipError_t upload_operators(int N, const double *divT, const double *interp) {
 #define CALL(n) upload_operators_n<n>(divT, interp)
   GOCFD_DISPATCH_ORDER(N, CALL)

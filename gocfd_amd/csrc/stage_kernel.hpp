@@ -113,34 +113,44 @@ struct Mma<double> {
 // operand read (row pitch 65 words) on even and odd LDS banks.
 __device__ __forceinline__ int mfma_elem(int b, int col) { return 2 * col + (b & 1) + 32 * (b >> 1); }
 
-// The operators of this translation unit's order in constant memory, so that
-// every coefficient is a uniform scalar load: DivInt^T [NpFlux][NpInt]
-// followed by FluxEdgeInterp [3 NpEdge][NpInt], in FP64 and FP32.
+// The operators of order N in constant memory (each translation unit holds
+// the orders it instantiates), so that every coefficient is a uniform scalar
+// load: DivInt^T [NpFlux][NpInt] followed by FluxEdgeInterp [3 NpEdge][NpInt],
+// in FP64 and FP32.
 constexpr int kOpsMax = ((4 + 2) * (4 + 4) + 3 * (4 + 2)) * 15;
+template <int N>
 static __constant__ double cOpsD[kOpsMax];
+template <int N>
 static __constant__ float cOpsF[kOpsMax];
 // DivInt as the A operand of the MFMA contraction: K-step s, lane l holds
 // DivInt[row l%16][DOF of K-row 4s + l/16] (zero outside the matrix)
 constexpr int kMfMax = 13 * 64;
+template <int N>
 static __constant__ float cMfA[kMfMax];
+template <int N>
 static __constant__ double cMfAD[kMfMax];
 // FluxEdgeInterp as the A operand: row block rb, K-step s, lane l holds
 // Interp[16 rb + l%16][4 s + l/16]
 constexpr int kMfIMax = 8 * 64;
+template <int N>
 static __constant__ float cMfI[kMfIMax];
+template <int N>
 static __constant__ double cMfID[kMfIMax];
-template <class C>
-__device__ __forceinline__ const C *mfa_div();
-template <>
-__device__ __forceinline__ const float *mfa_div<float>() { return cMfA; }
-template <>
-__device__ __forceinline__ const double *mfa_div<double>() { return cMfAD; }
-template <class C>
-__device__ __forceinline__ const C *mfa_interp();
-template <>
-__device__ __forceinline__ const float *mfa_interp<float>() { return cMfI; }
-template <>
-__device__ __forceinline__ const double *mfa_interp<double>() { return cMfID; }
+// The order-N tables in arithmetic type C
+template <int N, class C>
+struct Tabs;
+template <int N>
+struct Tabs<N, float> {
+  static __device__ __forceinline__ const float *ops() { return cOpsF<N>; }
+  static __device__ __forceinline__ const float *div() { return cMfA<N>; }
+  static __device__ __forceinline__ const float *interp() { return cMfI<N>; }
+};
+template <int N>
+struct Tabs<N, double> {
+  static __device__ __forceinline__ const double *ops() { return cOpsD<N>; }
+  static __device__ __forceinline__ const double *div() { return cMfAD<N>; }
+  static __device__ __forceinline__ const double *interp() { return cMfID<N>; }
+};
 
 // Face traces sT[var][p][e] = sum_i FluxEdgeInterp[p][i] sQ[var][i][e] of
 // variable tn (= the wave) for the tile's 64 elements on the matrix cores:
@@ -152,7 +162,7 @@ __device__ __forceinline__ void traces_mfma(const C *sQ, C *sT, int tn, int te, 
   using M = Mma<C>;
   constexpr int NpInt = Sh::NpInt, NF = Sh::NF, RB = Sh::RB, KI = Sh::KI, E = kTile, P = kTile + 1;
   const int col = te & 15, krow = te >> 4;
-  const C *tab = mfa_interp<C>();
+  const C *tab = Tabs<N, C>::interp();
   C ai[RB][KI];
 #pragma unroll
   for (int rb = 0; rb < RB; ++rb)
@@ -180,12 +190,6 @@ __device__ __forceinline__ void traces_mfma(const C *sQ, C *sT, int tn, int te, 
       }
   }
 }
-template <class C>
-__device__ __forceinline__ const C *ops_base();
-template <>
-__device__ __forceinline__ const double *ops_base<double>() { return cOpsD; }
-template <>
-__device__ __forceinline__ const float *ops_base<float>() { return cOpsF; }
 
 // Tile index of a workgroup: with workgroups dealt round-robin to the 8 XCDs,
 // XCD x processes the contiguous tile range [x*q, (x+1)*q), so neighbouring
@@ -271,6 +275,21 @@ __device__ __forceinline__ bool step_skipped(const StageArgs &a, const Ctl *ctl,
   return a.stage >= 0 && a.step_index > 0 && ctl->time[a.step_index & 1] >= final_time;
 }
 
+// The values of a stage launch that change from launch to launch (the rest
+// of StageArgs describes the solver or the ensemble group)
+struct StageLaunch {
+  int stage, step_index, ring, par2, hpar;
+  const void *qs;
+  double t_rhs;
+  double *rhs_out;
+};
+__device__ __forceinline__ StageLaunch launch_of(const StageArgs &a) {
+  return StageLaunch{a.stage, a.step_index, a.ring, a.par2, a.hpar, a.qs, a.t_rhs, a.rhs_out};
+}
+__device__ __forceinline__ bool step_skipped(const StageLaunch &l, const Ctl *ctl, double final_time) {
+  return l.stage >= 0 && l.step_index > 0 && ctl->time[l.step_index & 1] >= final_time;
+}
+
 // Waves per SIMD the register allocator targets. FP64 tiles need 80 KB of
 // LDS at N=4 (two workgroups per CU), so FP64 gains nothing above 2; the FP32
 // stages 0-3 fit 128 VGPRs without spilling.
@@ -293,8 +312,7 @@ struct StageWaves {
 // input (q0 at stage 0 and for the RHS evaluation, else S). q0 and the
 // stage-3 RHS R, which accumulate the step, are always FP64.
 template <int N, class S, class SIN, class C, int FLUX, bool LAST, bool SHOCK>
-__global__ void __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(StageWaves<S, LAST, SHOCK>::value)))
-k_stage(StageArgs a) {
+__device__ __forceinline__ void stage_tile(const StageArgs &a, const StageLaunch &l, const int tile) {
   using Sh = Shape<N>;
   constexpr int NpInt = Sh::NpInt, NpEdge = Sh::NpEdge, NF = Sh::NF, NV = Sh::NV, NR = Sh::NR;
   constexpr int E = kTile, P = kTile + 1;
@@ -326,12 +344,10 @@ k_stage(StageArgs a) {
   unsigned long long *sWf = reinterpret_cast<unsigned long long *>(sDT + kTile);
   __shared__ double red[kThreads / 64];
 
-  const int tile = tile_of_block(a.xcd_tiles);
-  if (tile >= a.ntiles) return;
   const int tid = threadIdx.x;
   const int k0 = tile * E;
   const int Kp = a.Kp;
-  const SIN *qs = static_cast<const SIN *>(a.qs);
+  const SIN *qs = static_cast<const SIN *>(l.qs);
 
   // The tile's work lists and state do not depend on the control block:
   // their loads go out first, so that their latency overlaps the chain
@@ -394,7 +410,7 @@ k_stage(StageArgs a) {
   // the neighbour traces of the cross-tile faces, under the state loads
   V4 hx[HX ? ITB : 1];
   if constexpr (HX) {
-    const V4 *hin = static_cast<const V4 *>(a.ht) + size_t(a.hpar) * 3 * size_t(Kp) * NpEdge;
+    const V4 *hin = static_cast<const V4 *>(a.ht) + size_t(l.hpar) * 3 * size_t(Kp) * NpEdge;
 #pragma unroll
     for (int j = 0; j < ITB; ++j) {
       const int it = tid + j * kThreads;
@@ -407,19 +423,19 @@ k_stage(StageArgs a) {
     }
   }
 
-  const bool rhs_only = a.stage < 0;
-  const int par = a.step_index & 1;
+  const bool rhs_only = l.stage < 0;
+  const int par = l.step_index & 1;
   const int cs = a.tile_case[tile];
   const CaseInfo &ci = a.cases[cs];
   const Phys &ph = ci.ph;
   const bool first = tile == ci.tile0;  // does the case's control bookkeeping
   Ctl *ctl = a.ctl + cs;
-  const double time = rhs_only ? a.t_rhs : ctl->time[par];
+  const double time = rhs_only ? l.t_rhs : ctl->time[par];
   // this stage's wave-speed maxima, written by the previous launch: read now
   // so that phase E does not wait for them
-  const unsigned long long wmax_bits = rhs_only ? 0ull : ctl->wmax[a.ring];
-  const unsigned long long wmaxv_bits = (rhs_only || !SHOCK) ? 0ull : ctl->wmaxv[a.ring];
-  const C *DvT = ops_base<C>();                         // DivInt^T [NpFlux][NpInt]
+  const unsigned long long wmax_bits = rhs_only ? 0ull : ctl->wmax[l.ring];
+  const unsigned long long wmaxv_bits = (rhs_only || !SHOCK) ? 0ull : ctl->wmaxv[l.ring];
+  const C *DvT = Tabs<N, C>::ops();                     // DivInt^T [NpFlux][NpInt]
   const C *Itp = DvT + Sh::NpFlux * NpInt;              // FluxEdgeInterp [NF][NpInt]
   const C g = C(ph.gamma);
   {
@@ -455,8 +471,8 @@ k_stage(StageArgs a) {
   __syncthreads();
   // the control-block chain (tile -> case -> time) has had the tile loads and
   // the LDS stores to arrive under
-  if (step_skipped(a, ctl, ph.final_time)) {
-    if (a.stage == 4 && first && threadIdx.x == 0) ctl->time[par ^ 1] = time;
+  if (step_skipped(l, ctl, ph.final_time)) {
+    if (l.stage == 4 && first && threadIdx.x == 0) ctl->time[par ^ 1] = time;
     return;
   }
   const int te = tid % E, tn = tid / E;  // (element, variable) of this thread
@@ -642,7 +658,7 @@ k_stage(StageArgs a) {
     const int col = te & 15, krow = te >> 4;
     C af[Sh::KS];
 #pragma unroll
-    for (int s2 = 0; s2 < Sh::KS; ++s2) af[s2] = mfa_div<C>()[s2 * 64 + te];
+    for (int s2 = 0; s2 < Sh::KS; ++s2) af[s2] = Tabs<N, C>::div()[s2 * 64 + te];
     typename M::vec out[4];
 #pragma unroll
     for (int b = 0; b < 4; ++b) {
@@ -739,7 +755,7 @@ k_stage(StageArgs a) {
   __syncthreads();
 
   if (rhs_only) {
-    double *out = a.rhs_out + size_t(k0) * NV;
+    double *out = l.rhs_out + size_t(k0) * NV;
     for (int idx = tid; idx < E * NV; idx += kThreads) {
       const int e = idx / NV, r = idx - e * NV;
       out[idx] = double(sR[r * P + e]);
@@ -750,7 +766,7 @@ k_stage(StageArgs a) {
   constexpr int ITG = (3 * E * NpEdge + kThreads - 1) / kThreads;
 
   // ---- E: SSP54 stage combination (euler.go:511-565), FP64 arithmetic ----
-  const int stage = a.stage;
+  const int stage = l.stage;
   const bool lts = a.lts.on;
   double dt = 0.0;  // GlobalDT; stays 0 with local time stepping (never computed)
   if (!lts) {
@@ -768,13 +784,13 @@ k_stage(StageArgs a) {
       const int kk = k0 + tid;
       double d = 0.0;
       if (kk < ci.k_end) {
-        const double *fw = a.lts.fw + size_t(a.par2) * 3 * Kp;
+        const double *fw = a.lts.fw + size_t(l.par2) * 3 * Kp;
         double w = stage == 0 ? -100.0 : a.lts.DTk[kk];
 #pragma unroll
         for (int f = 0; f < 3; ++f) w = fmax(w, fw[a.lts.wsrc[size_t(f) * Kp + kk]]);
         d = ph.cfl / w;
         if constexpr (SHOCK) {
-          const double *fwv = a.lts.fwv + size_t(a.par2) * 3 * Kp;
+          const double *fwv = a.lts.fwv + size_t(l.par2) * 3 * Kp;
           double dv = a.lts.DTv[kk];
 #pragma unroll
           for (int f = 0; f < 3; ++f) dv = fmax(dv, fwv[a.lts.wsrc[size_t(f) * Kp + kk]]);
@@ -936,7 +952,7 @@ k_stage(StageArgs a) {
         if (lts) atomicMax(&sWf[e * 3 + f], bits(wsp));
       }
     }
-    V4 *hout = static_cast<V4 *>(a.ht) + size_t(a.hpar ^ 1) * 3 * size_t(Kp) * NpEdge;
+    V4 *hout = static_cast<V4 *>(a.ht) + size_t(l.hpar ^ 1) * 3 * size_t(Kp) * NpEdge;
 #pragma unroll
     for (int j = 0; j < ITG; ++j) {
       const int it = tid + j * kThreads;
@@ -954,7 +970,7 @@ k_stage(StageArgs a) {
   bad = __syncthreads_or(bad);
   if constexpr (!SHOCK) {
     if (lts) {
-      double *fw = a.lts.fw + size_t(a.par2 ^ 1) * 3 * Kp;
+      double *fw = a.lts.fw + size_t(l.par2 ^ 1) * 3 * Kp;
       const int *wlist = a.g.wave_list + size_t(tile) * a.Wmax;
       for (int w = tid; w < a.g.wave_cnt[tile]; w += kThreads) {
         const int packed = wlist[w];
@@ -967,12 +983,12 @@ k_stage(StageArgs a) {
     if constexpr (!SHOCK) {
       double m = red[0];
       for (int w = 1; w < kThreads / 64; ++w) m = fmax(m, red[w]);
-      atomic_max_bits(&ctl->wmax[(a.ring + 1) % 3], m);
+      atomic_max_bits(&ctl->wmax[(l.ring + 1) % 3], m);
     }
     if (bad) atomicOr(&ctl->nan, 1);
     if (first) {
-      ctl->wmax[(a.ring + 2) % 3] = 0ull;
-      ctl->wmaxv[(a.ring + 2) % 3] = 0ull;
+      ctl->wmax[(l.ring + 2) % 3] = 0ull;
+      ctl->wmaxv[(l.ring + 2) % 3] = 0ull;
       ctl->dt[stage] = dt;
       if (LAST) {
         ctl->time[par ^ 1] = tnext;
@@ -980,6 +996,14 @@ k_stage(StageArgs a) {
       }
     }
   }
+}
+
+template <int N, class S, class SIN, class C, int FLUX, bool LAST, bool SHOCK>
+__global__ void __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(StageWaves<S, LAST, SHOCK>::value)))
+k_stage(StageArgs a) {
+  const int tile = tile_of_block(a.xcd_tiles);
+  if (tile >= a.ntiles) return;
+  stage_tile<N, S, SIN, C, FLUX, LAST, SHOCK>(a, launch_of(a), tile);
 }
 
 // Wave speeds of the current state (start of a step sequence): one thread per
@@ -1048,7 +1072,7 @@ __global__ void __launch_bounds__(256) k_halo_export(StageArgs a) {
   C q[NpInt];
 #pragma unroll
   for (int i = 0; i < NpInt; ++i) q[i] = C(qn[i]);
-  const C *Itp = ops_base<C>() + Sh::NpFlux * NpInt;
+  const C *Itp = Tabs<N, C>::ops() + Sh::NpFlux * NpInt;
   C *hout = static_cast<C *>(a.ht) + size_t(a.hpar) * 3 * size_t(a.Kp) * NpEdge * 4;
 #pragma unroll
   for (int pp = 0; pp < NpEdge; ++pp) {
@@ -1232,7 +1256,7 @@ __global__ void __launch_bounds__(kThreads) k_shock_grad(StageArgs a) {
   if (a.stage < 0 || step_skipped(a, ctl, ci.ph.final_time)) return;
   const int te = tid % E, tn = tid / E, k0 = tile * E, Kp = a.Kp;
   const SIN *Q = static_cast<const SIN *>(a.qs);
-  const D *Itp = ops_base<D>() + NpFlux * NpInt;
+  const D *Itp = Tabs<N, D>::ops() + NpFlux * NpInt;
   {
     constexpr int LPT = (E * NV + kThreads - 1) / kThreads;
     const SIN *qt = Q + size_t(k0) * NV;
@@ -1341,7 +1365,7 @@ __global__ void __launch_bounds__(256) k_shock_grad_act(StageArgs a) {
   const size_t n = size_t(*a.sh.act_cnt) * 8;
   const int Kp = a.Kp;
   const SIN *Q = static_cast<const SIN *>(a.qs);
-  const D *Itp = ops_base<D>() + NpFlux * NpInt;
+  const D *Itp = Tabs<N, D>::ops() + NpFlux * NpInt;
   const D *Div = shock_ops<D>(a.sh) + Sh::oDiv;
   for (size_t it = size_t(blockIdx.x) * blockDim.x + threadIdx.x; it < n; it += size_t(gridDim.x) * blockDim.x) {
     const int kk = a.sh.act_list[it >> 3], tn = int(it >> 1) & 3, dir = int(it) & 1;
@@ -1518,8 +1542,10 @@ hipError_t launch_shock_n(int prec, int pass, const StageArgs &a, hipStream_t s)
   return launch_shock_t<N, float, BulkC<N>>(pass, a, s);
 }
 
+// The order-N tables of the including translation unit (internal linkage:
+// every unit that instantiates a stage body uploads its own copies)
 template <int N>
-hipError_t upload_operators_n(const double *divT, const double *interp) {
+static hipError_t upload_tables(const double *divT, const double *interp) {
   using Sh = Shape<N>;
   static_assert(Sh::KS * 64 <= kMfMax && Sh::RB * Sh::KI * 64 <= kMfIMax, "MFMA operand tables");
   {
@@ -1532,10 +1558,10 @@ hipError_t upload_operators_n(const double *divT, const double *interp) {
           if (p < Sh::NF && i < Sh::NpInt) ai[(size_t(rb) * Sh::KI + s) * 64 + l] = interp[size_t(p) * Sh::NpInt + i];
         }
     const std::vector<float> aif(ai.begin(), ai.end());
-    hipError_t e = hipMemcpyToSymbol(HIP_SYMBOL(cMfI), aif.data(), aif.size() * sizeof(float), 0,
+    hipError_t e = hipMemcpyToSymbol(HIP_SYMBOL(cMfI<N>), aif.data(), aif.size() * sizeof(float), 0,
                                      hipMemcpyHostToDevice);
     if (e != hipSuccess) return e;
-    e = hipMemcpyToSymbol(HIP_SYMBOL(cMfID), ai.data(), ai.size() * sizeof(double), 0, hipMemcpyHostToDevice);
+    e = hipMemcpyToSymbol(HIP_SYMBOL(cMfID<N>), ai.data(), ai.size() * sizeof(double), 0, hipMemcpyHostToDevice);
     if (e != hipSuccess) return e;
   }
   {
@@ -1555,20 +1581,25 @@ hipError_t upload_operators_n(const double *divT, const double *interp) {
         if (row < Sh::NpInt && j >= 0) af[size_t(s) * 64 + l] = divT[size_t(j) * Sh::NpInt + row];
       }
     const std::vector<float> aff(af.begin(), af.end());
-    hipError_t e = hipMemcpyToSymbol(HIP_SYMBOL(cMfA), aff.data(), aff.size() * sizeof(float), 0,
+    hipError_t e = hipMemcpyToSymbol(HIP_SYMBOL(cMfA<N>), aff.data(), aff.size() * sizeof(float), 0,
                                      hipMemcpyHostToDevice);
     if (e != hipSuccess) return e;
-    e = hipMemcpyToSymbol(HIP_SYMBOL(cMfAD), af.data(), af.size() * sizeof(double), 0, hipMemcpyHostToDevice);
+    e = hipMemcpyToSymbol(HIP_SYMBOL(cMfAD<N>), af.data(), af.size() * sizeof(double), 0, hipMemcpyHostToDevice);
     if (e != hipSuccess) return e;
   }
   std::vector<double> d(static_cast<size_t>(Sh::ops));
   std::copy(divT, divT + size_t(Sh::NpFlux) * Sh::NpInt, d.begin());
   std::copy(interp, interp + size_t(Sh::NF) * Sh::NpInt, d.begin() + size_t(Sh::NpFlux) * Sh::NpInt);
   std::vector<float> f(d.begin(), d.end());
-  hipError_t e = hipMemcpyToSymbol(HIP_SYMBOL(cOpsD), d.data(), d.size() * sizeof(double), 0,
+  hipError_t e = hipMemcpyToSymbol(HIP_SYMBOL(cOpsD<N>), d.data(), d.size() * sizeof(double), 0,
                                    hipMemcpyHostToDevice);
   if (e != hipSuccess) return e;
-  return hipMemcpyToSymbol(HIP_SYMBOL(cOpsF), f.data(), f.size() * sizeof(float), 0, hipMemcpyHostToDevice);
+  return hipMemcpyToSymbol(HIP_SYMBOL(cOpsF<N>), f.data(), f.size() * sizeof(float), 0, hipMemcpyHostToDevice);
+}
+
+template <int N>
+hipError_t upload_operators_n(const double *divT, const double *interp) {
+  return upload_tables<N>(divT, interp);
 }
 
 }  // namespace dev
@@ -1585,3 +1616,97 @@ hipError_t upload_operators_n(const double *divT, const double *interp) {
   template hipError_t upload_operators_n<NN>(const double *, const double *);                           \
   }                                                                                                      \
   }
+
+// ---- All non-shock order groups of an ensemble in one launch per stage ----
+// (a translation unit per flux includes this header with
+// GOCFD_MULTI_FLUX defined and instantiates its entry points)
+#ifdef GOCFD_MULTI_FLUX
+namespace gocfd {
+namespace dev {
+
+template <int N, class S>
+using MultiC = typename std::conditional<(sizeof(S) == 8 || N <= 1), double, float>::type;
+
+// KIND 0: stage 0 (input q0), 1: stages 1-3, 2: stage 4
+template <class S, int FLUX, int KIND>
+__global__ void __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(StageWaves<S, KIND == 2, false>::value)))
+k_stage_multi(MultiArgs m) {
+  const int t = tile_of_block(m.xcd_tiles);
+  if (t >= m.ntiles) return;
+  int g = 0;
+  while (g + 1 < m.ngroups && t >= m.g[g + 1].tile0) ++g;
+  const StageArgs &a = *m.g[g].a;
+  const int st = m.l.stage;
+  const StageLaunch l{st, m.l.step_index, m.l.ring, m.l.par2, m.l.hpar,
+                      st == 0 ? a.q0 : st == 1 ? a.q1 : st == 2 ? a.q2 : st == 3 ? a.q3 : a.q4, 0., nullptr};
+  const int tile = t - m.g[g].tile0;
+  using SIN = typename std::conditional<KIND == 0, double, S>::type;
+  constexpr bool LAST = KIND == 2;
+  switch (m.g[g].N) {
+    case 0: stage_tile<0, S, SIN, MultiC<0, S>, FLUX, LAST, false>(a, l, tile); break;
+    case 1: stage_tile<1, S, SIN, MultiC<1, S>, FLUX, LAST, false>(a, l, tile); break;
+    case 2: stage_tile<2, S, SIN, MultiC<2, S>, FLUX, LAST, false>(a, l, tile); break;
+    case 3: stage_tile<3, S, SIN, MultiC<3, S>, FLUX, LAST, false>(a, l, tile); break;
+    default: stage_tile<4, S, SIN, MultiC<4, S>, FLUX, LAST, false>(a, l, tile); break;
+  }
+}
+
+template <class S, int FLUX, int KIND>
+static hipError_t launch_multi_k(const MultiArgs &m, hipStream_t s) {
+  size_t lds = 0;
+  for (int g = 0; g < m.ngroups; ++g) {
+    const MultiGroup &G = m.g[g];
+    const size_t cs = sizeof(S) == 8 || G.N <= 1 ? sizeof(double) : sizeof(float);
+    size_t b = 0;
+    switch (G.N) {
+      case 0: b = lds_bytes_n<0>(cs, 0, G.Fmax, G.WXmax, G.lts != 0); break;
+      case 1: b = lds_bytes_n<1>(cs, 0, G.Fmax, G.WXmax, G.lts != 0); break;
+      case 2: b = lds_bytes_n<2>(cs, 0, G.Fmax, G.WXmax, G.lts != 0); break;
+      case 3: b = lds_bytes_n<3>(cs, 0, G.Fmax, G.WXmax, G.lts != 0); break;
+      default: b = lds_bytes_n<4>(cs, 0, G.Fmax, G.WXmax, G.lts != 0); break;
+    }
+    lds = std::max(lds, b);
+  }
+  static size_t attr_bytes = 0;  // dynamic LDS the kernel is currently allowed
+  if (lds > 64 * 1024 && lds > attr_bytes) {
+    hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void *>(&k_stage_multi<S, FLUX, KIND>),
+                                       hipFuncAttributeMaxDynamicSharedMemorySize, int(lds));
+    if (e != hipSuccess) return e;
+    attr_bytes = lds;
+  }
+  const int q = (m.ntiles + 7) / 8;
+  hipLaunchKernelGGL((k_stage_multi<S, FLUX, KIND>), dim3(8 * q), dim3(kThreads), lds, s, m);
+  return hipGetLastError();
+}
+
+template <>
+hipError_t launch_stage_multi_f<GOCFD_MULTI_FLUX>(int prec, const MultiArgs &m, hipStream_t s) {
+  // FP64 only: the FP32 variant faulted the GPU in its first run and stays
+  // off until that is explained
+  if (prec != 0) return hipErrorNotSupported;
+  for (int g = 0; g < m.ngroups; ++g)
+    if (m.g[g].Fmax > kThreads || m.g[g].WXmax > 2 * kThreads) return hipErrorInvalidValue;
+  const int kind = m.l.stage == 0 ? 0 : m.l.stage == 4 ? 2 : 1;
+  if (prec == 0) {
+    if (kind == 0) return launch_multi_k<double, GOCFD_MULTI_FLUX, 0>(m, s);
+    if (kind == 2) return launch_multi_k<double, GOCFD_MULTI_FLUX, 2>(m, s);
+    return launch_multi_k<double, GOCFD_MULTI_FLUX, 1>(m, s);
+  }
+  return hipErrorNotSupported;
+}
+
+template <>
+hipError_t upload_operators_multi_f<GOCFD_MULTI_FLUX>(int N, const double *divT, const double *interp) {
+  switch (N) {
+    case 0: return upload_tables<0>(divT, interp);
+    case 1: return upload_tables<1>(divT, interp);
+    case 2: return upload_tables<2>(divT, interp);
+    case 3: return upload_tables<3>(divT, interp);
+    case 4: return upload_tables<4>(divT, interp);
+  }
+  return hipErrorInvalidValue;
+}
+
+}  // namespace dev
+}  // namespace gocfd
+#endif

@@ -61,15 +61,18 @@ class Ensemble:
         _lib.check(_lib.load().gocfd_ensemble_kernel_time(self._h, C.byref(ms), C.byref(n)))
         return ms.value, n.value
 
-    def set_launch(self, parallel: bool, graph_steps: int = 6, xcd_tiles: bool = True) -> None:
-        """Launch configuration: order groups as parallel graph branches, steps
-        per captured graph (a multiple of 6), XCD-contiguous tile order."""
+    def set_launch(self, parallel: int, graph_steps: int = 6, xcd_tiles: bool = True) -> None:
+        """Launch configuration: parallel 0 = order groups one after another,
+        1 (or True) = as parallel graph branches, 2 = the groups without shock
+        capturing in one multi-order launch per stage; steps per captured
+        graph (a multiple of 6, or 0 for eager launches); XCD-contiguous tile
+        order."""
         _lib.check(_lib.load().gocfd_ensemble_set_launch(self._h, int(parallel), int(graph_steps), int(xcd_tiles)))
 
     def launch(self) -> dict:
         p, g, x = C.c_int32(), C.c_int32(), C.c_int32()
         _lib.check(_lib.load().gocfd_ensemble_get_launch(self._h, C.byref(p), C.byref(g), C.byref(x)))
-        return {"parallel": bool(p.value), "graph_steps": g.value, "xcd_tiles": bool(x.value)}
+        return {"parallel": p.value, "graph_steps": g.value, "xcd_tiles": bool(x.value)}
 
     def tune(self, trial_blocks: int = 2) -> dict:
         """Autotune the launch configuration from the initial state (before

@@ -234,16 +234,20 @@ int gocfd_ensemble_time(gocfd_ensemble *h, int32_t c, double *time, double *glob
 /* Wall time in ms of the last gocfd_ensemble_step call measured with HIP
  * events on the ensemble's stream, and its number of steps */
 int gocfd_ensemble_kernel_time(gocfd_ensemble *h, double *ms, int32_t *steps);
-/* Launch configuration (no counterpart in the reference): parallel = the
- * order groups run as parallel branches of the captured graph (else one
- * after another on one stream); graph_steps = steps per captured graph, a
+/* Launch configuration (no counterpart in the reference): parallel = 1:
+ * the order groups run as parallel branches of the captured graph; 2: the
+ * groups without shock capturing run as one launch per stage of a
+ * multi-order kernel (each workgroup runs its tile's order; FP64 ensembles
+ * only; GOCFD_ERR_ARG where the groups do not allow it), the PerssonC0
+ * groups after it; 0: one
+ * group after another on one stream; graph_steps = steps per captured graph, a
  * multiple of 6, or 0 for eager launches without a graph (the default when
  * rocprofv3 is attached: ROCPROF_KERNEL_TRACE or ROCPROF_COUNTER_COLLECTION
  * set); xcd_tiles = each XCD takes a contiguous tile range (else
  * tile = workgroup id). None of them changes the results. */
 int gocfd_ensemble_set_launch(gocfd_ensemble *h, int32_t parallel, int32_t graph_steps, int32_t xcd_tiles);
 int gocfd_ensemble_get_launch(gocfd_ensemble *h, int32_t *parallel, int32_t *graph_steps, int32_t *xcd_tiles);
-/* Autotuner: times each candidate configuration (parallel 0/1, graph_steps
+/* Autotuner: times each candidate configuration (parallel 0/1/2, graph_steps
  * 6/24 -- only 0 under rocprofv3 --, xcd_tiles 1/0) over trial_blocks graph replays from the initial
  * state, keeps the fastest and restores the initial state. Only before the
  * first step. gocfd_ensemble_kernel_time then reports the chosen
