@@ -108,6 +108,67 @@ struct Mma<double> {
   static __device__ __forceinline__ int row(int krow, int v) { return krow + 4 * v; }
 };
 
+// Matrix-core path of the stage kernel: wave w owns the 16 elements 16 w +
+// col of the tile and lane (col, krow) holds the four variables of element
+// 16 w + col at the solution points row(krow, j), j < KI, in registers; these
+// are the rows the lane receives of a 16x16 MFMA result, so the same values
+// feed the B operand of K-step j of the FluxEdgeInterp and DivInt
+// contractions (whose A operands take their columns in that point order)
+// and the SSP combination with the divergence. FP32 results hold rows
+// 4 krow + j, which needs all four K-steps also for NpInt = 10; FP64 results
+// hold rows krow + 4 j, the natural order.
+template <int N, class C>
+struct MfShape {
+  using Sh = Shape<N>;
+  static constexpr int KI = sizeof(C) == 4 ? 4 : Sh::KI;  // K-steps over solution points
+  static constexpr int KS = 2 * KI + Sh::KE;               // DivInt K-steps: r, s components, faces
+  static __host__ __device__ constexpr int point(int krow, int j) {
+    return sizeof(C) == 4 ? 4 * krow + j : krow + 4 * j;
+  }
+};
+
+// The lane's KI values of one variable of its element (matrix-core path);
+// row points at the variable's NpInt values in HBM. An FP32 lane holds four
+// consecutive points 4 krow + j: one vector access of the run starting at
+// min(4 krow, NpInt - 4) (points past NpInt repeat the last one on loads and
+// are not stored). FP64 lanes hold points krow + 4 j, one access each.
+template <class T>
+struct Run4 {
+  T v[4];
+};
+template <int N, class C, class T>
+__device__ __forceinline__ void lane_load(const T *row, int krow, T (&out)[MfShape<N, C>::KI]) {
+  constexpr int NpInt = Shape<N>::NpInt, KI = MfShape<N, C>::KI;
+  if constexpr (sizeof(C) == 4) {
+    static_assert(KI == 4 && NpInt >= 4, "FP32 lanes hold four consecutive points");
+    const int o = min(4 * krow, NpInt - 4);
+    const Run4<T> r = *reinterpret_cast<const Run4<T> *>(row + o);
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const int d = min(4 * krow + j, NpInt - 1) - o;
+      out[j] = d == 0 ? r.v[0] : d == 1 ? r.v[1] : d == 2 ? r.v[2] : r.v[3];
+    }
+  } else {
+#pragma unroll
+    for (int j = 0; j < KI; ++j) out[j] = row[min(MfShape<N, C>::point(krow, j), NpInt - 1)];
+  }
+}
+template <int N, class C, class T>
+__device__ __forceinline__ void lane_store(T *row, int krow, const T (&val)[MfShape<N, C>::KI]) {
+  constexpr int NpInt = Shape<N>::NpInt, KI = MfShape<N, C>::KI;
+  if constexpr (sizeof(C) == 4) {
+    if (4 * krow + 3 < NpInt) {
+      *reinterpret_cast<Run4<T> *>(row + 4 * krow) = Run4<T>{{val[0], val[1], val[2], val[3]}};
+      return;
+    }
+  }
+#pragma unroll
+  for (int j = 0; j < KI; ++j) {
+    const int i = MfShape<N, C>::point(krow, j);
+    if (i < NpInt) row[i] = val[j];
+  }
+}
+
 // Element of column col of MFMA column block b. Pairing the blocks
 // (e = 2 col + (b & 1) + 32 (b >> 1)) puts the two K-rows of a half-wave
 // operand read (row pitch 65 words) on even and odd LDS banks.
@@ -152,41 +213,36 @@ struct Tabs<N, double> {
   static __device__ __forceinline__ const double *interp() { return cMfID<N>; }
 };
 
-// Face traces sT[var][p][e] = sum_i FluxEdgeInterp[p][i] sQ[var][i][e] of
-// variable tn (= the wave) for the tile's 64 elements on the matrix cores:
-// four 16-element column blocks, lane (col, krow) supplies the solution
-// point 4s + krow of element mfma_elem(b, col) as the B operand of K-step s.
+// Face traces of the lane's element (matrix-core path) from the state in
+// registers: for each variable, RB row blocks of 16 face points over KI
+// K-steps; lane (col, krow) stores face points 16 rb + row(krow, j).
 template <int N, class C>
-__device__ __forceinline__ void traces_mfma(const C *sQ, C *sT, int tn, int te, int TS) {
+__device__ __forceinline__ void traces_mfma_regs(const C (&qv)[4][MfShape<N, C>::KI], C *sT, int lane, int krow,
+                                                 int me, int TS) {
   using Sh = Shape<N>;
   using M = Mma<C>;
-  constexpr int NpInt = Sh::NpInt, NF = Sh::NF, RB = Sh::RB, KI = Sh::KI, E = kTile, P = kTile + 1;
-  const int col = te & 15, krow = te >> 4;
+  constexpr int NF = Sh::NF, RB = Sh::RB, KI = MfShape<N, C>::KI, E = kTile;
   const C *tab = Tabs<N, C>::interp();
   C ai[RB][KI];
 #pragma unroll
   for (int rb = 0; rb < RB; ++rb)
 #pragma unroll
-    for (int s = 0; s < KI; ++s) ai[rb][s] = tab[(rb * KI + s) * 64 + te];
+    for (int s = 0; s < KI; ++s) ai[rb][s] = tab[(rb * KI + s) * 64 + lane];
 #pragma unroll
-  for (int b = 0; b < 4; ++b) {
-    const int e = mfma_elem(b, col);
+  for (int v = 0; v < 4; ++v) {
     typename M::vec acc[RB];
 #pragma unroll
     for (int rb = 0; rb < RB; ++rb) acc[rb] = M::zero();
 #pragma unroll
-    for (int s = 0; s < KI; ++s) {
-      // past NpInt the A column is zero: any finite point value will do
-      const C qv = sQ[(tn * NpInt + min(4 * s + krow, NpInt - 1)) * P + e];
+    for (int s = 0; s < KI; ++s)
 #pragma unroll
-      for (int rb = 0; rb < RB; ++rb) acc[rb] = M::mma(ai[rb][s], qv, acc[rb]);
-    }
+      for (int rb = 0; rb < RB; ++rb) acc[rb] = M::mma(ai[rb][s], qv[v][s], acc[rb]);
 #pragma unroll
     for (int rb = 0; rb < RB; ++rb)
 #pragma unroll
-      for (int v = 0; v < 4; ++v) {
-        const int p = 16 * rb + M::row(krow, v);
-        if (p < NF) sT[tn * TS + p * E + e] = acc[rb][v];
+      for (int j = 0; j < 4; ++j) {
+        const int p = 16 * rb + M::row(krow, j);
+        if (p < NF) sT[v * TS + p * E + me] = acc[rb][j];
       }
   }
 }
@@ -251,21 +307,22 @@ __device__ __forceinline__ double point_speed(double g, const double q[4], doubl
   return oohK * (U + c);
 }
 
-// Dynamic LDS of the stage kernel: the tile state [NV][kTile+1] and the
-// trace/flux buffer [4][3 NpEdge kTile + NpEdge Hmax] (rounded to 32 bytes),
-// the Riemann items' geometry [Fmax][4] and the elements' metric terms
-// [6][kTile]; without shock capturing, phase G's lists: the elements' 1/hK
-// [kTile] (FP64), the owned faces with their codes [2][Wmax] and the export
-// items with their destinations [2][Xmax] (ints); local time stepping adds
-// the element dt and face-speed buffers.
+// Dynamic LDS of the stage kernel: the tile state [NV][kTile+1] (not on the
+// matrix-core path, whose state stays in registers) and the trace/flux
+// buffer [4][3 NpEdge kTile + NpEdge Hmax] (rounded to 32 bytes), the Riemann
+// items' geometry [Fmax][4] and the elements' metric terms [6][kTile];
+// without shock capturing, phase G's lists: the elements' 1/hK [kTile]
+// (FP64), the owned faces with their codes [2][Wmax] and the export items
+// with their destinations [2][Xmax] (ints); local time stepping adds the
+// element dt and face-speed buffers.
 template <int N>
-__host__ __device__ constexpr size_t lds_trace_bytes(size_t csize, int Hmax) {
-  return (csize * (size_t(Shape<N>::NV) * (kTile + 1) +
+__host__ __device__ constexpr size_t lds_trace_bytes(size_t csize, int Hmax, bool mf) {
+  return (csize * ((mf ? 0 : size_t(Shape<N>::NV) * (kTile + 1)) +
                    4 * (size_t(Shape<N>::NF) * kTile + size_t(Shape<N>::NpEdge) * Hmax)) + 31) / 32 * 32;
 }
 template <int N>
-__host__ __device__ constexpr size_t lds_tile_bytes(size_t csize, int Hmax, int Fmax, int WXmax) {
-  return lds_trace_bytes<N>(csize, Hmax) + csize * (4 * size_t(Fmax) + 6 * kTile) +
+__host__ __device__ constexpr size_t lds_tile_bytes(size_t csize, int Hmax, int Fmax, int WXmax, bool mf) {
+  return lds_trace_bytes<N>(csize, Hmax, mf) + csize * (4 * size_t(Fmax) + 6 * kTile) +
          (WXmax > 0 ? (sizeof(double) * kTile + 2 * sizeof(int) * size_t(WXmax) + 15) / 16 * 16 : 0);
 }
 constexpr size_t lds_lts_bytes = sizeof(double) * kTile + sizeof(unsigned long long) * 3 * kTile;
@@ -290,9 +347,9 @@ __device__ __forceinline__ bool step_skipped(const StageLaunch &l, const Ctl *ct
   return l.stage >= 0 && l.step_index > 0 && ctl->time[l.step_index & 1] >= final_time;
 }
 
-// Waves per SIMD the register allocator targets. FP64 tiles need 80 KB of
-// LDS at N=4 (two workgroups per CU), so FP64 gains nothing above 2; the FP32
-// stages 0-3 fit 128 VGPRs without spilling.
+// Waves per SIMD the register allocator targets. On the matrix-core path an
+// FP64 tile needs about 45 KB of LDS at N=4 (three workgroups per CU) and
+// fits 168 VGPRs; the FP32 stages fit 128 VGPRs without spilling.
 #ifndef GOCFD_WAVES_F32
 #define GOCFD_WAVES_F32 4
 #endif
@@ -300,7 +357,7 @@ __device__ __forceinline__ bool step_skipped(const StageLaunch &l, const Ctl *ct
 #define GOCFD_WAVES_F32_LAST 3
 #endif
 #ifndef GOCFD_WAVES_F64
-#define GOCFD_WAVES_F64 1
+#define GOCFD_WAVES_F64 3
 #endif
 template <class S, bool LAST, bool SHOCK>
 struct StageWaves {
@@ -324,13 +381,15 @@ __device__ __forceinline__ void stage_tile(const StageArgs &a, const StageLaunch
   // previous stage) each tile interpolates them from the neighbours' state
   // into a halo part of the trace buffer.
   constexpr bool HX = !SHOCK;
+  // Matrix-core path: the state stays in registers (no sQ, no sR)
+  constexpr bool MF = UseMfma<N, C, SHOCK>::value;
   using V4 = typename Vec4<C>::type;
-  C *sQ = reinterpret_cast<C *>(smem_raw);  // [NV][P]
+  C *sQ = reinterpret_cast<C *>(smem_raw);  // [NV][P] (VALU path)
   const int TS = NF * E + (HX ? 0 : NpEdge * a.Hmax);
-  C *sT = sQ + NV * P;                       // [4][TS]: traces, then fluxes
-  C *sR = sT;                                // [NV][P]: RHS (aliases sT)
+  C *sT = MF ? sQ : sQ + NV * P;             // [4][TS]: traces, then fluxes
+  C *sR = sT;                                // [NV][P]: RHS (aliases sT; VALU path)
   // the Riemann items' geometry and the metric terms (Jdet, Jinv[4], -1/Jdet)
-  V4 *sFG = reinterpret_cast<V4 *>(smem_raw + lds_trace_bytes<N>(sizeof(C), HX ? 0 : a.Hmax));  // [Fmax]
+  V4 *sFG = reinterpret_cast<V4 *>(smem_raw + lds_trace_bytes<N>(sizeof(C), HX ? 0 : a.Hmax, MF));  // [Fmax]
   C *sJ = reinterpret_cast<C *>(sFG + a.Fmax);                                                 // [6][E]
   // phase G's lists (without shock capturing)
   double *sOK = reinterpret_cast<double *>(sJ + 6 * E);  // [E] 1/hK
@@ -340,7 +399,7 @@ __device__ __forceinline__ void stage_tile(const StageArgs &a, const StageLaunch
   // local time stepping only: element dt and per-face speed maxima, after
   // the tile buffers (the launch adds their bytes only when LTS is on)
   double *sDT = reinterpret_cast<double *>(
-      smem_raw + lds_tile_bytes<N>(sizeof(C), HX ? 0 : a.Hmax, a.Fmax, HX ? a.Wmax + a.Xmax : 0));
+      smem_raw + lds_tile_bytes<N>(sizeof(C), HX ? 0 : a.Hmax, a.Fmax, HX ? a.Wmax + a.Xmax : 0, MF));
   unsigned long long *sWf = reinterpret_cast<unsigned long long *>(sDT + kTile);
   __shared__ double red[kThreads / 64];
 
@@ -394,12 +453,29 @@ __device__ __forceinline__ void stage_tile(const StageArgs &a, const StageLaunch
     hf0 = a.g.halo_face[size_t(tile) * a.Hmax + h];
   }
 
-  // ---- A: tile state -> LDS (coalesced, contiguous chunk) ----
+  // ---- A: tile state -> LDS (coalesced, contiguous chunk), or on the
+  // matrix-core path to the registers of the lanes that use it ----
   // Every load is issued before the first LDS store, so a thread's LPT
   // loads are in flight together.
   constexpr int LPT = (E * NV + kThreads - 1) / kThreads;
-  SIN v[LPT];
-  {
+  using MS = MfShape<N, C>;
+  constexpr int KI = MF ? MS::KI : 1;
+  const int lane = tid & 63, krow = lane >> 4;
+  const int me = (tid >> 6) * 16 + (lane & 15);  // element of the lane (matrix-core path)
+  SIN v[MF ? 1 : LPT];
+  C qv[4][KI];  // matrix-core path: the variables at points MS::point(krow, j)
+  if constexpr (MF) {
+    // a point past NpInt repeats the last one (a finite operand for the zero
+    // columns of the A fragments)
+    const SIN *qe = qs + size_t(k0 + me) * NV;
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+      SIN r[KI];
+      lane_load<N, C>(qe + c * NpInt, krow, r);
+#pragma unroll
+      for (int j = 0; j < KI; ++j) qv[c][j] = C(r[j]);
+    }
+  } else {
     const SIN *qt = qs + size_t(k0) * NV;
 #pragma unroll
     for (int j = 0; j < LPT; ++j) {
@@ -438,7 +514,7 @@ __device__ __forceinline__ void stage_tile(const StageArgs &a, const StageLaunch
   const C *DvT = Tabs<N, C>::ops();                     // DivInt^T [NpFlux][NpInt]
   const C *Itp = DvT + Sh::NpFlux * NpInt;              // FluxEdgeInterp [NF][NpInt]
   const C g = C(ph.gamma);
-  {
+  if constexpr (!MF) {
 #pragma unroll
     for (int j = 0; j < LPT; ++j) {
       const int idx = tid + j * kThreads;
@@ -475,9 +551,9 @@ __device__ __forceinline__ void stage_tile(const StageArgs &a, const StageLaunch
     if (l.stage == 4 && first && threadIdx.x == 0) ctl->time[par ^ 1] = time;
     return;
   }
-  const int te = tid % E, tn = tid / E;  // (element, variable) of this thread
-  if constexpr (UseMfma<N, C, SHOCK>::value) {
-    traces_mfma<N, C>(sQ, sT, tn, te, TS);
+  const int te = tid % E, tn = tid / E;  // (element, variable) of this thread (VALU path)
+  if constexpr (MF) {
+    traces_mfma_regs<N, C>(qv, sT, lane, krow, me, TS);
   } else {
     C q[NpInt];
 #pragma unroll
@@ -648,51 +724,41 @@ __device__ __forceinline__ void stage_tile(const StageArgs &a, const StageLaunch
   __syncthreads();
 
   // ---- C: interior flux + divergence ----
-  if constexpr (UseMfma<N, C, SHOCK>::value) {
-    // Matrix cores: wave tn forms variable tn of the tile's 64 elements in
-    // four 16-element column blocks; lane (col, krow) evaluates the interior
-    // flux at the points 4g + krow of element mfma_elem(b, col) and feeds the r and
-    // s components as the B operands of two K-steps, then the face fluxes.
-    static_assert(E == 64, "one wave per variable");
+  // (matrix-core path: the divergence stays in the lanes' registers, rows
+  // MS::point(krow, j) of element me, the points of qv)
+  typename Mma<C>::vec rq[MF ? 4 : 1];
+  if constexpr (MF) {
+    // K-steps 2j and 2j + 1: the r and s components of the contravariant
+    // interior flux at the lane's point j (all four variables from one flux
+    // evaluation); then the scaled face fluxes from LDS
     using M = Mma<C>;
-    const int col = te & 15, krow = te >> 4;
-    C af[Sh::KS];
+    C af[MS::KS];
 #pragma unroll
-    for (int s2 = 0; s2 < Sh::KS; ++s2) af[s2] = Tabs<N, C>::div()[s2 * 64 + te];
-    typename M::vec out[4];
+    for (int s2 = 0; s2 < MS::KS; ++s2) af[s2] = Tabs<N, C>::div()[s2 * 64 + lane];
+    const C Jdc = sJ[me], J0 = sJ[E + me], J1 = sJ[2 * E + me], J2 = sJ[3 * E + me], J3 = sJ[4 * E + me];
 #pragma unroll
-    for (int b = 0; b < 4; ++b) {
-      const int e = mfma_elem(b, col);
-      const C Jdc = sJ[e], J0 = sJ[E + e], J1 = sJ[2 * E + e], J2 = sJ[3 * E + e], J3 = sJ[4 * E + e];
-      typename M::vec acc = M::zero();
+    for (int c = 0; c < 4; ++c) rq[c] = M::zero();
 #pragma unroll
-      for (int g2 = 0; g2 < Sh::G; ++g2) {
-        // past NpInt the A column is zero: any finite point value will do
-        const int j = min(4 * g2 + krow, NpInt - 1);
-        const C r0 = sQ[(0 * NpInt + j) * P + e], r1 = sQ[(1 * NpInt + j) * P + e];
-        const C r2 = sQ[(2 * NpInt + j) * P + e], r3 = sQ[(3 * NpInt + j) * P + e];
-        C Fx, Fy;
-        flux_component(tn, g, r0, r1, r2, r3, Fx, Fy);
-        const C Fr = Jdc * (J0 * Fx + J1 * Fy), Fs = Jdc * (J2 * Fx + J3 * Fy);
-        acc = M::mma(af[2 * g2], Fr, acc);
-        acc = M::mma(af[2 * g2 + 1], Fs, acc);
-      }
+    for (int j = 0; j < KI; ++j) {
+      const C q4[4] = {qv[0][j], qv[1][j], qv[2][j], qv[3][j]};
+      C Fx[4], Fy[4];
+      flux_xy(g, q4, Fx, Fy);
 #pragma unroll
-      for (int s2 = 0; s2 < Sh::KE; ++s2) {
-        const int p = min(4 * s2 + krow, NF - 1);
-        acc = M::mma(af[2 * Sh::G + s2], sT[tn * TS + p * E + e], acc);
-      }
-      out[b] = acc * sJ[5 * E + e];  // -1/Jdet
-    }
-    __syncthreads();  // every wave has read its fluxes; sR may now overwrite sT
-#pragma unroll
-    for (int b = 0; b < 4; ++b) {
-#pragma unroll
-      for (int v = 0; v < 4; ++v) {
-        const int i = M::row(krow, v);
-        if (i < NpInt) sR[(tn * NpInt + i) * P + mfma_elem(b, col)] = out[b][v];
+      for (int c = 0; c < 4; ++c) {
+        const C Fr = Jdc * (J0 * Fx[c] + J1 * Fy[c]), Fs = Jdc * (J2 * Fx[c] + J3 * Fy[c]);
+        rq[c] = M::mma(af[2 * j], Fr, rq[c]);
+        rq[c] = M::mma(af[2 * j + 1], Fs, rq[c]);
       }
     }
+#pragma unroll
+    for (int s2 = 0; s2 < Sh::KE; ++s2) {
+      const int p = min(4 * s2 + krow, NF - 1);
+#pragma unroll
+      for (int c = 0; c < 4; ++c) rq[c] = M::mma(af[2 * KI + s2], sT[c * TS + p * E + me], rq[c]);
+    }
+    const C moojd = sJ[5 * E + me];  // -1/Jdet
+#pragma unroll
+    for (int c = 0; c < 4; ++c) rq[c] = rq[c] * moojd;
   } else {
     // VALU: thread (element te, variable tn)
     const int kk = k0 + te;
@@ -752,13 +818,23 @@ __device__ __forceinline__ void stage_tile(const StageArgs &a, const StageLaunch
 #pragma unroll
     for (int i = 0; i < NpInt; ++i) sR[(tn * NpInt + i) * P + te] = acc[i];
   }
-  __syncthreads();
+  if constexpr (!MF) __syncthreads();
 
   if (rhs_only) {
     double *out = l.rhs_out + size_t(k0) * NV;
-    for (int idx = tid; idx < E * NV; idx += kThreads) {
-      const int e = idx / NV, r = idx - e * NV;
-      out[idx] = double(sR[r * P + e]);
+    if constexpr (MF) {
+#pragma unroll
+      for (int j = 0; j < KI; ++j) {
+        const int i = MS::point(krow, j);
+        if (i < NpInt)
+#pragma unroll
+          for (int c = 0; c < 4; ++c) out[size_t(me) * NV + c * NpInt + i] = double(rq[c][j]);
+      }
+    } else {
+      for (int idx = tid; idx < E * NV; idx += kThreads) {
+        const int e = idx / NV, r = idx - e * NV;
+        out[idx] = double(sR[r * P + e]);
+      }
     }
     return;
   }
@@ -818,7 +894,71 @@ __device__ __forceinline__ void stage_tile(const StageArgs &a, const StageLaunch
     S *q4 = static_cast<S *>(a.q4) + base;
     double *Rr = static_cast<double *>(a.R) + base;
     const int nreal = min(E, ci.k_end - k0);
-    if constexpr (LAST) {
+    if constexpr (MF) {
+      // the lane's values: variable c at point MS::point(krow, j) of element
+      // me; the divergence is rq, the stage input qv, which takes the new
+      // state for phase F (a repeated point keeps its finite old value)
+      const size_t eb = size_t(me) * NV;
+      const double dte = lts ? sDT[me] : dt;
+      const bool real = me < nreal;
+      if constexpr (LAST) {
+#pragma unroll
+        for (int c = 0; c < 4; ++c) {
+          double v0[KI], vR[KI];
+          S v2[KI], v3[KI];
+          lane_load<N, C>(q0 + eb + c * NpInt, krow, v0);
+          lane_load<N, C>(q2 + eb + c * NpInt, krow, v2);
+          lane_load<N, C>(q3 + eb + c * NpInt, krow, v3);
+          lane_load<N, C>(Rr + eb + c * NpInt, krow, vR);
+#pragma unroll
+          for (int j = 0; j < KI; ++j) {
+            const double dtR = dte * double(rq[c][j]);
+            const double Rn = -v0[j] + 0.517231671970585 * double(v2[j]) + 0.096059710526146 * double(v3[j]) +
+                              0.386708617503269 * double(qv[c][j]) + 0.063692468666290 * (dte * vR[j]) +
+                              0.226007483236906 * dtR;
+            vR[j] = Rn;
+            v0[j] += Rn;
+            if (MS::point(krow, j) < NpInt) {
+              qv[c][j] = C(v0[j]);
+              if (real && !(v0[j] == v0[j])) bad = 1;
+            }
+          }
+          lane_store<N, C>(Rr + eb + c * NpInt, krow, vR);
+          lane_store<N, C>(q0 + eb + c * NpInt, krow, v0);
+        }
+      } else {
+        double v0[4][KI];
+        if (stage >= 1) {
+#pragma unroll
+          for (int c = 0; c < 4; ++c) lane_load<N, C>(q0 + eb + c * NpInt, krow, v0[c]);
+        }
+        S *qn = stage == 0 ? q1 : stage == 1 ? q2 : stage == 2 ? q3 : q4;
+#pragma unroll
+        for (int c = 0; c < 4; ++c) {
+          S nv[KI];
+          double rh[KI];
+#pragma unroll
+          for (int j = 0; j < KI; ++j) {
+            const double rhs = double(rq[c][j]);
+            const double qsv = double(qv[c][j]);
+            const double dtR = dte * rhs;
+            rh[j] = rhs;
+            switch (stage) {
+              case 0: nv[j] = S(qsv + 0.391752226571890 * dtR); break;
+              case 1: nv[j] = S(0.444370493651235 * v0[c][j] + 0.555629506348765 * qsv + 0.368410593050371 * dtR); break;
+              case 2: nv[j] = S(0.620101851488403 * v0[c][j] + 0.379898148511597 * qsv + 0.251891774271694 * dtR); break;
+              default: nv[j] = S(0.178079954393132 * v0[c][j] + 0.821920045606868 * qsv + 0.544974750228521 * dtR); break;
+            }
+            if (MS::point(krow, j) < NpInt) {
+              qv[c][j] = C(nv[j]);
+              if (real && !(nv[j] == nv[j])) bad = 1;
+            }
+          }
+          lane_store<N, C>(qn + eb + c * NpInt, krow, nv);
+          if (stage == 3) lane_store<N, C>(Rr + eb + c * NpInt, krow, rh);
+        }
+      }
+    } else if constexpr (LAST) {
       // stage 4: R = -q0 + a q2 + b q3 + c q4 + d dt R3 + e dt RHS, q0 += R;
       // with FP32 stage states the four operand arrays are loaded in two
       // halves (the register budget of three waves per SIMD)
@@ -908,8 +1048,8 @@ __device__ __forceinline__ void stage_tile(const StageArgs &a, const StageLaunch
     __syncthreads();
 
     // ---- F: traces of the new state (own faces) ----
-    if constexpr (UseMfma<N, C, SHOCK>::value) {
-      traces_mfma<N, C>(sQ, sT, tn, te, TS);
+    if constexpr (MF) {
+      traces_mfma_regs<N, C>(qv, sT, lane, krow, me, TS);
     } else {
       C q[NpInt];
 #pragma unroll
@@ -1435,8 +1575,8 @@ __global__ void __launch_bounds__(256) k_shock_grad_act(StageArgs a) {
 // ---- launchers ----
 
 template <int N>
-size_t lds_bytes_n(size_t csize, int Hmax, int Fmax, int WXmax, bool lts) {
-  return lds_tile_bytes<N>(csize, Hmax, Fmax, WXmax) + (lts ? lds_lts_bytes : 0);
+size_t lds_bytes_n(size_t csize, int Hmax, int Fmax, int WXmax, bool lts, bool mf) {
+  return lds_tile_bytes<N>(csize, Hmax, Fmax, WXmax, mf) + (lts ? lds_lts_bytes : 0);
 }
 
 template <int N, class S, class SIN, class C, int FLUX, bool LAST, bool SHOCK>
@@ -1444,7 +1584,7 @@ hipError_t launch_stage_t(const StageArgs &a, hipStream_t s) {
   // one thread stages each list item into LDS
   if (a.Fmax > kThreads || (!SHOCK && (a.Wmax > kThreads || a.Xmax > kThreads))) return hipErrorInvalidValue;
   const size_t lds = lds_bytes_n<N>(sizeof(C), SHOCK ? a.Hmax : 0, a.Fmax, SHOCK ? 0 : a.Wmax + a.Xmax,
-                                    a.lts.on != 0);
+                                    a.lts.on != 0, UseMfma<N, C, SHOCK>::value);
   static size_t attr_bytes = 0;  // dynamic LDS the kernel is currently allowed
   if (lds > 64 * 1024 && lds > attr_bytes) {
     hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void *>(&k_stage<N, S, SIN, C, FLUX, LAST, SHOCK>),
@@ -1547,41 +1687,47 @@ hipError_t launch_shock_n(int prec, int pass, const StageArgs &a, hipStream_t s)
 template <int N>
 static hipError_t upload_tables(const double *divT, const double *interp) {
   using Sh = Shape<N>;
-  static_assert(Sh::KS * 64 <= kMfMax && Sh::RB * Sh::KI * 64 <= kMfIMax, "MFMA operand tables");
-  {
-    // A fragments of the MFMA FluxEdgeInterp contraction
-    std::vector<double> ai(size_t(Sh::RB) * Sh::KI * 64, 0.);
+  // A fragments of the matrix-core contractions in the point order of the
+  // arithmetic type (MfShape): the FluxEdgeInterp [row block][K-step][lane]
+  // and the DivInt [K-step][lane] (r components, s components, face DOFs)
+  auto fragments = [&](auto tag, std::vector<double> &ai, std::vector<double> &af) {
+    using MS = MfShape<N, decltype(tag)>;
+    static_assert(MS::KS * 64 <= kMfMax && Sh::RB * MS::KI * 64 <= kMfIMax, "MFMA operand tables");
+    ai.assign(size_t(Sh::RB) * MS::KI * 64, 0.);
     for (int rb = 0; rb < Sh::RB; ++rb)
-      for (int s = 0; s < Sh::KI; ++s)
+      for (int s = 0; s < MS::KI; ++s)
         for (int l = 0; l < 64; ++l) {
-          const int p = 16 * rb + l % 16, i = 4 * s + l / 16;
-          if (p < Sh::NF && i < Sh::NpInt) ai[(size_t(rb) * Sh::KI + s) * 64 + l] = interp[size_t(p) * Sh::NpInt + i];
+          const int p = 16 * rb + l % 16, i = MS::point(l / 16, s);
+          if (p < Sh::NF && i < Sh::NpInt) ai[(size_t(rb) * MS::KI + s) * 64 + l] = interp[size_t(p) * Sh::NpInt + i];
         }
-    const std::vector<float> aif(ai.begin(), ai.end());
-    hipError_t e = hipMemcpyToSymbol(HIP_SYMBOL(cMfI<N>), aif.data(), aif.size() * sizeof(float), 0,
-                                     hipMemcpyHostToDevice);
-    if (e != hipSuccess) return e;
-    e = hipMemcpyToSymbol(HIP_SYMBOL(cMfID<N>), ai.data(), ai.size() * sizeof(double), 0, hipMemcpyHostToDevice);
-    if (e != hipSuccess) return e;
-  }
-  {
-    // A fragments of the MFMA DivInt contraction (the K-row order of phase C)
-    std::vector<double> af(size_t(Sh::KS) * 64, 0.);
-    for (int s = 0; s < Sh::KS; ++s)
+    af.assign(size_t(MS::KS) * 64, 0.);
+    for (int s = 0; s < MS::KS; ++s)
       for (int l = 0; l < 64; ++l) {
         const int row = l % 16, kr = l / 16;
         int j = -1;
-        if (s < 2 * Sh::G) {
-          const int i = 4 * (s / 2) + kr;
+        if (s < 2 * MS::KI) {
+          const int i = MS::point(kr, s / 2);
           if (i < Sh::NpInt) j = (s % 2) * Sh::NpInt + i;
         } else {
-          const int p = 4 * (s - 2 * Sh::G) + kr;
+          const int p = 4 * (s - 2 * MS::KI) + kr;
           if (p < Sh::NF) j = 2 * Sh::NpInt + p;
         }
         if (row < Sh::NpInt && j >= 0) af[size_t(s) * 64 + l] = divT[size_t(j) * Sh::NpInt + row];
       }
-    const std::vector<float> aff(af.begin(), af.end());
-    hipError_t e = hipMemcpyToSymbol(HIP_SYMBOL(cMfA<N>), aff.data(), aff.size() * sizeof(float), 0,
+  };
+  std::vector<double> ai, af;
+  fragments(float(), ai, af);
+  {
+    const std::vector<float> aif(ai.begin(), ai.end()), aff(af.begin(), af.end());
+    hipError_t e = hipMemcpyToSymbol(HIP_SYMBOL(cMfI<N>), aif.data(), aif.size() * sizeof(float), 0,
+                                     hipMemcpyHostToDevice);
+    if (e != hipSuccess) return e;
+    e = hipMemcpyToSymbol(HIP_SYMBOL(cMfA<N>), aff.data(), aff.size() * sizeof(float), 0, hipMemcpyHostToDevice);
+    if (e != hipSuccess) return e;
+  }
+  fragments(double(), ai, af);
+  {
+    hipError_t e = hipMemcpyToSymbol(HIP_SYMBOL(cMfID<N>), ai.data(), ai.size() * sizeof(double), 0,
                                      hipMemcpyHostToDevice);
     if (e != hipSuccess) return e;
     e = hipMemcpyToSymbol(HIP_SYMBOL(cMfAD<N>), af.data(), af.size() * sizeof(double), 0, hipMemcpyHostToDevice);
@@ -1659,11 +1805,11 @@ static hipError_t launch_multi_k(const MultiArgs &m, hipStream_t s) {
     const size_t cs = sizeof(S) == 8 || G.N <= 1 ? sizeof(double) : sizeof(float);
     size_t b = 0;
     switch (G.N) {
-      case 0: b = lds_bytes_n<0>(cs, 0, G.Fmax, G.WXmax, G.lts != 0); break;
-      case 1: b = lds_bytes_n<1>(cs, 0, G.Fmax, G.WXmax, G.lts != 0); break;
-      case 2: b = lds_bytes_n<2>(cs, 0, G.Fmax, G.WXmax, G.lts != 0); break;
-      case 3: b = lds_bytes_n<3>(cs, 0, G.Fmax, G.WXmax, G.lts != 0); break;
-      default: b = lds_bytes_n<4>(cs, 0, G.Fmax, G.WXmax, G.lts != 0); break;
+      case 0: b = lds_bytes_n<0>(cs, 0, G.Fmax, G.WXmax, G.lts != 0, false); break;
+      case 1: b = lds_bytes_n<1>(cs, 0, G.Fmax, G.WXmax, G.lts != 0, false); break;
+      case 2: b = lds_bytes_n<2>(cs, 0, G.Fmax, G.WXmax, G.lts != 0, false); break;
+      case 3: b = lds_bytes_n<3>(cs, 0, G.Fmax, G.WXmax, G.lts != 0, true); break;
+      default: b = lds_bytes_n<4>(cs, 0, G.Fmax, G.WXmax, G.lts != 0, true); break;
     }
     lds = std::max(lds, b);
   }
