@@ -544,6 +544,9 @@ __device__ __forceinline__ void stage_tile(const StageArgs &a, const StageLaunch
       sJ[5 * E + e] = -C(1. / jm0);
     }
   }
+  // matrix-core path: the traces need only the lane's registers, so they
+  // are formed before the one barrier of phase A
+  if constexpr (MF) traces_mfma_regs<N, C>(qv, sT, lane, krow, me, TS);
   __syncthreads();
   // the control-block chain (tile -> case -> time) has had the tile loads and
   // the LDS stores to arrive under
@@ -552,9 +555,7 @@ __device__ __forceinline__ void stage_tile(const StageArgs &a, const StageLaunch
     return;
   }
   const int te = tid % E, tn = tid / E;  // (element, variable) of this thread (VALU path)
-  if constexpr (MF) {
-    traces_mfma_regs<N, C>(qv, sT, lane, krow, me, TS);
-  } else {
+  if constexpr (!MF) {
     C q[NpInt];
 #pragma unroll
     for (int i = 0; i < NpInt; ++i) q[i] = sQ[(tn * NpInt + i) * P + te];
@@ -584,7 +585,7 @@ __device__ __forceinline__ void stage_tile(const StageArgs &a, const StageLaunch
       sT[n * TS + NF * E + h * NpEdge + pp] = t;
     }
   }
-  __syncthreads();
+  if constexpr (!MF) __syncthreads();
 
   // ---- B: numerical flux at every face point ----
   // Shared faces from the tile's work list: a face with an out-of-tile
@@ -619,8 +620,8 @@ __device__ __forceinline__ void stage_tile(const StageArgs &a, const StageLaunch
       }
       const C nx = gq[0], ny = gq[1];
       const bool owner = packed & (1 << 12);
-      if (owner) numflux(FLUX, g, own, oth, nx, ny, F);
-      else numflux(FLUX, g, oth, own, nx, ny, F);
+      if (owner) numflux<C, !SHOCK>(FLUX, g, own, oth, nx, ny, F);
+      else numflux<C, !SHOCK>(FLUX, g, oth, own, nx, ny, F);
       if constexpr (SHOCK) {
         // Viscous edge flux at owner point jo (StoreEdgeViscousFlux,
         // edges.go:151-244): average of the two sides' eps*grad Q along
@@ -723,6 +724,17 @@ __device__ __forceinline__ void stage_tile(const StageArgs &a, const StageLaunch
   }
   __syncthreads();
 
+  // FP32 matrix-core path, stages 1-3: phase E's q0 operand, loaded under
+  // phase C
+  constexpr bool PF = MF && !LAST && sizeof(C) == 4;
+  double q0p[PF ? 4 : 1][PF ? KI : 1];
+  if constexpr (PF) {
+    if (l.stage >= 1) {
+      const double *q0e = static_cast<const double *>(a.q0) + size_t(k0 + me) * NV;
+#pragma unroll
+      for (int c = 0; c < 4; ++c) lane_load<N, C>(q0e + c * NpInt, krow, q0p[c]);
+    }
+  }
   // ---- C: interior flux + divergence ----
   // (matrix-core path: the divergence stays in the lanes' registers, rows
   // MS::point(krow, j) of element me, the points of qv)
@@ -928,7 +940,12 @@ __device__ __forceinline__ void stage_tile(const StageArgs &a, const StageLaunch
         }
       } else {
         double v0[4][KI];
-        if (stage >= 1) {
+        if constexpr (PF) {
+#pragma unroll
+          for (int c = 0; c < 4; ++c)
+#pragma unroll
+            for (int j = 0; j < KI; ++j) v0[c][j] = q0p[c][j];  // loaded under phase C
+        } else if (stage >= 1) {
 #pragma unroll
           for (int c = 0; c < 4; ++c) lane_load<N, C>(q0 + eb + c * NpInt, krow, v0[c]);
         }
