@@ -7,9 +7,12 @@
 // One stage launch per Runge-Kutta stage. A workgroup owns a tile of kTile
 // consecutive elements (Hilbert order), whose element-major state is one
 // contiguous chunk of HBM. Per stage and tile:
-//   A. load the stage state into LDS; face traces by the FluxEdgeInterp
-//      contraction (model_problems/Euler2D/edges.go:485-491) for the tile's
-//      faces and for the out-of-tile neighbour faces listed in the halo table
+//   A. load the stage state (into LDS, or on the matrix-core path into the
+//      registers of the lanes that use it); face traces by the
+//      FluxEdgeInterp contraction (model_problems/Euler2D/edges.go:485-491)
+//      for the tile's faces; the cross-tile neighbour traces from the
+//      halo-trace buffer, or with shock capturing interpolated from the
+//      neighbours' state for the faces listed in the halo table
 //   B. Riemann / boundary flux at every face point (edges.go:325-452,
 //      bcs.go:11-133), scaled by ||n|| (SetRTFluxOnEdges edges.go:454-483);
 //      with shock capturing, minus the viscous edge flux
@@ -25,8 +28,9 @@
 //      and the owner-side CFL wave-speed maximum (edges.go:246-323), folded
 //      into one atomic per workgroup for the next stage's dt
 //      (calculateGlobalDT euler.go:945-971)
-// With FP32 arithmetic from N=3 on, the FluxEdgeInterp (A, F) and DivInt (C)
-// contractions run on the matrix cores as 16x16x4 FP32 MFMA tiles.
+// From N=3 on without shock capturing, the FluxEdgeInterp (A, F) and DivInt
+// (C) contractions run on the matrix cores as 16x16x4 FP32 or FP64 MFMA
+// tiles, with the element state held in the MFMA result-row layout.
 // With shock capturing three passes precede each stage launch: the sensor
 // with the vertex max-merge, the element viscosity fields (and the stage-2
 // state filter), and the RT gradient with the stage's wave speeds.
@@ -168,11 +172,6 @@ __device__ __forceinline__ void lane_store(T *row, int krow, const T (&val)[MfSh
     if (i < NpInt) row[i] = val[j];
   }
 }
-
-// Element of column col of MFMA column block b. Pairing the blocks
-// (e = 2 col + (b & 1) + 32 (b >> 1)) puts the two K-rows of a half-wave
-// operand read (row pitch 65 words) on even and odd LDS banks.
-__device__ __forceinline__ int mfma_elem(int b, int col) { return 2 * col + (b & 1) + 32 * (b >> 1); }
 
 // The operators of order N in constant memory (each translation unit holds
 // the orders it instantiates), so that every coefficient is a uniform scalar
