@@ -717,7 +717,7 @@ uint64_t hilbert(uint32_t x, uint32_t y) {
 }
 }  // namespace
 
-DeviceLayout build_device_layout(const Discretisation &d, int E) {
+DeviceLayout build_device_layout(const Discretisation &d, int E, int TP) {
   DeviceLayout L;
   const int K = d.K, NpEdge = d.ops.NpEdge, NpInt = d.ops.NpInt;
   L.K = K;
@@ -808,7 +808,7 @@ DeviceLayout build_device_layout(const Discretisation &d, int E) {
         const double len = Ed.enr[0] == 1 ? (2. * std::sqrt(2.)) * Ed.IInII[0] : 2. * Ed.IInII[0];
         L.vooL[fi] = 1. / len;
         if (pkk / E == tile) {
-          L.nbr_slot[fi] = pf * NpEdge * E + (pkk - tile * E);
+          L.nbr_slot[fi] = pf * NpEdge * TP + (pkk - tile * E);
         } else {
           auto key = std::make_pair(pkk, pf);
           auto it = halo_index[tile].find(key);
@@ -820,7 +820,7 @@ DeviceLayout build_device_layout(const Discretisation &d, int E) {
           } else {
             hidx = it->second;
           }
-          L.nbr_slot[fi] = int(0x80000000u | uint32_t(3 * NpEdge * E + hidx * NpEdge));
+          L.nbr_slot[fi] = int(0x80000000u | uint32_t(3 * NpEdge * TP + hidx * NpEdge));
         }
         code |= FK_Shared;
       } else {
@@ -868,8 +868,8 @@ DeviceLayout build_device_layout(const Discretisation &d, int E) {
         fl[kk / E].push_back((kk % E) | (f << 8) | ((in_tile && owner) ? 1 << 10 : 0) | (in_tile ? 0 : 1 << 11) |
                              (owner ? 1 << 12 : 0) | (base << 13));
         double w2 = 0.0;
-        if (in_tile) {  // the partner slot pf * NpEdge * E + e2
-          const int pf = base / (NpEdge * E), e2 = base - pf * NpEdge * E;
+        if (in_tile) {  // the partner slot pf * NpEdge * TP + e2
+          const int pf = base / (NpEdge * TP), e2 = base - pf * NpEdge * TP;
           w2 = -L.IInII[size_t(pf) * Kp + (kk / E) * E + e2];
         }
         fg[kk / E].push_back({L.Nx[fi], L.Ny[fi], owner ? L.IInII[fi] : -L.IInII[fi], w2});

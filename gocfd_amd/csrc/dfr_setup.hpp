@@ -138,7 +138,7 @@ struct DeviceLayout {
   std::vector<int> evert;           // [3][Kp] vertex ids
   std::vector<double> hK;           // EdgeLenMax / (N+1)^2
 };
-DeviceLayout build_device_layout(const Discretisation &d, int tile_elems);
+DeviceLayout build_device_layout(const Discretisation &d, int tile_elems, int trace_pitch);
 
 // Operator block of the shock-capturing kernels for order N (layout of
 // gocfd::dev::ShockArgs::ops): P = Mass (I - Clip), Mass, D = I - Clip,

@@ -599,7 +599,7 @@ int gocfd_ensemble_create(int32_t ncases, const gocfd_mesh *const *meshes, const
     c->prm = p;
     c->d = build_discretisation(meshes[i]->m, p.polynomial_order);
     init_case(c->d, p, c->fs, c->Q0);
-    c->L = build_device_layout(c->d, dev::kTile);
+    c->L = build_device_layout(c->d, dev::kTile, dev::kTP);
     h->cases.push_back(std::move(c));
   }
   // one group per order, limiter (with its Kappa) and time-stepping mode:

@@ -85,7 +85,7 @@ int gocfd_euler2d_create(const gocfd_mesh *mesh, const gocfd_params *p, gocfd_eu
   h->d = build_discretisation(mesh->m, p->polynomial_order);
   std::vector<double> Q0;
   init_case(h->d, *p, h->fs, Q0);
-  h->L = build_device_layout(h->d, dev::kTile);
+  h->L = build_device_layout(h->d, dev::kTile, dev::kTP);
   h->NV = 4 * h->d.ops.NpInt;
   if (p->device < 0) {  // host-only handle: setup and introspection, no device work
     h->device = -1;

@@ -13,6 +13,11 @@ namespace dev {
 #endif
 constexpr int kTile = GOCFD_TILE;  // elements per workgroup tile
 constexpr int kThreads = 4 * kTile; // threads per workgroup: one per (element, variable)
+// Row pitch (elements) of the stage kernel's LDS trace/flux rows [point][element]:
+// four more than the tile, so that the face points of one element, read by
+// neighbouring lanes in the Riemann and wave-speed phases, fall on different
+// LDS banks (a 64-element pitch put them all on one)
+constexpr int kTP = kTile + 4;
 static_assert(kThreads % 64 == 0, "whole wavefronts per workgroup");
 
 // Device control block. Time is double-buffered by step parity and the wave

@@ -69,13 +69,13 @@ hipError_t upload_operators(int N, const double *divT, const double *interp) {
 }
 
 // Dynamic LDS of the stage kernel, at most: the tile state [NV][kTile+1],
-// the trace/flux buffer [4][3 NpEdge kTile + NpEdge Hmax] (the halo part
+// the trace/flux buffer [4][3 NpEdge kTP + NpEdge Hmax] (the halo part
 // only with shock capturing), the Riemann items' geometry and the metric.
 size_t stage_lds_bytes(int N, int prec, int Hmax) {
   if (N < 0 || N > 4) return 0;
   const size_t cs = prec == 0 ? sizeof(double) : sizeof(float);
   const size_t NpInt = size_t(N + 1) * (N + 2) / 2, NpEdge = size_t(N) + 2;
-  return (cs * (4 * NpInt * (kTile + 1) + 4 * (3 * NpEdge * kTile + NpEdge * size_t(Hmax))) + 31) / 32 * 32 +
+  return (cs * (4 * NpInt * (kTile + 1) + 4 * (3 * NpEdge * kTP + NpEdge * size_t(Hmax))) + 31) / 32 * 32 +
          cs * (4 * 3 * kTile + 6 * kTile) +                                  // item geometry (Fmax <= 3 kTile), metric
          sizeof(double) * kTile + 2 * sizeof(int) * 6 * kTile +              // phase G lists (Wmax, Xmax <= 3 kTile)
          sizeof(double) * kTile + sizeof(unsigned long long) * 3 * kTile;  // with the LTS buffers

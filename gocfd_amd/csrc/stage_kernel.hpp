@@ -241,7 +241,7 @@ __device__ __forceinline__ void traces_mfma_regs(const C (&qv)[4][MfShape<N, C>:
 #pragma unroll
       for (int j = 0; j < 4; ++j) {
         const int p = 16 * rb + M::row(krow, j);
-        if (p < NF) sT[v * TS + p * E + me] = acc[rb][j];
+        if (p < NF) sT[v * TS + p * kTP + me] = acc[rb][j];
       }
   }
 }
@@ -308,7 +308,8 @@ __device__ __forceinline__ double point_speed(double g, const double q[4], doubl
 
 // Dynamic LDS of the stage kernel: the tile state [NV][kTile+1] (not on the
 // matrix-core path, whose state stays in registers) and the trace/flux
-// buffer [4][3 NpEdge kTile + NpEdge Hmax] (rounded to 32 bytes), the Riemann
+// buffer [4][3 NpEdge kTP + NpEdge Hmax] (rows of kTP elements; rounded to
+// 32 bytes), the Riemann
 // items' geometry [Fmax][4] and the elements' metric terms [6][kTile];
 // without shock capturing, phase G's lists: the elements' 1/hK [kTile]
 // (FP64), the owned faces with their codes [2][Wmax] and the export items
@@ -317,7 +318,7 @@ __device__ __forceinline__ double point_speed(double g, const double q[4], doubl
 template <int N>
 __host__ __device__ constexpr size_t lds_trace_bytes(size_t csize, int Hmax, bool mf) {
   return (csize * ((mf ? 0 : size_t(Shape<N>::NV) * (kTile + 1)) +
-                   4 * (size_t(Shape<N>::NF) * kTile + size_t(Shape<N>::NpEdge) * Hmax)) + 31) / 32 * 32;
+                   4 * (size_t(Shape<N>::NF) * kTP + size_t(Shape<N>::NpEdge) * Hmax)) + 31) / 32 * 32;
 }
 template <int N>
 __host__ __device__ constexpr size_t lds_tile_bytes(size_t csize, int Hmax, int Fmax, int WXmax, bool mf) {
@@ -384,7 +385,7 @@ __device__ __forceinline__ void stage_tile(const StageArgs &a, const StageLaunch
   constexpr bool MF = UseMfma<N, C, SHOCK>::value;
   using V4 = typename Vec4<C>::type;
   C *sQ = reinterpret_cast<C *>(smem_raw);  // [NV][P] (VALU path)
-  const int TS = NF * E + (HX ? 0 : NpEdge * a.Hmax);
+  const int TS = NF * kTP + (HX ? 0 : NpEdge * a.Hmax);
   C *sT = MF ? sQ : sQ + NV * P;             // [4][TS]: traces, then fluxes
   C *sR = sT;                                // [NV][P]: RHS (aliases sT; VALU path)
   // the Riemann items' geometry and the metric terms (Jdet, Jinv[4], -1/Jdet)
@@ -563,7 +564,7 @@ __device__ __forceinline__ void stage_tile(const StageArgs &a, const StageLaunch
       C t = C(0);
 #pragma unroll
       for (int i = 0; i < NpInt; ++i) t += Itp[p * NpInt + i] * q[i];
-      sT[tn * TS + p * E + te] = t;
+      sT[tn * TS + p * kTP + te] = t;
     }
   }
   for (int it = tid; it < (HX ? 0 : a.Hmax * 4); it += kThreads) {
@@ -581,7 +582,7 @@ __device__ __forceinline__ void stage_tile(const StageArgs &a, const StageLaunch
       C t = C(0);
 #pragma unroll
       for (int i = 0; i < NpInt; ++i) t += row[i] * q[i];
-      sT[n * TS + NF * E + h * NpEdge + pp] = t;
+      sT[n * TS + NF * kTP + h * NpEdge + pp] = t;
     }
   }
   if constexpr (!MF) __syncthreads();
@@ -608,13 +609,13 @@ __device__ __forceinline__ void stage_tile(const StageArgs &a, const StageLaunch
       const V4 gq = sFG[w];
       C own[4], oth[4], F[4];
       const int base = packed >> 13;                    // the neighbour's slot
-      const int stride = (packed & (1 << 11)) ? 1 : E;  // halo: point-contiguous
+      const int stride = (packed & (1 << 11)) ? 1 : kTP;  // halo: point-contiguous
       const int rp = NpEdge - 1 - pp;
       const bool xt = HX && (packed & (1 << 11));  // neighbour traces in hx[j]
-      const int idx = xt ? p * E + e : base + rp * stride;
+      const int idx = xt ? p * kTP + e : base + rp * stride;
 #pragma unroll
       for (int c = 0; c < 4; ++c) {
-        own[c] = sT[c * TS + p * E + e];
+        own[c] = sT[c * TS + p * kTP + e];
         oth[c] = sT[c * TS + idx];
         if (xt) oth[c] = hx[j][c];
       }
@@ -641,7 +642,7 @@ __device__ __forceinline__ void stage_tile(const StageArgs &a, const StageLaunch
           for (int c = 0; c < 4; ++c) {
             // owner traces at owner points jo and NpEdge-1-jo
             const C EQj = owner ? own[c] : oth[c];
-            const C EQr = owner ? sT[c * TS + (f * NpEdge + rp) * E + e] : sT[c * TS + base + pp * stride];
+            const C EQr = owner ? sT[c * TS + (f * NpEdge + rp) * kTP + e] : sT[c * TS + base + pp * stride];
             const C *dX = diss + size_t(c * 2 * NR) * Kp, *dY = dX + size_t(NR) * Kp;
             const C vFL = aL ? nx * dX[size_t(rowL) * Kp + kL] + ny * dY[size_t(rowL) * Kp + kL] : C(0);
             const C vFR = aR ? nx * dX[size_t(rowR) * Kp + kR] + ny * dY[size_t(rowR) * Kp + kR] : C(0);
@@ -660,7 +661,7 @@ __device__ __forceinline__ void stage_tile(const StageArgs &a, const StageLaunch
         // 81 us at order 4, 80,000 elements).
         const C wn = gq[2];  // IInII, negated on the non-owner side
 #pragma unroll
-        for (int c = 0; c < 4; ++c) sT[c * TS + p * E + e] = F[c] * wn;
+        for (int c = 0; c < 4; ++c) sT[c * TS + p * kTP + e] = F[c] * wn;
         if (packed & (1 << 10)) {
           const C w2 = gq[3];
 #pragma unroll
@@ -684,11 +685,11 @@ __device__ __forceinline__ void stage_tile(const StageArgs &a, const StageLaunch
       const V4 gq = sFG[w];
       const C wn = gq[2];  // IInII, negated on the non-owner side
 #pragma unroll
-      for (int c = 0; c < 4; ++c) sT[c * TS + p * E + e] = fl[j][c] * wn;
+      for (int c = 0; c < 4; ++c) sT[c * TS + p * kTP + e] = fl[j][c] * wn;
       if (packed & (1 << 10)) {
         // the in-tile partner's slot (pf*NpEdge*E + e2) at the reversed
         // point, with the partner's IInII negated
-        const int idx = (packed >> 13) + (NpEdge - 1 - pp) * E;
+        const int idx = (packed >> 13) + (NpEdge - 1 - pp) * kTP;
         const C w2 = gq[3];
 #pragma unroll
         for (int c = 0; c < 4; ++c) sT[c * TS + idx] = fl[j][c] * w2;
@@ -704,7 +705,7 @@ __device__ __forceinline__ void stage_tile(const StageArgs &a, const StageLaunch
     if (kind == FK_Shared) continue;
     C own[4];
 #pragma unroll
-    for (int c = 0; c < 4; ++c) own[c] = sT[c * TS + p * E + e];
+    for (int c = 0; c < 4; ++c) own[c] = sT[c * TS + p * kTP + e];
     const double nxd = a.g.Nx[fi], nyd = a.g.Ny[fi];
     const C nx = C(nxd), ny = C(nyd);
     C F[4] = {C(0), C(0), C(0), C(0)};
@@ -739,7 +740,7 @@ __device__ __forceinline__ void stage_tile(const StageArgs &a, const StageLaunch
     }
     const C w = C(a.g.IInII[fi]);
 #pragma unroll
-    for (int c = 0; c < 4; ++c) sT[c * TS + p * E + e] = F[c] * w;
+    for (int c = 0; c < 4; ++c) sT[c * TS + p * kTP + e] = F[c] * w;
   }
   __syncthreads();
 
@@ -783,9 +784,9 @@ __device__ __forceinline__ void stage_tile(const StageArgs &a, const StageLaunch
     }
 #pragma unroll
     for (int s2 = 0; s2 < Sh::KE; ++s2) {
-      const int p = min(4 * s2 + krow, NF - 1);
+      const int p = min(s2 + Sh::KE * krow, NF - 1);
 #pragma unroll
-      for (int c = 0; c < 4; ++c) rq[c] = M::mma(af[2 * KI + s2], sT[c * TS + p * E + me], rq[c]);
+      for (int c = 0; c < 4; ++c) rq[c] = M::mma(af[2 * KI + s2], sT[c * TS + p * kTP + me], rq[c]);
     }
     const C moojd = sJ[5 * E + me];  // -1/Jdet
 #pragma unroll
@@ -818,7 +819,7 @@ __device__ __forceinline__ void stage_tile(const StageArgs &a, const StageLaunch
     }
 #pragma unroll 2
     for (int p = 0; p < NF; ++p) {
-      const C Fe = sT[tn * TS + p * E + te];
+      const C Fe = sT[tn * TS + p * kTP + te];
 #pragma unroll
       for (int i = 0; i < NpInt; ++i) acc[i] += DvT[(2 * NpInt + p) * NpInt + i] * Fe;
     }
@@ -1095,7 +1096,7 @@ __device__ __forceinline__ void stage_tile(const StageArgs &a, const StageLaunch
         C t = C(0);
 #pragma unroll
         for (int i = 0; i < NpInt; ++i) t += Itp[p * NpInt + i] * q[i];
-        sT[tn * TS + p * E + te] = t;
+        sT[tn * TS + p * kTP + te] = t;
       }
     }
     __syncthreads();
@@ -1118,7 +1119,7 @@ __device__ __forceinline__ void stage_tile(const StageArgs &a, const StageLaunch
         const int kind = code & 15;
         C own[4];
 #pragma unroll
-        for (int c = 0; c < 4; ++c) own[c] = sT[c * TS + p * E + e];
+        for (int c = 0; c < 4; ++c) own[c] = sT[c * TS + p * kTP + e];
         double q[4] = {double(own[0]), double(own[1]), double(own[2]), double(own[3])};
         if (kind == FK_Far || kind == FK_IVortex)
           boundary_state(kind, ph, a.g.bxy, code >> 8, NpEdge, pp, tnext, own, a.g.Nx[fi], a.g.Ny[fi], q);
@@ -1137,7 +1138,7 @@ __device__ __forceinline__ void stage_tile(const StageArgs &a, const StageLaunch
         const int xk = sXP[w];
         const int e = xk & 255, f = (xk >> 8) & 3, p = f * NpEdge + pp;
         hout[size_t(sXD[w]) * NpEdge + (NpEdge - 1 - pp)] =
-            V4{sT[p * E + e], sT[TS + p * E + e], sT[2 * TS + p * E + e], sT[3 * TS + p * E + e]};
+            V4{sT[p * kTP + e], sT[TS + p * kTP + e], sT[2 * TS + p * kTP + e], sT[3 * TS + p * kTP + e]};
       }
     }
   }
@@ -1745,7 +1746,7 @@ static hipError_t upload_tables(const double *divT, const double *interp) {
           const int i = MS::point(kr, s / 2);
           if (i < Sh::NpInt) j = (s % 2) * Sh::NpInt + i;
         } else {
-          const int p = 4 * (s - 2 * MS::KI) + kr;
+          const int p = (s - 2 * MS::KI) + Sh::KE * kr;
           if (p < Sh::NF) j = 2 * Sh::NpInt + p;
         }
         if (row < Sh::NpInt && j >= 0) af[size_t(s) * 64 + l] = divT[size_t(j) * Sh::NpInt + row];
