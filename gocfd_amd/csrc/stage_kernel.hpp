@@ -1177,7 +1177,12 @@ __device__ __forceinline__ void stage_tile(const StageArgs &a, const StageLaunch
 
 template <int N, class S, class SIN, class C, int FLUX, bool LAST, bool SHOCK>
 __global__ void __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(StageWaves<S, LAST, SHOCK>::value)))
-k_stage(StageArgs a) {
+k_stage(StageArgs) {
+  // The argument is read through a pointer into the kernel-argument segment
+  // (the struct is the only explicit argument, at offset 0): each field is
+  // then loaded where it is used rather than all of them at kernel entry,
+  // which halves the scalar registers spilled to vector lanes.
+  const StageArgs &a = *(const StageArgs *)__builtin_amdgcn_kernarg_segment_ptr();
   const int tile = tile_of_block(a.xcd_tiles);
   if (tile >= a.ntiles) return;
   stage_tile<N, S, SIN, C, FLUX, LAST, SHOCK>(a, launch_of(a), tile);
