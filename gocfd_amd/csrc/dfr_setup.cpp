@@ -921,10 +921,15 @@ DeviceLayout build_device_layout(const Discretisation &d, int E, int TP) {
   L.Wmax = 1;
   for (auto &w : waves) L.Wmax = std::max<int>(L.Wmax, int(w.size()));
   L.wave_list.assign(size_t(L.ntiles) * L.Wmax, 0);
+  L.wave_code.assign(size_t(L.ntiles) * L.Wmax, 0);
   L.wave_cnt.assign(L.ntiles, 0);
   for (int t = 0; t < L.ntiles; ++t) {
     L.wave_cnt[t] = int(waves[t].size());
     std::copy(waves[t].begin(), waves[t].end(), L.wave_list.begin() + size_t(t) * L.Wmax);
+    for (size_t i = 0; i < waves[t].size(); ++i) {
+      const int w = waves[t][i];
+      L.wave_code[size_t(t) * L.Wmax + i] = L.face_code[size_t(w >> 8) * Kp + size_t(t) * E + (w & 255)];
+    }
   }
   L.Hmax = 1;
   for (auto &hl : halos) L.Hmax = std::max<int>(L.Hmax, int(hl.size()));

@@ -108,6 +108,7 @@ struct DeviceLayout {
   int Wmax = 0;                                         // owned (wave-speed) faces per tile, max
   std::vector<int> wave_list;                           // [ntiles x Wmax]: e | face << 8
   std::vector<int> wave_cnt;                            // [ntiles]
+  std::vector<int> wave_code;                           // [ntiles x Wmax]: face_code of each wave_list face
   // Shared faces whose Riemann flux this tile evaluates: every face with an
   // out-of-tile neighbour, and the owner side of in-tile pairs (which also
   // fills the partner's slot). [ntiles x Fmax]: e | face << 8 | partner << 10

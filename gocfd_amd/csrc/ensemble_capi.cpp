@@ -89,7 +89,7 @@ struct EnsGroup {
   int Nv = 0;                       // vertices of the group's cases
   std::vector<int> cases;
   DevBuf<double> Jdet, Jinv, IInII, Nx, Ny, oohK, bxy, Interp64;
-  DevBuf<int> code, nbr_slot, halo_elem, halo_face, wave_list, wave_cnt, flux_list, flux_cnt, bnd_cnt, tile_case;
+  DevBuf<int> code, nbr_slot, halo_elem, halo_face, wave_list, wave_cnt, wave_code, flux_list, flux_cnt, bnd_cnt, tile_case;
   // halo traces (without shock capturing): export lists and the two buffers
   DevBuf<int> xp_list, xp_dst, xp_cnt;
   DevBuf<double> ht;
@@ -172,7 +172,7 @@ struct gocfd_ensemble {
     a.g.Jdet = g.Jdet.p; a.g.Jinv = g.Jinv.p; a.g.IInII = g.IInII.p; a.g.Nx = g.Nx.p; a.g.Ny = g.Ny.p;
     a.g.oohK = g.oohK.p; a.g.code = g.code.p; a.g.nbr_slot = g.nbr_slot.p;
     a.g.halo_elem = g.halo_elem.p; a.g.halo_face = g.halo_face.p; a.g.bxy = g.bxy.p;
-    a.g.wave_list = g.wave_list.p; a.g.wave_cnt = g.wave_cnt.p;
+    a.g.wave_list = g.wave_list.p; a.g.wave_cnt = g.wave_cnt.p; a.g.wave_code = g.wave_code.p;
     a.g.flux_list = g.flux_list.p; a.g.flux_cnt = g.flux_cnt.p; a.g.bnd_cnt = g.bnd_cnt.p;
     a.g.flux_geo = g.flux_geo.ptr();
     a.g.xp_list = g.xp_list.p; a.g.xp_dst = g.xp_dst.p; a.g.xp_cnt = g.xp_cnt.p;
@@ -488,7 +488,14 @@ void build_group(gocfd_ensemble &h, EnsGroup &g) {
   g.Jdet.upload(Jdet); g.Jinv.upload(Jinv); g.IInII.upload(IInII); g.Nx.upload(Nx); g.Ny.upload(Ny);
   g.oohK.upload(oohK); g.bxy.upload(bxy); g.code.upload(code); g.nbr_slot.upload(nbr_slot);
   g.halo_elem.upload(halo_elem); g.halo_face.upload(halo_face);
-  g.wave_list.upload(wave_list); g.wave_cnt.upload(wave_cnt);
+  // the (group-offset) code of each wave-list face
+  std::vector<int> wave_code(size_t(g.ntiles) * g.Wmax, 0);
+  for (int t = 0; t < g.ntiles; ++t)
+    for (int i = 0; i < wave_cnt[t]; ++i) {
+      const int w = wave_list[size_t(t) * g.Wmax + i];
+      wave_code[size_t(t) * g.Wmax + i] = code[size_t(w >> 8) * Kp + size_t(t) * E + (w & 255)];
+    }
+  g.wave_list.upload(wave_list); g.wave_cnt.upload(wave_cnt); g.wave_code.upload(wave_code);
   g.flux_list.upload(flux_list); g.flux_cnt.upload(flux_cnt); g.bnd_cnt.upload(bnd_cnt);
   g.flux_geo.upload(flux_geo, h.prec, N);
   g.tile_case.upload(tile_case);

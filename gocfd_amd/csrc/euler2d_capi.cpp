@@ -113,7 +113,7 @@ int gocfd_euler2d_create(const gocfd_mesh *mesh, const gocfd_params *p, gocfd_eu
   h->bxy.upload(L.bpoint_xy);
   h->code.upload(L.face_code); h->nbr_slot.upload(L.nbr_slot);
   h->halo_elem.upload(L.halo_elem); h->halo_face.upload(L.halo_face);
-  h->wave_list.upload(L.wave_list); h->wave_cnt.upload(L.wave_cnt);
+  h->wave_list.upload(L.wave_list); h->wave_cnt.upload(L.wave_cnt); h->wave_code.upload(L.wave_code);
   h->flux_list.upload(L.flux_list); h->flux_cnt.upload(L.flux_cnt); h->bnd_cnt.upload(L.bnd_cnt);
   h->flux_geo.upload(L.flux_geo, h->prec, p->polynomial_order);
   h->perm.upload(L.perm);

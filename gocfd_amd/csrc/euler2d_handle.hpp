@@ -38,7 +38,7 @@ struct gocfd_euler2d {
   void *scratch_in = nullptr;
   DevBuf<double> rhs_out;
   DevBuf<double> Jdet, Jinv, IInII, Nx, Ny, oohK, bxy, Interp64;
-  DevBuf<int> code, nbr_slot, halo_elem, halo_face, wave_list, wave_cnt, flux_list, flux_cnt, bnd_cnt;
+  DevBuf<int> code, nbr_slot, halo_elem, halo_face, wave_list, wave_cnt, wave_code, flux_list, flux_cnt, bnd_cnt;
   DevBuf<int> perm;  // [Kp] device element -> reference element (-1: padding)
   StageTypeBuf flux_geo;
   // halo traces (without shock capturing): export lists and the two buffers
@@ -89,7 +89,7 @@ struct gocfd_euler2d {
     a.g.Jdet = Jdet.p; a.g.Jinv = Jinv.p; a.g.IInII = IInII.p; a.g.Nx = Nx.p; a.g.Ny = Ny.p;
     a.g.oohK = oohK.p; a.g.code = code.p; a.g.nbr_slot = nbr_slot.p;
     a.g.halo_elem = halo_elem.p; a.g.halo_face = halo_face.p; a.g.bxy = bxy.p;
-    a.g.wave_list = wave_list.p; a.g.wave_cnt = wave_cnt.p;
+    a.g.wave_list = wave_list.p; a.g.wave_cnt = wave_cnt.p; a.g.wave_code = wave_code.p;
     a.g.flux_list = flux_list.p; a.g.flux_cnt = flux_cnt.p; a.g.bnd_cnt = bnd_cnt.p;
     a.g.flux_geo = flux_geo.ptr();
     a.g.xp_list = xp_list.p; a.g.xp_dst = xp_dst.p; a.g.xp_cnt = xp_cnt.p;

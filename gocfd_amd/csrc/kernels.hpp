@@ -39,6 +39,7 @@ struct Geom {
   const int *code, *nbr_slot;                          // [3][Kp]
   const int *halo_elem, *halo_face;                    // [ntiles][Hmax]
   const int *wave_list, *wave_cnt;                     // [ntiles][Wmax], [ntiles]
+  const int *wave_code;                                // [ntiles][Wmax]: code of each wave_list face
   const int *flux_list, *flux_cnt;                     // [ntiles][Fmax], [ntiles]
   // per flux-list item, in the stage arithmetic type: owner normal (nx, ny),
   // this side's signed IInII, the in-tile partner's IInII negated

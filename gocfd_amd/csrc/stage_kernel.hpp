@@ -410,7 +410,9 @@ __device__ __forceinline__ void stage_tile(const StageArgs &a, const StageLaunch
 
   // The tile's work lists and state do not depend on the control block:
   // their loads go out first, so that their latency overlaps the chain
-  // tile -> case -> control block and each other.
+  // tile -> case -> control block and each other. The lists are read to
+  // their padded lengths (zero past each tile's count), so that no load
+  // waits for a count; the counts bound the loops that use the items.
   constexpr int ITB = (E * NF + kThreads - 1) / kThreads;
   const int nfl = a.g.flux_cnt[tile] * NpEdge;
   const int *flist = a.g.flux_list + size_t(tile) * a.Fmax;
@@ -418,11 +420,11 @@ __device__ __forceinline__ void stage_tile(const StageArgs &a, const StageLaunch
 #pragma unroll
   for (int j = 0; j < ITB; ++j) {
     const int it = tid + j * kThreads;
-    fpk[j] = it < nfl ? flist[it / NpEdge] : 0;
+    fpk[j] = it < a.Fmax * NpEdge ? flist[it / NpEdge] : 0;
   }
   // the items' geometry and the elements' metric terms, to LDS below
   V4 fgv = V4{C(0), C(0), C(0), C(0)};
-  if (tid < a.g.flux_cnt[tile]) fgv = static_cast<const V4 *>(a.g.flux_geo)[size_t(tile) * a.Fmax + tid];
+  if (tid < a.Fmax) fgv = static_cast<const V4 *>(a.g.flux_geo)[size_t(tile) * a.Fmax + tid];
   // (wave 0: Jdet and Jinv[3]; wave c = 1..3: Jinv[c-1] of element te)
   double jm0, jm1 = 0.;
   {
@@ -435,11 +437,11 @@ __device__ __forceinline__ void stage_tile(const StageArgs &a, const StageLaunch
   int gw = 0, gc = 0, gx = 0, gd = 0;
   double gok = 0.;
   if constexpr (HX) {
-    if (tid < nwc) {
+    if (tid < a.Wmax) {
       gw = a.g.wave_list[size_t(tile) * a.Wmax + tid];
-      gc = a.g.code[size_t(gw >> 8) * Kp + k0 + (gw & 255)];
+      gc = a.g.wave_code[size_t(tile) * a.Wmax + tid];
     }
-    if (tid < nxc) {
+    if (tid < a.Xmax) {
       gx = a.g.xp_list[size_t(tile) * a.Xmax + tid];
       gd = a.g.xp_dst[size_t(tile) * a.Xmax + tid];
     }
@@ -491,7 +493,7 @@ __device__ __forceinline__ void stage_tile(const StageArgs &a, const StageLaunch
     for (int j = 0; j < ITB; ++j) {
       const int it = tid + j * kThreads;
       hx[j] = V4{C(0), C(0), C(0), C(0)};
-      if (it < nfl && (fpk[j] & (1 << 11))) {
+      if (fpk[j] & (1 << 11)) {  // (zero past the tile's items)
         const int w = it / NpEdge, pp = it - w * NpEdge;
         const int e = fpk[j] & 255, f = (fpk[j] >> 8) & 3;
         hx[j] = hin[(size_t(f) * Kp + k0 + e) * NpEdge + pp];
