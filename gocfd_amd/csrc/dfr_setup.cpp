@@ -852,9 +852,13 @@ DeviceLayout build_device_layout(const Discretisation &d, int E, int TP) {
     }
   }
   // Shared faces with Riemann work, per tile
+  // (cross-tile faces first: item i of tile t receives its neighbour traces
+  // in halo-trace slot t * Fmax + i, written by the neighbour's tile)
+  std::vector<int> item_of(size_t(3) * Kp, -1);  // flux-list item of each cross-tile face
   {
     std::vector<std::vector<int>> fl(L.ntiles);
     std::vector<std::vector<std::array<double, 4>>> fg(L.ntiles);
+    for (int pass = 0; pass < 2; ++pass)
     for (int kk = 0; kk < K; ++kk)
       for (int f = 0; f < 3; ++f) {
         const size_t fi = size_t(f) * Kp + kk;
@@ -863,6 +867,8 @@ DeviceLayout build_device_layout(const Discretisation &d, int E, int TP) {
         const bool in_tile = !(uint32_t(L.nbr_slot[fi]) >> 31);
         const bool owner = code & FC_FLUX_OWNER;
         if (in_tile && !owner) continue;  // the in-tile owner fills this slot
+        if (in_tile != (pass == 1)) continue;
+        if (!in_tile) item_of[fi] = int(fl[kk / E].size());
         const int base = int(uint32_t(L.nbr_slot[fi]) & 0x7fffffffu);
         if (base >= (1 << 18)) throw SetupError(GOCFD_ERR_MESH, "tile halo too large for the flux-list encoding");
         fl[kk / E].push_back((kk % E) | (f << 8) | ((in_tile && owner) ? 1 << 10 : 0) | (in_tile ? 0 : 1 << 11) |
@@ -893,7 +899,10 @@ DeviceLayout build_device_layout(const Discretisation &d, int E, int TP) {
       for (int f = 0; f < 3; ++f) {
         const size_t fi = size_t(f) * Kp + kk;
         if ((L.face_code[fi] & 15) != FK_Shared || !(uint32_t(L.nbr_slot[fi]) >> 31)) continue;
-        xp[kk / E].push_back({(kk % E) | (f << 8), L.nbr_face[fi] * Kp + L.nbr_elem[fi]});
+        // destination: the item of the neighbour's face in its tile's list
+        const size_t ni = size_t(L.nbr_face[fi]) * Kp + L.nbr_elem[fi];
+        if (item_of[ni] < 0) throw SetupError(GOCFD_ERR_MESH, "cross-tile face without a flux-list item");
+        xp[kk / E].push_back({(kk % E) | (f << 8), (L.nbr_elem[fi] / E) * L.Fmax + item_of[ni]});
       }
     L.Xmax = 1;
     for (auto &v : xp) L.Xmax = std::max<int>(L.Xmax, int(v.size()));

@@ -125,8 +125,9 @@ struct DeviceLayout {
   // Tiles with none skip that loop. [ntiles]
   std::vector<int> bnd_cnt;
   // Faces whose neighbour lies in another tile, per tile (the export list of
-  // the halo traces): [ntiles x Xmax] e | face << 8, the neighbour's face
-  // index nbr_face * Kp + nbr_elem, and [ntiles] counts
+  // the halo traces): [ntiles x Xmax] e | face << 8, the halo-trace item slot
+  // of the neighbour's face (its tile * Fmax + its flux-list item; the
+  // cross-tile faces lead each tile's flux list), and [ntiles] counts
   int Xmax = 0;
   std::vector<int> xp_list, xp_dst, xp_cnt;
   std::vector<double> bpoint_xy;                        // IVortex boundary points [nb][NpEdge][2]

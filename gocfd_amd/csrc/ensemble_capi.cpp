@@ -477,10 +477,10 @@ void build_group(gocfd_ensemble &h, EnsGroup &g) {
       }
       xp_cnt[t0 + t] = L.xp_cnt[t];
       for (int i = 0; i < L.xp_cnt[t]; ++i) {
-        // the neighbour face f * kp + kk of the case -> f * Kp + off + kk of the group
-        const int d = L.xp_dst[size_t(t) * L.Xmax + i], f = d / int(kp), kk = d - f * int(kp);
+        // the item slot tile * Fmax + item of the case -> of the group
+        const int d = L.xp_dst[size_t(t) * L.Xmax + i], dt = d / L.Fmax, di = d - dt * L.Fmax;
         xp_list[size_t(t0 + t) * g.Xmax + i] = L.xp_list[size_t(t) * L.Xmax + i];
-        xp_dst[size_t(t0 + t) * g.Xmax + i] = f * int(Kp) + int(o) + kk;
+        xp_dst[size_t(t0 + t) * g.Xmax + i] = (t0 + dt) * g.Fmax + di;
       }
     }
     fill_element_major(L, NpInt, c.Q0.data(), q0.data() + o * g.NV);
@@ -501,7 +501,7 @@ void build_group(gocfd_ensemble &h, EnsGroup &g) {
   g.tile_case.upload(tile_case);
   if (!g.shock) {
     g.xp_list.upload(xp_list); g.xp_dst.upload(xp_dst); g.xp_cnt.upload(xp_cnt);
-    g.ht.alloc(size_t(2) * 3 * Kp * NpEdge * 4);  // FP64 size, enough for either arithmetic
+    g.ht.alloc(size_t(2) * g.ntiles * g.Fmax * NpEdge * 4);  // FP64 size, enough for either arithmetic
   }
   const EnsCase &c0 = *h.cases[g.cases[0]];
   g.Interp64.upload(c0.d.ops.Interp);

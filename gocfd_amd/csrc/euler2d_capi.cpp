@@ -119,7 +119,7 @@ int gocfd_euler2d_create(const gocfd_mesh *mesh, const gocfd_params *p, gocfd_eu
   h->perm.upload(L.perm);
   if (!h->shock) {
     h->xp_list.upload(L.xp_list); h->xp_dst.upload(L.xp_dst); h->xp_cnt.upload(L.xp_cnt);
-    h->ht.alloc(size_t(2) * 3 * L.Kp * (h->d.N + 2) * 4);  // FP64 size, enough for either arithmetic
+    h->ht.alloc(size_t(2) * L.ntiles * L.Fmax * (h->d.N + 2) * 4);  // FP64 size, enough for either arithmetic
   }
   noalloc_setup(h->device);
   {

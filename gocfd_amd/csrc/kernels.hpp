@@ -123,9 +123,10 @@ struct StageArgs {
   int ring;            // global stage counter mod 3
   int par2;            // global stage counter mod 2
   int hpar;            // halo-trace buffer holding this stage's input traces
-  // Halo-trace buffers [2][3 Kp][NpEdge][4] in the stage arithmetic type
-  // (without shock capturing): the traces of each cross-tile face's
-  // neighbour, at the face's own points, written by the neighbour's tile
+  // Halo-trace buffers [2][ntiles Fmax][NpEdge][4] in the stage arithmetic
+  // type (without shock capturing): by tile and flux-list item, the traces
+  // of each cross-tile face's neighbour at the face's own points, written by
+  // the neighbour's tile
   void *ht;
   double t_rhs;        // time for the RHS-only evaluation
   void *qs;            // stage input state (element-major)

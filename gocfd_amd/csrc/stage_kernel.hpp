@@ -485,19 +485,19 @@ __device__ __forceinline__ void stage_tile(const StageArgs &a, const StageLaunch
       v[j] = (idx < E * NV) ? qt[idx] : S(0);
     }
   }
-  // the neighbour traces of the cross-tile faces, under the state loads
+  // the neighbour traces of the cross-tile faces, under the state loads:
+  // the cross-tile faces lead the flux list and their traces sit at the
+  // tile's item slots, so these loads need nothing else loaded first (items
+  // below Xmax that are not cross-tile read slots they never use)
   V4 hx[HX ? ITB : 1];
   if constexpr (HX) {
-    const V4 *hin = static_cast<const V4 *>(a.ht) + size_t(l.hpar) * 3 * size_t(Kp) * NpEdge;
+    const V4 *hin = static_cast<const V4 *>(a.ht) + size_t(l.hpar) * a.ntiles * a.Fmax * NpEdge +
+                    size_t(tile) * a.Fmax * NpEdge;
 #pragma unroll
     for (int j = 0; j < ITB; ++j) {
       const int it = tid + j * kThreads;
       hx[j] = V4{C(0), C(0), C(0), C(0)};
-      if (fpk[j] & (1 << 11)) {  // (zero past the tile's items)
-        const int w = it / NpEdge, pp = it - w * NpEdge;
-        const int e = fpk[j] & 255, f = (fpk[j] >> 8) & 3;
-        hx[j] = hin[(size_t(f) * Kp + k0 + e) * NpEdge + pp];
-      }
+      if (it < a.Xmax * NpEdge) hx[j] = hin[it];
     }
   }
 
@@ -1131,7 +1131,7 @@ __device__ __forceinline__ void stage_tile(const StageArgs &a, const StageLaunch
         if (lts) atomicMax(&sWf[e * 3 + f], bits(wsp));
       }
     }
-    V4 *hout = static_cast<V4 *>(a.ht) + size_t(l.hpar ^ 1) * 3 * size_t(Kp) * NpEdge;
+    V4 *hout = static_cast<V4 *>(a.ht) + size_t(l.hpar ^ 1) * a.ntiles * a.Fmax * NpEdge;
 #pragma unroll
     for (int j = 0; j < ITG; ++j) {
       const int it = tid + j * kThreads;
@@ -1257,7 +1257,7 @@ __global__ void __launch_bounds__(256) k_halo_export(StageArgs a) {
 #pragma unroll
   for (int i = 0; i < NpInt; ++i) q[i] = C(qn[i]);
   const C *Itp = Tabs<N, C>::ops() + Sh::NpFlux * NpInt;
-  C *hout = static_cast<C *>(a.ht) + size_t(a.hpar) * 3 * size_t(a.Kp) * NpEdge * 4;
+  C *hout = static_cast<C *>(a.ht) + size_t(a.hpar) * a.ntiles * a.Fmax * NpEdge * 4;
 #pragma unroll
   for (int pp = 0; pp < NpEdge; ++pp) {
     const C *row = Itp + (f * NpEdge + pp) * NpInt;
