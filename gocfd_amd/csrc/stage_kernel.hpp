@@ -62,12 +62,11 @@ struct Shape {
                        oVi = 4 * NpInt * NpInt, omf = 5 * NpInt * NpInt, oDiv = omf + NpInt,
                        oBary = oDiv + NR * NpFlux, shock_ops = oBary + 3 * NR;
   static constexpr int ops = (NpFlux + NF) * NpInt;
-  // MFMA form of the DivInt contraction (16x16x4 FP32 tiles): the interior
-  // DOFs in G groups of 4 points (one K-step for the r and one for the s
-  // components of a group), then the face DOFs in KE steps of 4
-  static constexpr int G = (NpInt + 3) / 4, KE = (NF + 3) / 4, KS = 2 * G + KE;
-  // MFMA form of the FluxEdgeInterp contraction: RB row blocks of 16 face
-  // points, KI K-steps of 4 solution points
+  // MFMA forms (16x16x4 tiles): the face DOFs of the DivInt contraction in
+  // KE K-steps of 4; the FluxEdgeInterp contraction in RB row blocks of 16
+  // face points over KI K-steps of 4 solution points (MfShape sets the
+  // K-steps the matrix-core stage path uses)
+  static constexpr int KE = (NF + 3) / 4;
   static constexpr int RB = (NF + 15) / 16, KI = (NpInt + 3) / 4;
 };
 
