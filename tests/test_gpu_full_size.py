@@ -27,8 +27,8 @@ pytestmark = pytest.mark.gpu
 
 
 @pytest.fixture(scope="module")
-def cfg2(tmp_path_factory):
-    return meshgen.vortex_mesh(str(tmp_path_factory.mktemp("cfg2") / "vortex200.su2"), 200)
+def vortex200(tmp_path_factory):
+    return meshgen.vortex_mesh(str(tmp_path_factory.mktemp("vortex200") / "vortex200.su2"), 200)
 
 
 def _vortex_density(t, X, Y):
@@ -43,10 +43,10 @@ def _l2(a, b):
 
 
 @pytest.mark.parametrize("prec", ["fp64", "fp32"])
-def test_freestream_preserved_full_size(cfg2, prec):
+def test_freestream_preserved_full_size(vortex200, prec):
     ip = InputParameters2D(CFL=1.0, FluxType="Roe", InitType="Freestream", PolynomialOrder=4, Minf=0.5,
                            Alpha=2.0, FinalTime=1e9)
-    g = Euler(ip, cfg2, precision=prec)
+    g = Euler(ip, vortex200, precision=prec)
     assert g.K == 80000
     q0 = g.Q
     g.step(20)
@@ -54,10 +54,10 @@ def test_freestream_preserved_full_size(cfg2, prec):
     assert np.abs(g.Q - q0).max() < bar * np.abs(q0).max()
 
 
-def test_vortex_full_size_parity_and_analytic(cfg2):
+def test_vortex_full_size_parity_and_analytic(vortex200):
     ip = InputParameters2D(CFL=1.0, FluxType="Roe", InitType="IVortex", PolynomialOrder=4, FinalTime=1e9)
-    g = Euler(ip, cfg2, precision="fp64")
-    o = oracle.OracleSolver(cfg2, 4, flux="roe", init="ivortex", CFL=1.0)
+    g = Euler(ip, vortex200, precision="fp64")
+    o = oracle.OracleSolver(vortex200, 4, flux="roe", init="ivortex", CFL=1.0)
     g.step(5)
     for _ in range(5):
         o.step()
@@ -69,7 +69,7 @@ def test_vortex_full_size_parity_and_analytic(cfg2):
     t, _, _ = g.time_info()
     c = g.coords()
     assert _l2(g.Q[0], _vortex_density(t, c["SolX"], c["SolY"])) < 1e-6
-    g32 = Euler(ip, cfg2, precision="fp32")
+    g32 = Euler(ip, vortex200, precision="fp32")
     g32.step(50)
     assert _l2(g32.Q[0], g.Q[0]) < 1e-5
 
