@@ -595,9 +595,10 @@ __device__ __forceinline__ void stage_tile(const StageArgs &a, const StageLaunch
   // point, scaled by the partner's ||n||), which is the value the reference
   // stores once per edge and reads back from both sides (edges.go:454-483).
   // With shock capturing (whose viscous penalty also reads the owner's trace
-  // at the reversed point) and in FP64, results stay in registers until
-  // every trace has been read. Boundary faces follow in a rolled loop: a boundary face point
-  // reads and writes only its own LDS slot.
+  // at the reversed point) results stay in registers until every trace of
+  // the tile has been read; in FP64 until the thread's last item. Boundary
+  // faces follow in a rolled loop: a boundary face point reads and writes
+  // only its own LDS slot.
   C fl[SHOCK || sizeof(C) == 8 ? ITB : 1][4];
 #pragma unroll
   for (int j = 0; j < ITB; ++j) {
@@ -658,8 +659,8 @@ __device__ __forceinline__ void stage_tile(const StageArgs &a, const StageLaunch
         // which reads and writes the partner's point NpEdge-1-pp; a
         // cross-tile face by its own item), so the item can overwrite the
         // slots it read with the scaled fluxes at once. Measured faster in
-        // FP32 only (FP64 keeps its results until the barrier: 75 against
-        // 81 us at order 4, 80,000 elements).
+        // FP32 only (FP64 writes its results after its last item: 75
+        // against 81 us at order 4, 80,000 elements).
         const C wn = gq[2];  // IInII, negated on the non-owner side
 #pragma unroll
         for (int c = 0; c < 4; ++c) sT[c * TS + p * kTP + e] = F[c] * wn;
@@ -674,8 +675,10 @@ __device__ __forceinline__ void stage_tile(const StageArgs &a, const StageLaunch
       }
     }
   }
-  constexpr bool HOLD = SHOCK || sizeof(C) == 8;  // results held until the barrier
-  if constexpr (HOLD) __syncthreads();
+  // results held until every item of the thread is done (FP64), or until
+  // every trace of the tile has been read (shock capturing: a barrier)
+  constexpr bool HOLD = SHOCK || sizeof(C) == 8;
+  if constexpr (SHOCK) __syncthreads();
 #pragma unroll
   for (int j = 0; j < (HOLD ? ITB : 0); ++j) {
     const int it = tid + j * kThreads;
