@@ -87,6 +87,14 @@ struct Vec4<float> { using type = floatx4; };
 template <>
 struct Vec4<double> { using type = doublex4; };
 
+// The FP64 shock operators (V, Vi, filter weights, sensor matrices,
+// barycentric weights) through the constant address space (they are
+// read-only during every kernel): uniform scalar loads where they are used.
+// Through a generic pointer, which an FP32 state cannot alias, the compiler
+// hoisted whole matrices into vector registers.
+using ConstD = const __attribute__((address_space(4))) double;
+__device__ __forceinline__ ConstD *shock_ops_c(const ShockArgs &s) { return (ConstD *)s.ops; }
+
 // 16x16x4 MFMA tiles in FP32 and FP64. Operands: lane l holds A[l%16][l/16]
 // and B[l/16][l%16]; result element v of lane l is D[row(l/16, v)][l%16],
 // row = 4 krow + v in FP32 and krow + 4 v in FP64 (both checked on gfx950).
@@ -832,7 +840,7 @@ __device__ __forceinline__ void stage_tile(const StageArgs &a, const StageLaunch
     for (int i = 0; i < NpInt; ++i) acc[i] *= moojd;
     if constexpr (SHOCK) {
       // modal filter of the RHS (limitAndFilterSolution dissipation.go:606-622)
-      const double *V = a.sh.ops + Sh::oV, *Vi = a.sh.ops + Sh::oVi, *mf = a.sh.ops + Sh::omf;
+      ConstD *V = shock_ops_c(a.sh) + Sh::oV, *Vi = shock_ops_c(a.sh) + Sh::oVi, *mf = shock_ops_c(a.sh) + Sh::omf;
       const double fac = 1. - sin(0.5 * M_PI * a.sh.sigma[kk]);
       double m[NpInt];
 #pragma unroll
@@ -1300,7 +1308,7 @@ __global__ void __launch_bounds__(256) k_shock_sensor(StageArgs a) {
   const CaseInfo &ci = a.cases[cs];
   if (kk >= ci.k_end || step_skipped(a, a.ctl + cs, ci.ph.final_time)) return;
   const SIN *rho = static_cast<const SIN *>(a.qs) + size_t(kk) * NV;
-  const double *Pm = a.sh.ops + Sh::oP, *Mm = a.sh.ops + Sh::oM, *Dm = a.sh.ops + Sh::oD;
+  ConstD *Pm = shock_ops_c(a.sh) + Sh::oP, *Mm = shock_ops_c(a.sh) + Sh::oM, *Dm = shock_ops_c(a.sh) + Sh::oD;
   double r[NpInt];
 #pragma unroll
   for (int i = 0; i < NpInt; ++i) r[i] = rho[i];
@@ -1391,12 +1399,12 @@ __global__ void __launch_bounds__(256) k_shock_prep(StageArgs a) {
   // an inactive element's eps is zero at every RT point; readers check the
   // flag instead of reading epsF
   if (act) {
-    const double *B = a.sh.ops + Sh::oBary;
+    ConstD *B = shock_ops_c(a.sh) + Sh::oBary;
 #pragma unroll
     for (int r = 0; r < NR; ++r) epsF[size_t(r) * a.Kp] = B[3 * r] * e0 + B[3 * r + 1] * e1 + B[3 * r + 2] * e2;
   }
   if (a.sh.filter_state) {
-    const double *V = a.sh.ops + Sh::oV, *Vi = a.sh.ops + Sh::oVi, *mf = a.sh.ops + Sh::omf;
+    ConstD *V = shock_ops_c(a.sh) + Sh::oV, *Vi = shock_ops_c(a.sh) + Sh::oVi, *mf = shock_ops_c(a.sh) + Sh::omf;
     const double fac = 1. - sin(0.5 * M_PI * sg);
     SIN *q = static_cast<SIN *>(a.qs) + size_t(kk) * NV;
     for (int n = 0; n < 4; ++n) {
