@@ -17,6 +17,19 @@ __device__ __forceinline__ double fdiv(double a, double b) { return a / b; }
 __device__ __forceinline__ float fdiv(float a, float b) { return a * __builtin_amdgcn_rcpf(b); }
 __device__ __forceinline__ double fsqrt(double x) { return sqrt(x); }
 __device__ __forceinline__ float fsqrt(float x) { return __builtin_amdgcn_sqrtf(x); }
+// Reciprocal for the reciprocal form of the Roe flux: in FP64 the hardware
+// estimate (v_rcp_f64) refined by two Newton steps, within an ulp of 1/x for
+// the normal-range operands it gets (densities, sums of their square roots,
+// sound speeds), in a third of the IEEE division's instructions; in FP32 the
+// hardware reciprocal.
+__device__ __forceinline__ double frcp(double x) {
+  double y = __builtin_amdgcn_rcp(x);
+  double e = fma(-x, y, 1.0);
+  y = fma(y, e, y);
+  e = fma(-x, y, 1.0);
+  return fma(y, e, y);
+}
+__device__ __forceinline__ float frcp(float x) { return __builtin_amdgcn_rcpf(x); }
 
 // GetFlowFunctionBase StaticPressure (model_problems/Euler2D/fluids.go:289-336)
 template <class C>
@@ -93,24 +106,28 @@ __device__ __forceinline__ void roe_flux(C g, const C L[4], const C R[4], C nx, 
   const C rULr = L[1] * nx + L[2] * ny, rVLr = L[1] * (-ny) + L[2] * nx;
   const C rURr = R[1] * nx + R[2] * ny, rVRr = R[1] * (-ny) + R[2] * nx;
   const C rhoL = L[0], rhoR = R[0];
-  const C pL = pressure(g, L[0], L[1], L[2], L[3]), pR = pressure(g, R[0], R[1], R[2], R[3]);
   const C sL = fsqrt(rhoL), sR = fsqrt(rhoR), sLR = sL + sR;
   const C rho = sL * sR;
-  C uL, vL, uR, vR, u, v, h, c2, cc, dW1, dW2, dW4;
+  C uL, vL, uR, vR, pL, pR, u, v, h, c2, cc, dW1, dW2, dW4;
   if constexpr (RCP) {
-    const C ooL = fdiv(C(1), rhoL), ooR = fdiv(C(1), rhoR), ooS = fdiv(C(1), sLR);
+    const C ooL = frcp(rhoL), ooR = frcp(rhoR), ooS = frcp(sLR);
     uL = rULr * ooL; vL = rVLr * ooL; uR = rURr * ooR; vR = rVRr * ooR;
+    // the static pressure from the rotated velocities (|u| is invariant)
+    pL = GM1 * (L[3] - C(0.5) * rhoL * (uL * uL + vL * vL));
+    pR = GM1 * (R[3] - C(0.5) * rhoR * (uR * uR + vR * vR));
     const C hL = (L[3] + pL) * ooL, hR = (R[3] + pR) * ooR;
     u = (sL * uL + sR * uR) * ooS;
     v = (sL * vL + sR * vR) * ooS;
     h = (sL * hL + sR * hR) * ooS;
     c2 = GM1 * (h - C(0.5) * (u * u + v * v));
     cc = fsqrt(c2);
-    const C ooc = fdiv(C(1), cc), ooc2 = ooc * ooc;
+    const C ooc = frcp(cc), ooc2 = ooc * ooc;
     dW1 = -C(0.5) * (rho * (uR - uL)) * ooc + C(0.5) * (pR - pL) * ooc2;
     dW2 = (rhoR - rhoL) - (pR - pL) * ooc2;
     dW4 = C(0.5) * (rho * (uR - uL)) * ooc + C(0.5) * (pR - pL) * ooc2;
   } else {
+    pL = pressure(g, L[0], L[1], L[2], L[3]);
+    pR = pressure(g, R[0], R[1], R[2], R[3]);
     uL = fdiv(rULr, L[0]); vL = fdiv(rVLr, L[0]); uR = fdiv(rURr, R[0]); vR = fdiv(rVRr, R[0]);
     const C hL = fdiv(L[3] + pL, rhoL), hR = fdiv(R[3] + pR, rhoR);
     u = fdiv(sL * uL + sR * uR, sLR);
