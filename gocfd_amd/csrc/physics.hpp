@@ -98,8 +98,9 @@ __device__ __forceinline__ void lax_flux(C g, const C L[4], const C R[4], C nx, 
 // RoeFlux (fluxes.go:284-413): normal frame, Roe averages, no entropy fix.
 // With RCP (the stage kernel without shock capturing) each division by rho,
 // by the sum of the square roots, by c and by c^2 is a multiplication by one
-// reciprocal: within about an ulp per operation of the divisions, and in
-// FP64 a third of the instructions of the IEEE division sequences.
+// reciprocal (frcp), and the static pressures come from the rotated
+// velocities: within about an ulp per operation of the divisions, with no
+// IEEE division sequence left.
 template <class C, bool RCP = false>
 __device__ __forceinline__ void roe_flux(C g, const C L[4], const C R[4], C nx, C ny, C F[4]) {
   const C GM1 = g - C(1);
