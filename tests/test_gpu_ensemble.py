@@ -99,6 +99,10 @@ def test_launch_configurations_are_bitwise_identical(tmp_path, prec):
     configs = [(1, 6, True), (1, 24, False), (0, 12, False)]
     if prec == "fp64":  # the multi-order launch is FP64 only (its FP32 variant is switched off)
         configs += [(2, 6, True), (2, 24, False), (2, 0, True)]
+    else:
+        with pytest.raises(GocfdError, match="no multi-order launch"):
+            ref.set_launch(2, 6, True)
+        assert ref.launch() == {"parallel": 0, "graph_steps": 6, "xcd_tiles": True}
     for par, gs, xcd in configs:
         ens = Ensemble(cases, precision=prec)
         ens.set_launch(par, gs, xcd)
@@ -106,6 +110,21 @@ def test_launch_configurations_are_bitwise_identical(tmp_path, prec):
         for c in range(len(cases)):
             assert np.array_equal(ens.state(c), ref.state(c)), (par, gs, xcd, c)
             assert ens.time_info(c) == ref.time_info(c)
+
+
+@pytest.mark.skip(reason="the FP32 multi-order stage kernel faulted the GPU in its first run; reading the kernel "
+                         "and its launch found no cause, so the launch mode is switched off (set_launch refuses it)")
+def test_fp32_multi_order_launch_is_bitwise_identical(tmp_path):
+    """The FP32 ensemble in one multi-order launch per stage against its
+    groups launched in sequence, bit for bit."""
+    cases = _cases(tmp_path)
+    ref = Ensemble(cases, precision="fp32")
+    ref.step(31)
+    ens = Ensemble(cases, precision="fp32")
+    ens.set_launch(2, 6, True)
+    ens.step(31)
+    for c in range(len(cases)):
+        assert np.array_equal(ens.state(c), ref.state(c)), c
 
 
 def test_tune_restores_the_initial_state(tmp_path):
