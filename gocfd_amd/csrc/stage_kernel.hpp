@@ -1303,15 +1303,18 @@ __global__ void __launch_bounds__(256) k_shock_sensor(StageArgs a) {
   // the element-fields pass of this stage builds the active-element list
   if (kk == 0) *a.sh.act_cnt = 0u;
   if (kk >= a.Kp) return;
+  // the element's density and size go out first, under the chain element ->
+  // case -> control block of the skip check (kk < Kp keeps them in bounds)
+  const SIN *rho = static_cast<const SIN *>(a.qs) + size_t(kk) * NV;
+  double r[NpInt];
+#pragma unroll
+  for (int i = 0; i < NpInt; ++i) r[i] = rho[i];
+  const double hK = a.sh.hK[kk];
   // the element's case (a batch holds several, each with its own control)
   const int cs = a.tile_case[kk / kTile];
   const CaseInfo &ci = a.cases[cs];
   if (kk >= ci.k_end || step_skipped(a, a.ctl + cs, ci.ph.final_time)) return;
-  const SIN *rho = static_cast<const SIN *>(a.qs) + size_t(kk) * NV;
   ConstD *Pm = shock_ops_c(a.sh) + Sh::oP, *Mm = shock_ops_c(a.sh) + Sh::oM, *Dm = shock_ops_c(a.sh) + Sh::oD;
-  double r[NpInt];
-#pragma unroll
-  for (int i = 0; i < NpInt; ++i) r[i] = rho[i];
   double num = 0, den = 0;
 #pragma unroll
   for (int i = 0; i < NpInt; ++i) {
@@ -1332,7 +1335,7 @@ __global__ void __launch_bounds__(256) k_shock_sensor(StageArgs a) {
   if (se < left) sg = 0.;
   else if (se >= left && se <= right) sg = 0.5 * (1. + sin(M_PI * ook * (se - S0)));
   else if (se > right) sg = 1.;
-  const double eps = a.sh.eps0 * a.sh.hK[kk] * sg;
+  const double eps = a.sh.eps0 * hK * sg;
   a.sh.Se[kk] = se;
   a.sh.eps[kk] = eps;
   unsigned long long *sv = a.sh.sigV + size_t(a.sh.vbuf) * a.sh.Nv;
@@ -1368,6 +1371,9 @@ __global__ void __launch_bounds__(256) k_shock_prep(StageArgs a) {
   }
   const int kk = gid;
   if (kk >= a.Kp) return;
+  // the element's vertices go out first, under the chain element -> case ->
+  // control block of the skip check (kk < Kp keeps them in bounds)
+  const int v0 = a.sh.evert[kk], v1 = a.sh.evert[size_t(a.Kp) + kk], v2 = a.sh.evert[2 * size_t(a.Kp) + kk];
   const int cs = a.tile_case[kk / kTile];
   const CaseInfo &ci = a.cases[cs];
   if (step_skipped(a, a.ctl + cs, ci.ph.final_time)) return;
@@ -1379,7 +1385,6 @@ __global__ void __launch_bounds__(256) k_shock_prep(StageArgs a) {
   }
   const unsigned long long *sv = a.sh.sigV + size_t(a.sh.vbuf) * a.sh.Nv;
   const unsigned long long *ev = a.sh.epsV + size_t(a.sh.vbuf) * a.sh.Nv;
-  const int v0 = a.sh.evert[kk], v1 = a.sh.evert[size_t(a.Kp) + kk], v2 = a.sh.evert[2 * size_t(a.Kp) + kk];
   const double sg = ((from_bits(sv[v0]) + from_bits(sv[v1])) + from_bits(sv[v2])) / 3.;
   a.sh.sigma[kk] = sg;
   const double e0 = from_bits(ev[v0]), e1 = from_bits(ev[v1]), e2 = from_bits(ev[v2]);
@@ -1443,23 +1448,38 @@ __global__ void __launch_bounds__(kThreads) k_shock_grad(StageArgs a) {
   __shared__ D sbuf[LW];
   __shared__ double red[2][kThreads / 64];
   const int tile = blockIdx.x, tid = threadIdx.x;
-  // the tile's case (a batch holds several, each with its own control)
-  const int cs = a.tile_case[tile];
-  const CaseInfo &ci = a.cases[cs];
-  Ctl *ctl = a.ctl + cs;
-  if (a.stage < 0 || step_skipped(a, ctl, ci.ph.final_time)) return;
   const int te = tid % E, tn = tid / E, k0 = tile * E, Kp = a.Kp;
   const SIN *Q = static_cast<const SIN *>(a.qs);
   const D *Itp = Tabs<N, D>::ops() + NpFlux * NpInt;
+  // The tile's state and the owned faces' codes, 1/hK and active flags do
+  // not depend on the control block (every tile lies inside the padded
+  // range): their loads go out first, under the chain tile -> case ->
+  // control block of the skip check.
+  constexpr int LPT = (E * NV + kThreads - 1) / kThreads;
+  SIN v[LPT];
   {
-    constexpr int LPT = (E * NV + kThreads - 1) / kThreads;
     const SIN *qt = Q + size_t(k0) * NV;
-    SIN v[LPT];
 #pragma unroll
     for (int j = 0; j < LPT; ++j) {
       const int idx = tid + j * kThreads;
       v[j] = idx < E * NV ? qt[idx] : SIN(0);
     }
+  }
+  const int we = tid / 3, wf = tid - 3 * we;  // (element, face) of the wave-speed threads
+  int wcode = 0;
+  double woohK = 0.0;
+  bool wact = false;
+  if (tid < 3 * E) {
+    wcode = a.g.code[size_t(wf) * Kp + k0 + we];
+    woohK = a.g.oohK[k0 + we];
+    wact = a.sh.active[k0 + we] != 0;
+  }
+  // the tile's case (a batch holds several, each with its own control)
+  const int cs = a.tile_case[tile];
+  const CaseInfo &ci = a.cases[cs];
+  Ctl *ctl = a.ctl + cs;
+  if (a.stage < 0 || step_skipped(a, ctl, ci.ph.final_time)) return;
+  {
 #pragma unroll
     for (int j = 0; j < LPT; ++j) {
       const int idx = tid + j * kThreads;
@@ -1493,13 +1513,13 @@ __global__ void __launch_bounds__(kThreads) k_shock_grad(StageArgs a) {
   int bad = 0;
   const double time = ctl->time[a.step_index & 1];
   if (tid < 3 * E) {
-    const int e = tid / 3, f = tid - 3 * e, k = k0 + e;
+    const int e = we, f = wf, k = k0 + e;
     const size_t fi = size_t(f) * Kp + k;
-    const int code = a.g.code[fi];
+    const int code = wcode;
     if (k < ci.k_end && (code & FC_WAVE)) {
       const int kind = code & 15;
-      const double oohK = a.g.oohK[k];
-      const bool actk = a.sh.active[k] != 0;  // else eps = 0 on the face
+      const double oohK = woohK;
+      const bool actk = wact;  // else eps = 0 on the face
       const double *ef = a.sh.epsF + size_t(NpInt + f * NpEdge) * Kp + k;
       double fwl = 0.0, fwv = 0.0;
       for (int pp = 0; pp < NpEdge; ++pp) {
@@ -1566,15 +1586,33 @@ __global__ void __launch_bounds__(256) k_shock_grad_act(StageArgs a) {
     D u[NpInt];
 #pragma unroll
     for (int i = 0; i < NpInt; ++i) u[i] = D(Q[size_t(kk) * NV + tn * NpInt + i]);
-    // edge values: the owner's trace at this face's points, in this face's order
-    D ue[NF];
+    // edge values: the owner's trace at this face's points, in this face's
+    // order. The three faces' neighbour entries, then the three source
+    // states (the element's own where it owns the face, so every address is
+    // valid), are loaded before any face branches: one latency each instead
+    // of two per face one after another.
+    bool own[3];
+    int nbf[3];
+    const SIN *qsrc[3];
 #pragma unroll
     for (int f = 0; f < 3; ++f) {
       const size_t fi = size_t(f) * Kp + kk;
       const int code = a.g.code[fi];
+      const int nb = a.sh.nbr_elem[fi];
+      nbf[f] = a.sh.nbr_face[fi];
       // unshared faces (boundary, padding) are always their own edge's owner
-      const bool owner = (code & FC_FLUX_OWNER) || (code & 15) != FK_Shared;
-      if (owner) {
+      own[f] = (code & FC_FLUX_OWNER) || (code & 15) != FK_Shared;
+      qsrc[f] = Q + size_t(own[f] ? kk : nb) * NV + tn * NpInt;
+    }
+    D un[3][NpInt];
+#pragma unroll
+    for (int f = 0; f < 3; ++f)
+#pragma unroll
+      for (int j = 0; j < NpInt; ++j) un[f][j] = D(qsrc[f][j]);
+    D ue[NF];
+#pragma unroll
+    for (int f = 0; f < 3; ++f) {
+      if (own[f]) {
 #pragma unroll
         for (int i = 0; i < NpEdge; ++i) {
           D t = 0;
@@ -1583,17 +1621,13 @@ __global__ void __launch_bounds__(256) k_shock_grad_act(StageArgs a) {
           ue[f * NpEdge + i] = t;
         }
       } else {
-        const int nb = a.sh.nbr_elem[fi], nf = a.sh.nbr_face[fi];
-        const SIN *qn = Q + size_t(nb) * NV + tn * NpInt;
-        D un[NpInt];
-#pragma unroll
-        for (int j = 0; j < NpInt; ++j) un[j] = D(qn[j]);
+        const int nf = nbf[f];
 #pragma unroll
         for (int i = 0; i < NpEdge; ++i) {
           const int p = nf * NpEdge + (NpEdge - 1 - i);
           D t = 0;
 #pragma unroll
-          for (int j = 0; j < NpInt; ++j) t += Itp[p * NpInt + j] * un[j];
+          for (int j = 0; j < NpInt; ++j) t += Itp[p * NpInt + j] * un[f][j];
           ue[f * NpEdge + i] = t;
         }
       }
