@@ -1580,7 +1580,11 @@ __global__ void __launch_bounds__(256) k_shock_grad_act(StageArgs a) {
   const int Kp = a.Kp;
   const SIN *Q = static_cast<const SIN *>(a.qs);
   const D *Itp = Tabs<N, D>::ops() + NpFlux * NpInt;
-  const D *Div = shock_ops<D>(a.sh) + Sh::oDiv;
+  // the RT divergence rows through the constant address space (read-only
+  // here): uniform scalar loads, where a generic pointer, which the stores
+  // below may alias, leaves them vector loads awaited row block by row block
+  using CD = const __attribute__((address_space(4))) D;
+  CD *Div = (CD *)(shock_ops<D>(a.sh) + Sh::oDiv);
   for (size_t it = size_t(blockIdx.x) * blockDim.x + threadIdx.x; it < n; it += size_t(gridDim.x) * blockDim.x) {
     const int kk = a.sh.act_list[it >> 3], tn = int(it >> 1) & 3, dir = int(it) & 1;
     D u[NpInt];
