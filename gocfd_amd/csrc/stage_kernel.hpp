@@ -1569,8 +1569,16 @@ __global__ void __launch_bounds__(kThreads) k_shock_grad(StageArgs a) {
 // not read it), from interior DOFs Jinv*Q and edge DOFs n*IInII/Jdet times
 // the owner's trace (read reversed by the non-owner). Thread = (list entry,
 // variable, direction); a grid-stride loop over the list that the
-// element-fields pass built.
+// element-fields pass built. In FP32, blockIdx.y picks one of three blocks
+// of RT points: a few percent of the elements are active, so one thread per
+// (entry, variable, direction) would leave about one wave per SIMD to wait
+// on its loads; the row blocks triple the waves in flight (each recomputes
+// the element's DOFs) and keep the table rows uniform per workgroup, so they
+// stay scalar loads. In FP64 the recomputed DOFs cost more than the waves
+// gain (measured), so one block holds every row.
 constexpr int kGradActBlocks = 1024;
+template <class D>
+constexpr int grad_act_row_groups() { return sizeof(D) == 4 ? 3 : 1; }
 template <int N, class SIN, class D>
 __global__ void __launch_bounds__(256) k_shock_grad_act(StageArgs a) {
   using Sh = Shape<N>;
@@ -1654,12 +1662,27 @@ __global__ void __launch_bounds__(256) k_shock_grad_act(StageArgs a) {
     }
     const double *epsF = a.sh.epsF + kk;                                   // [NR][Kp]
     D *dout = static_cast<D *>(a.sh.diss) + size_t(tn * 2 * NR) * Kp + kk;  // [4][2][NR][Kp]
+    constexpr int RG = grad_act_row_groups<D>();
+    if constexpr (RG == 1) {
 #pragma unroll 3
-    for (int r = 0; r < NR; ++r) {
-      D acc = 0;
+      for (int r = 0; r < NR; ++r) {
+        D acc = 0;
 #pragma unroll
-      for (int j = 0; j < NpFlux; ++j) acc += Div[r * NpFlux + j] * dof[j];
-      dout[size_t(dir * NR + r) * Kp] = acc * D(epsF[size_t(r) * Kp]);
+        for (int j = 0; j < NpFlux; ++j) acc += Div[r * NpFlux + j] * dof[j];
+        dout[size_t(dir * NR + r) * Kp] = acc * D(epsF[size_t(r) * Kp]);
+      }
+    } else {
+      constexpr int RPG = (NR + RG - 1) / RG;
+      const int r0 = int(blockIdx.y) * RPG;
+#pragma unroll
+      for (int rr = 0; rr < RPG; ++rr) {
+        const int r = r0 + rr;
+        if (r >= NR) break;
+        D acc = 0;
+#pragma unroll
+        for (int j = 0; j < NpFlux; ++j) acc += Div[r * NpFlux + j] * dof[j];
+        dout[size_t(dir * NR + r) * Kp] = acc * D(epsF[size_t(r) * Kp]);
+      }
     }
   }
 }
@@ -1757,7 +1780,8 @@ hipError_t launch_shock_t(int pass, const StageArgs &a, hipStream_t s) {
       break;
     case 3:
       hipLaunchKernelGGL((k_shock_grad_act<N, SIN, D>),
-                         dim3(std::max(1, std::min(kGradActBlocks, (a.Kp * 8 + 255) / 256))), dim3(256), 0, s, a);
+                         dim3(std::max(1, std::min(kGradActBlocks, (a.Kp * 8 + 255) / 256)), grad_act_row_groups<D>()),
+                         dim3(256), 0, s, a);
       break;
     default:
       return hipErrorInvalidValue;
