@@ -239,3 +239,38 @@ def test_wedge_shock_path_fp32(tmp_path):
     _, sg_o, _ = ora.shock()
     assert (sg_o > 0).sum() > 100
     assert np.mean((sg_g > 0) == (sg_o > 0)) > 0.99
+
+
+@pytest.mark.parametrize("N", [2, 3, 4])
+def test_shock_terms_fp32(vortex_meshes, N):
+    """The shock-capturing terms alone in FP32 bulk: the RHS with PerssonC0
+    minus the RHS without a limiter, on a state at rest with a 2:1 jump that
+    cuts through a column of elements (about 120 flagged), against the same
+    difference in the oracle and in the FP64 solver. The difference is the
+    RHS filter plus AddDissipation and the viscous edge flux; the oracle's
+    eps * grad Q reaches over a tenth of the pressure (the Euler flux scale
+    at rest), so the dissipation is a first-order part of it. In FP32 the
+    gradient pass splits the RT rows over three workgroup row blocks (9 rows
+    each at N = 3, the last block short); a block left out or misplaced
+    would move the difference far above the FP32 bar. Measured relative to
+    the difference's max: FP32 1.6e-6, 1.2e-5, 1.9e-5 at N = 2, 3, 4 (bar
+    1e-4); FP64 1.2e-14 to 1.0e-13 (bar 1e-12)."""
+    gs, os_ = _pair(vortex_meshes[20], N, "Roe", precision="fp32", Kappa=10.0)
+    gp, op = _pair(vortex_meshes[20], N, "Roe", precision="fp32", Limiter="")
+    g64s, _ = _pair(vortex_meshes[20], N, "Roe", precision="fp64", Kappa=10.0)
+    g64p, _ = _pair(vortex_meshes[20], N, "Roe", precision="fp64", Limiter="")
+    X = os_.coords()["SolX"]
+    v = np.where(X < 5.2, 1.0, 0.5)
+    q = np.stack([v, 0 * v, 0 * v, v / 0.4])
+    d_gpu = gs.rhs(q, 0.0) - gp.rhs(q, 0.0)
+    d_64 = g64s.rhs(q, 0.0) - g64p.rhs(q, 0.0)
+    d_ora = os_.rhs(q, 0.0) - op.rhs(q, 0.0)
+    _, sg_o, _ = os_.shock()
+    dis = os_.dissipation()
+    e_ora, e_64, e_64o = _rel(d_gpu, d_ora), _rel(d_gpu, d_64), _rel(d_64, d_ora)
+    print(f"N={N}: {int((sg_o > 0).sum())} flagged, max |eps grad Q| {np.abs(dis['DissX']).max():.3f}, "
+          f"fp32 vs oracle {e_ora:.3e}, fp32 vs fp64 {e_64:.3e}, fp64 vs oracle {e_64o:.3e}", flush=True)
+    assert (sg_o > 0).sum() > 100
+    assert np.abs(dis["DissX"]).max() > 0.1 * 1.0  # the pressure is 1 and 0.5
+    assert e_64o < 1e-12
+    assert e_ora < 1e-4 and e_64 < 1e-4, (e_ora, e_64)
