@@ -52,18 +52,34 @@ bool profiler_attached() {
   return false;
 }
 
-// Ends a stream capture that an exception interrupts: the stream leaves
-// capture mode, the partial graph is destroyed and the stage counters that
-// the capture advanced are restored, so that the handle stays usable.
+// Test hook: with GOCFD_TEST_FAIL_CAPTURE set (and not "0"), a graph
+// capture throws after its first captured step, as an error from a launch
+// inside the capture would; the tests use it to check that the handle stays
+// usable. No kernel is affected.
+bool capture_failure_requested() {
+  const char *e = std::getenv("GOCFD_TEST_FAIL_CAPTURE");
+  return e && *e && std::strcmp(e, "0") != 0;
+}
+
+// Ends a stream capture that an exception interrupts: branch streams forked
+// into the capture are joined back (an unjoined fork would make the end of
+// the capture fail), the stream leaves capture mode, the partial graph is
+// destroyed and the stage counters that the capture advanced are restored,
+// so that the handle stays usable.
 struct CaptureGuard {
   hipStream_t stream;
   long long *gstage;
   int *step_index;
   long long g0;
   int s0;
+  std::vector<std::pair<hipStream_t, hipEvent_t>> joins;  // forked branch streams, their join events
   bool active = true;
   ~CaptureGuard() {
     if (!active) return;
+    for (auto &j : joins) {
+      (void)hipEventRecord(j.second, j.first);
+      (void)hipStreamWaitEvent(stream, j.second, 0);
+    }
     hipGraph_t partial = nullptr;
     (void)hipStreamEndCapture(stream, &partial);
     if (partial) (void)hipGraphDestroy(partial);
@@ -299,9 +315,16 @@ struct gocfd_ensemble {
       const bool br = parallel == 1 && groups.size() > 1;
       if (br) {
         HIPCHECK(hipEventRecord(fork, stream));
-        for (auto &gr : groups) HIPCHECK(hipStreamWaitEvent(gr->stream, fork, 0));
+        for (auto &gr : groups) {
+          HIPCHECK(hipStreamWaitEvent(gr->stream, fork, 0));
+          guard.joins.push_back({gr->stream, gr->done});
+        }
       }
-      for (int j = 0; j < graph_steps; ++j) enqueue_step(br);
+      for (int j = 0; j < graph_steps; ++j) {
+        enqueue_step(br);
+        if (j == 0 && capture_failure_requested())
+          throw SetupError(GOCFD_ERR_DEVICE, "graph capture interrupted (GOCFD_TEST_FAIL_CAPTURE is set)");
+      }
       if (br) {
         for (auto &gr : groups) {
           HIPCHECK(hipEventRecord(gr->done, gr->stream));

@@ -127,6 +127,31 @@ def test_fp32_multi_order_launch_is_bitwise_identical(tmp_path):
         assert np.array_equal(ens.state(c), ref.state(c)), c
 
 
+@pytest.mark.parametrize("par", [0, 1])
+def test_interrupted_capture_leaves_the_ensemble_usable(tmp_path, monkeypatch, par):
+    """An error inside a graph capture (forced from the host by the
+    GOCFD_TEST_FAIL_CAPTURE hook after the first captured step; no kernel is
+    affected) is reported, ends the capture (joining the forked branch
+    streams of the parallel schedule) and leaves the handle usable: the
+    steps taken afterwards give the states of an uninterrupted run of the
+    same total length, bit for bit."""
+    cases = _cases(tmp_path)
+    ens = Ensemble(cases, precision="fp64")
+    ens.set_launch(par, 6, True)
+    monkeypatch.setenv("GOCFD_TEST_FAIL_CAPTURE", "1")
+    with pytest.raises(GocfdError, match="graph capture interrupted"):
+        ens.step(12)
+    monkeypatch.delenv("GOCFD_TEST_FAIL_CAPTURE")
+    done = ens.time_info(0)[2]  # eager steps before the capture started
+    ens.step(13)
+    ref = Ensemble(cases, precision="fp64")
+    ref.set_launch(par, 6, True)
+    ref.step(done + 13)
+    for c in range(len(cases)):
+        assert np.array_equal(ens.state(c), ref.state(c)), (par, c)
+        assert ens.time_info(c) == ref.time_info(c)
+
+
 def test_tune_restores_the_initial_state(tmp_path):
     cases = _cases(tmp_path)
     ens = Ensemble(cases, precision="fp32")
